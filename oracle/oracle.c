@@ -1,0 +1,1351 @@
+/*
+ * CPU FP64 oracle -- TEST INFRASTRUCTURE ONLY (see oracle.h for the list of
+ * restated reference functions and how each part is pinned).
+ *
+ * Built with -ffp-contract=off so that a*b+c is never fused: the reference is
+ * compiled for baseline x86-64, where no fused multiply-add is emitted, and
+ * this file keeps its operation order.
+ */
+#include "oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* ct_defs.h:84-176 */
+#define AVOGADRO 6.02214076e26
+#define BOLTZMANN 1.380649e-23
+#define GAS_CONSTANT (AVOGADRO * BOLTZMANN)
+#define SMALL_NUMBER 1e-300
+#define BIG_NUMBER 1e300
+
+#define MAXR 8   /* max distinct species per reaction side */
+#define MAXTB 24 /* max explicit third-body efficiencies */
+
+static _Thread_local char g_err[512];
+const char* or_last_error(void) { return g_err; }
+static void set_err(const char* s) { snprintf(g_err, sizeof g_err, "%s", s); }
+
+/* ---- StoichManagerN restatement (StoichManager.h:131-366, 676-744) ---- */
+typedef struct { int rxn, n, k[3]; } c123_t;
+typedef struct { int rxn, n, k[MAXR]; double order[MAXR], stoich[MAXR]; } cn_t;
+typedef struct {
+    int n1, n2, n3, nn, cap;
+    c123_t *c1, *c2, *c3;
+    cn_t* cn;
+} stoich_t;
+
+static void stoich_init(stoich_t* s, int cap) {
+    memset(s, 0, sizeof *s);
+    s->cap = cap;
+    s->c1 = calloc(cap, sizeof(c123_t));
+    s->c2 = calloc(cap, sizeof(c123_t));
+    s->c3 = calloc(cap, sizeof(c123_t));
+    s->cn = calloc(cap, sizeof(cn_t));
+}
+static void stoich_free(stoich_t* s) { free(s->c1); free(s->c2); free(s->c3); free(s->cn); }
+
+/* StoichManagerN::add (StoichManager.h:676-723) */
+static void stoich_add(stoich_t* s, int rxn, int n, const int* k, const double* order, const double* st) {
+    int frac = 0;
+    for (int i = 0; i < n; i++)
+        if (fmod(st[i], 1.0) != 0.0 || st[i] != order[i]) frac = 1;
+    if (!frac && n <= 3) {
+        int krep[8], nrep = 0;
+        for (int i = 0; i < n; i++)
+            for (int j = 0; j < st[i]; j++)
+                if (nrep < 8) krep[nrep++] = k[i];
+        c123_t c = {rxn, nrep, {0, 0, 0}};
+        for (int j = 0; j < nrep && j < 3; j++) c.k[j] = krep[j];
+        if (nrep == 1) { s->c1[s->n1++] = c; return; }
+        if (nrep == 2) { s->c2[s->n2++] = c; return; }
+        if (nrep == 3) { s->c3[s->n3++] = c; return; }
+    }
+    cn_t c;
+    memset(&c, 0, sizeof c);
+    c.rxn = rxn;
+    c.n = n;
+    for (int i = 0; i < n; i++) { c.k[i] = k[i]; c.order[i] = order[i]; c.stoich[i] = st[i]; }
+    s->cn[s->nn++] = c;
+}
+
+static void stoich_multiply(const stoich_t* s, const double* S, double* R) {
+    for (int i = 0; i < s->n1; i++) R[s->c1[i].rxn] *= S[s->c1[i].k[0]];
+    for (int i = 0; i < s->n2; i++) {
+        const c123_t* c = &s->c2[i];
+        if (S[c->k[0]] < 0 && S[c->k[1]] < 0) R[c->rxn] = 0;
+        else R[c->rxn] *= S[c->k[0]] * S[c->k[1]];
+    }
+    for (int i = 0; i < s->n3; i++) {
+        const c123_t* c = &s->c3[i];
+        double a = S[c->k[0]], b = S[c->k[1]], d = S[c->k[2]];
+        if ((a < 0 && (b < 0 || d < 0)) || (b < 0 && d < 0)) R[c->rxn] = 0;
+        else R[c->rxn] *= a * b * d;
+    }
+    for (int i = 0; i < s->nn; i++) {
+        const cn_t* c = &s->cn[i];
+        for (int j = 0; j < c->n; j++) {
+            double order = c->order[j];
+            if (order != 0.0) {
+                double x = S[c->k[j]];
+                if (x > 0.0) R[c->rxn] *= pow(x, order);
+                else R[c->rxn] = 0.0;
+            }
+        }
+    }
+}
+
+static void stoich_inc_species(const stoich_t* s, const double* R, double* S, double sign) {
+    for (int i = 0; i < s->n1; i++) {
+        if (sign > 0) S[s->c1[i].k[0]] += R[s->c1[i].rxn];
+        else S[s->c1[i].k[0]] -= R[s->c1[i].rxn];
+    }
+    for (int i = 0; i < s->n2; i++) {
+        const c123_t* c = &s->c2[i];
+        if (sign > 0) { S[c->k[0]] += R[c->rxn]; S[c->k[1]] += R[c->rxn]; }
+        else { S[c->k[0]] -= R[c->rxn]; S[c->k[1]] -= R[c->rxn]; }
+    }
+    for (int i = 0; i < s->n3; i++) {
+        const c123_t* c = &s->c3[i];
+        for (int j = 0; j < 3; j++) {
+            if (sign > 0) S[c->k[j]] += R[c->rxn];
+            else S[c->k[j]] -= R[c->rxn];
+        }
+    }
+    for (int i = 0; i < s->nn; i++) {
+        const cn_t* c = &s->cn[i];
+        double x = R[c->rxn];
+        for (int j = 0; j < c->n; j++) {
+            if (sign > 0) S[c->k[j]] += c->stoich[j] * x;
+            else S[c->k[j]] -= c->stoich[j] * x;
+        }
+    }
+}
+
+static void stoich_inc_reactions(const stoich_t* s, const double* S, double* R, double sign) {
+    for (int i = 0; i < s->n1; i++) {
+        if (sign > 0) R[s->c1[i].rxn] += S[s->c1[i].k[0]];
+        else R[s->c1[i].rxn] -= S[s->c1[i].k[0]];
+    }
+    for (int i = 0; i < s->n2; i++) {
+        const c123_t* c = &s->c2[i];
+        double v = S[c->k[0]] + S[c->k[1]];
+        if (sign > 0) R[c->rxn] += v;
+        else R[c->rxn] -= v;
+    }
+    for (int i = 0; i < s->n3; i++) {
+        const c123_t* c = &s->c3[i];
+        double v = S[c->k[0]] + S[c->k[1]] + S[c->k[2]];
+        if (sign > 0) R[c->rxn] += v;
+        else R[c->rxn] -= v;
+    }
+    for (int i = 0; i < s->nn; i++) {
+        const cn_t* c = &s->cn[i];
+        for (int j = 0; j < c->n; j++) {
+            if (sign > 0) R[c->rxn] += c->stoich[j] * S[c->k[j]];
+            else R[c->rxn] -= c->stoich[j] * S[c->k[j]];
+        }
+    }
+}
+
+/* ---- mechanism ---- */
+struct or_mech {
+    int K, I, E;
+    char (*names)[48];
+    double *mw, *rmw, *comp, *tmin, *tmid, *tmax, *pref, *clow, *chigh;
+    int *type, *rev;
+    double *dn, *A, *b, *EaR, *A0, *b0, *EaR0, *ta, *rt3, *rt1, *t2;
+    int *nr, *np, *ntb;
+    int (*rk)[MAXR], (*pk)[MAXR], (*tbk)[MAXTB];
+    double (*rs)[MAXR], (*ro)[MAXR], (*ps)[MAXR], (*tbe)[MAXTB];
+    double* tbdef;
+    stoich_t reac, prod, revprod;
+};
+
+static int next_tok(char** p, char* out, int cap) {
+    char* s = *p;
+    while (*s == ' ' || *s == '\t') s++;
+    if (!*s || *s == '\n') return 0;
+    int n = 0;
+    while (*s && *s != ' ' && *s != '\t' && *s != '\n') {
+        if (n < cap - 1) out[n++] = *s;
+        s++;
+    }
+    out[n] = 0;
+    *p = s;
+    return 1;
+}
+static double tokd(char** p) {
+    char t[64];
+    if (!next_tok(p, t, sizeof t)) return NAN;
+    return strtod(t, NULL);
+}
+static int toki(char** p) {
+    char t[64];
+    if (!next_tok(p, t, sizeof t)) return -1;
+    return atoi(t);
+}
+
+or_mech* or_mech_load(const char* path) {
+    FILE* f = fopen(path, "r");
+    if (!f) { set_err("cannot open mechanism file"); return NULL; }
+    or_mech* m = calloc(1, sizeof *m);
+    size_t cap = 1 << 16;
+    char* line = malloc(cap);
+    char tok[256];
+    int ok = 0;
+    while (fgets(line, (int)cap, f)) {
+        char* p = line;
+        if (!next_tok(&p, tok, sizeof tok)) continue;
+        if (!strcmp(tok, "elements")) {
+            m->E = toki(&p);
+        } else if (!strcmp(tok, "species")) {
+            m->K = toki(&p);
+            int K = m->K;
+            m->names = calloc(K, sizeof *m->names);
+            m->mw = calloc(K, 8); m->rmw = calloc(K, 8); m->comp = calloc((size_t)K * (m->E > 0 ? m->E : 1), 8);
+            m->tmin = calloc(K, 8); m->tmid = calloc(K, 8); m->tmax = calloc(K, 8); m->pref = calloc(K, 8);
+            m->clow = calloc(7 * K, 8); m->chigh = calloc(7 * K, 8);
+        } else if (!strcmp(tok, "S")) {
+            int k = toki(&p);
+            next_tok(&p, m->names[k], 48);
+            m->mw[k] = tokd(&p);
+            m->rmw[k] = 1.0 / m->mw[k];
+            m->tmin[k] = tokd(&p); m->tmid[k] = tokd(&p); m->tmax[k] = tokd(&p); m->pref[k] = tokd(&p);
+            for (int j = 0; j < 7; j++) m->clow[7 * k + j] = tokd(&p);
+            for (int j = 0; j < 7; j++) m->chigh[7 * k + j] = tokd(&p);
+            for (int e = 0; e < m->E; e++) m->comp[k * m->E + e] = tokd(&p);
+        } else if (!strcmp(tok, "reactions")) {
+            int I = m->I = toki(&p);
+            m->type = calloc(I, 4); m->rev = calloc(I, 4);
+            m->dn = calloc(I, 8); m->A = calloc(I, 8); m->b = calloc(I, 8); m->EaR = calloc(I, 8);
+            m->A0 = calloc(I, 8); m->b0 = calloc(I, 8); m->EaR0 = calloc(I, 8);
+            m->ta = calloc(I, 8); m->rt3 = calloc(I, 8); m->rt1 = calloc(I, 8); m->t2 = calloc(I, 8);
+            m->nr = calloc(I, 4); m->np = calloc(I, 4); m->ntb = calloc(I, 4);
+            m->rk = calloc(I, sizeof *m->rk); m->pk = calloc(I, sizeof *m->pk); m->tbk = calloc(I, sizeof *m->tbk);
+            m->rs = calloc(I, sizeof *m->rs); m->ro = calloc(I, sizeof *m->ro); m->ps = calloc(I, sizeof *m->ps);
+            m->tbe = calloc(I, sizeof *m->tbe); m->tbdef = calloc(I, 8);
+            stoich_init(&m->reac, I + 1);
+            stoich_init(&m->prod, I + 1);
+            stoich_init(&m->revprod, I + 1);
+        } else if (!strcmp(tok, "R")) {
+            int i = toki(&p);
+            m->type[i] = toki(&p); m->rev[i] = toki(&p); m->dn[i] = tokd(&p);
+            m->A[i] = tokd(&p); m->b[i] = tokd(&p); m->EaR[i] = tokd(&p);
+            m->A0[i] = tokd(&p); m->b0[i] = tokd(&p); m->EaR0[i] = tokd(&p);
+            m->ta[i] = tokd(&p); m->rt3[i] = tokd(&p); m->rt1[i] = tokd(&p); m->t2[i] = tokd(&p);
+            m->nr[i] = toki(&p);
+            if (m->nr[i] > MAXR) { set_err("too many reactants"); goto fail; }
+            for (int j = 0; j < m->nr[i]; j++) { m->rk[i][j] = toki(&p); m->rs[i][j] = tokd(&p); m->ro[i][j] = tokd(&p); }
+            m->np[i] = toki(&p);
+            if (m->np[i] > MAXR) { set_err("too many products"); goto fail; }
+            for (int j = 0; j < m->np[i]; j++) { m->pk[i][j] = toki(&p); m->ps[i][j] = tokd(&p); }
+            m->tbdef[i] = tokd(&p);
+            m->ntb[i] = toki(&p);
+            if (m->ntb[i] > MAXTB) { set_err("too many efficiencies"); goto fail; }
+            for (int j = 0; j < m->ntb[i]; j++) { m->tbk[i][j] = toki(&p); m->tbe[i][j] = tokd(&p); }
+            /* Kinetics::addReaction (Kinetics.cpp:740-781) */
+            stoich_add(&m->reac, i, m->nr[i], m->rk[i], m->ro[i], m->rs[i]);
+            stoich_add(&m->prod, i, m->np[i], m->pk[i], m->ps[i], m->ps[i]);
+            if (m->rev[i]) stoich_add(&m->revprod, i, m->np[i], m->pk[i], m->ps[i], m->ps[i]);
+        } else if (!strcmp(tok, "end")) {
+            ok = 1;
+        }
+    }
+    free(line);
+    fclose(f);
+    if (!ok || m->K <= 0) { set_err("truncated mechanism file"); or_mech_free(m); return NULL; }
+    return m;
+fail:
+    free(line);
+    fclose(f);
+    or_mech_free(m);
+    return NULL;
+}
+
+void or_mech_free(or_mech* m) {
+    if (!m) return;
+    free(m->names); free(m->mw); free(m->rmw); free(m->comp); free(m->tmin); free(m->tmid); free(m->tmax);
+    free(m->pref); free(m->clow); free(m->chigh); free(m->type); free(m->rev); free(m->dn); free(m->A);
+    free(m->b); free(m->EaR); free(m->A0); free(m->b0); free(m->EaR0); free(m->ta); free(m->rt3);
+    free(m->rt1); free(m->t2); free(m->nr); free(m->np); free(m->ntb); free(m->rk); free(m->pk);
+    free(m->tbk); free(m->rs); free(m->ro); free(m->ps); free(m->tbe); free(m->tbdef);
+    if (m->reac.cap) { stoich_free(&m->reac); stoich_free(&m->prod); stoich_free(&m->revprod); }
+    free(m);
+}
+
+int or_mech_nspecies(const or_mech* m) { return m->K; }
+int or_mech_nreactions(const or_mech* m) { return m->I; }
+int or_mech_nelements(const or_mech* m) { return m->E; }
+void or_mech_mw(const or_mech* m, double* mw) { memcpy(mw, m->mw, sizeof(double) * m->K); }
+void or_mech_composition(const or_mech* m, double* comp) { memcpy(comp, m->comp, sizeof(double) * m->K * m->E); }
+int or_mech_species_index(const or_mech* m, const char* name) {
+    for (int k = 0; k < m->K; k++)
+        if (!strcmp(m->names[k], name)) return k;
+    return -1;
+}
+
+/* ---- thermo (NasaPoly1.h:83-122, NasaPoly2.h:105-111) ---- */
+static void nasa7(const or_mech* m, int k, const double* tt, double* cp, double* h, double* s) {
+    const double* c = (tt[0] <= m->tmid[k]) ? &m->clow[7 * k] : &m->chigh[7 * k];
+    double ct0 = c[0];
+    double ct1 = c[1] * tt[0];
+    double ct2 = c[2] * tt[1];
+    double ct3 = c[3] * tt[2];
+    double ct4 = c[4] * tt[3];
+    *cp = ct0 + ct1 + ct2 + ct3 + ct4;
+    *h = ct0 + 0.5 * ct1 + 1.0 / 3.0 * ct2 + 0.25 * ct3 + 0.2 * ct4 + c[5] * tt[4];
+    *s = ct0 * tt[5] + ct1 + 0.5 * ct2 + 1.0 / 3.0 * ct3 + 0.25 * ct4 + c[6];
+}
+static void tpoly(double T, double* tt) {
+    tt[0] = T;
+    tt[1] = T * T;
+    tt[2] = tt[1] * T;
+    tt[3] = tt[2] * T;
+    tt[4] = 1.0 / T;
+    tt[5] = log(T);
+}
+
+void or_thermo(const or_mech* m, double T, double* cp_R, double* h_RT, double* s_R) {
+    double tt[6];
+    tpoly(T, tt);
+    for (int k = 0; k < m->K; k++) nasa7(m, k, tt, &cp_R[k], &h_RT[k], &s_R[k]);
+}
+
+/* ---- workspace + state ---- */
+typedef struct {
+    int K, I;
+    double *ym, *cp_R, *h_RT, *s_R, *grt, *conc, *concm, *delta, *rkcn, *kf, *ropf, *ropr, *ropnet, *wdot;
+    double T, P, mmw, RT, dens, molar_density, pressure, ctot;
+} work_t;
+
+static work_t* work_new(const or_mech* m) {
+    work_t* w = calloc(1, sizeof *w);
+    int K = m->K, I = m->I;
+    w->K = K; w->I = I;
+    double* buf = calloc((size_t)7 * K + 8 * (size_t)I + 8, 8);
+    w->ym = buf; w->cp_R = w->ym + K; w->h_RT = w->cp_R + K; w->s_R = w->h_RT + K; w->grt = w->s_R + K;
+    w->conc = w->grt + K; w->wdot = w->conc + K;
+    w->concm = w->wdot + K; w->delta = w->concm + I; w->rkcn = w->delta + I; w->kf = w->rkcn + I;
+    w->ropf = w->kf + I; w->ropr = w->ropf + I; w->ropnet = w->ropr + I;
+    return w;
+}
+static void work_free(work_t* w) { if (w) { free(w->ym); free(w); } }
+
+/* setMassFractions_NoNorm + setState_TP (Phase.cpp:362-372; ThermoPhase.cpp:187-198;
+ * IdealGasPhase.h:338-354) */
+static void set_state(const or_mech* m, work_t* w, double T, double P, const double* Y) {
+    double sum = 0.0;
+    for (int k = 0; k < m->K; k++) w->ym[k] = Y[k] * m->rmw[k];
+    for (int k = 0; k < m->K; k++) sum += w->ym[k];
+    w->mmw = 1.0 / sum;
+    w->T = T;
+    w->RT = T * GAS_CONSTANT;
+    w->dens = P * w->mmw / w->RT;
+    w->molar_density = w->dens / w->mmw;
+    w->pressure = GAS_CONSTANT * w->molar_density * T;
+    w->P = P;
+}
+
+/* BulkKinetics::updateROP (BulkKinetics.cpp:457-545) + getNetProductionRates */
+static void update_rop(const or_mech* m, work_t* w) {
+    const int K = m->K, I = m->I;
+    const double T = w->T, RT = w->RT;
+    double tt[6];
+    tpoly(T, tt);
+    for (int k = 0; k < K; k++) nasa7(m, k, tt, &w->cp_R[k], &w->h_RT[k], &w->s_R[k]);
+    /* getStandardChemPotentials: g0_RT*RT + RT*ln(P/Pref) */
+    double tmp = log(w->pressure / m->pref[0]) * RT;
+    for (int k = 0; k < K; k++) {
+        double g0 = w->h_RT[k] - w->s_R[k];
+        w->grt[k] = g0 * RT;
+        w->grt[k] += tmp;
+    }
+    for (int i = 0; i < I; i++) w->delta[i] = 0.0;
+    stoich_inc_reactions(&m->revprod, w->grt, w->delta, +1);
+    stoich_inc_reactions(&m->reac, w->grt, w->delta, -1);
+    double logStandConc = log(w->pressure / RT);
+    double rrt = 1.0 / RT;
+    for (int i = 0; i < I; i++) {
+        if (m->rev[i]) {
+            double v = exp(w->delta[i] * rrt - m->dn[i] * logStandConc);
+            w->rkcn[i] = v < BIG_NUMBER ? v : BIG_NUMBER;
+        } else {
+            w->rkcn[i] = 0.0;
+        }
+    }
+    /* concentrations and third bodies */
+    for (int k = 0; k < K; k++) w->conc[k] = w->ym[k] * w->dens;
+    w->ctot = w->dens / w->mmw;
+    for (int i = 0; i < I; i++) {
+        if (m->type[i] == 0) continue;
+        double sum = 0.0;
+        for (int j = 0; j < m->ntb[i]; j++) sum += m->tbe[i][j] * w->conc[m->tbk[i][j]];
+        w->concm[i] = m->tbdef[i] * w->ctot + sum;
+    }
+    /* rate constants */
+    double logT = log(T), recipT = 1.0 / T;
+    for (int i = 0; i < I; i++) {
+        if (m->type[i] <= 1) {
+            w->kf[i] = m->A[i] * exp(m->b[i] * logT - m->EaR[i] * recipT);
+        } else {
+            double work = 0.0;
+            if (m->type[i] == 3) {
+                double Fcent = (1.0 - m->ta[i]) * exp(-T * m->rt3[i]) + m->ta[i] * exp(-T * m->rt1[i]);
+                if (m->t2[i] != 0.0) Fcent += exp(-m->t2[i] / T);
+                work = log10(Fcent > SMALL_NUMBER ? Fcent : SMALL_NUMBER);
+            }
+            double rc_low = m->A0[i] * exp(m->b0[i] * logT - m->EaR0[i] * recipT);
+            double rc_high = m->A[i] * exp(m->b[i] * logT - m->EaR[i] * recipT);
+            double pr = w->concm[i] * rc_low / (rc_high + SMALL_NUMBER);
+            double F = 1.0;
+            if (m->type[i] == 3) {
+                double lpr = log10(pr > SMALL_NUMBER ? pr : SMALL_NUMBER);
+                double cc = -0.4 - 0.67 * work;
+                double nn = 0.75 - 1.27 * work;
+                double f1 = (lpr + cc) / (nn - 0.14 * (lpr + cc));
+                double lgf = work / (1.0 + f1 * f1);
+                F = pow(10.0, lgf);
+            }
+            pr *= F / (1.0 + pr);
+            w->kf[i] = pr * rc_high;
+        }
+    }
+    for (int i = 0; i < I; i++) {
+        w->ropf[i] = w->kf[i] * 1.0; /* m_perturb */
+        if (m->type[i] == 1) w->ropf[i] *= w->concm[i];
+        w->ropr[i] = w->ropf[i];
+        w->ropr[i] *= w->rkcn[i];
+    }
+    stoich_multiply(&m->reac, w->conc, w->ropf);
+    stoich_multiply(&m->revprod, w->conc, w->ropr);
+    for (int i = 0; i < I; i++) w->ropnet[i] = w->ropf[i] - w->ropr[i];
+    for (int k = 0; k < K; k++) w->wdot[k] = 0.0;
+    stoich_inc_species(&m->prod, w->ropnet, w->wdot, +1);
+    stoich_inc_species(&m->reac, w->ropnet, w->wdot, -1);
+}
+
+int or_rates(const or_mech* m, double T, double P, const double* Y, double* wdot,
+             double* ropf, double* ropr, double* kf, double* rkcn) {
+    work_t* w = work_new(m);
+    set_state(m, w, T, P, Y);
+    update_rop(m, w);
+    if (wdot) memcpy(wdot, w->wdot, 8 * m->K);
+    if (ropf) memcpy(ropf, w->ropf, 8 * m->I);
+    if (ropr) memcpy(ropr, w->ropr, 8 * m->I);
+    if (kf) memcpy(kf, w->kf, 8 * m->I);
+    if (rkcn) memcpy(rkcn, w->rkcn, 8 * m->I);
+    work_free(w);
+    return 0;
+}
+
+/* BulkKinetics::getEquilibriumConstants (BulkKinetics.cpp:163-179) */
+int or_equilibrium_constants(const or_mech* m, double T, double P, const double* Y, double* Kc) {
+    work_t* w = work_new(m);
+    set_state(m, w, T, P, Y);
+    update_rop(m, w);
+    for (int i = 0; i < m->I; i++) w->delta[i] = 0.0;
+    stoich_inc_reactions(&m->prod, w->grt, w->delta, +1);
+    stoich_inc_reactions(&m->reac, w->grt, w->delta, -1);
+    double rrt = 1.0 / w->RT;
+    double logStandConc = log(w->pressure / w->RT);
+    for (int i = 0; i < m->I; i++) Kc[i] = exp(-w->delta[i] * rrt + m->dn[i] * logStandConc);
+    work_free(w);
+    return 0;
+}
+
+int or_rates_batch(const or_mech* m, int n, const double* T, const double* P, const double* Y,
+                   double* wdot, int nthreads) {
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+    (void)nthreads;
+#endif
+    const int K = m->K;
+#pragma omp parallel
+    {
+        work_t* w = work_new(m);
+        double* y = malloc(8 * K);
+#pragma omp for schedule(static)
+        for (int s = 0; s < n; s++) {
+            for (int k = 0; k < K; k++) y[k] = Y[(size_t)k * n + s];
+            set_state(m, w, T[s], P[s], y);
+            update_rop(m, w);
+            for (int k = 0; k < K; k++) wdot[(size_t)k * n + s] = w->wdot[k];
+        }
+        free(y);
+        work_free(w);
+    }
+    return 0;
+}
+
+/* ---- reactor RHS: IdealGasConstPressureReactor::updateState/eval and
+ * ReactorNet::eval ---- */
+static int rhs_w(const or_mech* m, work_t* w, double P, const double* y, double* ydot) {
+    const int K = m->K;
+    double mass = y[0];
+    set_state(m, w, y[1], P, y + 2);
+    double vol = mass / w->dens;
+    update_rop(m, w);
+    double mcpdTdt = 0.0;
+    mcpdTdt += 0.0;        /* m_Qdot */
+    mcpdTdt += 0.0 * vol;  /* intrinsicHeating */
+    for (int n = 0; n < K; n++) {
+        double hk = w->h_RT[n] * w->RT;
+        mcpdTdt -= w->wdot[n] * hk * vol;
+        ydot[2 + n] = (w->wdot[n] * vol) * m->mw[n] / mass;
+    }
+    double inner = 0.0;
+    for (int k = 0; k < K; k++) inner += w->ym[k] * w->cp_R[k];
+    double cp_mole = GAS_CONSTANT * (w->mmw * inner);
+    double cp_mass = cp_mole / w->mmw;
+    ydot[0] = 0.0 / 1.0;
+    ydot[1] = mcpdTdt / (mass * cp_mass);
+    for (int i = 0; i < K + 2; i++)
+        if (!isfinite(ydot[i])) return 1; /* checkFinite -> recoverable */
+    return 0;
+}
+
+int or_igcpr_rhs(const or_mech* m, double P, const double* y, double* ydot) {
+    work_t* w = work_new(m);
+    int r = rhs_w(m, w, P, y, ydot);
+    work_free(w);
+    return r;
+}
+
+/* Analytic Jacobian d(ydot)/dy for y = [m, T, Y] (row-major, neq x neq).
+ * Y columns: exact chain rule through C_j = rho*Y_j/W_j at fixed T (and hence
+ * fixed molar density P/RT), with dwdot/dC from mass action, third-body and
+ * falloff [M] dependence (rate constants and Kc at fixed T).
+ * T column: one-sided difference with increment sqrt(eps)*T.
+ * The GPU kernels evaluate the same closed form. */
+static void jac_analytic(const or_mech* m, work_t* w, double P, const double* y, double* J) {
+    const int K = m->K, I = m->I, N = K + 2;
+    double* f0 = calloc(N, 8);
+    double* f1 = calloc(N, 8);
+    double* y1 = calloc(N, 8);
+    double* JC = calloc((size_t)K * K, 8);
+    memset(J, 0, 8 * (size_t)N * N);
+    /* T column by difference */
+    double dT = sqrt(DBL_EPSILON) * y[1];
+    memcpy(y1, y, 8 * N);
+    y1[1] = y[1] + dT;
+    rhs_w(m, w, P, y1, f1);
+    rhs_w(m, w, P, y, f0); /* leaves the base state in w */
+    for (int i = 0; i < N; i++) J[i * N + 1] = (f1[i] - f0[i]) / dT;
+    /* dwdot/dC */
+    for (int i = 0; i < I; i++) {
+        const double* C = w->conc;
+        double kfM = w->kf[i];
+        double Mi = (m->type[i] != 0) ? w->concm[i] : 1.0;
+        if (m->type[i] == 1) kfM *= Mi;
+        /* products of concentrations */
+        double pr_r = 1.0, pr_p = 1.0;
+        for (int a = 0; a < m->nr[i]; a++) pr_r *= pow(C[m->rk[i][a]], m->ro[i][a]);
+        if (m->rev[i])
+            for (int a = 0; a < m->np[i]; a++) pr_p *= pow(C[m->pk[i][a]], m->ps[i][a]);
+        double dR[MAXR + MAXR + MAXTB];
+        int dk[MAXR + MAXR + MAXTB], nd = 0;
+        for (int a = 0; a < m->nr[i]; a++) {
+            double o = m->ro[i][a];
+            if (o == 0.0) continue;
+            double d = kfM * o * pow(C[m->rk[i][a]], o - 1.0);
+            for (int c = 0; c < m->nr[i]; c++)
+                if (c != a) d *= pow(C[m->rk[i][c]], m->ro[i][c]);
+            dk[nd] = m->rk[i][a]; dR[nd++] = d;
+        }
+        if (m->rev[i]) {
+            for (int a = 0; a < m->np[i]; a++) {
+                double o = m->ps[i][a];
+                double d = -kfM * w->rkcn[i] * o * pow(C[m->pk[i][a]], o - 1.0);
+                for (int c = 0; c < m->np[i]; c++)
+                    if (c != a) d *= pow(C[m->pk[i][c]], m->ps[i][c]);
+                dk[nd] = m->pk[i][a]; dR[nd++] = d;
+            }
+        }
+        if (m->type[i] != 0) {
+            double q; /* d ropnet / d[M] */
+            double diff = pr_r - w->rkcn[i] * pr_p;
+            if (m->type[i] == 1) {
+                q = w->kf[i] * diff;
+            } else {
+                double T = w->T, logT = log(T), recipT = 1.0 / T;
+                double rc_low = m->A0[i] * exp(m->b0[i] * logT - m->EaR0[i] * recipT);
+                double rc_high = m->A[i] * exp(m->b[i] * logT - m->EaR[i] * recipT);
+                double Pr = Mi * rc_low / (rc_high + SMALL_NUMBER);
+                double dlogF = 0.0, F = 1.0;
+                if (m->type[i] == 3) {
+                    double Fcent = (1.0 - m->ta[i]) * exp(-T * m->rt3[i]) + m->ta[i] * exp(-T * m->rt1[i]);
+                    if (m->t2[i] != 0.0) Fcent += exp(-m->t2[i] / T);
+                    double lFc = log10(Fcent > SMALL_NUMBER ? Fcent : SMALL_NUMBER);
+                    double lpr = log10(Pr > SMALL_NUMBER ? Pr : SMALL_NUMBER);
+                    double cc = -0.4 - 0.67 * lFc, nn = 0.75 - 1.27 * lFc;
+                    double x = lpr + cc, den = nn - 0.14 * x;
+                    double f1v = x / den;
+                    double df1 = nn / (den * den);
+                    F = pow(10.0, lFc / (1.0 + f1v * f1v));
+                    dlogF = (Pr > SMALL_NUMBER) ? -lFc * 2.0 * f1v * df1 / ((1.0 + f1v * f1v) * (1.0 + f1v * f1v)) : 0.0;
+                }
+                /* dk/d[M] = k0' F/(1+Pr) * (1/(1+Pr) + dlogF) */
+                double k0p = rc_low * rc_high / (rc_high + SMALL_NUMBER);
+                double dkdM = k0p * F / (1.0 + Pr) * (1.0 / (1.0 + Pr) + dlogF);
+                q = dkdM * diff;
+            }
+            for (int a = 0; a < m->ntb[i]; a++) { dk[nd] = m->tbk[i][a]; dR[nd++] = q * m->tbe[i][a]; }
+        }
+        /* scatter nu_ki * dR into JC, with nu the net stoichiometric
+         * coefficient (a species on both sides, e.g. an explicit collider
+         * kept in an elementary reaction, has nu = 0) */
+        int nk[2 * MAXR], nnu = 0;
+        double nuv[2 * MAXR];
+        for (int a = 0; a < m->nr[i] + m->np[i]; a++) {
+            int k = a < m->nr[i] ? m->rk[i][a] : m->pk[i][a - m->nr[i]];
+            double v = a < m->nr[i] ? -m->rs[i][a] : m->ps[i][a - m->nr[i]];
+            int found = 0;
+            for (int c = 0; c < nnu; c++)
+                if (nk[c] == k) { nuv[c] += v; found = 1; }
+            if (!found) { nk[nnu] = k; nuv[nnu++] = v; }
+        }
+        for (int c = 0; c < nnu; c++) {
+            if (nuv[c] == 0.0) continue;
+            for (int d = 0; d < nd; d++) JC[nk[c] * K + dk[d]] += nuv[c] * dR[d];
+        }
+    }
+    /* chain rule to Y and the energy row */
+    double rho = w->dens, Ct = w->ctot, Wbar = w->mmw;
+    double inner = 0.0;
+    for (int k = 0; k < K; k++) inner += w->ym[k] * w->cp_R[k];
+    double cp_mass = GAS_CONSTANT * (w->mmw * inner) / w->mmw;
+    double H = 0.0, hD = 0.0;
+    double* D = calloc(K, 8);
+    double* hk = calloc(K, 8);
+    for (int k = 0; k < K; k++) {
+        hk[k] = w->h_RT[k] * w->RT;
+        double s = 0.0;
+        for (int l = 0; l < K; l++) s += JC[k * K + l] * w->conc[l];
+        D[k] = s;
+        H += hk[k] * w->wdot[k];
+        hD += hk[k] * D[k];
+    }
+    for (int j = 0; j < K; j++) {
+        double hJ = 0.0;
+        for (int k = 0; k < K; k++) {
+            J[(2 + k) * N + 2 + j] = m->mw[k] / m->mw[j] * (JC[k * K + j] + (w->wdot[k] - D[k]) / Ct);
+            hJ += hk[k] * JC[k * K + j];
+        }
+        double cpj = GAS_CONSTANT * w->cp_R[j] / m->mw[j];
+        J[1 * N + 2 + j] = -1.0 / (rho * cp_mass) *
+                           ((rho * hJ - Wbar * hD + Wbar * H) / m->mw[j] - H * cpj / cp_mass);
+    }
+    free(D); free(hk); free(f0); free(f1); free(y1); free(JC);
+}
+
+static void jac_fd(const or_mech* m, work_t* w, double P, const double* y, double* J) {
+    const int N = m->K + 2;
+    double* f0 = calloc(N, 8);
+    double* f1 = calloc(N, 8);
+    double* y1 = calloc(N, 8);
+    rhs_w(m, w, P, y, f0);
+    for (int j = 0; j < N; j++) {
+        memcpy(y1, y, 8 * N);
+        double d = fmax(fabs(y[j]), 1e-12) * sqrt(DBL_EPSILON);
+        y1[j] = y[j] + d;
+        rhs_w(m, w, P, y1, f1);
+        for (int i = 0; i < N; i++) J[i * N + j] = (f1[i] - f0[i]) / d;
+    }
+    free(f0); free(f1); free(y1);
+}
+
+int or_igcpr_jacobian(const or_mech* m, double P, const double* y, double* J, int mode) {
+    work_t* w = work_new(m);
+    if (mode == 1) jac_analytic(m, w, P, y, J);
+    else jac_fd(m, w, P, y, J);
+    work_free(w);
+    return 0;
+}
+
+/* ======================================================================
+ * Variable-order BDF in Nordsieck form (CVODE method, restated).
+ * ====================================================================== */
+#define QMAX 5
+#define LMAX (QMAX + 1)
+#define MXSTEP 20000
+#define MAXCOR 3
+#define MXNEF 7
+#define MXNCF 10
+#define MXNEF1 3
+#define SMALL_NEF 2
+#define LONG_WAIT 10
+#define SMALL_NST 10
+#define NLSCOEF 0.1
+#define ETAMX1 10000.0
+#define ETAMX2 10.0
+#define ETAMX3 10.0
+#define ETAMXF 0.2
+#define ETAMIN 0.1
+#define ETACF 0.25
+#define ADDON 1e-6
+#define BIAS1 6.0
+#define BIAS2 6.0
+#define BIAS3 10.0
+#define THRESH 1.5
+#define CRDOWN 0.3
+#define DGMAX_LSETUP 0.3
+#define RDIV 2.0
+#define MSBP 20
+#define MSBJ 51
+#define DGMAX_JBAD 0.2
+#define HLB_FACTOR 100.0
+#define HUB_FACTOR 0.1
+#define H_BIAS 0.5
+#define HIN_ITERS 4
+#define MIN_INC_MULT 1000.0
+#define UROUND DBL_EPSILON
+
+enum { FIRST_CALL = 0, PREV_CONV_FAIL = 1, PREV_ERR_FAIL = 2 };
+enum { NO_FAILURES = 0, FAIL_BAD_J = 1, FAIL_OTHER = 2 };
+
+struct or_reactor {
+    const or_mech* m;
+    work_t* w;
+    int N;
+    double P, rtol, atol;
+    int jac_mode;
+    double* zn[LMAX + 1];
+    double *ewt, *acor, *y, *ftemp, *tempv, *savedJ, *M;
+    int* piv;
+    double tn, h, hprime, hscale, eta, etamax, hu, tau[LMAX + 2], tq[6], l[LMAX + 1];
+    double rl1, gamma, gammap, gamrat, crate, delp, acnrm, saved_tq5;
+    int q, qprime, qwait, L, nst, nstlp, nstlj, nscon;
+    int convfail, jcur;
+    long nfe, nje, nsetups, nni, ncfn, netf, nfe_dq;
+    int started;
+    double t_user;
+    double* yout;
+};
+
+static double wrms(const double* v, const double* w, int n) {
+    double s = 0.0;
+    for (int i = 0; i < n; i++) { double x = v[i] * w[i]; s += x * x; }
+    return sqrt(s / n);
+}
+
+static int f_eval(or_reactor* r, const double* y, double* ydot) {
+    r->nfe++;
+    return rhs_w(r->m, r->w, r->P, y, ydot);
+}
+
+static void ewt_set(or_reactor* r, const double* y) {
+    for (int i = 0; i < r->N; i++) r->ewt[i] = 1.0 / (r->rtol * fabs(y[i]) + r->atol);
+}
+
+/* dense LU with partial pivoting (column-oriented, first maximal pivot) */
+static int lu_factor(double* A, int n, int* piv) {
+    for (int k = 0; k < n; k++) {
+        int p = k;
+        double amax = fabs(A[k * n + k]);
+        for (int i = k + 1; i < n; i++)
+            if (fabs(A[i * n + k]) > amax) { amax = fabs(A[i * n + k]); p = i; }
+        piv[k] = p;
+        if (A[p * n + k] == 0.0) return k + 1;
+        if (p != k)
+            for (int j = 0; j < n; j++) { double t = A[k * n + j]; A[k * n + j] = A[p * n + j]; A[p * n + j] = t; }
+        double mult = 1.0 / A[k * n + k];
+        for (int i = k + 1; i < n; i++) A[i * n + k] *= mult;
+        for (int j = k + 1; j < n; j++) {
+            double akj = A[k * n + j];
+            if (akj != 0.0)
+                for (int i = k + 1; i < n; i++) A[i * n + j] -= A[i * n + k] * akj;
+        }
+    }
+    return 0;
+}
+static void lu_solve(const double* A, int n, const int* piv, double* b) {
+    for (int k = 0; k < n; k++) {
+        int p = piv[k];
+        if (p != k) { double t = b[k]; b[k] = b[p]; b[p] = t; }
+    }
+    for (int k = 0; k < n; k++) {
+        double bk = b[k];
+        for (int i = k + 1; i < n; i++) b[i] -= A[i * n + k] * bk;
+    }
+    for (int k = n - 1; k >= 0; k--) {
+        b[k] /= A[k * n + k];
+        double bk = b[k];
+        for (int i = 0; i < k; i++) b[i] -= A[i * n + k] * bk;
+    }
+}
+
+/* Difference-quotient dense Jacobian (CVODE's DQ method) */
+static int jac_dq(or_reactor* r, const double* y0, const double* fy, double* J) {
+    const int N = r->N;
+    double srur = sqrt(UROUND);
+    double fnorm = wrms(fy, r->ewt, N);
+    double minInc = (fnorm != 0.0) ? (MIN_INC_MULT * fabs(r->h) * UROUND * N * fnorm) : 1.0;
+    double* yy = calloc(N, 8);
+    double* col = calloc(N, 8);
+    memcpy(yy, y0, 8 * N);
+    for (int j = 0; j < N; j++) {
+        double ysave = yy[j];
+        double inc = fmax(srur * fabs(ysave), minInc / r->ewt[j]);
+        yy[j] += inc;
+        r->nfe_dq++;
+        int ret = rhs_w(r->m, r->w, r->P, yy, col);
+        yy[j] = ysave;
+        if (ret) { free(col); free(yy); return ret; }
+        double inc_inv = 1.0 / inc;
+        for (int i = 0; i < N; i++) J[i * N + j] = inc_inv * col[i] - inc_inv * fy[i];
+    }
+    free(col);
+    free(yy);
+    return 0;
+}
+
+/* linear-solver setup: decide on Jacobian reuse, form M = I - gamma*J, factor */
+static int lsetup(or_reactor* r, const double* ypred, const double* fpred) {
+    const int N = r->N;
+    double dgamma = fabs(r->gamma / r->gammap - 1.0);
+    int jbad = (r->nst == 0) || (r->nst >= r->nstlj + MSBJ) ||
+               ((r->convfail == FAIL_BAD_J) && (dgamma < DGMAX_JBAD)) || (r->convfail == FAIL_OTHER);
+    if (!jbad) {
+        r->jcur = 0;
+    } else {
+        r->nje++;
+        r->nstlj = r->nst;
+        r->jcur = 1;
+        if (r->jac_mode == OR_JAC_ANALYTIC) {
+            jac_analytic(r->m, r->w, r->P, ypred, r->savedJ);
+            r->nfe_dq += 2;
+        } else {
+            int ret = jac_dq(r, ypred, fpred, r->savedJ);
+            if (ret) return ret;
+        }
+    }
+    for (int i = 0; i < N * N; i++) r->M[i] = -r->gamma * r->savedJ[i];
+    for (int i = 0; i < N; i++) r->M[i * N + i] += 1.0;
+    r->nsetups++;
+    if (lu_factor(r->M, N, r->piv)) return 1;
+    r->gamrat = 1.0;
+    r->gammap = r->gamma;
+    r->crate = 1.0;
+    r->nstlp = r->nst;
+    return 0;
+}
+
+/* coefficients l[] and tq[] of the fixed-leading-coefficient BDF */
+static void set_bdf(or_reactor* r) {
+    double alpha0, alpha0_hat, xi_inv, xistar_inv, hsum;
+    int q = r->q;
+    r->l[0] = r->l[1] = xi_inv = xistar_inv = 1.0;
+    for (int i = 2; i <= q; i++) r->l[i] = 0.0;
+    alpha0 = alpha0_hat = -1.0;
+    hsum = r->h;
+    if (q > 1) {
+        for (int j = 2; j < q; j++) {
+            hsum += r->tau[j - 1];
+            xi_inv = r->h / hsum;
+            alpha0 -= 1.0 / j;
+            for (int i = j; i >= 1; i--) r->l[i] += r->l[i - 1] * xi_inv;
+        }
+        alpha0 -= 1.0 / q;
+        xistar_inv = -r->l[1] - alpha0;
+        hsum += r->tau[q - 1];
+        xi_inv = r->h / hsum;
+        alpha0_hat = -r->l[1] - xi_inv;
+        for (int i = q; i >= 1; i--) r->l[i] += r->l[i - 1] * xistar_inv;
+    }
+    /* error-test constants */
+    double A1 = 1.0 - alpha0_hat + alpha0;
+    double A2 = 1.0 + q * A1;
+    r->tq[2] = fabs(A1 / (alpha0 * A2));
+    r->tq[5] = fabs(A2 * xistar_inv / (r->l[q] * xi_inv));
+    if (r->qwait == 1) {
+        if (q > 1) {
+            double C = xistar_inv / r->l[q];
+            double A3 = alpha0 + 1.0 / q;
+            double A4 = alpha0_hat + xi_inv;
+            double Cpinv = (1.0 - A4 + A3) / A3;
+            r->tq[1] = fabs(C * Cpinv);
+        } else {
+            r->tq[1] = 1.0;
+        }
+        hsum += r->tau[q];
+        xi_inv = r->h / hsum;
+        double A5 = alpha0 - 1.0 / (q + 1);
+        double A6 = alpha0_hat - xi_inv;
+        double Cppinv = (1.0 - A6 + A5) / A2;
+        r->tq[3] = fabs(Cppinv / (xi_inv * (q + 2) * A5));
+    }
+    r->tq[4] = NLSCOEF / r->tq[2];
+}
+
+static void predict(or_reactor* r) {
+    r->tn += r->h;
+    for (int k = 1; k <= r->q; k++)
+        for (int j = r->q; j >= k; j--)
+            for (int i = 0; i < r->N; i++) r->zn[j - 1][i] += r->zn[j][i];
+}
+static void restore(or_reactor* r, double saved_t) {
+    r->tn = saved_t;
+    for (int k = 1; k <= r->q; k++)
+        for (int j = r->q; j >= k; j--)
+            for (int i = 0; i < r->N; i++) r->zn[j - 1][i] -= r->zn[j][i];
+}
+static void rescale(or_reactor* r) {
+    double factor = r->eta;
+    for (int j = 1; j <= r->q; j++) {
+        for (int i = 0; i < r->N; i++) r->zn[j][i] *= factor;
+        factor *= r->eta;
+    }
+    r->h = r->hscale * r->eta;
+    r->hscale = r->h;
+    r->nscon = 0;
+}
+
+static void increase_bdf(or_reactor* r) {
+    double l[LMAX + 1];
+    for (int i = 0; i <= QMAX; i++) l[i] = 0.0;
+    double alpha1, prod, xiold, alpha0, hsum, xi;
+    l[2] = alpha1 = prod = xiold = 1.0;
+    alpha0 = -1.0;
+    hsum = r->hscale;
+    if (r->q > 1) {
+        for (int j = 1; j < r->q; j++) {
+            hsum += r->tau[j + 1];
+            xi = hsum / r->hscale;
+            prod *= xi;
+            alpha0 -= 1.0 / (j + 1);
+            alpha1 += 1.0 / xi;
+            for (int i = j + 2; i >= 2; i--) l[i] = l[i] * xiold + l[i - 1];
+            xiold = xi;
+        }
+    }
+    double A1 = (-alpha0 - alpha1) / prod;
+    int L = r->q + 1;
+    for (int i = 0; i < r->N; i++) r->zn[L][i] = A1 * r->zn[QMAX][i];
+    for (int j = 2; j <= r->q; j++)
+        for (int i = 0; i < r->N; i++) r->zn[j][i] += l[j] * r->zn[L][i];
+}
+static void decrease_bdf(or_reactor* r) {
+    double l[LMAX + 1];
+    for (int i = 0; i <= QMAX; i++) l[i] = 0.0;
+    l[2] = 1.0;
+    double hsum = 0.0, xi;
+    for (int j = 1; j <= r->q - 2; j++) {
+        hsum += r->tau[j];
+        xi = hsum / r->hscale;
+        for (int i = j + 2; i >= 2; i--) l[i] = l[i] * xi + l[i - 1];
+    }
+    for (int j = 2; j < r->q; j++)
+        for (int i = 0; i < r->N; i++) r->zn[j][i] = -l[j] * r->zn[r->q][i] + r->zn[j][i];
+}
+static void adjust_order(or_reactor* r, int dq) {
+    if (dq == 1) increase_bdf(r);
+    else if (dq == -1) decrease_bdf(r);
+}
+
+/* Newton iteration for the corrector; returns 0 on success, 1 recoverable failure,
+ * -1 fatal */
+static int nls(or_reactor* r, int nflag) {
+    const int N = r->N;
+    int callSetup;
+    r->convfail = (nflag == FIRST_CALL || nflag == PREV_ERR_FAIL) ? NO_FAILURES : FAIL_OTHER;
+    callSetup = (nflag == PREV_CONV_FAIL) || (nflag == PREV_ERR_FAIL) || (r->nst == 0) ||
+                (r->nst >= r->nstlp + MSBP) || (fabs(r->gamrat - 1.0) > DGMAX_LSETUP);
+    double* ycor = r->acor;
+    double* delta = r->tempv;
+    int jbad = 0;
+    for (;;) {
+        for (int i = 0; i < N; i++) ycor[i] = 0.0;
+        /* residual at the predictor */
+        for (int i = 0; i < N; i++) r->y[i] = r->zn[0][i] + ycor[i];
+        int ret = f_eval(r, r->y, r->ftemp);
+        if (ret) return 1;
+        for (int i = 0; i < N; i++) delta[i] = r->rl1 * r->zn[1][i] + ycor[i];
+        for (int i = 0; i < N; i++) delta[i] = -r->gamma * r->ftemp[i] + delta[i];
+        if (callSetup) {
+            if (jbad) r->convfail = FAIL_BAD_J;
+            ret = lsetup(r, r->zn[0], r->ftemp);
+            if (ret) return ret > 0 ? 1 : -1;
+            callSetup = 0;
+        }
+        int m = 0;
+        int status;
+        for (;;) {
+            r->nni++;
+            for (int i = 0; i < N; i++) delta[i] = -delta[i];
+            lu_solve(r->M, N, r->piv, delta);
+            if (r->gamrat != 1.0) {
+                double s = 2.0 / (1.0 + r->gamrat);
+                for (int i = 0; i < N; i++) delta[i] *= s;
+            }
+            for (int i = 0; i < N; i++) ycor[i] += delta[i];
+            /* convergence test */
+            double del = wrms(delta, r->ewt, N);
+            if (m > 0) r->crate = fmax(CRDOWN * r->crate, del / r->delp);
+            double dcon = del * fmin(1.0, r->crate) / r->tq[4];
+            if (dcon <= 1.0) {
+                r->acnrm = (m == 0) ? del : wrms(ycor, r->ewt, N);
+                status = 0;
+                break;
+            }
+            if (m >= 1 && del > RDIV * r->delp) { status = 1; break; }
+            r->delp = del;
+            m++;
+            if (m >= MAXCOR) { status = 1; break; }
+            for (int i = 0; i < N; i++) r->y[i] = r->zn[0][i] + ycor[i];
+            ret = f_eval(r, r->y, r->ftemp);
+            if (ret) { status = 1; break; }
+            for (int i = 0; i < N; i++) delta[i] = r->rl1 * r->zn[1][i] + ycor[i];
+            for (int i = 0; i < N; i++) delta[i] = -r->gamma * r->ftemp[i] + delta[i];
+        }
+        if (status == 0) {
+            r->jcur = 0;
+            for (int i = 0; i < N; i++) r->y[i] = r->zn[0][i] + ycor[i];
+            return 0;
+        }
+        if (!r->jcur) {
+            callSetup = 1;
+            jbad = 1;
+            continue;
+        }
+        return 1;
+    }
+}
+
+static int do_error_test(or_reactor* r, int* nflag, double saved_t, int* nef, double* dsm) {
+    *dsm = r->acnrm * r->tq[2];
+    if (*dsm <= 1.0) return 0;
+    (*nef)++;
+    r->netf++;
+    *nflag = PREV_ERR_FAIL;
+    restore(r, saved_t);
+    if (*nef == MXNEF) return -1;
+    r->etamax = 1.0;
+    if (*nef <= MXNEF1) {
+        r->eta = 1.0 / (pow(BIAS2 * (*dsm), 1.0 / r->L) + ADDON);
+        r->eta = fmax(ETAMIN, r->eta);
+        if (*nef >= SMALL_NEF) r->eta = fmin(r->eta, ETAMXF);
+        rescale(r);
+        return 2;
+    }
+    if (r->q > 1) {
+        r->eta = ETAMIN;
+        adjust_order(r, -1);
+        r->L = r->q;
+        r->q--;
+        r->qwait = r->L;
+        rescale(r);
+        return 2;
+    }
+    r->eta = ETAMIN;
+    r->h *= r->eta;
+    r->hscale = r->h;
+    r->qwait = LONG_WAIT;
+    r->nscon = 0;
+    int ret = f_eval(r, r->zn[0], r->tempv);
+    if (ret) return -1;
+    for (int i = 0; i < r->N; i++) r->zn[1][i] = r->h * r->tempv[i];
+    return 2;
+}
+
+static void complete_step(or_reactor* r) {
+    r->nst++;
+    r->nscon++;
+    r->hu = r->h;
+    for (int i = r->q; i >= 2; i--) r->tau[i] = r->tau[i - 1];
+    if (r->q == 1 && r->nst > 1) r->tau[2] = r->tau[1];
+    r->tau[1] = r->h;
+    for (int j = 0; j <= r->q; j++)
+        for (int i = 0; i < r->N; i++) r->zn[j][i] += r->l[j] * r->acor[i];
+    r->qwait--;
+    if (r->qwait == 1 && r->q != QMAX) {
+        memcpy(r->zn[QMAX], r->acor, 8 * r->N);
+        r->saved_tq5 = r->tq[5];
+    }
+}
+
+static void set_eta(or_reactor* r) {
+    if (r->eta < THRESH) {
+        r->eta = 1.0;
+        r->hprime = r->h;
+    } else {
+        r->eta = fmin(r->eta, r->etamax);
+        r->hprime = r->h * r->eta;
+    }
+}
+
+static void prepare_next_step(or_reactor* r, double dsm) {
+    if (r->etamax == 1.0) {
+        r->qwait = r->qwait > 2 ? r->qwait : 2;
+        r->qprime = r->q;
+        r->hprime = r->h;
+        r->eta = 1.0;
+        return;
+    }
+    double etaq = 1.0 / (pow(BIAS2 * dsm, 1.0 / r->L) + ADDON);
+    if (r->qwait != 0) {
+        r->eta = etaq;
+        r->qprime = r->q;
+        set_eta(r);
+        return;
+    }
+    r->qwait = 2;
+    double etaqm1 = 0.0, etaqp1 = 0.0;
+    if (r->q > 1) {
+        double ddn = wrms(r->zn[r->q], r->ewt, r->N) * r->tq[1];
+        etaqm1 = 1.0 / (pow(BIAS1 * ddn, 1.0 / r->q) + ADDON);
+    }
+    if (r->q != QMAX && r->saved_tq5 != 0.0) {
+        double cquot = (r->tq[5] / r->saved_tq5) * pow(r->h / r->tau[2], r->L);
+        for (int i = 0; i < r->N; i++) r->tempv[i] = -cquot * r->zn[QMAX][i] + r->acor[i];
+        double dup = wrms(r->tempv, r->ewt, r->N) * r->tq[3];
+        etaqp1 = 1.0 / (pow(BIAS3 * dup, 1.0 / (r->L + 1)) + ADDON);
+    }
+    double etam = fmax(etaqm1, fmax(etaq, etaqp1));
+    if (etam < THRESH) {
+        r->eta = 1.0;
+        r->qprime = r->q;
+    } else if (etam == etaq) {
+        r->eta = etaq;
+        r->qprime = r->q;
+    } else if (etam == etaqm1) {
+        r->eta = etaqm1;
+        r->qprime = r->q - 1;
+    } else {
+        r->eta = etaqp1;
+        r->qprime = r->q + 1;
+        memcpy(r->zn[QMAX], r->acor, 8 * r->N);
+    }
+    set_eta(r);
+}
+
+/* one internal step; 0 ok, <0 failure */
+static int bdf_step(or_reactor* r) {
+    double saved_t = r->tn;
+    int ncf = 0, nef = 0, nflag = FIRST_CALL;
+    double dsm = 0.0;
+    if (r->nst > 0 && r->hprime != r->h) {
+        if (r->qprime != r->q) {
+            adjust_order(r, r->qprime - r->q);
+            r->q = r->qprime;
+            r->L = r->q + 1;
+            r->qwait = r->L;
+        }
+        rescale(r);
+    }
+    for (;;) {
+        predict(r);
+        set_bdf(r);
+        r->rl1 = 1.0 / r->l[1];
+        r->gamma = r->h * r->rl1;
+        if (r->nst == 0) r->gammap = r->gamma;
+        r->gamrat = (r->nst > 0) ? r->gamma / r->gammap : 1.0;
+        int ret = nls(r, nflag);
+        if (ret != 0) {
+            r->ncfn++;
+            restore(r, saved_t);
+            if (ret < 0) return -2;
+            ncf++;
+            r->etamax = 1.0;
+            if (ncf == MXNCF) return -3;
+            r->eta = ETACF;
+            nflag = PREV_CONV_FAIL;
+            rescale(r);
+            continue;
+        }
+        int e = do_error_test(r, &nflag, saved_t, &nef, &dsm);
+        if (e == 2) continue;
+        if (e < 0) return -4;
+        break;
+    }
+    complete_step(r);
+    prepare_next_step(r, dsm);
+    r->etamax = (r->nst <= SMALL_NST) ? ETAMX2 : ETAMX3;
+    for (int i = 0; i < r->N; i++) r->acor[i] *= r->tq[2];
+    return 0;
+}
+
+/* initial step size (CVODE's h0 estimate from an estimate of y'') */
+static int initial_h(or_reactor* r, double tout) {
+    const int N = r->N;
+    double tdist = fabs(tout - r->tn);
+    double tround = UROUND * fmax(fabs(r->tn), fabs(tout));
+    if (tdist < 2.0 * tround) return -1;
+    double hlb = HLB_FACTOR * tround;
+    /* upper bound */
+    double hub_inv = 0.0;
+    for (int i = 0; i < N; i++) {
+        double t1 = HUB_FACTOR * fabs(r->zn[0][i]) + (r->rtol * fabs(r->zn[0][i]) + r->atol);
+        double v = fabs(r->zn[1][i]) / t1;
+        if (v > hub_inv) hub_inv = v;
+    }
+    double hub = HUB_FACTOR * tdist;
+    if (hub * hub_inv > 1.0) hub = 1.0 / hub_inv;
+    double hg = sqrt(hlb * hub);
+    if (hub < hlb) { r->h = hg; return 0; }
+    double hnew = hg;
+    double* yy = calloc(N, 8);
+    double* ft = calloc(N, 8);
+    for (int c1 = 1; c1 <= HIN_ITERS; c1++) {
+        double yddnrm = 0.0;
+        int ok = 0;
+        for (int c2 = 1; c2 <= HIN_ITERS; c2++) {
+            for (int i = 0; i < N; i++) yy[i] = hg * r->zn[1][i] + r->zn[0][i];
+            int ret = f_eval(r, yy, ft);
+            if (ret == 0) {
+                for (int i = 0; i < N; i++) ft[i] = (ft[i] - r->zn[1][i]) * (1.0 / hg);
+                yddnrm = wrms(ft, r->ewt, N);
+                ok = 1;
+                break;
+            }
+            hg *= 0.2;
+        }
+        if (!ok) { free(yy); free(ft); return -2; }
+        hnew = (yddnrm * hub * hub > 2.0) ? sqrt(2.0 / yddnrm) : sqrt(hg * hub);
+        if (c1 == HIN_ITERS) break;
+        double hrat = hnew / hg;
+        if (hrat > 0.5 && hrat < 2.0) break;
+        if (c1 > 1 && hrat > 2.0) { hnew = hg; break; }
+        hg = hnew;
+    }
+    free(yy);
+    free(ft);
+    double h0 = H_BIAS * hnew;
+    if (h0 < hlb) h0 = hlb;
+    if (h0 > hub) h0 = hub;
+    r->h = h0;
+    return 0;
+}
+
+or_reactor* or_reactor_new(const or_mech* m, double T0, double P, const double* Y0,
+                           double rtol, double atol, int jac_mode) {
+    or_reactor* r = calloc(1, sizeof *r);
+    r->m = m;
+    r->w = work_new(m);
+    r->N = m->K + 2;
+    r->P = P;
+    r->rtol = rtol;
+    r->atol = atol;
+    r->jac_mode = jac_mode;
+    int N = r->N;
+    for (int j = 0; j <= LMAX; j++) r->zn[j] = calloc(N, 8);
+    r->ewt = calloc(N, 8); r->acor = calloc(N, 8); r->y = calloc(N, 8); r->ftemp = calloc(N, 8);
+    r->tempv = calloc(N, 8); r->savedJ = calloc((size_t)N * N, 8); r->M = calloc((size_t)N * N, 8);
+    r->piv = calloc(N, sizeof(int));
+    r->yout = calloc(N, 8);
+    /* IdealGasConstPressureReactor::getState with V = 1 m^3 */
+    set_state(m, r->w, T0, P, Y0);
+    r->zn[0][0] = r->w->dens * 1.0;
+    r->zn[0][1] = T0;
+    memcpy(r->zn[0] + 2, Y0, 8 * m->K);
+    memcpy(r->yout, r->zn[0], 8 * N);
+    r->q = 1;
+    r->L = 2;
+    r->qwait = r->L;
+    r->etamax = ETAMX1;
+    r->tn = 0.0;
+    r->t_user = 0.0;
+    return r;
+}
+
+void or_reactor_free(or_reactor* r) {
+    if (!r) return;
+    for (int j = 0; j <= LMAX; j++) free(r->zn[j]);
+    free(r->ewt); free(r->acor); free(r->y); free(r->ftemp); free(r->tempv); free(r->savedJ);
+    free(r->M); free(r->piv); free(r->yout);
+    work_free(r->w);
+    free(r);
+}
+
+static int start(or_reactor* r, double tout) {
+    ewt_set(r, r->zn[0]);
+    if (f_eval(r, r->zn[0], r->zn[1])) return -1;
+    if (initial_h(r, tout)) return -1;
+    for (int i = 0; i < r->N; i++) r->zn[1][i] *= r->h;
+    r->hscale = r->h;
+    r->hprime = r->h;
+    r->started = 1;
+    return 0;
+}
+
+/* one CVode(ONE_STEP) call */
+static int one_step(or_reactor* r) {
+    if (r->nst > 0) ewt_set(r, r->zn[0]);
+    double nrm = wrms(r->zn[0], r->ewt, r->N);
+    if (UROUND * nrm > 1.0) return -10;
+    int ret = bdf_step(r);
+    return ret;
+}
+
+int or_reactor_advance(or_reactor* r, double tout) {
+    if (tout == r->t_user) return 0;
+    if (tout < r->t_user) return -20;
+    if (!r->started && start(r, tout)) return -21;
+    int nsteps = 0;
+    while (r->tn < tout) {
+        if (nsteps >= MXSTEP) return -22;
+        int ret = one_step(r);
+        if (ret) return ret;
+        nsteps++;
+    }
+    /* CVodeGetDky(tout, 0) */
+    double s = (tout - r->tn) / r->h;
+    for (int i = 0; i < r->N; i++) {
+        double v = r->zn[r->q][i];
+        for (int j = r->q - 1; j >= 0; j--) v = r->zn[j][i] + s * v;
+        r->yout[i] = v;
+    }
+    r->t_user = tout;
+    return 0;
+}
+
+int or_reactor_step(or_reactor* r) {
+    if (!r->started && start(r, r->t_user + 1.0)) return -21;
+    int ret = one_step(r);
+    if (ret) return ret;
+    memcpy(r->yout, r->zn[0], 8 * r->N);
+    r->t_user = r->tn;
+    return 0;
+}
+
+double or_reactor_time(const or_reactor* r) { return r->t_user; }
+void or_reactor_state(const or_reactor* r, double* y) { memcpy(y, r->yout, 8 * r->N); }
+void or_reactor_stats(const or_reactor* r, long* s) {
+    s[0] = r->nst; s[1] = r->nfe; s[2] = r->nje; s[3] = r->nsetups; s[4] = r->nni;
+    s[5] = r->ncfn; s[6] = r->netf; s[7] = r->q; s[8] = r->nfe_dq;
+}
+
+int or_reactors_run(const or_mech* m, int n, const double* T0, const double* P, const double* Y0,
+                    const double* dt, int nout, double rtol, double atol, int jac_mode,
+                    double* y_out, long* stats_out, int nthreads) {
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+    (void)nthreads;
+#endif
+    const int K = m->K, N = K + 2;
+    int fail = 0;
+#pragma omp parallel for schedule(static, 1)
+    for (int i = 0; i < n; i++) {
+        or_reactor* r = or_reactor_new(m, T0[i], P[i], Y0 + (size_t)i * K, rtol, atol, jac_mode);
+        if (y_out) or_reactor_state(r, y_out + ((size_t)i * (nout + 1)) * N);
+        for (int k = 1; k <= nout; k++) {
+            int ret = or_reactor_advance(r, k * dt[i]);
+            if (ret) {
+#pragma omp critical
+                { if (!fail) fail = -(i + 1); }
+                break;
+            }
+            if (y_out) or_reactor_state(r, y_out + ((size_t)i * (nout + 1) + k) * N);
+        }
+        if (stats_out) or_reactor_stats(r, stats_out + (size_t)i * 9);
+        or_reactor_free(r);
+    }
+    return fail;
+}
