@@ -1,0 +1,162 @@
+"""cantera_amd: MI355X-native batched gas kinetics and constant-pressure reactors.
+
+Host-side mirror of the reference interface for this path:
+
+* ``Solution(mech)`` -- an ideal-gas phase with bulk kinetics loaded from a
+  compiled mechanism (reference ``newSolution``, ``src/base/Solution.cpp:242-282``).
+* ``Solution.net_production_rates(T, P, Y)`` -- batched
+  ``Kinetics::getNetProductionRates`` (``src/kinetics/Kinetics.cpp:447-457``).
+* ``ReactorBatch`` -- n independent ``IdealGasConstPressureReactor`` +
+  ``ReactorNet`` pairs with ``advance(t)`` (``src/zeroD/ReactorNet.cpp:351-357``).
+
+All compute runs in libcantera_amd.so on the GPU; nothing here falls back to
+the CPU.
+"""
+from __future__ import annotations
+
+import os
+
+import numpy as np
+
+from . import _lib
+from ._lib import CG_FP64, CG_MIXED, CanteraGpuError, check, lib
+
+MECH_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "mechanisms")
+
+__all__ = ["Solution", "ReactorBatch", "CanteraGpuError", "CG_FP64", "CG_MIXED", "mech_path", "lib"]
+
+
+def mech_path(name: str) -> str:
+    """Resolve a mechanism name ('gri30', 'gri30.yaml', or a path) to a .mech file."""
+    if os.path.exists(name) and name.endswith(".mech"):
+        return name
+    base = os.path.splitext(os.path.basename(name))[0]
+    alias = {"nDodecane_Reitz": "nDodecane"}.get(base, base)
+    p = os.path.join(MECH_DIR, alias + ".mech")
+    if os.path.exists(p):
+        return p
+    if os.path.exists(name) and name.endswith((".yaml", ".yml")):
+        from .mechc import compile_mechanism, write_mech
+        out = os.path.join(MECH_DIR, base + ".mech")
+        write_mech(compile_mechanism(name), out)
+        return out
+    raise FileNotFoundError(f"no compiled mechanism for {name!r}")
+
+
+class Solution:
+    """Ideal-gas phase + bulk kinetics on the GPU (one mechanism handle)."""
+
+    def __init__(self, mechanism: str = "gri30"):
+        self.path = mech_path(mechanism)
+        L = lib()
+        self._h = check(L.cg_mech_load(self.path.encode()), "cg_mech_load")
+        self.n_species = check(L.cg_mech_nspecies(self._h))
+        self.n_reactions = check(L.cg_mech_nreactions(self._h))
+        self.n_elements = check(L.cg_mech_nelements(self._h))
+        self.molecular_weights = np.zeros(self.n_species)
+        check(L.cg_mech_molecular_weights(self._h, self.n_species, self.molecular_weights))
+        buf = _lib.C.create_string_buffer(256)
+        names = []
+        for k in range(self.n_species):
+            check(L.cg_mech_species_name(self._h, k, buf, 256))
+            names.append(buf.value.decode())
+        self.species_names = names
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and _lib._lib is not None:
+            _lib._lib.cg_mech_free(h)
+            self._h = None
+
+    @property
+    def handle(self) -> int:
+        return self._h
+
+    def species_index(self, name: str) -> int:
+        k = lib().cg_mech_species_index(self._h, name.encode())
+        if k < 0:
+            raise KeyError(name)
+        return k
+
+    def mass_fractions_from_moles(self, X) -> np.ndarray:
+        X = np.asarray(X, dtype=np.float64)
+        Y = X * self.molecular_weights
+        return Y / Y.sum(axis=-1, keepdims=True)
+
+    def net_production_rates(self, T, P, Y) -> np.ndarray:
+        """wdot [kmol/m^3/s] for n states.  Y is species-major (K, n); returns (K, n)."""
+        T = np.ascontiguousarray(np.atleast_1d(T), dtype=np.float64)
+        n = T.shape[0]
+        P = np.ascontiguousarray(np.broadcast_to(np.asarray(P, dtype=np.float64), (n,)))
+        Y = np.ascontiguousarray(np.asarray(Y, dtype=np.float64).reshape(self.n_species, n))
+        wdot = np.zeros((self.n_species, n))
+        check(lib().cg_rates_batch(self._h, n, T, P, Y, wdot, CG_FP64), "cg_rates_batch")
+        return wdot
+
+    def last_rates_kernel_ms(self) -> float:
+        return float(lib().cg_rates_last_kernel_ms(self._h))
+
+
+class ReactorBatch:
+    """n independent constant-pressure ideal-gas reactors, each with its own
+    ReactorNet; ``advance`` mirrors ``ReactorNet::advance``."""
+
+    def __init__(self, sol: Solution, T0, P, Y0, rtol=1e-9, atol=1e-15, precision=CG_FP64):
+        self.sol = sol
+        T0 = np.ascontiguousarray(np.atleast_1d(T0), dtype=np.float64)
+        n = T0.shape[0]
+        P = np.ascontiguousarray(np.broadcast_to(np.asarray(P, dtype=np.float64), (n,)))
+        Y0 = np.ascontiguousarray(np.asarray(Y0, dtype=np.float64).reshape(n, sol.n_species))
+        self.n = n
+        self.neq = sol.n_species + 2
+        self._h = check(lib().cg_reactors_create(sol.handle, n, T0, P, Y0, float(rtol), float(atol), int(precision)),
+                        "cg_reactors_create")
+        self.time = np.zeros(n)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and _lib._lib is not None:
+            _lib._lib.cg_reactors_free(h)
+            self._h = None
+
+    @property
+    def handle(self) -> int:
+        return self._h
+
+    def set_options(self, max_steps=0, waves_per_cu=-1):
+        check(lib().cg_reactors_set_options(self._h, int(max_steps), int(waves_per_cu)))
+
+    def advance(self, t):
+        t = np.ascontiguousarray(np.broadcast_to(np.asarray(t, dtype=np.float64), (self.n,)))
+        check(lib().cg_reactors_advance(self._h, t), "cg_reactors_advance")
+        self.time = t.copy()
+
+    @property
+    def state(self) -> np.ndarray:
+        y = np.zeros((self.n, self.neq))
+        check(lib().cg_reactors_get_state(self._h, y), "cg_reactors_get_state")
+        return y
+
+    @property
+    def T(self):
+        return self.state[:, 1]
+
+    @property
+    def Y(self):
+        return self.state[:, 2:]
+
+    @property
+    def status(self) -> np.ndarray:
+        s = np.zeros(self.n, dtype=np.int32)
+        check(lib().cg_reactors_status(self._h, s))
+        return s
+
+    @property
+    def stats(self) -> dict:
+        s = np.zeros((self.n, 9), dtype=np.int64)
+        check(lib().cg_reactors_stats(self._h, s))
+        keys = ("steps", "rhs", "jac", "lu", "newton", "conv_fail", "err_fail", "order", "rhs_jac")
+        return {k: s[:, i] for i, k in enumerate(keys)}
+
+    def last_kernel_ms(self) -> float:
+        return float(lib().cg_reactors_last_kernel_ms(self._h))

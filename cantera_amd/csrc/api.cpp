@@ -1,0 +1,494 @@
+// extern "C" boundary of libcantera_amd.so (declarations and reference
+// citations in include/cantera_gpu.h).  Handles index owned objects; errors are
+// reported as negative return codes plus a per-thread message, and no C++
+// exception crosses the boundary.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include <hip/hip_runtime.h>
+
+#include "../../include/cantera_gpu.h"
+#include "bdf.hpp"
+#include "mech.hpp"
+
+namespace {
+
+using namespace cg;
+
+thread_local std::string g_error;
+
+void check(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+int device_cu_count() {
+    static int cus = 0;
+    if (!cus) {
+        int dev = 0;
+        check(hipGetDevice(&dev), "hipGetDevice");
+        check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute");
+    }
+    return cus;
+}
+
+struct MechObj {
+    std::unique_ptr<DeviceMech> dm;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    double last_ms = 0.0;
+    int ns = 1;
+    ~MechObj() {
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+    }
+};
+
+struct Batch {
+    MechObj* mech = nullptr;
+    int n = 0, N = 0, K = 0, ns = 1, ld = 0;
+    bool fp32 = false;
+    double rtol = 1e-9, atol = 1e-15;
+    double* rec = nullptr;
+    double* jsave = nullptr;
+    void* lusave = nullptr;
+    int* piv = nullptr;
+    double* tout = nullptr;
+    int* queue = nullptr;
+    size_t rec_stride = 0, j_stride = 0, lu_stride = 0, piv_stride = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    double last_ms = 0.0;
+    int max_steps = 20000;
+    int waves_per_cu = 0;
+    size_t lds = 0;
+    ~Batch() {
+        for (void* p : {static_cast<void*>(rec), static_cast<void*>(jsave), lusave, static_cast<void*>(piv),
+                        static_cast<void*>(tout), static_cast<void*>(queue)})
+            if (p) (void)hipFree(p);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+std::mutex g_mu;
+std::vector<std::unique_ptr<MechObj>> g_mechs;
+std::vector<std::unique_ptr<Batch>> g_batches;
+
+MechObj& get_mech(int32_t h) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (h < 0 || h >= static_cast<int32_t>(g_mechs.size()) || !g_mechs[h]) throw std::runtime_error("invalid mechanism handle");
+    return *g_mechs[h];
+}
+Batch& get_batch(int32_t h) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (h < 0 || h >= static_cast<int32_t>(g_batches.size()) || !g_batches[h]) throw std::runtime_error("invalid reactor-batch handle");
+    return *g_batches[h];
+}
+
+template <class F>
+int32_t guarded(F&& f) {
+    try {
+        return f();
+    } catch (const std::exception& e) {
+        g_error = e.what();
+    } catch (...) {
+        g_error = "unknown error";
+    }
+    return CG_ERR;
+}
+
+int grid_for(const void* kernel, size_t lds, int waves_per_cu, long work_items) {
+    int per_cu = waves_per_cu;
+    if (per_cu <= 0) {
+        int nb = 0;
+        check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, 64, lds), "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+        per_cu = std::max(nb, 1);
+    }
+    long g = static_cast<long>(per_cu) * device_cu_count();
+    return static_cast<int>(std::max<long>(1, std::min(g, work_items)));
+}
+
+void set_lds_limit(const void* kernel, size_t lds) {
+    if (lds > 65536)
+        check(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)),
+              "hipFuncSetAttribute(MaxDynamicSharedMemorySize)");
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t cg_version(void) { return 1; }
+
+int32_t cg_last_error(char* buf, int32_t len) {
+    if (buf && len > 0) {
+        std::strncpy(buf, g_error.c_str(), static_cast<size_t>(len) - 1);
+        buf[len - 1] = 0;
+    }
+    return static_cast<int32_t>(g_error.size());
+}
+
+int32_t cg_device_count(void) {
+    return guarded([&] {
+        int n = 0;
+        check(hipGetDeviceCount(&n), "hipGetDeviceCount");
+        return n;
+    });
+}
+
+void* cg_dev_alloc(int64_t bytes) {
+    void* p = nullptr;
+    if (bytes <= 0) { g_error = "cg_dev_alloc: size must be positive"; return nullptr; }
+    hipError_t e = hipMalloc(&p, static_cast<size_t>(bytes));
+    if (e != hipSuccess) { g_error = std::string("hipMalloc: ") + hipGetErrorString(e); return nullptr; }
+    return p;
+}
+int32_t cg_dev_free(void* p) {
+    return guarded([&] { check(hipFree(p), "hipFree"); return 0; });
+}
+int32_t cg_memcpy_h2d(void* dst, const void* src, int64_t bytes) {
+    return guarded([&] { check(hipMemcpy(dst, src, static_cast<size_t>(bytes), hipMemcpyHostToDevice), "hipMemcpy H2D"); return 0; });
+}
+int32_t cg_memcpy_d2h(void* dst, const void* src, int64_t bytes) {
+    return guarded([&] { check(hipMemcpy(dst, src, static_cast<size_t>(bytes), hipMemcpyDeviceToHost), "hipMemcpy D2H"); return 0; });
+}
+int32_t cg_device_synchronize(void) {
+    return guarded([&] { check(hipDeviceSynchronize(), "hipDeviceSynchronize"); return 0; });
+}
+
+int32_t cg_mech_load(const char* path) {
+    return guarded([&] {
+        if (!path) throw std::runtime_error("null path");
+        auto obj = std::make_unique<MechObj>();
+        HostMech hm = load_mech(path);
+        obj->dm = std::make_unique<DeviceMech>(hm);
+        obj->ns = (obj->dm->view().N + 63) / 64;
+        check(hipEventCreate(&obj->e0), "hipEventCreate");
+        check(hipEventCreate(&obj->e1), "hipEventCreate");
+        std::lock_guard<std::mutex> lk(g_mu);
+        g_mechs.push_back(std::move(obj));
+        return static_cast<int32_t>(g_mechs.size() - 1);
+    });
+}
+
+int32_t cg_mech_free(int32_t h) {
+    return guarded([&] {
+        std::lock_guard<std::mutex> lk(g_mu);
+        if (h < 0 || h >= static_cast<int32_t>(g_mechs.size()) || !g_mechs[h]) throw std::runtime_error("invalid mechanism handle");
+        for (auto& b : g_batches)
+            if (b && b->mech == g_mechs[h].get()) throw std::runtime_error("mechanism still used by a reactor batch");
+        g_mechs[h].reset();
+        return 0;
+    });
+}
+
+int32_t cg_mech_nspecies(int32_t h) { return guarded([&] { return get_mech(h).dm->view().K; }); }
+int32_t cg_mech_nreactions(int32_t h) { return guarded([&] { return get_mech(h).dm->view().I; }); }
+int32_t cg_mech_nelements(int32_t h) { return guarded([&] { return get_mech(h).dm->view().E; }); }
+
+int32_t cg_mech_species_index(int32_t h, const char* name) {
+    return guarded([&] {
+        const auto& names = get_mech(h).dm->host().names;
+        for (size_t k = 0; k < names.size(); k++)
+            if (names[k] == name) return static_cast<int32_t>(k);
+        return -1;
+    });
+}
+
+int32_t cg_mech_species_name(int32_t h, int32_t k, char* buf, int32_t len) {
+    return guarded([&] {
+        const auto& names = get_mech(h).dm->host().names;
+        if (k < 0 || k >= static_cast<int32_t>(names.size())) throw std::runtime_error("species index out of range");
+        if (buf && len > 0) {
+            std::strncpy(buf, names[k].c_str(), static_cast<size_t>(len) - 1);
+            buf[len - 1] = 0;
+        }
+        return static_cast<int32_t>(names[k].size());
+    });
+}
+
+int32_t cg_mech_molecular_weights(int32_t h, int32_t len, double* mw) {
+    return guarded([&] {
+        const auto& w = get_mech(h).dm->host().mw;
+        if (len < static_cast<int32_t>(w.size())) throw std::runtime_error("buffer too small");
+        std::copy(w.begin(), w.end(), mw);
+        return 0;
+    });
+}
+
+int32_t cg_rates_batch_device(int32_t h, int64_t n, const double* T, const double* P, const double* Y,
+                              double* wdot, int32_t precision, void* stream) {
+    return guarded([&] {
+        MechObj& mo = get_mech(h);
+        if (precision != CG_FP64) throw std::runtime_error("cg_rates_batch: only CG_FP64 is available");
+        if (n <= 0) return 0;
+        if (!T || !P || !Y || !wdot) throw std::runtime_error("null array");
+        const DevMech& m = mo.dm->view();
+        const size_t lds = rates_lds_bytes(m, mo.ns);
+        const void* kp = rates_kernel_ptr(mo.ns);
+        set_lds_limit(kp, lds);
+        const int grid = grid_for(kp, lds, 0, (n + 63) / 64);
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        check(hipEventRecord(mo.e0, s), "hipEventRecord");
+        check(rates_launch(m, mo.ns, n, T, P, Y, wdot, grid, lds, s), "rates kernel launch");
+        check(hipEventRecord(mo.e1, s), "hipEventRecord");
+        return 0;
+    });
+}
+
+int32_t cg_rates_batch(int32_t h, int64_t n, const double* T, const double* P, const double* Y, double* wdot,
+                       int32_t precision) {
+    return guarded([&] {
+        MechObj& mo = get_mech(h);
+        if (n <= 0) return 0;
+        if (!T || !P || !Y || !wdot) throw std::runtime_error("null array");
+        const int K = mo.dm->view().K;
+        const size_t bytes = static_cast<size_t>(n) * sizeof(double);
+        double* d = nullptr;
+        check(hipMalloc(&d, bytes * (2 + 2 * static_cast<size_t>(K))), "hipMalloc");
+        std::unique_ptr<double, void (*)(double*)> guard(d, [](double* p) { (void)hipFree(p); });
+        double *dT = d, *dP = d + n, *dY = d + 2 * n, *dW = d + (2 + static_cast<size_t>(K)) * n;
+        check(hipMemcpy(dT, T, bytes, hipMemcpyHostToDevice), "hipMemcpy");
+        check(hipMemcpy(dP, P, bytes, hipMemcpyHostToDevice), "hipMemcpy");
+        check(hipMemcpy(dY, Y, bytes * K, hipMemcpyHostToDevice), "hipMemcpy");
+        if (cg_rates_batch_device(h, n, dT, dP, dY, dW, precision, nullptr) < 0) throw std::runtime_error(g_error);
+        check(hipMemcpy(wdot, dW, bytes * K, hipMemcpyDeviceToHost), "hipMemcpy");
+        check(hipEventSynchronize(mo.e1), "hipEventSynchronize");
+        float ms = 0.f;
+        check(hipEventElapsedTime(&ms, mo.e0, mo.e1), "hipEventElapsedTime");
+        mo.last_ms = ms;
+        return 0;
+    });
+}
+
+double cg_rates_last_kernel_ms(int32_t h) {
+    try {
+        MechObj& mo = get_mech(h);
+        check(hipEventSynchronize(mo.e1), "hipEventSynchronize");
+        float ms = 0.f;
+        check(hipEventElapsedTime(&ms, mo.e0, mo.e1), "hipEventElapsedTime");
+        mo.last_ms = ms;
+        return ms;
+    } catch (const std::exception& e) {
+        g_error = e.what();
+        return -1.0;
+    }
+}
+
+int32_t cg_reactors_create(int32_t h, int64_t n, const double* T0, const double* P, const double* Y0, double rtol,
+                           double atol, int32_t precision) {
+    return guarded([&] {
+        MechObj& mo = get_mech(h);
+        if (n <= 0 || n > (1LL << 31) - 1) throw std::runtime_error("reactor count out of range");
+        if (!T0 || !P || !Y0) throw std::runtime_error("null array");
+        if (!(rtol > 0) || !(atol > 0)) throw std::runtime_error("tolerances must be positive");
+        if (precision != CG_FP64 && precision != CG_MIXED) throw std::runtime_error("unknown precision");
+        const DevMech& m = mo.dm->view();
+        const HostMech& hm = mo.dm->host();
+        auto b = std::make_unique<Batch>();
+        b->mech = &mo;
+        b->n = static_cast<int>(n);
+        b->K = m.K;
+        b->N = m.N;
+        b->ns = mo.ns;
+        b->fp32 = precision == CG_MIXED;
+        b->ld = (m.N % 2) ? m.N : m.N + 1;
+        b->rtol = rtol;
+        b->atol = atol;
+        const int NP = 64 * b->ns;
+        b->rec_stride = 7 * static_cast<size_t>(NP) + C_COUNT;
+        b->j_stride = static_cast<size_t>(m.N) * m.N;
+        b->lu_stride = static_cast<size_t>(m.N) * b->ld;
+        b->piv_stride = static_cast<size_t>(NP);
+        const size_t es = b->fp32 ? 4 : 8;
+        check(hipMalloc(&b->rec, b->rec_stride * n * 8), "hipMalloc(records)");
+        check(hipMalloc(&b->jsave, b->j_stride * n * 8), "hipMalloc(jacobians)");
+        check(hipMalloc(&b->lusave, b->lu_stride * n * es), "hipMalloc(factors)");
+        check(hipMalloc(&b->piv, b->piv_stride * n * 4), "hipMalloc(pivots)");
+        check(hipMalloc(&b->tout, n * 8), "hipMalloc(tout)");
+        check(hipMalloc(&b->queue, 64), "hipMalloc(queue)");
+        check(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "hipStreamCreate");
+        check(hipEventCreate(&b->e0), "hipEventCreate");
+        check(hipEventCreate(&b->e1), "hipEventCreate");
+        // initial records: zn[0] = y0 in device order [Y, T, m]; V = 1 m^3
+        std::vector<double> rec(b->rec_stride * n, 0.0);
+        const double R = 6.02214076e26 * 1.380649e-23;
+        for (int64_t s = 0; s < n; s++) {
+            double* r = rec.data() + s * b->rec_stride;
+            const double* y = Y0 + s * m.K;
+            double sum = 0.0;
+            for (int k = 0; k < m.K; k++) sum += y[k] * (1.0 / hm.mw[k]);
+            const double mmw = 1.0 / sum;
+            const double dens = P[s] * mmw / (T0[s] * R);
+            for (int k = 0; k < m.K; k++) r[k] = y[k];
+            r[m.K] = T0[s];
+            r[m.K + 1] = dens * 1.0;
+            for (int i = 0; i < m.N; i++) r[6 * NP + i] = r[i];   // yout = y0
+            double* c = r + 7 * NP;
+            c[C_Q] = 1; c[C_L] = 2; c[C_QWAIT] = 2; c[C_QPRIME] = 1; c[C_ETAMAX] = 10000.0; c[C_P] = P[s];
+            c[C_ETA] = 1.0;
+        }
+        check(hipMemcpy(b->rec, rec.data(), rec.size() * 8, hipMemcpyHostToDevice), "hipMemcpy(records)");
+        b->lds = bdf_lds_bytes(m, b->ns, b->ld, b->fp32);
+        set_lds_limit(bdf_kernel_ptr(b->ns, b->fp32), b->lds);
+        std::lock_guard<std::mutex> lk(g_mu);
+        g_batches.push_back(std::move(b));
+        return static_cast<int32_t>(g_batches.size() - 1);
+    });
+}
+
+int32_t cg_reactors_free(int32_t h) {
+    return guarded([&] {
+        std::lock_guard<std::mutex> lk(g_mu);
+        if (h < 0 || h >= static_cast<int32_t>(g_batches.size()) || !g_batches[h]) throw std::runtime_error("invalid reactor-batch handle");
+        (void)hipStreamSynchronize(g_batches[h]->stream);
+        g_batches[h].reset();
+        return 0;
+    });
+}
+
+int32_t cg_reactors_set_options(int32_t h, int32_t max_steps, int32_t waves_per_cu) {
+    return guarded([&] {
+        Batch& b = get_batch(h);
+        if (max_steps > 0) b.max_steps = max_steps;
+        if (waves_per_cu >= 0) b.waves_per_cu = waves_per_cu;
+        return 0;
+    });
+}
+
+static void launch_advance(Batch& b) {
+    BdfArgs a{};
+    a.mech = b.mech->dm->view();
+    a.n = b.n;
+    a.neq_norm = b.N;
+    a.ld = b.ld;
+    a.max_steps = b.max_steps;
+    a.rtol = b.rtol;
+    a.atol = b.atol;
+    a.rec = b.rec;
+    a.rec_stride = b.rec_stride;
+    a.j_save = b.jsave;
+    a.j_stride = b.j_stride;
+    a.lu_save_raw = b.lusave;
+    a.lu_stride = b.lu_stride;
+    a.piv_save = b.piv;
+    a.piv_stride = b.piv_stride;
+    a.tout = b.tout;
+    a.order = nullptr;
+    a.queue = b.queue;
+    const int grid = grid_for(bdf_kernel_ptr(b.ns, b.fp32), b.lds, b.waves_per_cu, b.n);
+    check(hipEventRecord(b.e0, b.stream), "hipEventRecord");
+    check(bdf_launch(a, b.ns, b.fp32, grid, b.lds, b.stream), "integrator kernel launch");
+    check(hipEventRecord(b.e1, b.stream), "hipEventRecord");
+}
+
+int32_t cg_reactors_advance_device(int32_t h, const double* t_out_dev) {
+    return guarded([&] {
+        Batch& b = get_batch(h);
+        check(hipMemcpyAsync(b.tout, t_out_dev, static_cast<size_t>(b.n) * 8, hipMemcpyDeviceToDevice, b.stream),
+              "hipMemcpyAsync(tout)");
+        launch_advance(b);
+        return 0;
+    });
+}
+
+int32_t cg_reactors_status(int32_t h, int32_t* status) {
+    return guarded([&] {
+        Batch& b = get_batch(h);
+        check(hipStreamSynchronize(b.stream), "hipStreamSynchronize");
+        std::vector<double> st(b.n);
+        const int NP = 64 * b.ns;
+        check(hipMemcpy2D(st.data(), 8, b.rec + 7 * NP + C_STATUS, b.rec_stride * 8, 8, b.n, hipMemcpyDeviceToHost),
+              "hipMemcpy2D(status)");
+        int bad = 0;
+        for (int s = 0; s < b.n; s++) {
+            if (status) status[s] = static_cast<int32_t>(st[s]);
+            if (st[s] < 0) bad++;
+        }
+        return bad;
+    });
+}
+
+int32_t cg_reactors_advance(int32_t h, const double* t_out) {
+    return guarded([&] {
+        Batch& b = get_batch(h);
+        if (!t_out) throw std::runtime_error("null t_out");
+        check(hipMemcpyAsync(b.tout, t_out, static_cast<size_t>(b.n) * 8, hipMemcpyHostToDevice, b.stream),
+              "hipMemcpyAsync(tout)");
+        launch_advance(b);
+        check(hipStreamSynchronize(b.stream), "hipStreamSynchronize");
+        float ms = 0.f;
+        check(hipEventElapsedTime(&ms, b.e0, b.e1), "hipEventElapsedTime");
+        b.last_ms = ms;
+        int bad = cg_reactors_status(h, nullptr);
+        if (bad < 0) throw std::runtime_error(g_error);
+        if (bad > 0) throw std::runtime_error(std::to_string(bad) + " reactor(s) failed to integrate; see cg_reactors_status");
+        return 0;
+    });
+}
+
+int32_t cg_reactors_get_state(int32_t h, double* y) {
+    return guarded([&] {
+        Batch& b = get_batch(h);
+        check(hipStreamSynchronize(b.stream), "hipStreamSynchronize");
+        const int NP = 64 * b.ns;
+        std::vector<double> buf(static_cast<size_t>(b.n) * b.N);
+        check(hipMemcpy2D(buf.data(), b.N * 8, b.rec + 6 * NP, b.rec_stride * 8, b.N * 8, b.n, hipMemcpyDeviceToHost),
+              "hipMemcpy2D(state)");
+        for (int s = 0; s < b.n; s++) {
+            const double* src = buf.data() + static_cast<size_t>(s) * b.N;
+            double* dst = y + static_cast<size_t>(s) * b.N;
+            dst[0] = src[b.K + 1];
+            dst[1] = src[b.K];
+            for (int k = 0; k < b.K; k++) dst[2 + k] = src[k];
+        }
+        return 0;
+    });
+}
+
+int32_t cg_reactors_stats(int32_t h, int64_t* stats) {
+    return guarded([&] {
+        Batch& b = get_batch(h);
+        check(hipStreamSynchronize(b.stream), "hipStreamSynchronize");
+        const int NP = 64 * b.ns;
+        std::vector<double> c(static_cast<size_t>(b.n) * C_COUNT);
+        check(hipMemcpy2D(c.data(), C_COUNT * 8, b.rec + 7 * NP, b.rec_stride * 8, C_COUNT * 8, b.n, hipMemcpyDeviceToHost),
+              "hipMemcpy2D(stats)");
+        for (int s = 0; s < b.n; s++) {
+            const double* x = c.data() + static_cast<size_t>(s) * C_COUNT;
+            int64_t* o = stats + static_cast<size_t>(s) * 9;
+            o[0] = static_cast<int64_t>(x[C_NST]);
+            o[1] = static_cast<int64_t>(x[C_NFE]);
+            o[2] = static_cast<int64_t>(x[C_NJE]);
+            o[3] = static_cast<int64_t>(x[C_NSETUPS]);
+            o[4] = static_cast<int64_t>(x[C_NNI]);
+            o[5] = static_cast<int64_t>(x[C_NCFN]);
+            o[6] = static_cast<int64_t>(x[C_NETF]);
+            o[7] = static_cast<int64_t>(x[C_Q]);
+            o[8] = static_cast<int64_t>(x[C_NFE_DQ]);
+        }
+        return 0;
+    });
+}
+
+double cg_reactors_last_kernel_ms(int32_t h) {
+    try {
+        Batch& b = get_batch(h);
+        check(hipEventSynchronize(b.e1), "hipEventSynchronize");
+        float ms = 0.f;
+        check(hipEventElapsedTime(&ms, b.e0, b.e1), "hipEventElapsedTime");
+        b.last_ms = ms;
+        return ms;
+    } catch (const std::exception& e) {
+        g_error = e.what();
+        return -1.0;
+    }
+}
+
+}  // extern "C"

@@ -1,0 +1,55 @@
+// Host <-> device contract of the batched BDF integrator.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "mech.hpp"
+
+namespace cg {
+
+// Per-reactor control block layout (doubles), after 6 Nordsieck columns and the
+// interpolated output vector, each 64*NS doubles.
+enum CtlIndex {
+    C_TN = 0, C_H, C_HPRIME, C_HSCALE, C_ETA, C_ETAMAX, C_HU,
+    C_TAU = 7,            // 7 entries
+    C_TQ = 14,            // 6 entries
+    C_RL1 = 20, C_GAMMA, C_GAMMAP, C_GAMRAT, C_CRATE, C_DELP, C_ACNRM, C_SAVED_TQ5,
+    C_Q = 28, C_QPRIME, C_QWAIT, C_L, C_NST, C_NSTLP, C_NSTLJ, C_NSCON, C_STARTED, C_TUSER,
+    C_NFE = 38, C_NJE, C_NSETUPS, C_NNI, C_NCFN, C_NETF, C_STATUS, C_P, C_HAVE_LU, C_NFE_DQ,
+    C_COUNT = 64
+};
+
+struct BdfArgs {
+    DevMech mech;
+    int n;                // reactors
+    int neq_norm;         // N used in the WRMS norm (K+2, includes the mass)
+    int ld;               // LDS matrix leading dimension
+    int max_steps;        // per advance call
+    double rtol, atol;
+    double* rec;          // n x rec_stride doubles
+    size_t rec_stride;
+    double* j_save;       // n x j_stride doubles (N x N row-major)
+    size_t j_stride;
+    void* lu_save_raw;    // n x lu_stride elements of the matrix type
+    size_t lu_stride;
+    int* piv_save;        // n x piv_stride
+    size_t piv_stride;
+    const double* tout;   // n, device
+    const int* order;     // optional processing order (n), device
+    int* queue;           // device work-queue head
+    // typed view used by the kernel
+    template <class T> __host__ __device__ T* lu() const { return static_cast<T*>(lu_save_raw); }
+#ifdef __HIP_DEVICE_COMPILE__
+#endif
+};
+
+size_t bdf_lds_bytes(const DevMech& m, int ns, int ld, bool fp32);
+hipError_t bdf_launch(const BdfArgs& a, int ns, bool fp32, int grid, size_t lds, hipStream_t stream);
+const void* bdf_kernel_ptr(int ns, bool fp32);
+
+// batched rate evaluation (species-major SoA in and out)
+size_t rates_lds_bytes(const DevMech& m, int ns);
+hipError_t rates_launch(const DevMech& m, int ns, long n, const double* T, const double* P, const double* Y,
+                        double* wdot, int grid, size_t lds, hipStream_t stream);
+const void* rates_kernel_ptr(int ns);
+
+}  // namespace cg

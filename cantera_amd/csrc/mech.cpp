@@ -1,0 +1,309 @@
+#include "mech.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <numeric>
+#include <sstream>
+#include <stdexcept>
+
+namespace cg {
+
+namespace {
+
+double to_d(const std::string& s) {
+    char* end = nullptr;
+    double v = std::strtod(s.c_str(), &end);
+    if (end == s.c_str()) throw std::runtime_error("bad number '" + s + "' in mechanism file");
+    return v;
+}
+
+int to_i(const std::string& s) {
+    char* end = nullptr;
+    long v = std::strtol(s.c_str(), &end, 10);
+    if (end == s.c_str()) throw std::runtime_error("bad integer '" + s + "' in mechanism file");
+    return static_cast<int>(v);
+}
+
+void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+}  // namespace
+
+HostMech load_mech(const std::string& path) {
+    std::ifstream f(path);
+    if (!f) throw std::runtime_error("cannot open mechanism file " + path);
+    HostMech m;
+    std::string line;
+    bool header = false, ended = false;
+    int last_r = -1;
+    while (std::getline(f, line)) {
+        if (line.empty()) continue;
+        if (line.rfind("E ", 0) == 0) {
+            if (last_r >= 0) m.rxn[last_r].equation = line.substr(2);
+            continue;
+        }
+        std::istringstream is(line);
+        std::vector<std::string> t;
+        std::string w;
+        while (is >> w) t.push_back(w);
+        if (t.empty()) continue;
+        size_t p = 1;
+        auto next = [&]() -> const std::string& {
+            if (p >= t.size()) throw std::runtime_error("truncated record in " + path + ": " + line);
+            return t[p++];
+        };
+        const std::string& key = t[0];
+        if (key == "CGMECH") {
+            header = true;
+        } else if (key == "phase") {
+            m.phase = next();
+        } else if (key == "source") {
+        } else if (key == "elements") {
+            m.E = to_i(next());
+            for (int e = 0; e < m.E; e++) {
+                m.elements.push_back(next());
+                m.aw.push_back(to_d(next()));
+            }
+        } else if (key == "species") {
+            m.K = to_i(next());
+            m.names.resize(m.K);
+            m.mw.resize(m.K);
+            m.comp.assign(static_cast<size_t>(m.K) * m.E, 0.0);
+            m.tmin.resize(m.K); m.tmid.resize(m.K); m.tmax.resize(m.K); m.pref.resize(m.K);
+            m.clow.resize(7 * m.K); m.chigh.resize(7 * m.K);
+        } else if (key == "S") {
+            int k = to_i(next());
+            if (k < 0 || k >= m.K) throw std::runtime_error("species index out of range");
+            m.names[k] = next();
+            m.mw[k] = to_d(next());
+            m.tmin[k] = to_d(next()); m.tmid[k] = to_d(next()); m.tmax[k] = to_d(next()); m.pref[k] = to_d(next());
+            for (int j = 0; j < 7; j++) m.clow[7 * k + j] = to_d(next());
+            for (int j = 0; j < 7; j++) m.chigh[7 * k + j] = to_d(next());
+            for (int e = 0; e < m.E; e++) m.comp[static_cast<size_t>(k) * m.E + e] = to_d(next());
+        } else if (key == "reactions") {
+            m.I = to_i(next());
+            m.rxn.resize(m.I);
+        } else if (key == "R") {
+            int i = to_i(next());
+            if (i < 0 || i >= m.I) throw std::runtime_error("reaction index out of range");
+            HostReaction& r = m.rxn[i];
+            r.type = to_i(next());
+            r.rev = to_i(next()) != 0;
+            r.dn = to_d(next());
+            r.A = to_d(next()); r.b = to_d(next()); r.EaR = to_d(next());
+            r.A0 = to_d(next()); r.b0 = to_d(next()); r.EaR0 = to_d(next());
+            for (int j = 0; j < 4; j++) r.troe[j] = to_d(next());
+            int nr = to_i(next());
+            for (int j = 0; j < nr; j++) {
+                r.rk.push_back(to_i(next())); r.rs.push_back(to_d(next())); r.ro.push_back(to_d(next()));
+            }
+            int np = to_i(next());
+            for (int j = 0; j < np; j++) {
+                r.pk.push_back(to_i(next())); r.ps.push_back(to_d(next()));
+            }
+            r.tbdef = to_d(next());
+            int ntb = to_i(next());
+            for (int j = 0; j < ntb; j++) {
+                r.tbk.push_back(to_i(next())); r.tbe.push_back(to_d(next()));
+            }
+            for (int k : r.rk) if (k < 0 || k >= m.K) throw std::runtime_error("reactant index out of range");
+            for (int k : r.pk) if (k < 0 || k >= m.K) throw std::runtime_error("product index out of range");
+            for (int k : r.tbk) if (k < 0 || k >= m.K) throw std::runtime_error("efficiency index out of range");
+            if (r.type < 0 || r.type > 3) throw std::runtime_error("unknown reaction type");
+            last_r = i;
+        } else if (key == "end") {
+            ended = true;
+        }
+    }
+    if (!header || !ended || m.K <= 0) throw std::runtime_error("not a complete .mech file: " + path);
+    return m;
+}
+
+template <class T>
+const T* DeviceMech::upload(const std::vector<T>& v) {
+    size_t bytes = std::max<size_t>(v.size(), 1) * sizeof(T);
+    void* p = nullptr;
+    hip_check(hipMalloc(&p, bytes), "hipMalloc(mechanism table)");
+    allocs_.push_back(p);
+    if (!v.empty()) hip_check(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice), "hipMemcpy(mechanism table)");
+    bytes_ += bytes;
+    return static_cast<const T*>(p);
+}
+
+namespace {
+
+// StoichManagerN::add (StoichManager.h:676-723): unit-stoichiometry sides with
+// at most three molecules become repeated-species lists; everything else is
+// the general (power-law) form.
+bool expand_side(const std::vector<int>& k, const std::vector<double>& order,
+                 const std::vector<double>& st, std::vector<int>& krep) {
+    bool frac = false;
+    for (size_t n = 0; n < st.size(); n++)
+        if (std::fmod(st[n], 1.0) != 0.0 || st[n] != order[n]) frac = true;
+    krep.clear();
+    if (frac || k.size() > 3) return false;
+    for (size_t n = 0; n < k.size(); n++)
+        for (int i = 0; i < st[n]; i++) krep.push_back(k[n]);
+    return krep.size() >= 1 && krep.size() <= 3;
+}
+
+}  // namespace
+
+DeviceMech::DeviceMech(const HostMech& h) : h_(h) {
+    const int K = h.K, I = h.I;
+    if (K > 126) throw std::runtime_error("mechanisms with more than 126 species are not supported");
+    perm_.resize(I);
+    std::iota(perm_.begin(), perm_.end(), 0);
+    std::stable_sort(perm_.begin(), perm_.end(), [&](int a, int b) { return h.rxn[a].type < h.rxn[b].type; });
+
+    std::vector<double> rd(4 * static_cast<size_t>(I)), tbdef(I), fod, gend;
+    std::vector<RxnI> ri(I);
+    std::vector<int> tbk;
+    std::vector<double> tbe;
+    int n_el = 0, n_tb = 0, n_fo = 0;
+    for (int p = 0; p < I; p++) {
+        const HostReaction& r = h.rxn[perm_[p]];
+        RxnI& x = ri[p];
+        std::memset(&x, 0, sizeof x);
+        for (int j = 0; j < kMaxSide; j++) { x.rk[j] = -1; x.pk[j] = -1; }
+        x.orig = perm_[p];
+        x.fo = -1;
+        x.gen = -1;
+        x.flags = (r.rev ? 1 : 0) | (r.type << 1) | (r.type != kElementary ? 32 : 0);
+        if (r.type == kElementary) n_el++;
+        else if (r.type == kThreeBody) n_tb++;
+        else n_fo++;
+        rd[4 * p + 0] = r.A; rd[4 * p + 1] = r.b; rd[4 * p + 2] = r.EaR; rd[4 * p + 3] = r.dn;
+        tbdef[p] = r.tbdef;
+        std::vector<int> krep;
+        bool rgen = false, pgen = false;
+        if (expand_side(r.rk, r.ro, r.rs, krep)) {
+            x.nr = static_cast<int>(krep.size());
+            for (int j = 0; j < x.nr; j++) x.rk[j] = static_cast<short>(krep[j]);
+        } else {
+            rgen = true;
+            if (r.rk.size() > kMaxSide) throw std::runtime_error("reaction '" + r.equation + "' has too many reactants");
+            x.nr = static_cast<int>(r.rk.size());
+            for (int j = 0; j < x.nr; j++) x.rk[j] = static_cast<short>(r.rk[j]);
+        }
+        // products are kept for every reaction (the Jacobian needs them); only
+        // reversible reactions multiply them into the reverse rate
+        {
+            if (expand_side(r.pk, r.ps, r.ps, krep)) {
+                x.np = static_cast<int>(krep.size());
+                for (int j = 0; j < x.np; j++) x.pk[j] = static_cast<short>(krep[j]);
+            } else {
+                pgen = true;
+                if (r.pk.size() > kMaxSide) throw std::runtime_error("reaction '" + r.equation + "' has too many products");
+                x.np = static_cast<int>(r.pk.size());
+                for (int j = 0; j < x.np; j++) x.pk[j] = static_cast<short>(r.pk[j]);
+            }
+        }
+        if (rgen || pgen) {
+            x.flags |= (rgen ? 8 : 0) | (pgen ? 16 : 0);
+            x.gen = static_cast<int>(gend.size() / 12);
+            double g[12] = {0};
+            if (rgen)
+                for (int j = 0; j < x.nr; j++) { g[j] = r.ro[j]; g[4 + j] = r.rs[j]; }
+            if (pgen)
+                for (int j = 0; j < x.np; j++) g[8 + j] = r.ps[j];
+            gend.insert(gend.end(), g, g + 12);
+        }
+        if (r.type >= kLindemann) {
+            x.fo = static_cast<int>(fod.size() / 8);
+            double fo[8] = {r.A0, r.b0, r.EaR0, r.troe[0], r.troe[1], r.troe[2], r.troe[3], 0.0};
+            fod.insert(fod.end(), fo, fo + 8);
+        }
+        if (r.type != kElementary) {
+            if (r.tbk.size() > kMaxTB) throw std::runtime_error("too many third-body efficiencies");
+            x.tbo = static_cast<int>(tbk.size());
+            x.tbn = static_cast<int>(r.tbk.size());
+            tbk.insert(tbk.end(), r.tbk.begin(), r.tbk.end());
+            tbe.insert(tbe.end(), r.tbe.begin(), r.tbe.end());
+        }
+    }
+
+    // wdot gather: net stoichiometric coefficient per (species, permuted reaction)
+    std::vector<std::vector<std::pair<int, double>>> rows(K);
+    for (int p = 0; p < I; p++) {
+        const HostReaction& r = h.rxn[perm_[p]];
+        std::map<int, double> net;
+        for (size_t j = 0; j < r.rk.size(); j++) net[r.rk[j]] -= r.rs[j];
+        for (size_t j = 0; j < r.pk.size(); j++) net[r.pk[j]] += r.ps[j];
+        for (auto& [k, v] : net)
+            if (v != 0.0) rows[k].push_back({p, v});
+    }
+    size_t total = 0, longest = 0;
+    for (auto& r : rows) { total += r.size(); longest = std::max(longest, r.size()); }
+    // choose the sub-row length L minimising (passes * L + longest combine)
+    int bestL = 1;
+    double bestc = 1e300;
+    for (int L = 1; L <= std::max<int>(1, static_cast<int>(longest)); L++) {
+        int nsub = 0, maxsub = 0;
+        for (auto& r : rows) {
+            int s = std::max<int>(1, static_cast<int>((r.size() + L - 1) / L));
+            nsub += s;
+            maxsub = std::max(maxsub, s);
+        }
+        double c = static_cast<double>((nsub + 63) / 64) * L + maxsub;
+        if (c < bestc) { bestc = c; bestL = L; }
+    }
+    sub_len_ = bestL;
+    std::vector<int> sub_start{0}, sp_sub{0}, ent_r;
+    std::vector<double> ent_nu;
+    std::vector<float> ent_nuf;
+    for (int k = 0; k < K; k++) {
+        const auto& r = rows[k];
+        size_t i = 0;
+        do {
+            size_t e = std::min(r.size(), i + bestL);
+            for (; i < e; i++) {
+                ent_r.push_back(r[i].first);
+                ent_nu.push_back(r[i].second);
+                ent_nuf.push_back(static_cast<float>(r[i].second));
+            }
+            sub_start.push_back(static_cast<int>(ent_r.size()));
+        } while (i < r.size());
+        sp_sub.push_back(static_cast<int>(sub_start.size()) - 1);
+    }
+    (void)total;
+
+    std::vector<double> nasa(16 * static_cast<size_t>(K), 0.0), rmw(K);
+    for (int k = 0; k < K; k++) {
+        nasa[16 * k] = h.tmid[k];
+        for (int j = 0; j < 7; j++) { nasa[16 * k + 1 + j] = h.clow[7 * k + j]; nasa[16 * k + 8 + j] = h.chigh[7 * k + j]; }
+        rmw[k] = 1.0 / h.mw[k];
+    }
+
+    d_.K = K; d_.I = I; d_.N = K + 2; d_.E = h.E;
+    d_.nsub = static_cast<int>(sub_start.size()) - 1;
+    d_.n_el = n_el; d_.n_tb = n_tb; d_.n_fo = n_fo;
+    d_.pref = h.pref.empty() ? 101325.0 : h.pref[0];
+    d_.mw = upload(h.mw);
+    d_.rmw = upload(rmw);
+    d_.nasa = upload(nasa);
+    d_.rd = upload(rd);
+    d_.tbdef = upload(tbdef);
+    d_.ri = upload(ri);
+    d_.fod = upload(fod);
+    d_.gend = upload(gend);
+    d_.tbk = upload(tbk);
+    d_.tbe = upload(tbe);
+    d_.sub_start = upload(sub_start);
+    d_.sp_sub = upload(sp_sub);
+    d_.ent_r = upload(ent_r);
+    d_.ent_nu = upload(ent_nu);
+    d_.ent_nuf = upload(ent_nuf);
+    d_.comp = upload(h.comp);
+}
+
+DeviceMech::~DeviceMech() {
+    for (void* p : allocs_) (void)hipFree(p);
+}
+
+}  // namespace cg

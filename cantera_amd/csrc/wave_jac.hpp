@@ -1,0 +1,234 @@
+// Analytic reactor Jacobian d(ydot)/dy for one reactor, assembled by one
+// wavefront into an LDS matrix (row-major, leading dimension LD, internal
+// state order [Y_0..Y_{K-1}, T, m]).
+//
+// Y columns: exact chain rule through C_j = rho Y_j / W_j at fixed T (so the
+// molar density P/RT is fixed), with dwdot/dC from mass action, third-body and
+// falloff [M] dependence -- the structure of BulkKinetics::netProductionRates_ddCi
+// (src/kinetics/BulkKinetics.cpp:761-811) and ThirdBodyCalc::derivatives
+// (include/cantera/kinetics/ThirdBodyCalc.h:97-100), at fixed rate constants.
+// T column: one-sided difference with increment sqrt(eps) T, as the reference's
+// mole-reactor Jacobian does for temperature
+// (src/zeroD/IdealGasConstPressureMoleReactor.cpp:126-152).
+// Per-reaction contributions are scattered with LDS atomics; a single wavefront
+// owns the matrix, so the accumulation order is fixed.
+#pragma once
+#include "wave_kinetics.hpp"
+
+namespace cg {
+
+constexpr double kSqrtEps = 1.4901161193847656e-08;  // sqrt(DBL_EPSILON)
+
+template <typename T>
+__device__ __forceinline__ void lds_add(T* p, T v) {
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// d ropnet / d C_j contributions of reaction p, scattered into JC (K x K block
+// of A).  All per-reaction lists use static indices (registers only).
+template <typename T>
+__device__ __forceinline__ void scatter_reaction_derivs(const DevMech& m, int p, const GasState& g,
+                                                        const double* __restrict__ C, const double* __restrict__ MU,
+                                                        T* A, int LD) {
+    RxnRegs x;
+    load_rxn(m, p, x);
+    RxnVals v;
+    eval_reaction(m, p, g, C, MU, x, v);
+    const int type = (x.flags >> 1) & 3;
+    const bool rev = x.flags & 1;
+    const bool rgen = x.flags & 8, pgen = x.flags & 16;
+    const double* gs = (x.gen >= 0) ? m.gend + 12 * x.gen : m.gend;
+    const double kfM = (type == kThreeBody) ? v.kf * v.M : v.kf;
+    constexpr int E = 2 * kMaxSide;
+    // entries 0..3 reactants, 4..7 products
+    int ek[E];
+    double est[E];
+    bool eval_[E];
+#pragma unroll
+    for (int a = 0; a < kMaxSide; a++) {
+        eval_[a] = a < x.nr;
+        ek[a] = x.rk[a];
+        est[a] = rgen ? gs[4 + a] : 1.0;
+        eval_[kMaxSide + a] = a < x.np;
+        ek[kMaxSide + a] = x.pk[a];
+        est[kMaxSide + a] = pgen ? gs[8 + a] : 1.0;
+    }
+    // net stoichiometric coefficient of each distinct species (first entry leads)
+    double rnu[E];
+    bool lead[E];
+#pragma unroll
+    for (int a = 0; a < E; a++) {
+        double nu = 0.0;
+        bool first = eval_[a];
+#pragma unroll
+        for (int b = 0; b < E; b++) {
+            if (eval_[b] && ek[b] == ek[a]) {
+                nu += (b < kMaxSide) ? -est[b] : est[b];
+                if (b < a) first = false;
+            }
+        }
+        rnu[a] = nu;
+        lead[a] = first && nu != 0.0;
+    }
+    double ordr[kMaxSide], ordp[kMaxSide];
+#pragma unroll
+    for (int a = 0; a < kMaxSide; a++) {
+        ordr[a] = rgen ? gs[a] : 1.0;
+        ordp[a] = pgen ? gs[8 + a] : 1.0;
+    }
+    double pr_r = 1.0, pr_p = 1.0;
+#pragma unroll
+    for (int a = 0; a < kMaxSide; a++) {
+        if (a < x.nr) pr_r *= (ordr[a] == 1.0) ? C[x.rk[a]] : pow(C[x.rk[a]], ordr[a]);
+        if (rev && a < x.np) pr_p *= (ordp[a] == 1.0) ? C[x.pk[a]] : pow(C[x.pk[a]], ordp[a]);
+    }
+    auto scatter = [&](int j, double d) {
+#pragma unroll
+        for (int r = 0; r < E; r++)
+            if (lead[r]) lds_add(&A[ek[r] * LD + j], static_cast<T>(rnu[r] * d));
+    };
+#pragma unroll
+    for (int a = 0; a < kMaxSide; a++) {
+        if (a < x.nr && ordr[a] != 0.0) {
+            double d = kfM * ordr[a] * ((ordr[a] == 1.0) ? 1.0 : pow(C[x.rk[a]], ordr[a] - 1.0));
+#pragma unroll
+            for (int c = 0; c < kMaxSide; c++)
+                if (c != a && c < x.nr) d *= (ordr[c] == 1.0) ? C[x.rk[c]] : pow(C[x.rk[c]], ordr[c]);
+            scatter(x.rk[a], d);
+        }
+    }
+    if (rev) {
+#pragma unroll
+        for (int a = 0; a < kMaxSide; a++) {
+            if (a < x.np) {
+                double d = -kfM * v.rkcn * ordp[a] * ((ordp[a] == 1.0) ? 1.0 : pow(C[x.pk[a]], ordp[a] - 1.0));
+#pragma unroll
+                for (int c = 0; c < kMaxSide; c++)
+                    if (c != a && c < x.np) d *= (ordp[c] == 1.0) ? C[x.pk[c]] : pow(C[x.pk[c]], ordp[c]);
+                scatter(x.pk[a], d);
+            }
+        }
+    }
+    if (type != kElementary && x.tbn > 0) {
+        const double diff = pr_r - v.rkcn * pr_p;
+        double q;
+        if (type == kThreeBody) {
+            q = v.kf * diff;
+        } else {
+            const double* fo = m.fod + 8 * x.fo;
+            const double4 rd = reinterpret_cast<const double4*>(m.rd)[p];
+            double rc_low = fo[0] * exp(fo[1] * g.logT - fo[2] * g.recipT);
+            double rc_high = rd.x * exp(rd.y * g.logT - rd.z * g.recipT);
+            double Pr = v.M * rc_low / (rc_high + kSmallNumber);
+            double F = 1.0, dlogF = 0.0;
+            if (type == kTroe) {
+                double Fcent = (1.0 - fo[3]) * exp(-g.T * fo[4]) + fo[3] * exp(-g.T * fo[5]);
+                if (fo[6] != 0.0) Fcent += exp(-fo[6] / g.T);
+                double lFc = log10(fmax(Fcent, kSmallNumber));
+                double lpr = log10(fmax(Pr, kSmallNumber));
+                double cc = -0.4 - 0.67 * lFc, nn = 0.75 - 1.27 * lFc;
+                double xx = lpr + cc, den = nn - 0.14 * xx;
+                double f1 = xx / den;
+                double df1 = nn / (den * den);
+                double w = 1.0 + f1 * f1;
+                F = exp10(lFc / w);
+                dlogF = (Pr > kSmallNumber) ? -lFc * 2.0 * f1 * df1 / (w * w) : 0.0;
+            }
+            double k0p = rc_low * rc_high / (rc_high + kSmallNumber);
+            q = k0p * F / (1.0 + Pr) * (1.0 / (1.0 + Pr) + dlogF) * diff;
+        }
+        for (int t = 0; t < x.tbn; t++) scatter(m.tbk[x.tbo + t], q * m.tbe[x.tbo + t]);
+    }
+}
+
+// Build J = d(ydot)/dy into A.  y/f0: state and RHS at the expansion point
+// (lane-distributed).  Uses S for the RHS evaluations; on return the species
+// scratch holds the base state.
+template <int NS, typename T>
+__device__ void reactor_jacobian(const DevMech& m, const WaveScratch& S, double P, const double (&y)[NS],
+                                 const double (&f0)[NS], T* A, int LD) {
+    const int lane = lane_id();
+    const int K = m.K, N = m.N;
+    GasState g;
+    SpeciesRegs<NS> sr;
+    RhsScalars rs;
+    // temperature column by a one-sided difference
+    double y1[NS], f1[NS], colT[NS];
+#pragma unroll
+    for (int s = 0; s < NS; s++) y1[s] = y[s];
+    // find T (component K) and perturb it
+    double Tval = bcast<NS>(y, K);
+    const double dT = kSqrtEps * Tval;
+#pragma unroll
+    for (int s = 0; s < NS; s++)
+        if (lane + kWave * s == K) y1[s] = Tval + dT;
+    reactor_rhs<NS>(m, S, P, y1, f1, g, sr, rs);
+#pragma unroll
+    for (int s = 0; s < NS; s++) colT[s] = (f1[s] - f0[s]) / dT;
+    // base state (species scratch, wdot)
+    reactor_rhs<NS>(m, S, P, y, f1, g, sr, rs);
+    // zero the matrix
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        int j = lane + kWave * s;
+        if (j < N)
+            for (int i = 0; i < N; i++) A[i * LD + j] = T(0);
+    }
+    __syncthreads();
+    for (int p = lane; p < m.I; p += kWave) scatter_reaction_derivs<T>(m, p, g, S.C, S.MU, A, LD);
+    __syncthreads();
+    // D_k = sum_l JC[k][l] C_l ; keep wdot and D in W2, h_k in MU
+    double hD = 0.0, H = 0.0;
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        int k = lane + kWave * s;
+        if (k < K) {
+            double d = 0.0;
+            for (int l = 0; l < K; l++) d += static_cast<double>(A[k * LD + l]) * S.C[l];
+            const double hk = sr.hrt[s] * g.RT;
+            S.W2[k] = sr.wdot[s];
+            S.W2[K + k] = d;
+            hD += hk * d;
+            H += hk * sr.wdot[s];
+        }
+    }
+    hD = uni(wave_sum(hD));
+    H = uni(wave_sum(H));
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        int k = lane + kWave * s;
+        if (k < K) S.MU[k] = sr.hrt[s] * g.RT;
+    }
+    __syncthreads();
+    const double rho = g.dens, Ct = g.ctot, Wbar = g.mmw, cp = rs.cp_mass;
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        int j = lane + kWave * s;
+        if (j < K) {
+            double hJ = 0.0;
+            for (int k = 0; k < K; k++) hJ += S.MU[k] * static_cast<double>(A[k * LD + j]);
+            const double wj = m.mw[j];
+            for (int k = 0; k < K; k++) {
+                const double jc = static_cast<double>(A[k * LD + j]);
+                A[k * LD + j] = static_cast<T>(m.mw[k] / wj * (jc + (S.W2[k] - S.W2[K + k]) / Ct));
+            }
+            // pick cp_R of species j from its owner lane
+            const double cpj = kGasConstant * sr.cpr[s] / wj;
+            A[K * LD + j] = static_cast<T>(-1.0 / (rho * cp) * ((rho * hJ - Wbar * hD + Wbar * H) / wj - H * cpj / cp));
+            A[(K + 1) * LD + j] = T(0);
+        }
+    }
+    // T and m columns
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        int i = lane + kWave * s;
+        if (i < N) {
+            A[i * LD + K] = (i == K + 1) ? T(0) : static_cast<T>(colT[s]);
+            A[i * LD + K + 1] = T(0);
+        }
+    }
+    __syncthreads();
+}
+
+}  // namespace cg

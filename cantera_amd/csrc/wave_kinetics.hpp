@@ -1,0 +1,402 @@
+// Wavefront-cooperative gas kinetics for one state at a time: lanes over
+// species for thermo and concentrations, lanes over reactions for rate
+// constants, equilibrium constants and rates of progress, and a balanced
+// sub-row gather for the net production rates.  Shared by the batched rate
+// kernel and the per-reactor BDF integrator.
+//
+// Reference algorithm (file:line in the Cantera sources):
+//   BulkKinetics::updateROP            src/kinetics/BulkKinetics.cpp:457-545
+//   Kinetics::getNetProductionRates    src/kinetics/Kinetics.cpp:447-457
+//   StoichManagerN multiply rules      include/cantera/kinetics/StoichManager.h:147-366
+//   ThirdBodyCalc::update/multiply     include/cantera/kinetics/ThirdBodyCalc.h:75-91
+//   ArrheniusRate::evalFromStruct      include/cantera/kinetics/Arrhenius.h:196-198
+//   FalloffRate::evalFromStruct        include/cantera/kinetics/Falloff.h:154-176
+//   TroeRate::updateTemp / F           src/kinetics/Falloff.cpp:283-300
+//   NasaPoly1/NasaPoly2                include/cantera/thermo/NasaPoly1.h:83-122, NasaPoly2.h:105-111
+//   IdealGasConstPressureReactor::eval src/zeroD/IdealGasConstPressureReactor.cpp:46-110
+//
+// Internal state order on the device is [Y_0 .. Y_{K-1}, T, m]: component i
+// lives in lane i % 64, register slot i / 64, so species k is owned by the
+// lane that also owns Y_k (no cross-lane moves between thermo and state).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "mech.hpp"
+
+namespace cg {
+
+constexpr double kGasConstant = 6.02214076e26 * 1.380649e-23;  // ct_defs.h:84-123
+constexpr double kSmallNumber = 1e-300;
+constexpr double kBigNumber = 1e300;
+constexpr int kWave = 64;
+
+__device__ __forceinline__ int lane_id() { return static_cast<int>(threadIdx.x) & (kWave - 1); }
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, kWave);
+    return v;
+}
+__device__ __forceinline__ float wave_sumf(float v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, kWave);
+    return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v = fmax(v, __shfl_xor(v, off, kWave));
+    return v;
+}
+
+// make a value that all lanes hold identically explicitly wave-uniform
+__device__ __forceinline__ double uni(double x) {
+    long long v = __double_as_longlong(x);
+    int lo = __builtin_amdgcn_readfirstlane(static_cast<int>(v & 0xffffffffLL));
+    int hi = __builtin_amdgcn_readfirstlane(static_cast<int>(v >> 32));
+    return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+}
+__device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
+
+// broadcast element k of a lane-distributed vector (k wave-uniform)
+template <int NS, typename T>
+__device__ __forceinline__ T bcast(const T (&v)[NS], int k) {
+    const int s = k >> 6;
+    T src = v[0];
+#pragma unroll
+    for (int t = 1; t < NS; t++)
+        if (s == t) src = v[t];
+    if constexpr (sizeof(T) == 8) {
+        long long x = __double_as_longlong(src);
+        int lo = __builtin_amdgcn_readlane(static_cast<int>(x & 0xffffffffLL), k & 63);
+        int hi = __builtin_amdgcn_readlane(static_cast<int>(x >> 32), k & 63);
+        return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+    } else {
+        return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(src), k & 63));
+    }
+}
+
+// Per-wave LDS scratch for one state evaluation.
+struct WaveScratch {
+    double* yb;    // N (state broadcast)
+    double* C;     // K concentrations
+    double* MU;    // K standard chemical potentials [J/kmol]
+    double* ROP;   // I net rates of progress
+    double* SUB;   // nsub gather partial sums
+    double* W2;    // 2K (Jacobian: wdot, D)
+};
+
+// Scalar thermodynamic state derived from (T, P, Y): setMassFractions_NoNorm +
+// setState_TP (Phase.cpp:362-372; ThermoPhase.cpp:187-198; IdealGasPhase.h:338-354).
+struct GasState {
+    double T, P, mmw, RT, dens, ctot, pressure, logT, recipT, rrt, logC0, tmp;
+};
+
+// Values one lane keeps for the species it owns (slot s = species lane + 64 s).
+template <int NS>
+struct SpeciesRegs {
+    double ym[NS], hrt[NS], cpr[NS], wdot[NS];
+};
+
+template <typename R>
+__device__ __forceinline__ void nasa7(const double* __restrict__ c16, double T, const double* tt, R& cp, R& h, R& s, R& g) {
+    const double* c = (T <= c16[0]) ? c16 + 1 : c16 + 8;
+    double ct0 = c[0];
+    double ct1 = c[1] * tt[0];
+    double ct2 = c[2] * tt[1];
+    double ct3 = c[3] * tt[2];
+    double ct4 = c[4] * tt[3];
+    double cpv = ct0 + ct1 + ct2 + ct3 + ct4;
+    double hv = ct0 + 0.5 * ct1 + 1.0 / 3.0 * ct2 + 0.25 * ct3 + 0.2 * ct4 + c[5] * tt[4];
+    double sv = ct0 * tt[5] + ct1 + 0.5 * ct2 + 1.0 / 3.0 * ct3 + 0.25 * ct4 + c[6];
+    cp = cpv;
+    h = hv;
+    s = sv;
+    g = hv - sv;
+}
+
+// Per-reaction results.
+struct RxnVals {
+    double kf;     // forward rate constant (with falloff)
+    double M;      // third-body concentration (1 if none)
+    double rkcn;   // 1/Kc (0 if irreversible)
+    double ropf, ropr;
+};
+
+// Reaction record unpacked into registers (static indexing only, so nothing
+// spills to scratch).
+struct RxnRegs {
+    int flags, nr, np, tbo, tbn, fo, gen;
+    int rk[kMaxSide], pk[kMaxSide];
+};
+
+__device__ __forceinline__ void load_rxn(const DevMech& m, int p, RxnRegs& x) {
+    const int4* q = reinterpret_cast<const int4*>(m.ri + p);
+    const int4 a = q[0], b = q[1], c = q[2];
+    x.flags = a.x; x.nr = a.y; x.np = a.z; x.tbo = a.w;
+    x.tbn = b.x; x.fo = b.y; x.gen = b.z;
+    x.rk[0] = static_cast<short>(c.x & 0xffff); x.rk[1] = static_cast<short>(c.x >> 16);
+    x.rk[2] = static_cast<short>(c.y & 0xffff); x.rk[3] = static_cast<short>(c.y >> 16);
+    x.pk[0] = static_cast<short>(c.z & 0xffff); x.pk[1] = static_cast<short>(c.z >> 16);
+    x.pk[2] = static_cast<short>(c.w & 0xffff); x.pk[3] = static_cast<short>(c.w >> 16);
+}
+
+// Evaluate reaction p (permuted index) at the gas state; C and MU live in LDS.
+__device__ __forceinline__ void eval_reaction(const DevMech& m, int p, const GasState& g,
+                                              const double* __restrict__ C, const double* __restrict__ MU,
+                                              const RxnRegs& x, RxnVals& o) {
+    const double4 rd = reinterpret_cast<const double4*>(m.rd)[p];
+    const int type = (x.flags >> 1) & 3;
+    const bool rev = x.flags & 1;
+    double M = 1.0;
+    if (x.flags & 32) {
+        double sum = 0.0;
+        for (int j = 0; j < x.tbn; j++) sum += m.tbe[x.tbo + j] * C[m.tbk[x.tbo + j]];
+        M = m.tbdef[p] * g.ctot + sum;
+    }
+    double kf;
+    if (type <= kThreeBody) {
+        kf = rd.x * exp(rd.y * g.logT - rd.z * g.recipT);
+    } else {
+        const double* fo = m.fod + 8 * x.fo;
+        double work = 0.0;
+        if (type == kTroe) {
+            double Fcent = (1.0 - fo[3]) * exp(-g.T * fo[4]) + fo[3] * exp(-g.T * fo[5]);
+            if (fo[6] != 0.0) Fcent += exp(-fo[6] / g.T);
+            work = log10(fmax(Fcent, kSmallNumber));
+        }
+        double rc_low = fo[0] * exp(fo[1] * g.logT - fo[2] * g.recipT);
+        double rc_high = rd.x * exp(rd.y * g.logT - rd.z * g.recipT);
+        double pr = M * rc_low / (rc_high + kSmallNumber);
+        double F = 1.0;
+        if (type == kTroe) {
+            double lpr = log10(fmax(pr, kSmallNumber));
+            double cc = -0.4 - 0.67 * work;
+            double nn = 0.75 - 1.27 * work;
+            double f1 = (lpr + cc) / (nn - 0.14 * (lpr + cc));
+            double lgf = work / (1.0 + f1 * f1);
+            F = exp10(lgf);
+        }
+        pr *= F / (1.0 + pr);
+        kf = pr * rc_high;
+    }
+    o.kf = kf;
+    o.M = M;
+    // inverse equilibrium constant (BulkKinetics.cpp:465-484)
+    double rkcn = 0.0;
+    if (rev) {
+        double dp = 0.0;
+        if (x.flags & 16) {
+            const double* gs = m.gend + 12 * x.gen;
+#pragma unroll
+            for (int j = 0; j < kMaxSide; j++)
+                if (j < x.np) dp += gs[8 + j] * MU[x.pk[j]];
+        } else {
+            dp = MU[x.pk[0]];
+            if (x.np > 1) dp += MU[x.pk[1]];
+            if (x.np > 2) dp += MU[x.pk[2]];
+        }
+        double delta = dp;
+        if (x.flags & 8) {
+            const double* gs = m.gend + 12 * x.gen;
+#pragma unroll
+            for (int j = 0; j < kMaxSide; j++)
+                if (j < x.nr) delta -= gs[4 + j] * MU[x.rk[j]];
+        } else {
+            double dr = MU[x.rk[0]];
+            if (x.nr > 1) dr += MU[x.rk[1]];
+            if (x.nr > 2) dr += MU[x.rk[2]];
+            delta -= dr;
+        }
+        rkcn = fmin(exp(delta * g.rrt - rd.w * g.logC0), kBigNumber);
+    }
+    o.rkcn = rkcn;
+    double ropf = kf * 1.0;
+    if (type == kThreeBody) ropf *= M;
+    double ropr = ropf * rkcn;
+    // concentration products with the reference's negative-concentration rules
+    if (x.flags & 8) {
+        const double* gs = m.gend + 12 * x.gen;
+#pragma unroll
+        for (int j = 0; j < kMaxSide; j++) {
+            if (j < x.nr) {
+                double order = gs[j];
+                if (order != 0.0) {
+                    double c = C[x.rk[j]];
+                    ropf = (c > 0.0) ? ropf * pow(c, order) : 0.0;
+                }
+            }
+        }
+    } else {
+        double a = C[x.rk[0]];
+        if (x.nr == 1) {
+            ropf *= a;
+        } else if (x.nr == 2) {
+            double b = C[x.rk[1]];
+            ropf = (a < 0 && b < 0) ? 0.0 : ropf * (a * b);
+        } else {
+            double b = C[x.rk[1]], d = C[x.rk[2]];
+            ropf = ((a < 0 && (b < 0 || d < 0)) || (b < 0 && d < 0)) ? 0.0 : ropf * (a * b * d);
+        }
+    }
+    if (rev) {
+        if (x.flags & 16) {
+            const double* gs = m.gend + 12 * x.gen;
+#pragma unroll
+            for (int j = 0; j < kMaxSide; j++) {
+                if (j < x.np) {
+                    double c = C[x.pk[j]];
+                    ropr = (c > 0.0) ? ropr * pow(c, gs[8 + j]) : 0.0;
+                }
+            }
+        } else {
+            double a = C[x.pk[0]];
+            if (x.np == 1) {
+                ropr *= a;
+            } else if (x.np == 2) {
+                double b = C[x.pk[1]];
+                ropr = (a < 0 && b < 0) ? 0.0 : ropr * (a * b);
+            } else {
+                double b = C[x.pk[1]], d = C[x.pk[2]];
+                ropr = ((a < 0 && (b < 0 || d < 0)) || (b < 0 && d < 0)) ? 0.0 : ropr * (a * b * d);
+            }
+        }
+    }
+    o.ropf = ropf;
+    o.ropr = ropr;
+}
+
+// Gas state + per-species thermo, concentrations and chemical potentials.
+// Y values: lane owns species lane + 64 s in yk[s].  Leaves C and MU in LDS.
+template <int NS>
+__device__ __forceinline__ void species_pass(const DevMech& m, const WaveScratch& S, double T, double P,
+                                             const double (&yk)[NS], GasState& g, SpeciesRegs<NS>& sr) {
+    const int lane = lane_id();
+    double sum = 0.0;
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        int k = lane + kWave * s;
+        sr.ym[s] = (k < m.K) ? yk[s] * m.rmw[k] : 0.0;
+        sum += sr.ym[s];
+    }
+    sum = uni(wave_sum(sum));
+    g.T = T;
+    g.P = P;
+    g.mmw = 1.0 / sum;
+    g.RT = T * kGasConstant;
+    g.dens = P * g.mmw / g.RT;
+    g.ctot = g.dens / g.mmw;
+    g.pressure = kGasConstant * g.ctot * T;
+    g.logT = log(T);
+    g.recipT = 1.0 / T;
+    g.rrt = 1.0 / g.RT;
+    g.logC0 = log(g.pressure / g.RT);
+    g.tmp = log(g.pressure / m.pref) * g.RT;
+    double tt[6];
+    tt[0] = T;
+    tt[1] = T * T;
+    tt[2] = tt[1] * T;
+    tt[3] = tt[2] * T;
+    tt[4] = 1.0 / T;
+    tt[5] = g.logT;
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        int k = lane + kWave * s;
+        if (k < m.K) {
+            double cp, h, sR, g0;
+            nasa7(m.nasa + 16 * k, T, tt, cp, h, sR, g0);
+            sr.cpr[s] = cp;
+            sr.hrt[s] = h;
+            double mu = g0 * g.RT;
+            mu += g.tmp;
+            S.MU[k] = mu;
+            S.C[k] = sr.ym[s] * g.dens;
+        } else {
+            sr.cpr[s] = 0.0;
+            sr.hrt[s] = 0.0;
+        }
+    }
+}
+
+// Net production rates for the species each lane owns (after species_pass).
+template <int NS>
+__device__ __forceinline__ void production_rates(const DevMech& m, const WaveScratch& S, const GasState& g,
+                                                 SpeciesRegs<NS>& sr) {
+    const int lane = lane_id();
+    __syncthreads();
+    for (int p = lane; p < m.I; p += kWave) {
+        RxnVals v;
+        RxnRegs x;
+        load_rxn(m, p, x);
+        eval_reaction(m, p, g, S.C, S.MU, x, v);
+        S.ROP[p] = v.ropf - v.ropr;
+    }
+    __syncthreads();
+    for (int sb = lane; sb < m.nsub; sb += kWave) {
+        double acc = 0.0;
+        const int e1 = m.sub_start[sb + 1];
+        for (int e = m.sub_start[sb]; e < e1; e++) acc += m.ent_nu[e] * S.ROP[m.ent_r[e]];
+        S.SUB[sb] = acc;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        int k = lane + kWave * s;
+        double w = 0.0;
+        if (k < m.K) {
+            const int b1 = m.sp_sub[k + 1];
+            for (int b = m.sp_sub[k]; b < b1; b++) w += S.SUB[b];
+        }
+        sr.wdot[s] = w;
+    }
+}
+
+// Scalars of the reactor energy equation kept for the Jacobian.
+struct RhsScalars {
+    double mass, vol, cp_mass, H;   // H = sum_k h_k wdot_k [J/m^3/s]
+};
+
+// IdealGasConstPressureReactor::eval + ReactorNet::eval: ydot for the lane's
+// components (internal order).  y holds [Y_0..Y_{K-1}, T, m] distributed.
+template <int NS>
+__device__ __forceinline__ void reactor_rhs(const DevMech& m, const WaveScratch& S, double P, const double (&y)[NS],
+                                            double (&f)[NS], GasState& g, SpeciesRegs<NS>& sr, RhsScalars& rs) {
+    const int lane = lane_id();
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        int i = lane + kWave * s;
+        if (i < m.N) S.yb[i] = y[s];
+    }
+    __syncthreads();
+    const double T = uni(S.yb[m.K]);
+    const double mass = uni(S.yb[m.K + 1]);
+    species_pass<NS>(m, S, T, P, y, g, sr);
+    production_rates<NS>(m, S, g, sr);
+    const double vol = mass / g.dens;
+    double e = 0.0, inner = 0.0;
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        int k = lane + kWave * s;
+        if (k < m.K) {
+            e += sr.wdot[s] * (sr.hrt[s] * g.RT) * vol;
+            inner += sr.ym[s] * sr.cpr[s];
+        }
+    }
+    e = uni(wave_sum(e));
+    inner = uni(wave_sum(inner));
+    const double cp_mass = kGasConstant * (g.mmw * inner) / g.mmw;
+    const double mcpdTdt = -e;
+    rs.mass = mass;
+    rs.vol = vol;
+    rs.cp_mass = cp_mass;
+    rs.H = e / vol;
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        int i = lane + kWave * s;
+        double v = 0.0;
+        if (i < m.K) v = (sr.wdot[s] * vol) * m.mw[i] / mass;
+        else if (i == m.K) v = mcpdTdt / (mass * cp_mass);
+        f[s] = v;
+    }
+}
+
+}  // namespace cg

@@ -1,0 +1,121 @@
+// Dense LU with partial pivoting and triangular solves for one matrix held in
+// LDS, one wavefront per matrix.  Row i of the solution vector lives in lane
+// i % 64, slot i / 64.  Pivot search is a wavefront arg-max with shuffles
+// (first maximal entry wins, as in a column-oriented getrf).
+#pragma once
+#include "wave_kinetics.hpp"
+
+namespace cg {
+
+// Factor A (N x N, leading dimension LD, row-major) in place: PA = LU with unit
+// L below the diagonal.  pmap[i] = original row of permuted row i.  Returns
+// false on an exactly zero pivot.
+template <int NS, typename T>
+__device__ bool lu_factor(T* A, int LD, int N, int* pmap) {
+    const int lane = lane_id();
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        int i = lane + kWave * s;
+        if (i < N) pmap[i] = i;
+    }
+    __syncthreads();
+    for (int k = 0; k < N; k++) {
+        double best = -1.0;
+        int bi = N;
+#pragma unroll
+        for (int s = 0; s < NS; s++) {
+            int i = lane + kWave * s;
+            if (i >= k && i < N) {
+                double v = fabs(static_cast<double>(A[i * LD + k]));
+                if (v > best) { best = v; bi = i; }
+            }
+        }
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            double ob = __shfl_xor(best, off, kWave);
+            int oi = __shfl_xor(bi, off, kWave);
+            if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+        }
+        const int p = uni(bi);
+        if (p >= N || A[p * LD + k] == T(0)) return false;
+        if (p != k) {
+#pragma unroll
+            for (int s = 0; s < NS; s++) {
+                int j = lane + kWave * s;
+                if (j < N) {
+                    T a = A[k * LD + j];
+                    A[k * LD + j] = A[p * LD + j];
+                    A[p * LD + j] = a;
+                }
+            }
+            if (lane == 0) {
+                int t = pmap[k];
+                pmap[k] = pmap[p];
+                pmap[p] = t;
+            }
+        }
+        __syncthreads();
+        const T mult = T(1) / A[k * LD + k];
+#pragma unroll
+        for (int s = 0; s < NS; s++) {
+            int i = lane + kWave * s;
+            if (i > k && i < N) A[i * LD + k] *= mult;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int s = 0; s < NS; s++) {
+            int j = lane + kWave * s;
+            if (j > k && j < N) {
+                const T akj = A[k * LD + j];
+                if (akj != T(0)) {
+                    for (int i = k + 1; i < N; i++) A[i * LD + j] -= A[i * LD + k] * akj;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    return true;
+}
+
+// Solve (PA) x = P b in place: b holds the right-hand side distributed over
+// lanes and returns x.  bv is an LDS scratch vector of length N.
+template <int NS, typename T, typename V>
+__device__ void lu_solve(const T* A, int LD, int N, const int* pmap, V* bv, V (&b)[NS]) {
+    const int lane = lane_id();
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        int i = lane + kWave * s;
+        if (i < N) bv[i] = b[s];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        int i = lane + kWave * s;
+        b[s] = (i < N) ? bv[pmap[i]] : V(0);
+    }
+    __syncthreads();
+    // forward substitution with unit lower factor
+    for (int k = 0; k < N - 1; k++) {
+        const V bk = bcast<NS>(b, k);
+#pragma unroll
+        for (int s = 0; s < NS; s++) {
+            int i = lane + kWave * s;
+            if (i > k && i < N) b[s] -= static_cast<V>(A[i * LD + k]) * bk;
+        }
+    }
+    // back substitution
+    for (int k = N - 1; k >= 0; k--) {
+        const int s_k = k >> 6;
+#pragma unroll
+        for (int s = 0; s < NS; s++)
+            if (s == s_k && lane == (k & 63)) b[s] /= static_cast<V>(A[k * LD + k]);
+        const V bk = bcast<NS>(b, k);
+#pragma unroll
+        for (int s = 0; s < NS; s++) {
+            int i = lane + kWave * s;
+            if (i < k) b[s] -= static_cast<V>(A[i * LD + k]) * bk;
+        }
+    }
+}
+
+}  // namespace cg

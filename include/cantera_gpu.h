@@ -1,0 +1,126 @@
+/*
+ * cantera_gpu.h -- C ABI of libcantera_amd.so, the MI355X (gfx950) engine for
+ * batched ideal-gas kinetics and constant-pressure reactor integration.
+ *
+ * Conventions (mirroring the reference CLib, interfaces/clib + the generated
+ * recipes in interfaces/sourcegen/headers/):
+ *   - objects are integer handles (reference: Cabinet<T>, src/clib/Cabinet.h:64-121);
+ *   - every call returns >= 0 on success and a negative code on failure; the
+ *     message is fetched with cg_last_error (reference: ERR = -999 and the
+ *     Application error stack, include/cantera/clib/clib_defs.h:27-33,
+ *     src/clib/clib_utils.h:30-52).  No C++ exception crosses this boundary.
+ *   - units are SI with kmol (reference ct_defs.h); species order is mechanism
+ *     order.
+ *   - batched arrays are species-major SoA: Y[k*n + s] is species k of state s,
+ *     unless a function says otherwise.
+ *   - one reactor batch owns one HIP stream; calls on different handles may run
+ *     concurrently, calls on one handle must be serialised by the caller
+ *     (reference objects are not thread-safe either, openmp_ignition.cpp:28-31).
+ */
+#ifndef CANTERA_GPU_H
+#define CANTERA_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CG_ERR (-999)
+
+enum cg_precision {
+    CG_FP64 = 0,   /* plain: everything in FP64 */
+    CG_MIXED = 1   /* FP32 iteration matrix / LU / solves, FP64 residual, state and step control */
+};
+
+/* ---- library ------------------------------------------------------------ */
+int32_t cg_version(void);
+/* Copy the last error message of the calling thread into buf (NUL-terminated).
+ * Returns the full message length.  Reference: clib_utils.h:30-52 handleAllExceptions. */
+int32_t cg_last_error(char* buf, int32_t len);
+int32_t cg_device_count(void);
+/* Device memory helpers (so callers need no HIP runtime of their own). */
+void* cg_dev_alloc(int64_t bytes);
+int32_t cg_dev_free(void* p);
+int32_t cg_memcpy_h2d(void* dst, const void* src, int64_t bytes);
+int32_t cg_memcpy_d2h(void* dst, const void* src, int64_t bytes);
+int32_t cg_device_synchronize(void);
+
+/* ---- mechanisms ----------------------------------------------------------- */
+/* Load a compiled mechanism (.mech, written by cantera_amd/mechc.py from a
+ * Cantera YAML file) and upload its tables.  Replaces newSolution(file, phase)
+ * for the ideal-gas/bulk-kinetics path (src/base/Solution.cpp:242-282;
+ * KineticsFactory "gas"/"bulk", src/kinetics/KineticsFactory.cpp:24-37).
+ * Returns a mechanism handle. */
+int32_t cg_mech_load(const char* path);
+int32_t cg_mech_free(int32_t mech);
+/* Kinetics::nTotalSpecies / nReactions / Phase::nElements */
+int32_t cg_mech_nspecies(int32_t mech);
+int32_t cg_mech_nreactions(int32_t mech);
+int32_t cg_mech_nelements(int32_t mech);
+/* Phase::speciesIndex (returns -1 if absent) / speciesName */
+int32_t cg_mech_species_index(int32_t mech, const char* name);
+int32_t cg_mech_species_name(int32_t mech, int32_t k, char* buf, int32_t len);
+/* Phase::getMolecularWeights (bufLen/buf like kin_getNetProductionRates) */
+int32_t cg_mech_molecular_weights(int32_t mech, int32_t len, double* mw);
+
+/* ---- batched rates ---------------------------------------------------------- */
+/* Net production rates wdot [kmol/m^3/s] of n independent states given T [K],
+ * P [Pa] and (unnormalised, possibly negative) mass fractions Y.  The batched
+ * form of Kinetics::getNetProductionRates (include/cantera/kinetics/Kinetics.h:572,
+ * src/kinetics/Kinetics.cpp:447-457; CLib kin_getNetProductionRates,
+ * interfaces/sourcegen/headers/ctkin.yaml:46) with the state set as
+ * IdealGasConstPressureReactor::updateState does
+ * (src/zeroD/IdealGasConstPressureReactor.cpp:34-44).
+ * Host pointers; copies in and out.  precision: CG_FP64. */
+int32_t cg_rates_batch(int32_t mech, int64_t n, const double* T, const double* P,
+                       const double* Y, double* wdot, int32_t precision);
+/* Same on device pointers, no copies, asynchronous on `stream` (hipStream_t,
+ * NULL for the default stream). */
+int32_t cg_rates_batch_device(int32_t mech, int64_t n, const double* T, const double* P,
+                              const double* Y, double* wdot, int32_t precision, void* stream);
+
+/* ---- batched reactors ------------------------------------------------------- */
+/* Create n IdealGasConstPressureReactor + ReactorNet pairs (one per state) with
+ * V = 1 m^3, integrator tolerances rtol/atol for every component
+ * (ReactorNet::setTolerances, src/zeroD/ReactorNet.cpp:233-245).  Y0 is AoS
+ * n x K (row s = mass fractions of reactor s).  Returns a batch handle.
+ * Reference: IdealGasConstPressureReactor(sol) + ReactorNet(reactor),
+ * samples/cxx/openmp_ignition/openmp_ignition.cpp:36-40. */
+int32_t cg_reactors_create(int32_t mech, int64_t n, const double* T0, const double* P,
+                           const double* Y0, double rtol, double atol, int32_t precision);
+int32_t cg_reactors_free(int32_t batch);
+/* Advance every reactor s to t_out[s] (host array of n).  History persists
+ * between calls; each reactor steps past t_out[s] and interpolates back, as
+ * ReactorNet::advance does (src/zeroD/ReactorNet.cpp:351-357;
+ * CVodesIntegrator::integrate, src/numerics/CVodesIntegrator.cpp:479-529).
+ * Returns 0, or CG_ERR if any reactor failed (see cg_reactors_status). */
+int32_t cg_reactors_advance(int32_t batch, const double* t_out);
+/* Same with t_out a device pointer; asynchronous (no error check on reactor
+ * status until cg_reactors_status / cg_reactors_get_state). */
+int32_t cg_reactors_advance_device(int32_t batch, const double* t_out_dev);
+/* State y = [m, T, Y_1..Y_K] per reactor, AoS n x (K+2)
+ * (ReactorNet::getState, src/zeroD/ReactorNet.cpp:754-759). */
+int32_t cg_reactors_get_state(int32_t batch, double* y);
+/* Integrator counters per reactor, n x 9 int64: steps, rhs evaluations,
+ * Jacobian evaluations, LU factorisations, Newton iterations, nonlinear
+ * convergence failures, error-test failures, current order, extra rhs
+ * evaluations for Jacobians (CVodesIntegrator::solverStats,
+ * src/numerics/CVodesIntegrator.cpp:572-621). */
+int32_t cg_reactors_stats(int32_t batch, int64_t* stats);
+/* Status per reactor: 0 ok, negative = integration failure code. */
+int32_t cg_reactors_status(int32_t batch, int32_t* status);
+/* Options: max internal steps per advance call (default 20000,
+ * CVodesIntegrator.h:120-132) and the persistent grid size in wavefronts per
+ * CU (0 = from occupancy). */
+int32_t cg_reactors_set_options(int32_t batch, int32_t max_steps, int32_t waves_per_cu);
+/* Duration [ms] of the last integrator kernel of this batch, from HIP events
+ * on the batch stream. */
+double cg_reactors_last_kernel_ms(int32_t batch);
+/* Duration [ms] of the last batched-rates kernel on `mech`. */
+double cg_rates_last_kernel_ms(int32_t mech);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
