@@ -1,0 +1,271 @@
+"""Pins of the CPU oracle (no GPU needed).
+
+Each check is independent of the oracle's own formulas: reference known-answer
+values, closed forms evaluated from separately computed inputs, conservation
+laws, the reference's blessed regression outputs, and an independent
+integrator (scipy Radau) on the same right-hand side.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, mech_file, parse_stoich
+
+R = 6.02214076e26 * 1.380649e-23
+ONE_ATM = 101325.0
+
+
+@pytest.fixture(scope="module")
+def gri(oracle_mod):
+    return oracle_mod.Mech(mech_file("gri30"))
+
+
+def test_nasa7_chemkin_known_answer(gri):
+    """test/thermo/nasapoly.cpp:62-78: GRI-3.0 CO2 low-range polynomial against
+    CHEMKIN-II values (1e-7 / 1e-5 / 1e-6 absolute)."""
+    k = gri.index("CO2")
+    for T, cp_ref, h_ref, s_ref in ((298.15, 4.46633496, -158.739244, 25.7125777),
+                                    (876.54, 6.33029000, -50.3179924, 31.5401226)):
+        cp, h, s = gri.thermo(T)
+        assert abs(cp[k] - cp_ref) < 1e-7
+        assert abs(h[k] - h_ref) < 1e-5
+        assert abs(s[k] - s_ref) < 1e-6
+
+
+def _nox_state(m):
+    X = np.zeros(m.K)
+    for name, v in (("H2O", 1.0), ("H", 0.2), ("O2", 0.3), ("NH", 0.05), ("NO", 0.05), ("N2O", 0.05)):
+        X[m.index(name)] = v
+    X /= X.sum()
+    Y = X * m.mw
+    return Y / Y.sum()
+
+
+def test_rates_known_answer_noxneg(oracle_mod):
+    """test/kinetics/rates.cpp:128-150 (noxNeg.yaml, negative A, three-body,
+    Lindemann): wdot, ropf, ropr at 2000 K, 1 atm within 2e-5 rel + 1e-9."""
+    m = oracle_mod.Mech(mech_file("noxNeg"))
+    r = m.rates(2000.0, ONE_ATM, _nox_state(m))
+    wref = np.array([0.44705, -0.0021443, 0, -279.36, 0.0021432, 278.92, 0.4449, -279.36, 279.36, 0, 0, 0])
+    fref = np.array([479.305, -128.202, 0, -0, 0, 0, 0, 0, 0, 0.4449, 0, 0])
+    rref = np.array([97.94, -26.1964, 0, -0, 1.10334e-06, 0, 0, 0, 6.58592e-06, 0, 0, 0.00214319])
+    for got, ref in ((r["wdot"], wref), (r["ropf"], fref), (r["ropr"], rref)):
+        assert np.all(np.abs(got - ref) <= np.abs(ref) * 2e-5 + 1e-9)
+
+
+def test_equilibrium_constants_closed_form(gri):
+    """Kc_i = exp(-dG0_i/RT) (P_ref/RT)^dn_i (rates.cpp:99-115 pattern, 1e-13 rel),
+    with dG0 assembled here from the NASA7 g0/RT values."""
+    nu_r, nu_p = parse_stoich(mech_file("gri30"))
+    nu = nu_p - nu_r
+    dn = nu.sum(axis=0)
+    Y = np.full(gri.K, 1.0 / gri.K)
+    for T in (600.0, 1300.0, 2400.0):
+        cp, h, s = gri.thermo(T)
+        g0 = h - s
+        Kc = gri.equilibrium_constants(T, 3 * ONE_ATM, Y)
+        ref = np.exp(-(nu.T @ g0)) * (ONE_ATM / (R * T)) ** dn
+        assert np.max(np.abs(Kc - ref) / ref) < 1e-12
+
+
+def test_reverse_rates_detailed_balance(gri):
+    """ropf / ropr = Kc prod(C_reac) / prod(C_prod) for every reversible reaction."""
+    nu_r, nu_p = parse_stoich(mech_file("gri30"))
+    g = np.random.default_rng(3)
+    for _ in range(5):
+        T = g.uniform(800, 2500)
+        P = ONE_ATM * g.uniform(0.5, 20)
+        X = 10 ** g.uniform(-6, 0, gri.K)
+        Y = X * gri.mw
+        Y /= Y.sum()
+        r = gri.rates(T, P, Y)
+        Kc = gri.equilibrium_constants(T, P, Y)
+        C = Y / gri.mw * (P / (R * T)) / np.sum(Y / gri.mw)
+        lnC = np.log(C)
+        rev = r["ropr"] != 0
+        lhs = np.log(r["ropf"][rev] / r["ropr"][rev])
+        rhs = np.log(Kc[rev]) + (nu_r.T @ lnC)[rev] - (nu_p.T @ lnC)[rev]
+        assert np.max(np.abs(lhs - rhs)) < 1e-9
+
+
+def test_mass_and_element_conservation(oracle_mod):
+    for mech in ("h2o2", "gri30", "nDodecane"):
+        m = oracle_mod.Mech(mech_file(mech))
+        g = np.random.default_rng(5)
+        for _ in range(4):
+            Y = 10 ** g.uniform(-8, 0, m.K)
+            Y /= Y.sum()
+            r = m.rates(g.uniform(900, 2500), ONE_ATM * g.uniform(0.5, 30), Y)
+            w = r["wdot"]
+            scale = np.abs(m.mw * w).sum() + 1e-300
+            assert abs(m.mw @ w) / scale < 1e-13
+            el = m.composition.T @ w
+            el_scale = m.composition.T @ np.abs(w) + 1e-300
+            assert np.max(np.abs(el) / el_scale) < 1e-12
+
+
+def test_troe_falloff_center_and_lindemann(gri, oracle_mod):
+    """Troe: at log10(Pr) = -c = 0.4 + 0.67 log10(Fcent), f1 = 0 so F = Fcent
+    exactly and k = kinf Pr/(1+Pr) Fcent.  Lindemann: F = 1.
+    (Falloff.cpp:283-300; Gilbert et al. 1983.)  Pure N2 bath gas makes [M] = P/RT
+    for reactions whose efficiency list does not name N2."""
+    recs = {}
+    with open(mech_file("gri30")) as f:
+        lines = f.read().splitlines()
+    for i, line in enumerate(lines):
+        t = line.split()
+        if t and t[0] == "R" and int(t[2]) >= 2:
+            idx = int(t[1])
+            eq = lines[i + 1][2:]
+            vals = [float(x) for x in t[4:15]]
+            rest = t[15:]
+            nr = int(rest[0]); p = 1 + 3 * nr
+            np_ = int(rest[p]); p += 1 + 2 * np_
+            tbdef = float(rest[p]); ntb = int(rest[p + 1])
+            tbk = [int(rest[p + 2 + 2 * j]) for j in range(ntb)]
+            recs[idx] = (int(t[2]), vals, tbdef, tbk, eq)
+    n2 = gri.index("N2")
+    Y = np.zeros(gri.K)
+    Y[n2] = 1.0
+    checked_troe = checked_lind = 0
+    for idx, (typ, v, tbdef, tbk, eq) in recs.items():
+        if tbdef != 1.0 or n2 in tbk:
+            continue
+        dn, A, b, EaR, A0, b0, EaR0, a, rt3, rt1, t2 = v
+        T = 1500.0
+        kinf = A * np.exp(b * np.log(T) - EaR / T)
+        k0 = A0 * np.exp(b0 * np.log(T) - EaR0 / T)
+        if typ == 3:
+            Fcent = (1 - a) * np.exp(-T * rt3) + a * np.exp(-T * rt1) + (np.exp(-t2 / T) if t2 else 0.0)
+            lFc = np.log10(Fcent)
+            Pr = 10 ** (0.4 + 0.67 * lFc)
+            expect_F = Fcent
+        else:
+            Pr = 0.37
+            expect_F = 1.0
+        P = Pr * kinf / k0 * R * T
+        kf = gri.rates(T, P, Y)["kf"][idx]
+        Pr_act = (P / (R * T)) * k0 / kinf
+        expect = kinf * Pr_act / (1 + Pr_act) * expect_F
+        assert abs(kf - expect) / expect < 1e-10, eq
+        if typ == 3:
+            checked_troe += 1
+        else:
+            checked_lind += 1
+    assert checked_troe >= 5 and checked_lind >= 1
+
+
+def test_analytic_jacobian_matches_finite_differences(oracle_mod):
+    """Analytic d(ydot)/dy against central differences of the RHS
+    (test_jacobian.py:886-1167 style; row-scaled tolerance 1e-6)."""
+    for mech, fuel, ox, n2 in (("h2o2", "H2", "O2", "N2"), ("gri30", "CH4", "O2", "N2"),
+                               ("nDodecane", "c12h26", "o2", "n2")):
+        m = oracle_mod.Mech(mech_file(mech))
+        g = np.random.default_rng(1)
+        X = g.random(m.K) * 1e-3
+        X[m.index(fuel)] += 0.1
+        X[m.index(ox)] += 0.2
+        X[m.index(n2)] += 0.7
+        X /= X.sum()
+        Y = X * m.mw
+        Y /= Y.sum()
+        for T in (900.0, 1600.0, 2400.0):
+            y = np.concatenate([[0.3, T], Y])
+            P = 3 * ONE_ATM
+            Ja = m.jacobian(P, y, True)
+            N = m.K + 2
+            Jf = np.zeros((N, N))
+            for j in range(1, N):
+                d = max(abs(y[j]), 1e-8) * 1e-6
+                yp, ym = y.copy(), y.copy()
+                yp[j] += d
+                ym[j] -= d
+                Jf[:, j] = (m.rhs(P, yp) - m.rhs(P, ym)) / (2 * d)
+            scale = np.abs(Jf).max(axis=1, keepdims=True) + 1e-30 * np.abs(Jf).max()
+            assert np.max(np.abs(Ja - Jf)[1:, 1:] / scale[1:]) < 1e-6, (mech, T)
+
+
+def _kin1_rows(m, analytic):
+    from oracle.oracle import Reactor
+    X = np.zeros(m.K)
+    X[m.index("H2")] = 2.0
+    X[m.index("O2")] = 1.0
+    X[m.index("N2")] = 4.0
+    X /= X.sum()
+    Y = X * m.mw
+    Y /= Y.sum()
+    r = Reactor(m, 1001.0, ONE_ATM, Y, analytic=analytic)
+
+    def row(t, y):
+        Yk = y[2:]
+        mmw = 1 / np.sum(Yk / m.mw)
+        rho = ONE_ATM * mmw / (R * y[1])
+        return [float("%.6g" % v) for v in [t, y[1], rho, ONE_ATM, *(Yk / m.mw * mmw)]]
+
+    rows = [row(0.0, r.state)]
+    for i in range(1, 101):
+        r.advance(i * 1e-5)
+        rows.append(row(i * 1e-5, r.state))
+    return np.array(rows)
+
+
+@pytest.mark.parametrize("analytic", [False, True])
+def test_kin1_blessed_trajectory(oracle_mod, analytic):
+    """samples/cxx/kinetics1 against test_problems/cxx_samples/kin1_blessed.csv
+    with the reference comparator (2e-4, threshold 1e-14)."""
+    m = oracle_mod.Mech(mech_file("h2o2"))
+    ours = _kin1_rows(m, analytic)
+    ref = np.genfromtxt(os.path.join(GOLDEN, "kin1_blessed.csv"), delimiter=",", skip_header=1)
+    rel = np.abs(ours - ref) / (np.maximum(np.abs(ours), np.abs(ref)) + 1e-14)
+    assert np.nanmax(rel) < 2e-4
+
+
+def test_gri30_ignition_delays_blessed(oracle_mod):
+    """samples/cxx/openmp_ignition (gri30, CH4:0.5 O2:1 N2:3.76, 1 atm, 50 T0 in
+    [1000, 1490] K): time of the first internal step with T >= T0 + 500 K, at the
+    4 significant figures of opmenmp_ignition_blessed.txt."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    from oracle.oracle import Reactor
+    m = oracle_mod.Mech(mech_file("gri30"))
+    X = np.zeros(m.K)
+    X[m.index("CH4")] = 0.5
+    X[m.index("O2")] = 1.0
+    X[m.index("N2")] = 3.76
+    X /= X.sum()
+    Y = X * m.mw
+    Y /= Y.sum()
+    ref = []
+    for line in open(os.path.join(GOLDEN, "openmp_ignition_blessed.txt")):
+        p = line.split()
+        try:
+            ref.append((float(p[0]), float(p[1])))
+        except (ValueError, IndexError):
+            pass
+    assert len(ref) == 50
+
+    def run(T0):
+        r = Reactor(m, T0, ONE_ATM, Y)
+        while r.state[1] < T0 + 500:
+            r.step()
+        return r.time
+
+    T0s = [1000 + 500 * float(np.float32(i) / np.float32(50)) for i in range(50)]
+    with ThreadPoolExecutor(8) as ex:
+        t_ig = list(ex.map(run, T0s))
+    for (T, t_ref), t in zip(ref, t_ig):
+        assert float("%.3e" % t) == t_ref, (T, t, t_ref)
+
+
+def test_bdf_against_radau_golden(oracle_mod):
+    """The BDF (rtol 1e-10) against scipy Radau IIA at rtol 1e-12 on the same RHS
+    (tests/golden/radau_h2o2.npz): |dT| <= 1e-6 T, |dY| <= 1e-9 + 1e-5 |Y|."""
+    d = np.load(os.path.join(GOLDEN, "radau_h2o2.npz"))
+    m = oracle_mod.Mech(mech_file("h2o2"))
+    touts = d["touts"]
+    dt = np.full(len(d["T0"]), touts[1] - touts[0])
+    y, _ = m.run_reactors(d["T0"], d["P"], d["Y0"], dt, len(touts) - 1, 1e-10, 1e-20, analytic=False, nthreads=4)
+    ref = d["states"]
+    assert np.max(np.abs(y[:, :, 1] - ref[:, :, 1]) / ref[:, :, 1]) < 1e-6
+    assert np.all(np.abs(y[:, :, 2:] - ref[:, :, 2:]) <= 1e-9 + 1e-5 * np.abs(ref[:, :, 2:]))
