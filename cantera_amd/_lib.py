@@ -49,6 +49,7 @@ SIGNATURES = [
     ("cg_reactors_get_state", C.c_int32, [C.c_int32, _dp]),
     ("cg_reactors_stats", C.c_int32, [C.c_int32, _i64p]),
     ("cg_reactors_status", C.c_int32, [C.c_int32, _i32p]),
+    ("cg_reactors_get_control", C.c_int32, [C.c_int32, _dp]),
     ("cg_reactors_set_options", C.c_int32, [C.c_int32, C.c_int32, C.c_int32]),
     ("cg_reactors_last_kernel_ms", C.c_double, [C.c_int32]),
     ("cg_rates_last_kernel_ms", C.c_double, [C.c_int32]),

@@ -16,7 +16,8 @@ OUT = os.path.join(HERE, "libcantera_amd.so")
 OBJDIR = os.path.join(HERE, "_build")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["mech.cpp", "api.cpp", "rates.hip", "bdf.hip"]
+SOURCES = ["mech.cpp", "api.cpp", "rates.hip", "bdf.hip", "bdf_r16.hip", "bdf_r32.hip", "bdf_r56.hip",
+           "bdf_r64.hip", "bdf_lds.hip"]
 
 COMMON = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",
@@ -41,7 +42,7 @@ def _compile(src: str) -> str:
     return obj
 
 
-def build(jobs: int = 4) -> str:
+def build(jobs: int = 8) -> str:
     os.makedirs(OBJDIR, exist_ok=True)
     with ThreadPoolExecutor(jobs) as ex:
         objs = list(ex.map(_compile, SOURCES))

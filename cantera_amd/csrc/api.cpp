@@ -305,7 +305,8 @@ int32_t cg_reactors_create(int32_t h, int64_t n, const double* T0, const double*
         b->rec_stride = 7 * static_cast<size_t>(NP) + C_COUNT;
         b->j_stride = static_cast<size_t>(m.N) * m.N;
         b->lu_stride = static_cast<size_t>(m.N) * b->ld;
-        b->piv_stride = static_cast<size_t>(NP);
+        b->piv_stride = 2 * static_cast<size_t>(NP);
+        if (bdf_variant(m.N) < 0) throw std::runtime_error("mechanisms with more than 126 species are not supported");
         const size_t es = b->fp32 ? 4 : 8;
         check(hipMalloc(&b->rec, b->rec_stride * n * 8), "hipMalloc(records)");
         check(hipMalloc(&b->jsave, b->j_stride * n * 8), "hipMalloc(jacobians)");
@@ -335,8 +336,8 @@ int32_t cg_reactors_create(int32_t h, int64_t n, const double* T0, const double*
             c[C_ETA] = 1.0;
         }
         check(hipMemcpy(b->rec, rec.data(), rec.size() * 8, hipMemcpyHostToDevice), "hipMemcpy(records)");
-        b->lds = bdf_lds_bytes(m, b->ns, b->ld, b->fp32);
-        set_lds_limit(bdf_kernel_ptr(b->ns, b->fp32), b->lds);
+        b->lds = bdf_lds_bytes(m, b->ld, b->fp32);
+        set_lds_limit(bdf_kernel_ptr(m.N, b->fp32), b->lds);
         std::lock_guard<std::mutex> lk(g_mu);
         g_batches.push_back(std::move(b));
         return static_cast<int32_t>(g_batches.size() - 1);
@@ -382,9 +383,9 @@ static void launch_advance(Batch& b) {
     a.tout = b.tout;
     a.order = nullptr;
     a.queue = b.queue;
-    const int grid = grid_for(bdf_kernel_ptr(b.ns, b.fp32), b.lds, b.waves_per_cu, b.n);
+    const int grid = grid_for(bdf_kernel_ptr(b.N, b.fp32), b.lds, b.waves_per_cu, b.n);
     check(hipEventRecord(b.e0, b.stream), "hipEventRecord");
-    check(bdf_launch(a, b.ns, b.fp32, grid, b.lds, b.stream), "integrator kernel launch");
+    check(bdf_launch(a, b.fp32, grid, b.lds, b.stream), "integrator kernel launch");
     check(hipEventRecord(b.e1, b.stream), "hipEventRecord");
 }
 
@@ -473,6 +474,18 @@ int32_t cg_reactors_stats(int32_t h, int64_t* stats) {
             o[7] = static_cast<int64_t>(x[C_Q]);
             o[8] = static_cast<int64_t>(x[C_NFE_DQ]);
         }
+        return 0;
+    });
+}
+
+int32_t cg_reactors_get_control(int32_t h, double* ctl) {
+    return guarded([&] {
+        Batch& b = get_batch(h);
+        if (!ctl) throw std::runtime_error("null array");
+        check(hipStreamSynchronize(b.stream), "hipStreamSynchronize");
+        const int NP = 64 * b.ns;
+        check(hipMemcpy2D(ctl, C_COUNT * 8, b.rec + 7 * NP, b.rec_stride * 8, C_COUNT * 8, b.n, hipMemcpyDeviceToHost),
+              "hipMemcpy2D(control)");
         return 0;
     });
 }

@@ -38,13 +38,14 @@ struct BdfArgs {
     int* queue;           // device work-queue head
     // typed view used by the kernel
     template <class T> __host__ __device__ T* lu() const { return static_cast<T*>(lu_save_raw); }
-#ifdef __HIP_DEVICE_COMPILE__
-#endif
 };
 
-size_t bdf_lds_bytes(const DevMech& m, int ns, int ld, bool fp32);
-hipError_t bdf_launch(const BdfArgs& a, int ns, bool fp32, int grid, size_t lds, hipStream_t stream);
-const void* bdf_kernel_ptr(int ns, bool fp32);
+// kernel variant for a state size N: 16/32/56/64 = matrix in registers with
+// that many columns, 0 = matrix in LDS (N <= 128), -1 = unsupported
+int bdf_variant(int N);
+size_t bdf_lds_bytes(const DevMech& m, int ld, bool fp32);
+hipError_t bdf_launch(const BdfArgs& a, bool fp32, int grid, size_t lds, hipStream_t stream);
+const void* bdf_kernel_ptr(int N, bool fp32);
 
 // batched rate evaluation (species-major SoA in and out)
 size_t rates_lds_bytes(const DevMech& m, int ns);

@@ -1,0 +1,898 @@
+// Batched stiff integration of IdealGasConstPressureReactor, one wavefront per
+// reactor, persistent kernel drawing reactors from a device work queue.
+//
+// Algorithm: variable-order (q <= 5), variable-step BDF in fixed-leading-
+// coefficient Nordsieck form with a modified Newton corrector on a dense LU of
+// M = I - gamma J -- the method CVODES runs for the reference
+// (ReactorNet -> CVodesIntegrator, src/numerics/CVodesIntegrator.cpp:281-549;
+// SUNDIALS 7.5 is not vendored, the method is restated from Hindmarsh et al.,
+// ACM TOMS 31 (2005) and the CVODE user guide).  advance(tout) steps past tout
+// and interpolates (CVodesIntegrator::integrate, :479-529); history, the saved
+// Jacobian and the factored iteration matrix persist between calls.
+//
+// Layout: the Nordsieck history and work vectors are lane-distributed
+// registers (component i in lane i % 64, slot i / 64); the step-control block
+// lives in the wavefront's LDS; the iteration matrix is held by the linear-
+// system policy LS (registers or LDS).
+//
+// Precision modes: plain (FP64 everywhere) and mixed (iteration matrix, its
+// LU and the triangular solves in FP32; Newton residual, state, error test and
+// step control in FP64 -- an inexact Newton corrected by the FP64 residual).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "bdf.hpp"
+#include "linsys.hpp"
+#include "wave_jac.hpp"
+
+namespace cg {
+
+namespace bdfc {
+constexpr int QMAX = 5;
+constexpr int MAXCOR = 3;
+constexpr int MXNEF = 7;
+constexpr int MXNCF = 10;
+constexpr int MXNEF1 = 3;
+constexpr int SMALL_NEF = 2;
+constexpr int LONG_WAIT = 10;
+constexpr int SMALL_NST = 10;
+constexpr double NLSCOEF = 0.1;
+constexpr double ETAMX2 = 10.0, ETAMX3 = 10.0, ETAMXF = 0.2, ETAMIN = 0.1, ETACF = 0.25;
+constexpr double ADDON = 1e-6, BIAS1 = 6.0, BIAS2 = 6.0, BIAS3 = 10.0, THRESH = 1.5;
+constexpr double CRDOWN = 0.3, DGMAX_LSETUP = 0.3, RDIV = 2.0, DGMAX_JBAD = 0.2;
+constexpr int MSBP = 20, MSBJ = 51;
+constexpr double HLB_FACTOR = 100.0, HUB_FACTOR = 0.1, H_BIAS = 0.5;
+constexpr int HIN_ITERS = 4;
+constexpr double UROUND = 2.220446049250313e-16;
+enum { FIRST_CALL = 0, PREV_CONV_FAIL = 1, PREV_ERR_FAIL = 2 };
+enum { NO_FAILURES = 0, FAIL_BAD_J = 1, FAIL_OTHER = 2 };
+}  // namespace bdfc
+
+// step-control block (wave-uniform), kept in LDS
+struct Ctl {
+    double tn, h, hprime, hscale, eta, etamax, hu;
+    double tau[bdfc::QMAX + 2], tq[6], l[bdfc::QMAX + 1];
+    double rl1, gamma, gammap, gamrat, crate, delp, acnrm, saved_tq5, t_user, P;
+    double nfe, nje, nsetups, nni, ncfn, netf, nfe_dq;
+    int q, qprime, qwait, L, nst, nstlp, nstlj, nscon, started, convfail, jcur, lu_dirty;
+};
+constexpr size_t kCtlDoubles = (sizeof(Ctl) + 7) / 8;
+
+template <int NS>
+__device__ __forceinline__ double wrms(const double (&v)[NS], const double (&w)[NS], int N) {
+    double s = 0.0;
+#pragma unroll
+    for (int t = 0; t < NS; t++) {
+        double x = v[t] * w[t];
+        s += x * x;
+    }
+    return uni(sqrt(wave_sum(s) / N));
+}
+
+template <int NS, class LS, typename MT>
+struct Integrator {
+    const DevMech& m;
+    const BdfArgs& a;
+    WaveScratch S;
+    Ctl& c;
+    LS ls;
+    int N, lane;
+    double rtol, atol;
+    double zn[bdfc::QMAX + 1][NS];
+    double ewt[NS], acor[NS], y[NS], ftemp[NS], tempv[NS];
+    double* rec;
+    size_t rid;
+
+    __device__ __forceinline__ Integrator(const DevMech& m_, const BdfArgs& a_, Ctl& c_) : m(m_), a(a_), c(c_) {}
+
+    __device__ __forceinline__ int comp(int s) const { return lane + kWave * s; }
+
+    __device__ __forceinline__ void load(size_t r) {
+        rid = r;
+        rec = a.rec + r * a.rec_stride;
+        const int NP = kWave * NS;
+#pragma unroll
+        for (int j = 0; j <= bdfc::QMAX; j++)
+#pragma unroll
+            for (int s = 0; s < NS; s++) zn[j][s] = rec[j * NP + comp(s)];
+        const double* x = rec + 7 * NP;
+        if (lane == 0) {
+            c.tn = x[C_TN]; c.h = x[C_H]; c.hprime = x[C_HPRIME]; c.hscale = x[C_HSCALE]; c.eta = x[C_ETA];
+            c.etamax = x[C_ETAMAX]; c.hu = x[C_HU];
+            for (int i = 0; i < bdfc::QMAX + 2; i++) c.tau[i] = x[C_TAU + i];
+            for (int i = 0; i < 6; i++) c.tq[i] = x[C_TQ + i];
+            c.rl1 = x[C_RL1]; c.gamma = x[C_GAMMA]; c.gammap = x[C_GAMMAP]; c.gamrat = x[C_GAMRAT];
+            c.crate = x[C_CRATE]; c.delp = x[C_DELP]; c.acnrm = x[C_ACNRM]; c.saved_tq5 = x[C_SAVED_TQ5];
+            c.q = static_cast<int>(x[C_Q]); c.qprime = static_cast<int>(x[C_QPRIME]);
+            c.qwait = static_cast<int>(x[C_QWAIT]); c.L = static_cast<int>(x[C_L]);
+            c.nst = static_cast<int>(x[C_NST]); c.nstlp = static_cast<int>(x[C_NSTLP]);
+            c.nstlj = static_cast<int>(x[C_NSTLJ]); c.nscon = static_cast<int>(x[C_NSCON]);
+            c.started = static_cast<int>(x[C_STARTED]);
+            c.t_user = x[C_TUSER];
+            c.nfe = x[C_NFE]; c.nje = x[C_NJE]; c.nsetups = x[C_NSETUPS]; c.nni = x[C_NNI];
+            c.ncfn = x[C_NCFN]; c.netf = x[C_NETF]; c.nfe_dq = x[C_NFE_DQ];
+            c.P = x[C_P];
+            c.jcur = 0;
+            c.lu_dirty = 0;
+        }
+        const bool have_lu = uni(x[C_HAVE_LU]) != 0.0;
+        wave_sync();
+        if (have_lu)
+            ls.restore(a.template lu<MT>() + r * a.lu_stride, a.piv_save + r * a.piv_stride, N);
+    }
+
+    __device__ __forceinline__ void store(int status) {
+        const int NP = kWave * NS;
+#pragma unroll
+        for (int j = 0; j <= bdfc::QMAX; j++)
+#pragma unroll
+            for (int s = 0; s < NS; s++) rec[j * NP + comp(s)] = zn[j][s];
+        const bool dirty = uni(c.lu_dirty) != 0;
+        if (lane == 0) {
+            double* x = rec + 7 * NP;
+            x[C_TN] = uni(c.tn); x[C_H] = uni(c.h); x[C_HPRIME] = uni(c.hprime); x[C_HSCALE] = uni(c.hscale); x[C_ETA] = uni(c.eta);
+            x[C_ETAMAX] = uni(c.etamax); x[C_HU] = uni(c.hu);
+            for (int i = 0; i < bdfc::QMAX + 2; i++) x[C_TAU + i] = uni(c.tau[i]);
+            for (int i = 0; i < 6; i++) x[C_TQ + i] = uni(c.tq[i]);
+            x[C_RL1] = uni(c.rl1); x[C_GAMMA] = uni(c.gamma); x[C_GAMMAP] = uni(c.gammap); x[C_GAMRAT] = uni(c.gamrat);
+            x[C_CRATE] = uni(c.crate); x[C_DELP] = uni(c.delp); x[C_ACNRM] = uni(c.acnrm); x[C_SAVED_TQ5] = uni(c.saved_tq5);
+            x[C_Q] = uni(c.q); x[C_QPRIME] = uni(c.qprime); x[C_QWAIT] = uni(c.qwait); x[C_L] = uni(c.L); x[C_NST] = uni(c.nst);
+            x[C_NSTLP] = uni(c.nstlp); x[C_NSTLJ] = uni(c.nstlj); x[C_NSCON] = uni(c.nscon); x[C_STARTED] = uni(c.started);
+            x[C_TUSER] = uni(c.t_user);
+            x[C_NFE] = uni(c.nfe); x[C_NJE] = uni(c.nje); x[C_NSETUPS] = uni(c.nsetups); x[C_NNI] = uni(c.nni); x[C_NCFN] = uni(c.ncfn);
+            x[C_NETF] = uni(c.netf); x[C_NFE_DQ] = uni(c.nfe_dq);
+            x[C_STATUS] = status;
+            if (dirty) x[C_HAVE_LU] = 1.0;
+        }
+        if (dirty) ls.save(a.template lu<MT>() + rid * a.lu_stride, a.piv_save + rid * a.piv_stride, N);
+    }
+
+    __device__ __forceinline__ int f_eval(const double (&yy)[NS], double (&ff)[NS]) {
+        if (lane == 0) c.nfe += 1;
+        GasState g;
+        SpeciesRegs<NS> sr;
+        RhsScalars rs;
+        reactor_rhs<NS>(m, S, uni(c.P), yy, ff, g, sr, rs);
+        int bad = 0;
+#pragma unroll
+        for (int s = 0; s < NS; s++)
+            if (comp(s) < N && !isfinite(ff[s])) bad = 1;
+        return __any(bad) ? 1 : 0;
+    }
+
+    __device__ __forceinline__ void ewt_set(const double (&yy)[NS]) {
+#pragma unroll
+        for (int s = 0; s < NS; s++) ewt[s] = (comp(s) < N) ? 1.0 / (rtol * fabs(yy[s]) + atol) : 0.0;
+    }
+
+    // linear-system setup: Jacobian reuse decision, M = I - gamma J, LU
+    __device__ __forceinline__ int lsetup() {
+        using namespace bdfc;
+        const double gamma = uni(c.gamma);
+        const double dgamma = fabs(gamma / uni(c.gammap) - 1.0);
+        const int nst = uni(c.nst), convfail = uni(c.convfail);
+        const bool jbad = (nst == 0) || (nst >= uni(c.nstlj) + MSBJ) ||
+                          ((convfail == FAIL_BAD_J) && (dgamma < DGMAX_JBAD)) || (convfail == FAIL_OTHER);
+        double* Js = a.j_save + rid * a.j_stride;
+        wave_sync();
+        if (!jbad) {
+            if (lane == 0) c.jcur = 0;
+        } else {
+            if (lane == 0) {
+                c.nje += 1;
+                c.nstlj = nst;
+                c.jcur = 1;
+                c.nfe_dq += 2;
+            }
+            reactor_jacobian<NS>(m, S, uni(c.P), zn[0], ftemp, Js);
+        }
+        ls.form(Js, N, gamma);
+        const bool ok = ls.factor(N);
+        if (lane == 0) {
+            c.nsetups += 1;
+            c.lu_dirty = 1;
+            if (ok) {
+                c.gamrat = 1.0;
+                c.gammap = gamma;
+                c.crate = 1.0;
+                c.nstlp = nst;
+            }
+        }
+        wave_sync();
+        return ok ? 0 : 1;
+    }
+
+    __device__ __forceinline__ void set_bdf() {
+        using namespace bdfc;
+        const int q = uni(c.q);
+        const double h = uni(c.h);
+        double l[QMAX + 1];
+        double alpha0, alpha0_hat, xi_inv, xistar_inv, hsum;
+        l[0] = l[1] = xi_inv = xistar_inv = 1.0;
+#pragma unroll
+        for (int i = 2; i <= QMAX; i++) l[i] = 0.0;
+        alpha0 = alpha0_hat = -1.0;
+        hsum = h;
+        if (q > 1) {
+#pragma unroll
+            for (int j = 2; j < QMAX; j++) {
+                if (j < q) {
+                    hsum += uni(c.tau[j - 1]);
+                    xi_inv = h / hsum;
+                    alpha0 -= 1.0 / j;
+#pragma unroll
+                    for (int i = j; i >= 1; i--) l[i] += l[i - 1] * xi_inv;
+                }
+            }
+            alpha0 -= 1.0 / q;
+            xistar_inv = -l[1] - alpha0;
+            hsum += uni(c.tau[q - 1]);
+            xi_inv = h / hsum;
+            alpha0_hat = -l[1] - xi_inv;
+#pragma unroll
+            for (int i = QMAX; i >= 1; i--)
+                if (i <= q) l[i] += l[i - 1] * xistar_inv;
+        }
+        double lq = 0.0;
+#pragma unroll
+        for (int i = 1; i <= QMAX; i++)
+            if (i == q) lq = l[i];
+        const double A1 = 1.0 - alpha0_hat + alpha0;
+        const double A2 = 1.0 + q * A1;
+        double tq1 = uni(c.tq[1]), tq3 = uni(c.tq[3]);
+        const double tq2 = fabs(A1 / (alpha0 * A2));
+        const double tq5 = fabs(A2 * xistar_inv / (lq * xi_inv));
+        if (uni(c.qwait) == 1) {
+            if (q > 1) {
+                const double C = xistar_inv / lq;
+                const double A3 = alpha0 + 1.0 / q;
+                const double A4 = alpha0_hat + xi_inv;
+                const double Cpinv = (1.0 - A4 + A3) / A3;
+                tq1 = fabs(C * Cpinv);
+            } else {
+                tq1 = 1.0;
+            }
+            hsum += uni(c.tau[q]);
+            xi_inv = h / hsum;
+            const double A5 = alpha0 - 1.0 / (q + 1);
+            const double A6 = alpha0_hat - xi_inv;
+            const double Cppinv = (1.0 - A6 + A5) / A2;
+            tq3 = fabs(Cppinv / (xi_inv * (q + 2) * A5));
+        }
+        wave_sync();
+        if (lane == 0) {
+#pragma unroll
+            for (int i = 0; i <= QMAX; i++) c.l[i] = l[i];
+            c.tq[1] = tq1;
+            c.tq[2] = tq2;
+            c.tq[3] = tq3;
+            c.tq[4] = NLSCOEF / tq2;
+            c.tq[5] = tq5;
+        }
+        wave_sync();
+    }
+
+    __device__ __forceinline__ void predict() {
+        const int q = uni(c.q);
+        if (lane == 0) c.tn += uni(c.h);
+#pragma unroll
+        for (int k = 1; k <= bdfc::QMAX; k++)
+#pragma unroll
+            for (int j = bdfc::QMAX; j >= 1; j--)
+                if (k <= q && j >= k && j <= q)
+#pragma unroll
+                    for (int s = 0; s < NS; s++) zn[j - 1][s] += zn[j][s];
+        wave_sync();
+    }
+    __device__ __forceinline__ void restore(double saved_t) {
+        const int q = uni(c.q);
+        if (lane == 0) c.tn = saved_t;
+#pragma unroll
+        for (int k = 1; k <= bdfc::QMAX; k++)
+#pragma unroll
+            for (int j = bdfc::QMAX; j >= 1; j--)
+                if (k <= q && j >= k && j <= q)
+#pragma unroll
+                    for (int s = 0; s < NS; s++) zn[j - 1][s] -= zn[j][s];
+        wave_sync();
+    }
+    __device__ __forceinline__ void rescale() {
+        const int q = uni(c.q);
+        const double eta = uni(c.eta);
+        double factor = eta;
+#pragma unroll
+        for (int j = 1; j <= bdfc::QMAX; j++) {
+            if (j <= q) {
+#pragma unroll
+                for (int s = 0; s < NS; s++) zn[j][s] *= factor;
+                factor *= eta;
+            }
+        }
+        const double h = uni(c.hscale) * eta;
+        wave_sync();
+        if (lane == 0) {
+            c.h = h;
+            c.hscale = h;
+            c.nscon = 0;
+        }
+        wave_sync();
+    }
+
+    __device__ __forceinline__ void pick(int j, double (&out)[NS]) {
+#pragma unroll
+        for (int s = 0; s < NS; s++) out[s] = zn[0][s];
+#pragma unroll
+        for (int t = 1; t <= bdfc::QMAX; t++)
+            if (t == j)
+#pragma unroll
+                for (int s = 0; s < NS; s++) out[s] = zn[t][s];
+    }
+
+    __device__ __forceinline__ void increase_bdf() {
+        using namespace bdfc;
+        const int q = uni(c.q);
+        double ll[QMAX + 1];
+#pragma unroll
+        for (int i = 0; i <= QMAX; i++) ll[i] = 0.0;
+        double alpha1, prod, xiold, alpha0, hsum, xi;
+        ll[2] = alpha1 = prod = xiold = 1.0;
+        alpha0 = -1.0;
+        const double hscale = uni(c.hscale);
+        hsum = hscale;
+        if (q > 1) {
+#pragma unroll
+            for (int j = 1; j < QMAX; j++) {
+                if (j < q) {
+                    hsum += uni(c.tau[j + 1]);
+                    xi = hsum / hscale;
+                    prod *= xi;
+                    alpha0 -= 1.0 / (j + 1);
+                    alpha1 += 1.0 / xi;
+#pragma unroll
+                    for (int i = j + 2; i >= 2; i--) ll[i] = ll[i] * xiold + ll[i - 1];
+                    xiold = xi;
+                }
+            }
+        }
+        const double A1 = (-alpha0 - alpha1) / prod;
+        double zL[NS];
+#pragma unroll
+        for (int s = 0; s < NS; s++) zL[s] = A1 * zn[QMAX][s];
+#pragma unroll
+        for (int j = 2; j <= QMAX; j++)
+            if (j == q + 1)
+#pragma unroll
+                for (int s = 0; s < NS; s++) zn[j][s] = zL[s];
+#pragma unroll
+        for (int j = 2; j <= QMAX; j++)
+            if (j <= q)
+#pragma unroll
+                for (int s = 0; s < NS; s++) zn[j][s] += ll[j] * zL[s];
+    }
+    __device__ __forceinline__ void decrease_bdf() {
+        using namespace bdfc;
+        const int q = uni(c.q);
+        double ll[QMAX + 1];
+#pragma unroll
+        for (int i = 0; i <= QMAX; i++) ll[i] = 0.0;
+        ll[2] = 1.0;
+        double hsum = 0.0, xi;
+        const double hscale = uni(c.hscale);
+#pragma unroll
+        for (int j = 1; j <= QMAX - 2; j++) {
+            if (j <= q - 2) {
+                hsum += uni(c.tau[j]);
+                xi = hsum / hscale;
+#pragma unroll
+                for (int i = j + 2; i >= 2; i--) ll[i] = ll[i] * xi + ll[i - 1];
+            }
+        }
+        double zq[NS];
+        pick(q, zq);
+#pragma unroll
+        for (int j = 2; j < QMAX; j++)
+            if (j < q)
+#pragma unroll
+                for (int s = 0; s < NS; s++) zn[j][s] = -ll[j] * zq[s] + zn[j][s];
+    }
+
+    // modified Newton iteration; 0 ok, 1 recoverable failure
+    __device__ __forceinline__ int nls(int nflag) {
+        using namespace bdfc;
+        wave_sync();
+        if (lane == 0) c.convfail = (nflag == FIRST_CALL || nflag == PREV_ERR_FAIL) ? NO_FAILURES : FAIL_OTHER;
+        bool callSetup = (nflag == PREV_CONV_FAIL) || (nflag == PREV_ERR_FAIL) || (uni(c.nst) == 0) ||
+                         (uni(c.nst) >= uni(c.nstlp) + MSBP) || (fabs(uni(c.gamrat) - 1.0) > DGMAX_LSETUP);
+        wave_sync();
+        bool jbad = false;
+        double delta[NS];
+        for (;;) {
+#pragma unroll
+            for (int s = 0; s < NS; s++) {
+                acor[s] = 0.0;
+                y[s] = zn[0][s];
+            }
+            if (f_eval(y, ftemp)) return 1;
+            double gamma = uni(c.gamma), rl1 = uni(c.rl1);
+#pragma unroll
+            for (int s = 0; s < NS; s++) delta[s] = -gamma * ftemp[s] + (rl1 * zn[1][s] + acor[s]);
+            if (callSetup) {
+                if (jbad && lane == 0) c.convfail = FAIL_BAD_J;
+                wave_sync();
+                if (lsetup()) return 1;
+                callSetup = false;
+            }
+            const double tq4 = uni(c.tq[4]);
+            int m_it = 0;
+            int status;
+            double crate = uni(c.crate), delp = uni(c.delp), gamrat = uni(c.gamrat);
+            for (;;) {
+                if constexpr (sizeof(MT) == 8) {
+#pragma unroll
+                    for (int s = 0; s < NS; s++) delta[s] = -delta[s];
+                    ls.solve(delta, N);
+                } else {
+                    float bf[NS];
+#pragma unroll
+                    for (int s = 0; s < NS; s++) bf[s] = static_cast<float>(-delta[s]);
+                    ls.solve(bf, N);
+#pragma unroll
+                    for (int s = 0; s < NS; s++) delta[s] = static_cast<double>(bf[s]);
+                }
+                if (gamrat != 1.0) {
+                    const double sc = 2.0 / (1.0 + gamrat);
+#pragma unroll
+                    for (int s = 0; s < NS; s++) delta[s] *= sc;
+                }
+#pragma unroll
+                for (int s = 0; s < NS; s++) acor[s] += delta[s];
+                const double del = wrms<NS>(delta, ewt, a.neq_norm);
+                if (m_it > 0) crate = fmax(CRDOWN * crate, del / delp);
+                const double dcon = del * fmin(1.0, crate) / tq4;
+                m_it++;
+                if (dcon <= 1.0) {
+                    const double acnrm = (m_it == 1) ? del : wrms<NS>(acor, ewt, a.neq_norm);
+                    if (lane == 0) c.acnrm = acnrm;
+                    status = 0;
+                    break;
+                }
+                if (m_it >= 2 && del > RDIV * delp) { status = 1; break; }
+                delp = del;
+                if (m_it >= MAXCOR) { status = 1; break; }
+#pragma unroll
+                for (int s = 0; s < NS; s++) y[s] = zn[0][s] + acor[s];
+                if (f_eval(y, ftemp)) { status = 1; break; }
+#pragma unroll
+                for (int s = 0; s < NS; s++) delta[s] = -gamma * ftemp[s] + (rl1 * zn[1][s] + acor[s]);
+            }
+            wave_sync();
+            if (lane == 0) {
+                c.nni += m_it;
+                c.crate = crate;
+                c.delp = delp;
+            }
+            wave_sync();
+            if (status == 0) {
+                if (lane == 0) c.jcur = 0;
+#pragma unroll
+                for (int s = 0; s < NS; s++) y[s] = zn[0][s] + acor[s];
+                wave_sync();
+                return 0;
+            }
+            if (!uni(c.jcur)) {
+                callSetup = true;
+                jbad = true;
+                continue;
+            }
+            return 1;
+        }
+    }
+
+    // 0 pass, 2 retry, -1 fail
+    __device__ __forceinline__ int do_error_test(int& nflag, double saved_t, int& nef, double& dsm) {
+        using namespace bdfc;
+        dsm = uni(c.acnrm) * uni(c.tq[2]);
+        if (dsm <= 1.0) return 0;
+        nef++;
+        nflag = PREV_ERR_FAIL;
+        wave_sync();
+        if (lane == 0) c.netf += 1;
+        restore(saved_t);
+        if (nef == MXNEF) return -1;
+        if (nef <= MXNEF1) {
+            double eta = 1.0 / (pow(BIAS2 * dsm, 1.0 / uni(c.L)) + ADDON);
+            eta = fmax(ETAMIN, eta);
+            if (nef >= SMALL_NEF) eta = fmin(eta, ETAMXF);
+            wave_sync();
+            if (lane == 0) {
+                c.etamax = 1.0;
+                c.eta = eta;
+            }
+            wave_sync();
+            rescale();
+            return 2;
+        }
+        if (uni(c.q) > 1) {
+            decrease_bdf();
+            const int q = uni(c.q);
+            wave_sync();
+            if (lane == 0) {
+                c.etamax = 1.0;
+                c.eta = ETAMIN;
+                c.L = q;
+                c.q = q - 1;
+                c.qwait = q;
+            }
+            wave_sync();
+            rescale();
+            return 2;
+        }
+        const double h = uni(c.h) * ETAMIN;
+        wave_sync();
+        if (lane == 0) {
+            c.etamax = 1.0;
+            c.eta = ETAMIN;
+            c.h = h;
+            c.hscale = h;
+            c.qwait = LONG_WAIT;
+            c.nscon = 0;
+        }
+        wave_sync();
+        if (f_eval(zn[0], tempv)) return -1;
+#pragma unroll
+        for (int s = 0; s < NS; s++) zn[1][s] = h * tempv[s];
+        return 2;
+    }
+
+    __device__ __forceinline__ void complete_step() {
+        using namespace bdfc;
+        const int q = uni(c.q);
+        const double h = uni(c.h);
+        double l[QMAX + 1];
+#pragma unroll
+        for (int j = 0; j <= QMAX; j++) l[j] = uni(c.l[j]);
+#pragma unroll
+        for (int j = 0; j <= QMAX; j++)
+            if (j <= q)
+#pragma unroll
+                for (int s = 0; s < NS; s++) zn[j][s] += l[j] * acor[s];
+        const int nst = uni(c.nst) + 1;
+        const int qwait = uni(c.qwait) - 1;
+        double tau[QMAX + 2];
+#pragma unroll
+        for (int i = 0; i < QMAX + 2; i++) tau[i] = uni(c.tau[i]);
+#pragma unroll
+        for (int i = QMAX; i >= 2; i--)
+            if (i <= q) tau[i] = tau[i - 1];
+        if (q == 1 && nst > 1) tau[2] = tau[1];
+        tau[1] = h;
+        if (qwait == 1 && q != QMAX) {
+#pragma unroll
+            for (int s = 0; s < NS; s++) zn[QMAX][s] = acor[s];
+        }
+        const double tq5 = uni(c.tq[5]);
+        wave_sync();
+        if (lane == 0) {
+            c.nst = nst;
+            c.nscon += 1;
+            c.hu = h;
+#pragma unroll
+            for (int i = 0; i < QMAX + 2; i++) c.tau[i] = tau[i];
+            c.qwait = qwait;
+            if (qwait == 1 && q != QMAX) c.saved_tq5 = tq5;
+        }
+        wave_sync();
+    }
+
+    __device__ __forceinline__ void set_eta(double eta, int qprime) {
+        using namespace bdfc;
+        double hprime;
+        const double h = uni(c.h);
+        if (eta < THRESH) {
+            eta = 1.0;
+            hprime = h;
+        } else {
+            eta = fmin(eta, uni(c.etamax));
+            hprime = h * eta;
+        }
+        wave_sync();
+        if (lane == 0) {
+            c.eta = eta;
+            c.hprime = hprime;
+            c.qprime = qprime;
+        }
+        wave_sync();
+    }
+
+    __device__ __forceinline__ void prepare_next_step(double dsm) {
+        using namespace bdfc;
+        const int q = uni(c.q), L = uni(c.L);
+        if (uni(c.etamax) == 1.0) {
+            const int qwait = uni(c.qwait);
+            wave_sync();
+            if (lane == 0) {
+                c.qwait = qwait > 2 ? qwait : 2;
+                c.qprime = q;
+                c.hprime = uni(c.h);
+                c.eta = 1.0;
+            }
+            wave_sync();
+            return;
+        }
+        const double etaq = 1.0 / (pow(BIAS2 * dsm, 1.0 / L) + ADDON);
+        if (uni(c.qwait) != 0) {
+            set_eta(etaq, q);
+            return;
+        }
+        double etaqm1 = 0.0, etaqp1 = 0.0;
+        if (q > 1) {
+            double zq[NS];
+            pick(q, zq);
+            const double ddn = wrms<NS>(zq, ewt, a.neq_norm) * uni(c.tq[1]);
+            etaqm1 = 1.0 / (pow(BIAS1 * ddn, 1.0 / q) + ADDON);
+        }
+        const double saved_tq5 = uni(c.saved_tq5);
+        if (q != QMAX && saved_tq5 != 0.0) {
+            const double cquot = (uni(c.tq[5]) / saved_tq5) * pow(uni(c.h) / uni(c.tau[2]), static_cast<double>(L));
+#pragma unroll
+            for (int s = 0; s < NS; s++) tempv[s] = -cquot * zn[QMAX][s] + acor[s];
+            const double dup = wrms<NS>(tempv, ewt, a.neq_norm) * uni(c.tq[3]);
+            etaqp1 = 1.0 / (pow(BIAS3 * dup, 1.0 / (L + 1)) + ADDON);
+        }
+        wave_sync();
+        if (lane == 0) c.qwait = 2;
+        wave_sync();
+        const double etam = fmax(etaqm1, fmax(etaq, etaqp1));
+        double eta;
+        int qprime;
+        if (etam < THRESH) {
+            eta = 1.0;
+            qprime = q;
+        } else if (etam == etaq) {
+            eta = etaq;
+            qprime = q;
+        } else if (etam == etaqm1) {
+            eta = etaqm1;
+            qprime = q - 1;
+        } else {
+            eta = etaqp1;
+            qprime = q + 1;
+#pragma unroll
+            for (int s = 0; s < NS; s++) zn[QMAX][s] = acor[s];
+        }
+        set_eta(eta, qprime);
+    }
+
+    __device__ __forceinline__ int bdf_step() {
+        using namespace bdfc;
+        const double saved_t = uni(c.tn);
+        int ncf = 0, nef = 0, nflag = FIRST_CALL;
+        double dsm = 0.0;
+        if (uni(c.nst) > 0 && uni(c.hprime) != uni(c.h)) {
+            const int q = uni(c.q), qprime = uni(c.qprime);
+            if (qprime != q) {
+                if (qprime == q + 1) increase_bdf();
+                else if (qprime == q - 1) decrease_bdf();
+                wave_sync();
+                if (lane == 0) {
+                    c.q = qprime;
+                    c.L = qprime + 1;
+                    c.qwait = qprime + 1;
+                }
+                wave_sync();
+            }
+            rescale();
+        }
+        for (;;) {
+            predict();
+            set_bdf();
+            {
+                const double rl1 = 1.0 / uni(c.l[1]);
+                const double gamma = uni(c.h) * rl1;
+                const int nst = uni(c.nst);
+                const double gammap = (nst == 0) ? gamma : uni(c.gammap);
+                const double gamrat = (nst > 0) ? gamma / gammap : 1.0;
+                wave_sync();
+                if (lane == 0) {
+                    c.rl1 = rl1;
+                    c.gamma = gamma;
+                    c.gammap = gammap;
+                    c.gamrat = gamrat;
+                }
+                wave_sync();
+            }
+            const int ret = nls(nflag);
+            if (ret != 0) {
+                wave_sync();
+                if (lane == 0) c.ncfn += 1;
+                restore(saved_t);
+                ncf++;
+                if (ncf == MXNCF) return -3;
+                wave_sync();
+                if (lane == 0) {
+                    c.etamax = 1.0;
+                    c.eta = ETACF;
+                }
+                wave_sync();
+                nflag = PREV_CONV_FAIL;
+                rescale();
+                continue;
+            }
+            const int e = do_error_test(nflag, saved_t, nef, dsm);
+            if (e == 2) continue;
+            if (e < 0) return -4;
+            break;
+        }
+        complete_step();
+        prepare_next_step(dsm);
+        const double etamax = (uni(c.nst) <= SMALL_NST) ? ETAMX2 : ETAMX3;
+        wave_sync();
+        if (lane == 0) c.etamax = etamax;
+        wave_sync();
+        return 0;
+    }
+
+    __device__ __forceinline__ int initial_h(double tout) {
+        using namespace bdfc;
+        const double tn = uni(c.tn);
+        const double tdist = fabs(tout - tn);
+        const double tround = UROUND * fmax(fabs(tn), fabs(tout));
+        if (tdist < 2.0 * tround) return -1;
+        const double hlb = HLB_FACTOR * tround;
+        double hub_inv = 0.0;
+#pragma unroll
+        for (int s = 0; s < NS; s++)
+            if (comp(s) < N) {
+                const double t1 = HUB_FACTOR * fabs(zn[0][s]) + (rtol * fabs(zn[0][s]) + atol);
+                hub_inv = fmax(hub_inv, fabs(zn[1][s]) / t1);
+            }
+        hub_inv = uni(wave_max(hub_inv));
+        double hub = HUB_FACTOR * tdist;
+        if (hub * hub_inv > 1.0) hub = 1.0 / hub_inv;
+        double hg = sqrt(hlb * hub);
+        double h0;
+        if (hub < hlb) {
+            h0 = hg;
+        } else {
+            double hnew = hg;
+            double yy[NS], ft[NS];
+            for (int c1 = 1; c1 <= HIN_ITERS; c1++) {
+                double yddnrm = 0.0;
+                bool ok = false;
+                for (int c2 = 1; c2 <= HIN_ITERS; c2++) {
+#pragma unroll
+                    for (int s = 0; s < NS; s++) yy[s] = hg * zn[1][s] + zn[0][s];
+                    if (f_eval(yy, ft) == 0) {
+#pragma unroll
+                        for (int s = 0; s < NS; s++) ft[s] = (ft[s] - zn[1][s]) * (1.0 / hg);
+                        yddnrm = wrms<NS>(ft, ewt, a.neq_norm);
+                        ok = true;
+                        break;
+                    }
+                    hg *= 0.2;
+                }
+                if (!ok) return -2;
+                hnew = (yddnrm * hub * hub > 2.0) ? sqrt(2.0 / yddnrm) : sqrt(hg * hub);
+                if (c1 == HIN_ITERS) break;
+                const double hrat = hnew / hg;
+                if (hrat > 0.5 && hrat < 2.0) break;
+                if (c1 > 1 && hrat > 2.0) {
+                    hnew = hg;
+                    break;
+                }
+                hg = hnew;
+            }
+            h0 = H_BIAS * hnew;
+            if (h0 < hlb) h0 = hlb;
+            if (h0 > hub) h0 = hub;
+        }
+        wave_sync();
+        if (lane == 0) c.h = h0;
+        wave_sync();
+        return 0;
+    }
+
+    __device__ __forceinline__ int start(double tout) {
+        ewt_set(zn[0]);
+        if (f_eval(zn[0], zn[1])) return -5;
+        if (initial_h(tout)) return -6;
+        const double h = uni(c.h);
+#pragma unroll
+        for (int s = 0; s < NS; s++) zn[1][s] *= h;
+        wave_sync();
+        if (lane == 0) {
+            c.hscale = h;
+            c.hprime = h;
+            c.started = 1;
+        }
+        wave_sync();
+        return 0;
+    }
+
+    __device__ __forceinline__ int one_step() {
+        if (uni(c.nst) > 0) ewt_set(zn[0]);
+        const double nrm = wrms<NS>(zn[0], ewt, a.neq_norm);
+        if (bdfc::UROUND * nrm > 1.0) return -10;
+        return bdf_step();
+    }
+
+    // advance to tout; writes the interpolated state to out
+    __device__ __forceinline__ int advance(double tout, double* out, int max_steps) {
+        const double t_user = uni(c.t_user);
+        if (tout == t_user) return 0;
+        if (tout < t_user) return -20;
+        if (!uni(c.started)) {
+            const int r = start(tout);
+            if (r) return r;
+        }
+        int nsteps = 0;
+        while (uni(c.tn) < tout) {
+            if (nsteps >= max_steps) return -22;
+            const int ret = one_step();
+            if (ret) return ret;
+            nsteps++;
+        }
+        const double sarg = (tout - uni(c.tn)) / uni(c.h);
+        const int q = uni(c.q);
+        double v[NS];
+        pick(q, v);
+#pragma unroll
+        for (int j = bdfc::QMAX - 1; j >= 0; j--)
+            if (j < q)
+#pragma unroll
+                for (int s = 0; s < NS; s++) v[s] = zn[j][s] + sarg * v[s];
+#pragma unroll
+        for (int s = 0; s < NS; s++)
+            if (comp(s) < N) out[comp(s)] = v[s];
+        wave_sync();
+        if (lane == 0) c.t_user = tout;
+        wave_sync();
+        return 0;
+    }
+};
+
+// LDS per wavefront: state broadcast, species/reaction scratch, control block,
+// then the linear-system area (LDS policy only).
+__host__ __device__ inline size_t bdf_scratch_doubles(int K, int I, int nsub, int ns) {
+    return static_cast<size_t>(kWave) * ns + K + K + I + nsub + 2 * K + kCtlDoubles;
+}
+
+template <int NS, class LS, typename MT>
+__device__ __forceinline__ void bdf_advance_body(const BdfArgs& a) {
+    extern __shared__ double lds_raw[];
+    const DevMech& m = a.mech;
+    double* base = lds_raw;
+    WaveScratch S;
+    S.yb = base;  base += kWave * NS;
+    S.C = base;   base += m.K;
+    S.MU = base;  base += m.K;
+    S.ROP = base; base += m.I;
+    S.SUB = base; base += m.nsub;
+    S.W2 = base;  base += 2 * m.K;
+    Ctl& c = *reinterpret_cast<Ctl*>(base);
+    base += kCtlDoubles;
+    Integrator<NS, LS, MT> it(m, a, c);
+    it.S = S;
+    it.lane = lane_id();
+    it.N = m.N;
+    it.rtol = a.rtol;
+    it.atol = a.atol;
+    it.ls.init(base);
+    for (;;) {
+        int r = 0;
+        if (it.lane == 0) r = atomicAdd(a.queue, 1);
+        r = uni(__shfl(r, 0, kWave));
+        if (r >= a.n) break;
+        const int rid = a.order ? uni(a.order[r]) : r;
+        double* rec = a.rec + static_cast<size_t>(rid) * a.rec_stride;
+        const double status = uni(rec[7 * kWave * NS + C_STATUS]);
+        if (status < 0.0) continue;   // failed earlier: leave untouched
+        it.load(static_cast<size_t>(rid));
+        const double tout = uni(a.tout[rid]);
+        const int ret = it.advance(tout, rec + 6 * kWave * NS, a.max_steps);
+        it.store(ret < 0 ? ret : 0);
+        wave_sync();
+    }
+}
+
+}  // namespace cg

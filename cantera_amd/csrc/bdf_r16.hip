@@ -1,0 +1,22 @@
+// Integrator kernels with the iteration matrix in registers, N <= 16.
+#include "bdf_integrator.hpp"
+
+namespace cg {
+
+template <typename MT>
+__global__ __launch_bounds__(64) void bdf_advance_r16(BdfArgs a) {
+    bdf_advance_body<1, RegLinSys<16, MT>, MT>(a);
+}
+
+const void* bdf_kernel_r16(bool fp32) {
+    return fp32 ? reinterpret_cast<const void*>(&bdf_advance_r16<float>)
+                : reinterpret_cast<const void*>(&bdf_advance_r16<double>);
+}
+
+hipError_t bdf_launch_r16(const BdfArgs& a, bool fp32, int grid, size_t lds, hipStream_t stream) {
+    if (fp32) hipLaunchKernelGGL(bdf_advance_r16<float>, dim3(grid), dim3(64), lds, stream, a);
+    else hipLaunchKernelGGL(bdf_advance_r16<double>, dim3(grid), dim3(64), lds, stream, a);
+    return hipGetLastError();
+}
+
+}  // namespace cg

@@ -1,6 +1,6 @@
 // Analytic reactor Jacobian d(ydot)/dy for one reactor, assembled by one
-// wavefront into an LDS matrix (row-major, leading dimension LD, internal
-// state order [Y_0..Y_{K-1}, T, m]).
+// wavefront into an FP64 matrix in global memory (row-major, internal state
+// order [Y_0..Y_{K-1}, T, m]).
 //
 // Y columns: exact chain rule through C_j = rho Y_j / W_j at fixed T (so the
 // molar density P/RT is fixed), with dwdot/dC from mass action, third-body and
@@ -10,8 +10,7 @@
 // T column: one-sided difference with increment sqrt(eps) T, as the reference's
 // mole-reactor Jacobian does for temperature
 // (src/zeroD/IdealGasConstPressureMoleReactor.cpp:126-152).
-// Per-reaction contributions are scattered with LDS atomics; a single wavefront
-// owns the matrix, so the accumulation order is fixed.
+// Per-reaction contributions are scattered with FP64 global atomics (L2).
 #pragma once
 #include "wave_kinetics.hpp"
 
@@ -19,17 +18,13 @@ namespace cg {
 
 constexpr double kSqrtEps = 1.4901161193847656e-08;  // sqrt(DBL_EPSILON)
 
-template <typename T>
-__device__ __forceinline__ void lds_add(T* p, T v) {
-    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-}
+__device__ __forceinline__ void mat_add(double* p, double v) { unsafeAtomicAdd(p, v); }
 
 // d ropnet / d C_j contributions of reaction p, scattered into JC (K x K block
 // of A).  All per-reaction lists use static indices (registers only).
-template <typename T>
 __device__ __forceinline__ void scatter_reaction_derivs(const DevMech& m, int p, const GasState& g,
                                                         const double* __restrict__ C, const double* __restrict__ MU,
-                                                        T* A, int LD) {
+                                                        double* A, int LD) {
     RxnRegs x;
     load_rxn(m, p, x);
     RxnVals v;
@@ -85,7 +80,7 @@ __device__ __forceinline__ void scatter_reaction_derivs(const DevMech& m, int p,
     auto scatter = [&](int j, double d) {
 #pragma unroll
         for (int r = 0; r < E; r++)
-            if (lead[r]) lds_add(&A[ek[r] * LD + j], static_cast<T>(rnu[r] * d));
+            if (lead[r]) mat_add(&A[ek[r] * LD + j], rnu[r] * d);
     };
 #pragma unroll
     for (int a = 0; a < kMaxSide; a++) {
@@ -141,12 +136,13 @@ __device__ __forceinline__ void scatter_reaction_derivs(const DevMech& m, int p,
     }
 }
 
-// Build J = d(ydot)/dy into A.  y/f0: state and RHS at the expansion point
-// (lane-distributed).  Uses S for the RHS evaluations; on return the species
-// scratch holds the base state.
-template <int NS, typename T>
+// Build J = d(ydot)/dy into the FP64 matrix J (global memory, N x N row-major).
+// y/f0: state and RHS at the expansion point (lane-distributed).  The
+// per-reaction derivatives are accumulated with FP64 global atomics, so their
+// summation order is not fixed; J only enters the Newton iteration matrix.
+template <int NS>
 __device__ void reactor_jacobian(const DevMech& m, const WaveScratch& S, double P, const double (&y)[NS],
-                                 const double (&f0)[NS], T* A, int LD) {
+                                 const double (&f0)[NS], double* __restrict__ J) {
     const int lane = lane_id();
     const int K = m.K, N = m.N;
     GasState g;
@@ -156,8 +152,7 @@ __device__ void reactor_jacobian(const DevMech& m, const WaveScratch& S, double 
     double y1[NS], f1[NS], colT[NS];
 #pragma unroll
     for (int s = 0; s < NS; s++) y1[s] = y[s];
-    // find T (component K) and perturb it
-    double Tval = bcast<NS>(y, K);
+    const double Tval = bcast<NS>(y, K);
     const double dT = kSqrtEps * Tval;
 #pragma unroll
     for (int s = 0; s < NS; s++)
@@ -167,16 +162,17 @@ __device__ void reactor_jacobian(const DevMech& m, const WaveScratch& S, double 
     for (int s = 0; s < NS; s++) colT[s] = (f1[s] - f0[s]) / dT;
     // base state (species scratch, wdot)
     reactor_rhs<NS>(m, S, P, y, f1, g, sr, rs);
-    // zero the matrix
 #pragma unroll
     for (int s = 0; s < NS; s++) {
         int j = lane + kWave * s;
         if (j < N)
-            for (int i = 0; i < N; i++) A[i * LD + j] = T(0);
+            for (int i = 0; i < N; i++) J[i * N + j] = 0.0;
     }
-    __syncthreads();
-    for (int p = lane; p < m.I; p += kWave) scatter_reaction_derivs<T>(m, p, g, S.C, S.MU, A, LD);
-    __syncthreads();
+    __threadfence();
+    wave_sync();
+    for (int p = lane; p < m.I; p += kWave) scatter_reaction_derivs(m, p, g, S.C, S.MU, J, N);
+    __threadfence();
+    wave_sync();
     // D_k = sum_l JC[k][l] C_l ; keep wdot and D in W2, h_k in MU
     double hD = 0.0, H = 0.0;
 #pragma unroll
@@ -184,7 +180,7 @@ __device__ void reactor_jacobian(const DevMech& m, const WaveScratch& S, double 
         int k = lane + kWave * s;
         if (k < K) {
             double d = 0.0;
-            for (int l = 0; l < K; l++) d += static_cast<double>(A[k * LD + l]) * S.C[l];
+            for (int l = 0; l < K; l++) d += J[k * N + l] * S.C[l];
             const double hk = sr.hrt[s] * g.RT;
             S.W2[k] = sr.wdot[s];
             S.W2[K + k] = d;
@@ -194,41 +190,40 @@ __device__ void reactor_jacobian(const DevMech& m, const WaveScratch& S, double 
     }
     hD = uni(wave_sum(hD));
     H = uni(wave_sum(H));
-    __syncthreads();
+    wave_sync();
 #pragma unroll
     for (int s = 0; s < NS; s++) {
         int k = lane + kWave * s;
         if (k < K) S.MU[k] = sr.hrt[s] * g.RT;
     }
-    __syncthreads();
+    wave_sync();
     const double rho = g.dens, Ct = g.ctot, Wbar = g.mmw, cp = rs.cp_mass;
 #pragma unroll
     for (int s = 0; s < NS; s++) {
         int j = lane + kWave * s;
         if (j < K) {
             double hJ = 0.0;
-            for (int k = 0; k < K; k++) hJ += S.MU[k] * static_cast<double>(A[k * LD + j]);
+            for (int k = 0; k < K; k++) hJ += S.MU[k] * J[k * N + j];
             const double wj = m.mw[j];
             for (int k = 0; k < K; k++) {
-                const double jc = static_cast<double>(A[k * LD + j]);
-                A[k * LD + j] = static_cast<T>(m.mw[k] / wj * (jc + (S.W2[k] - S.W2[K + k]) / Ct));
+                const double jc = J[k * N + j];
+                J[k * N + j] = m.mw[k] / wj * (jc + (S.W2[k] - S.W2[K + k]) / Ct);
             }
-            // pick cp_R of species j from its owner lane
             const double cpj = kGasConstant * sr.cpr[s] / wj;
-            A[K * LD + j] = static_cast<T>(-1.0 / (rho * cp) * ((rho * hJ - Wbar * hD + Wbar * H) / wj - H * cpj / cp));
-            A[(K + 1) * LD + j] = T(0);
+            J[K * N + j] = -1.0 / (rho * cp) * ((rho * hJ - Wbar * hD + Wbar * H) / wj - H * cpj / cp);
+            J[(K + 1) * N + j] = 0.0;
         }
     }
-    // T and m columns
 #pragma unroll
     for (int s = 0; s < NS; s++) {
         int i = lane + kWave * s;
         if (i < N) {
-            A[i * LD + K] = (i == K + 1) ? T(0) : static_cast<T>(colT[s]);
-            A[i * LD + K + 1] = T(0);
+            J[i * N + K] = (i == K + 1) ? 0.0 : colT[s];
+            J[i * N + K + 1] = 0.0;
         }
     }
-    __syncthreads();
+    __threadfence();
+    wave_sync();
 }
 
 }  // namespace cg

@@ -48,6 +48,12 @@ __device__ __forceinline__ double wave_max(double v) {
     return v;
 }
 
+// Order LDS traffic between the lanes of one wavefront.  Every kernel here
+// runs one wavefront per workgroup, so the workgroup barrier is a cheap,
+// compiler-visible ordering point (a wavefront-scope fence alone does not stop
+// plain LDS loads from moving above another lane's stores).
+__device__ __forceinline__ void wave_sync() { __syncthreads(); }
+
 // make a value that all lanes hold identically explicitly wave-uniform
 __device__ __forceinline__ double uni(double x) {
     long long v = __double_as_longlong(x);
@@ -56,6 +62,7 @@ __device__ __forceinline__ double uni(double x) {
     return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
 }
 __device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ float uni(float x) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x))); }
 
 // broadcast element k of a lane-distributed vector (k wave-uniform)
 template <int NS, typename T>
@@ -322,7 +329,7 @@ template <int NS>
 __device__ __forceinline__ void production_rates(const DevMech& m, const WaveScratch& S, const GasState& g,
                                                  SpeciesRegs<NS>& sr) {
     const int lane = lane_id();
-    __syncthreads();
+    wave_sync();
     for (int p = lane; p < m.I; p += kWave) {
         RxnVals v;
         RxnRegs x;
@@ -330,14 +337,14 @@ __device__ __forceinline__ void production_rates(const DevMech& m, const WaveScr
         eval_reaction(m, p, g, S.C, S.MU, x, v);
         S.ROP[p] = v.ropf - v.ropr;
     }
-    __syncthreads();
+    wave_sync();
     for (int sb = lane; sb < m.nsub; sb += kWave) {
         double acc = 0.0;
         const int e1 = m.sub_start[sb + 1];
         for (int e = m.sub_start[sb]; e < e1; e++) acc += m.ent_nu[e] * S.ROP[m.ent_r[e]];
         S.SUB[sb] = acc;
     }
-    __syncthreads();
+    wave_sync();
 #pragma unroll
     for (int s = 0; s < NS; s++) {
         int k = lane + kWave * s;
@@ -366,7 +373,7 @@ __device__ __forceinline__ void reactor_rhs(const DevMech& m, const WaveScratch&
         int i = lane + kWave * s;
         if (i < m.N) S.yb[i] = y[s];
     }
-    __syncthreads();
+    wave_sync();
     const double T = uni(S.yb[m.K]);
     const double mass = uni(S.yb[m.K + 1]);
     species_pass<NS>(m, S, T, P, y, g, sr);

@@ -18,7 +18,7 @@ __device__ bool lu_factor(T* A, int LD, int N, int* pmap) {
         int i = lane + kWave * s;
         if (i < N) pmap[i] = i;
     }
-    __syncthreads();
+    wave_sync();
     for (int k = 0; k < N; k++) {
         double best = -1.0;
         int bi = N;
@@ -37,7 +37,8 @@ __device__ bool lu_factor(T* A, int LD, int N, int* pmap) {
             if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
         }
         const int p = uni(bi);
-        if (p >= N || A[p * LD + k] == T(0)) return false;
+        if (p >= N) return false;
+        if (uni(A[p * LD + k]) == T(0)) return false;
         if (p != k) {
 #pragma unroll
             for (int s = 0; s < NS; s++) {
@@ -54,14 +55,14 @@ __device__ bool lu_factor(T* A, int LD, int N, int* pmap) {
                 pmap[p] = t;
             }
         }
-        __syncthreads();
+        wave_sync();
         const T mult = T(1) / A[k * LD + k];
 #pragma unroll
         for (int s = 0; s < NS; s++) {
             int i = lane + kWave * s;
             if (i > k && i < N) A[i * LD + k] *= mult;
         }
-        __syncthreads();
+        wave_sync();
 #pragma unroll
         for (int s = 0; s < NS; s++) {
             int j = lane + kWave * s;
@@ -72,7 +73,7 @@ __device__ bool lu_factor(T* A, int LD, int N, int* pmap) {
                 }
             }
         }
-        __syncthreads();
+        wave_sync();
     }
     return true;
 }
@@ -87,13 +88,13 @@ __device__ void lu_solve(const T* A, int LD, int N, const int* pmap, V* bv, V (&
         int i = lane + kWave * s;
         if (i < N) bv[i] = b[s];
     }
-    __syncthreads();
+    wave_sync();
 #pragma unroll
     for (int s = 0; s < NS; s++) {
         int i = lane + kWave * s;
         b[s] = (i < N) ? bv[pmap[i]] : V(0);
     }
-    __syncthreads();
+    wave_sync();
     // forward substitution with unit lower factor
     for (int k = 0; k < N - 1; k++) {
         const V bk = bcast<NS>(b, k);

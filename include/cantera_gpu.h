@@ -108,6 +108,12 @@ int32_t cg_reactors_get_state(int32_t batch, double* y);
  * evaluations for Jacobians (CVodesIntegrator::solverStats,
  * src/numerics/CVodesIntegrator.cpp:572-621). */
 int32_t cg_reactors_stats(int32_t batch, int64_t* stats);
+/* Integrator control block per reactor, n x 64 doubles: [0] internal time
+ * t_n, [1] current step h, [2] next step, [28] order q, [32] steps, [37] user
+ * time, [38..43] rhs/jacobian/LU/Newton/failure counters, [44] status, [45]
+ * pressure (CVodesIntegrator::getCurrentStepSize / getLastOrder and the
+ * solverStats fields, src/numerics/CVodesIntegrator.cpp:572-621). */
+int32_t cg_reactors_get_control(int32_t batch, double* ctl);
 /* Status per reactor: 0 ok, negative = integration failure code. */
 int32_t cg_reactors_status(int32_t batch, int32_t* status);
 /* Options: max internal steps per advance call (default 20000,
