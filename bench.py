@@ -88,6 +88,31 @@ def flop_model(mech_path, stats):
     return float(W), F_rhs
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "bdf_advance_gri30_65536_traffic.md")
+
+
+def measured_traffic(n):
+    """HBM bytes per integrator launch from the committed rocprofv3 --pmc summary.
+
+    The summary holds FETCH_SIZE and WRITE_SIZE (KiB per dispatch) of the
+    integrator kernel and of a plain copy kernel of known size; each counter is
+    scaled by (copy bytes / copy counter).  None when the summary is absent or
+    was taken at another batch size.
+    """
+    try:
+        kv = {}
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                if line.startswith("traffic_") and ":" in line:
+                    k, v = line.split(":", 1)
+                    kv[k.strip()] = float(v.split()[0])
+        if int(kv["traffic_n_reactors"]) != n:
+            return None
+        return kv["traffic_bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def cpu_baseline(wl, steps, sample, threads):
     from oracle.oracle import Mech
     om = Mech(os.path.join(ROOT, "cantera_amd", "mechanisms", "gri30.mech"))
@@ -109,8 +134,12 @@ def main(argv=None):
     ap.add_argument("--n", type=int, default=65536)
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
     ap.add_argument("--waves-per-cu", type=int, default=0)
-    ap.add_argument("--cpu-sample", type=int, default=2048)
+    ap.add_argument("--schedule", choices=["longest-first", "index"], default="longest-first")
+    ap.add_argument("--cpu-sample", type=int, default=16384)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
+                    help="after the timed region, run one device-to-device copy (blit kernel) of MIB MiB "
+                         "(known HBM bytes, to scale FETCH_SIZE/WRITE_SIZE counters)")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (the reactors are independent, "
@@ -142,6 +171,7 @@ def main(argv=None):
 
     rb = ca.ReactorBatch(sol, wl.T0, wl.P, wl.Y0, rtol=wl.rtol, atol=wl.atol, precision=prec)
     rb.set_options(0, a.waves_per_cu)
+    rb.set_schedule(a.schedule == "longest-first")
     check(L.cg_device_synchronize())
     kernel_ms = []
     t0 = time.perf_counter()
@@ -180,14 +210,16 @@ def main(argv=None):
                         f"T0 U[900,2000] K, CH4/air phi U[0.5,3], P 1-40 atm, rtol 1e-8 atol 1e-15",
             "precision": a.precision,
             "parallelism": "1 GPU, one wavefront per reactor, persistent work-queue kernel",
+            "schedule": a.schedule,
         },
         "roofline": {
-            "bound": "fp64",
+            "bound": "mfma",
+            "pipe": "FP64 (MI355X vector FMA and MFMA share the 78.6 TFLOP/s FP64 peak)",
             "achieved": achieved,
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / FP64_PEAK_TFLOPS,
-            "traffic": None,
+            "traffic": measured_traffic(n),
             "kernel": "bdf_advance_kernel",
             "avg_launch_ms": avg_launch_s * 1e3,
             "flops_per_launch": W / a.steps,
@@ -199,6 +231,13 @@ def main(argv=None):
         threads = max(1, min(threads, 16))
         out["cpu_baseline"] = cpu_baseline(wl, a.steps, a.cpu_sample, threads)
     L.cg_dev_free(d_tout)
+    if a.copy_calib > 0:
+        nb = a.copy_calib << 20
+        src, dst = L.cg_dev_alloc(nb), L.cg_dev_alloc(nb)
+        check(L.cg_memcpy_d2d(dst, src, nb))
+        check(L.cg_device_synchronize())
+        L.cg_dev_free(src)
+        L.cg_dev_free(dst)
     print(json.dumps(out), flush=True)
     return 0
 

@@ -16,6 +16,8 @@ from __future__ import annotations
 
 import os
 
+import ctypes as C
+
 import numpy as np
 
 from . import _lib
@@ -96,6 +98,19 @@ class Solution:
     def last_rates_kernel_ms(self) -> float:
         return float(lib().cg_rates_last_kernel_ms(self._h))
 
+    def reactor_jacobians(self, T, P, Y) -> np.ndarray:
+        """d(ydot)/dy of a constant-pressure reactor at n states, shape (n, K+2, K+2),
+        state order [m, T, Y_0..Y_{K-1}] (as ``Reactor.jacobian`` orders it).
+        Y is species-major (K, n)."""
+        T = np.ascontiguousarray(np.atleast_1d(T), dtype=np.float64)
+        n = T.shape[0]
+        P = np.ascontiguousarray(np.broadcast_to(np.asarray(P, dtype=np.float64), (n,)))
+        Y = np.ascontiguousarray(np.asarray(Y, dtype=np.float64).reshape(self.n_species, n))
+        N = self.n_species + 2
+        J = np.zeros((n, N, N))
+        check(lib().cg_jac_batch(self._h, n, T, P, Y, J, CG_FP64), "cg_jac_batch")
+        return J
+
 
 class ReactorBatch:
     """n independent constant-pressure ideal-gas reactors, each with its own
@@ -125,6 +140,20 @@ class ReactorBatch:
 
     def set_options(self, max_steps=0, waves_per_cu=-1):
         check(lib().cg_reactors_set_options(self._h, int(max_steps), int(waves_per_cu)))
+
+    def advance_surrogate(self, t, surrogate, dT_tol=0.05, rtol_g=1e-3) -> np.ndarray:
+        """advance() with guarded surrogate steps where t is a whole number of
+        surrogate steps away; returns [surrogate steps kept, reactors advanced
+        by the surrogate alone, reactors that ran the integrator]."""
+        t_out = np.ascontiguousarray(np.broadcast_to(np.asarray(t, dtype=np.float64), (self.n,)))
+        counts = (C.c_int64 * 3)()
+        check(lib().cg_reactors_advance_surrogate(self._h, surrogate.handle, t_out, float(dT_tol), float(rtol_g),
+                                                  counts), "cg_reactors_advance_surrogate")
+        self.time = t_out.copy()
+        return np.array(counts[:], dtype=np.int64)
+
+    def set_schedule(self, longest_first=True):
+        check(lib().cg_reactors_set_schedule(self._h, 1 if longest_first else 0))
 
     def advance(self, t):
         t = np.ascontiguousarray(np.broadcast_to(np.asarray(t, dtype=np.float64), (self.n,)))

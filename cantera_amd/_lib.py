@@ -42,6 +42,10 @@ SIGNATURES = [
     ("cg_rates_batch", C.c_int32, [C.c_int32, C.c_int64, _dp, _dp, _dp, _dp, C.c_int32]),
     ("cg_rates_batch_device", C.c_int32, [C.c_int32, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p,
                                           C.c_void_p, C.c_int32, C.c_void_p]),
+    ("cg_jac_batch", C.c_int32, [C.c_int32, C.c_int64, _dp, _dp, _dp, _dp, C.c_int32]),
+    ("cg_jac_batch_device", C.c_int32, [C.c_int32, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p,
+                                        C.c_void_p, C.c_int32, C.c_void_p]),
+    ("cg_jac_last_kernel_ms", C.c_double, [C.c_int32]),
     ("cg_reactors_create", C.c_int32, [C.c_int32, C.c_int64, _dp, _dp, _dp, C.c_double, C.c_double, C.c_int32]),
     ("cg_reactors_free", C.c_int32, [C.c_int32]),
     ("cg_reactors_advance", C.c_int32, [C.c_int32, _dp]),
@@ -51,8 +55,25 @@ SIGNATURES = [
     ("cg_reactors_status", C.c_int32, [C.c_int32, _i32p]),
     ("cg_reactors_get_control", C.c_int32, [C.c_int32, _dp]),
     ("cg_reactors_set_options", C.c_int32, [C.c_int32, C.c_int32, C.c_int32]),
+    ("cg_reactors_set_schedule", C.c_int32, [C.c_int32, C.c_int32]),
     ("cg_reactors_last_kernel_ms", C.c_double, [C.c_int32]),
     ("cg_rates_last_kernel_ms", C.c_double, [C.c_int32]),
+    ("cg_memcpy_d2d", C.c_int32, [C.c_void_p, C.c_void_p, C.c_int64]),
+    ("cg_surrogate_create", C.c_int32, [C.c_int32, C.c_double, C.c_int32, C.c_int32, C.c_uint64]),
+    ("cg_surrogate_free", C.c_int32, [C.c_int32]),
+    ("cg_surrogate_nparams", C.c_int64, [C.c_int32]),
+    ("cg_surrogate_fit_normalization", C.c_int32, [C.c_int32, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("cg_surrogate_train_epoch", C.c_int32, [C.c_int32, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32,
+                                             C.c_double, C.POINTER(C.c_double)]),
+    ("cg_surrogate_loss_grad", C.c_int32, [C.c_int32, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p,
+                                           C.POINTER(C.c_double), C.POINTER(C.c_float)]),
+    ("cg_surrogate_predict", C.c_int32, [C.c_int32, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("cg_surrogate_get_params", C.c_int32, [C.c_int32, C.POINTER(C.c_float)]),
+    ("cg_surrogate_set_params", C.c_int32, [C.c_int32, C.POINTER(C.c_float)]),
+    ("cg_surrogate_get_normalization", C.c_int32, [C.c_int32, C.POINTER(C.c_double)]),
+    ("cg_surrogate_set_normalization", C.c_int32, [C.c_int32, C.POINTER(C.c_double)]),
+    ("cg_reactors_advance_surrogate", C.c_int32, [C.c_int32, C.c_int32, _dp, C.c_double, C.c_double,
+                                                  C.POINTER(C.c_int64)]),
 ]
 
 

@@ -16,8 +16,8 @@ OUT = os.path.join(HERE, "libcantera_amd.so")
 OBJDIR = os.path.join(HERE, "_build")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["mech.cpp", "api.cpp", "rates.hip", "bdf.hip", "bdf_r16.hip", "bdf_r32.hip", "bdf_r56.hip",
-           "bdf_r64.hip", "bdf_lds.hip"]
+SOURCES = ["mech.cpp", "api.cpp", "rates.hip", "jacobian.hip", "order.hip", "bdf.hip", "bdf_r16.hip", "bdf_r32.hip", "bdf_l1.hip",
+           "bdf_l2.hip", "surrogate.hip", "surrogate_reactors.hip"]
 
 COMMON = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",

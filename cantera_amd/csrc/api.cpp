@@ -16,6 +16,7 @@
 #include "../../include/cantera_gpu.h"
 #include "bdf.hpp"
 #include "mech.hpp"
+#include "surrogate.hpp"
 
 namespace {
 
@@ -39,12 +40,13 @@ int device_cu_count() {
 
 struct MechObj {
     std::unique_ptr<DeviceMech> dm;
-    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;   // last rates kernel
+    hipEvent_t j0 = nullptr, j1 = nullptr;   // last Jacobian kernel
     double last_ms = 0.0;
     int ns = 1;
     ~MechObj() {
-        if (e0) (void)hipEventDestroy(e0);
-        if (e1) (void)hipEventDestroy(e1);
+        for (hipEvent_t e : {e0, e1, j0, j1})
+            if (e) (void)hipEventDestroy(e);
     }
 };
 
@@ -59,6 +61,10 @@ struct Batch {
     int* piv = nullptr;
     double* tout = nullptr;
     int* queue = nullptr;
+    int* lpt = nullptr;          // 5n ints: previous step counts, keys in/out, indices, order
+    void* sort_tmp = nullptr;
+    size_t sort_tmp_bytes = 0;
+    bool lpt_on = true, have_order = false;
     size_t rec_stride = 0, j_stride = 0, lu_stride = 0, piv_stride = 0;
     hipStream_t stream = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -68,7 +74,7 @@ struct Batch {
     size_t lds = 0;
     ~Batch() {
         for (void* p : {static_cast<void*>(rec), static_cast<void*>(jsave), lusave, static_cast<void*>(piv),
-                        static_cast<void*>(tout), static_cast<void*>(queue)})
+                        static_cast<void*>(tout), static_cast<void*>(queue), static_cast<void*>(lpt), sort_tmp})
             if (p) (void)hipFree(p);
         if (e0) (void)hipEventDestroy(e0);
         if (e1) (void)hipEventDestroy(e1);
@@ -79,6 +85,18 @@ struct Batch {
 std::mutex g_mu;
 std::vector<std::unique_ptr<MechObj>> g_mechs;
 std::vector<std::unique_ptr<Batch>> g_batches;
+
+struct SurrogateObj {
+    MechObj* mech = nullptr;
+    std::unique_ptr<SurrogateNet> net;
+};
+std::vector<std::unique_ptr<SurrogateObj>> g_surr;
+
+SurrogateObj& get_surr(int32_t h) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (h < 0 || h >= static_cast<int32_t>(g_surr.size()) || !g_surr[h]) throw std::runtime_error("invalid surrogate handle");
+    return *g_surr[h];
+}
 
 MechObj& get_mech(int32_t h) {
     std::lock_guard<std::mutex> lk(g_mu);
@@ -158,6 +176,9 @@ int32_t cg_memcpy_h2d(void* dst, const void* src, int64_t bytes) {
 int32_t cg_memcpy_d2h(void* dst, const void* src, int64_t bytes) {
     return guarded([&] { check(hipMemcpy(dst, src, static_cast<size_t>(bytes), hipMemcpyDeviceToHost), "hipMemcpy D2H"); return 0; });
 }
+int32_t cg_memcpy_d2d(void* dst, const void* src, int64_t bytes) {
+    return guarded([&] { check(hipMemcpy(dst, src, static_cast<size_t>(bytes), hipMemcpyDeviceToDevice), "hipMemcpy D2D"); return 0; });
+}
 int32_t cg_device_synchronize(void) {
     return guarded([&] { check(hipDeviceSynchronize(), "hipDeviceSynchronize"); return 0; });
 }
@@ -171,6 +192,8 @@ int32_t cg_mech_load(const char* path) {
         obj->ns = (obj->dm->view().N + 63) / 64;
         check(hipEventCreate(&obj->e0), "hipEventCreate");
         check(hipEventCreate(&obj->e1), "hipEventCreate");
+        check(hipEventCreate(&obj->j0), "hipEventCreate");
+        check(hipEventCreate(&obj->j1), "hipEventCreate");
         std::lock_guard<std::mutex> lk(g_mu);
         g_mechs.push_back(std::move(obj));
         return static_cast<int32_t>(g_mechs.size() - 1);
@@ -281,6 +304,61 @@ double cg_rates_last_kernel_ms(int32_t h) {
     }
 }
 
+int32_t cg_jac_batch_device(int32_t h, int64_t n, const double* T, const double* P, const double* Y, double* J,
+                            int32_t precision, void* stream) {
+    return guarded([&] {
+        MechObj& mo = get_mech(h);
+        if (precision != CG_FP64) throw std::runtime_error("cg_jac_batch: only CG_FP64 is available");
+        if (n <= 0) return 0;
+        if (!T || !P || !Y || !J) throw std::runtime_error("null array");
+        const DevMech& m = mo.dm->view();
+        const size_t lds = jac_lds_bytes(m, mo.ns);
+        const void* kp = jac_kernel_ptr(mo.ns);
+        set_lds_limit(kp, lds);
+        const int grid = grid_for(kp, lds, 0, n);
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        check(hipEventRecord(mo.j0, s), "hipEventRecord");
+        check(jac_launch(m, mo.ns, n, T, P, Y, J, grid, lds, s), "Jacobian kernel launch");
+        check(hipEventRecord(mo.j1, s), "hipEventRecord");
+        return 0;
+    });
+}
+
+int32_t cg_jac_batch(int32_t h, int64_t n, const double* T, const double* P, const double* Y, double* J,
+                     int32_t precision) {
+    return guarded([&] {
+        MechObj& mo = get_mech(h);
+        if (n <= 0) return 0;
+        if (!T || !P || !Y || !J) throw std::runtime_error("null array");
+        const int K = mo.dm->view().K, N = mo.dm->view().N;
+        const size_t bytes = static_cast<size_t>(n) * sizeof(double);
+        const size_t jbytes = bytes * N * N;
+        double* d = nullptr;
+        check(hipMalloc(&d, bytes * (2 + static_cast<size_t>(K)) + jbytes), "hipMalloc");
+        std::unique_ptr<double, void (*)(double*)> guard(d, [](double* p) { (void)hipFree(p); });
+        double *dT = d, *dP = d + n, *dY = d + 2 * n, *dJ = d + (2 + static_cast<size_t>(K)) * n;
+        check(hipMemcpy(dT, T, bytes, hipMemcpyHostToDevice), "hipMemcpy");
+        check(hipMemcpy(dP, P, bytes, hipMemcpyHostToDevice), "hipMemcpy");
+        check(hipMemcpy(dY, Y, bytes * K, hipMemcpyHostToDevice), "hipMemcpy");
+        if (cg_jac_batch_device(h, n, dT, dP, dY, dJ, precision, nullptr) < 0) throw std::runtime_error(g_error);
+        check(hipMemcpy(J, dJ, jbytes, hipMemcpyDeviceToHost), "hipMemcpy");
+        return 0;
+    });
+}
+
+double cg_jac_last_kernel_ms(int32_t h) {
+    try {
+        MechObj& mo = get_mech(h);
+        check(hipEventSynchronize(mo.j1), "hipEventSynchronize");
+        float ms = 0.f;
+        check(hipEventElapsedTime(&ms, mo.j0, mo.j1), "hipEventElapsedTime");
+        return ms;
+    } catch (const std::exception& e) {
+        g_error = e.what();
+        return -1.0;
+    }
+}
+
 int32_t cg_reactors_create(int32_t h, int64_t n, const double* T0, const double* P, const double* Y0, double rtol,
                            double atol, int32_t precision) {
     return guarded([&] {
@@ -314,6 +392,10 @@ int32_t cg_reactors_create(int32_t h, int64_t n, const double* T0, const double*
         check(hipMalloc(&b->piv, b->piv_stride * n * 4), "hipMalloc(pivots)");
         check(hipMalloc(&b->tout, n * 8), "hipMalloc(tout)");
         check(hipMalloc(&b->queue, 64), "hipMalloc(queue)");
+        check(hipMalloc(&b->lpt, 5 * n * sizeof(int)), "hipMalloc(order)");
+        check(hipMemset(b->lpt, 0, n * sizeof(int)), "hipMemset(order)");
+        b->sort_tmp_bytes = order_temp_bytes(static_cast<int>(n));
+        check(hipMalloc(&b->sort_tmp, b->sort_tmp_bytes > 0 ? b->sort_tmp_bytes : 16), "hipMalloc(sort)");
         check(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "hipStreamCreate");
         check(hipEventCreate(&b->e0), "hipEventCreate");
         check(hipEventCreate(&b->e1), "hipEventCreate");
@@ -363,10 +445,18 @@ int32_t cg_reactors_set_options(int32_t h, int32_t max_steps, int32_t waves_per_
     });
 }
 
-static void launch_advance(Batch& b) {
+int32_t cg_reactors_set_schedule(int32_t h, int32_t longest_first) {
+    return guarded([&] {
+        Batch& b = get_batch(h);
+        b.lpt_on = longest_first != 0;
+        return 0;
+    });
+}
+
+static void launch_advance(Batch& b, const int* list = nullptr, int count = 0) {
     BdfArgs a{};
     a.mech = b.mech->dm->view();
-    a.n = b.n;
+    a.n = list ? count : b.n;
     a.neq_norm = b.N;
     a.ld = b.ld;
     a.max_steps = b.max_steps;
@@ -381,12 +471,19 @@ static void launch_advance(Batch& b) {
     a.piv_save = b.piv;
     a.piv_stride = b.piv_stride;
     a.tout = b.tout;
-    a.order = nullptr;
+    int* order = b.lpt + 4 * static_cast<size_t>(b.n);
+    a.order = list ? list : ((b.lpt_on && b.have_order) ? order : nullptr);
     a.queue = b.queue;
     const int grid = grid_for(bdf_kernel_ptr(b.N, b.fp32), b.lds, b.waves_per_cu, b.n);
     check(hipEventRecord(b.e0, b.stream), "hipEventRecord");
     check(bdf_launch(a, b.fp32, grid, b.lds, b.stream), "integrator kernel launch");
     check(hipEventRecord(b.e1, b.stream), "hipEventRecord");
+    // cost keys of this call -> processing order of the next
+    const size_t n = static_cast<size_t>(b.n);
+    check(order_update(b.rec, b.rec_stride, 7 * 64 * b.ns, b.n, b.lpt, b.lpt + n, b.lpt + 2 * n, b.lpt + 3 * n, order,
+                       b.sort_tmp, b.sort_tmp_bytes, b.stream),
+          "work-queue order update");
+    b.have_order = true;
 }
 
 int32_t cg_reactors_advance_device(int32_t h, const double* t_out_dev) {
@@ -502,6 +599,198 @@ double cg_reactors_last_kernel_ms(int32_t h) {
         g_error = e.what();
         return -1.0;
     }
+}
+
+/* ---- learned surrogate ---------------------------------------------------- */
+
+int32_t cg_surrogate_create(int32_t mech, double dt, int32_t hidden, int32_t depth, uint64_t seed) {
+    return guarded([&] {
+        MechObj& mo = get_mech(mech);
+        auto obj = std::make_unique<SurrogateObj>();
+        obj->mech = &mo;
+        obj->net = std::make_unique<SurrogateNet>(mo.dm->view().K, dt, hidden, depth, seed);
+        std::lock_guard<std::mutex> lk(g_mu);
+        g_surr.push_back(std::move(obj));
+        return static_cast<int32_t>(g_surr.size() - 1);
+    });
+}
+
+int32_t cg_surrogate_free(int32_t s) {
+    return guarded([&] {
+        get_surr(s);
+        std::lock_guard<std::mutex> lk(g_mu);
+        g_surr[s].reset();
+        return 0;
+    });
+}
+
+int64_t cg_surrogate_nparams(int32_t s) {
+    try {
+        return static_cast<int64_t>(get_surr(s).net->n_params());
+    } catch (const std::exception& e) {
+        g_error = e.what();
+        return CG_ERR;
+    }
+}
+
+int32_t cg_surrogate_fit_normalization(int32_t s, int64_t n, const double* y0, const double* P, const double* y1) {
+    return guarded([&] {
+        if (n <= 0 || !y0 || !P || !y1) throw std::runtime_error("empty or null training pairs");
+        get_surr(s).net->fit_normalization(n, y0, P, y1, 1 << 20);
+        return 0;
+    });
+}
+
+int32_t cg_surrogate_train_epoch(int32_t s, int64_t n, const double* y0, const double* P, const double* y1,
+                                 int32_t batch, double lr, double* loss) {
+    return guarded([&] {
+        if (n <= 0 || !y0 || !P || !y1) throw std::runtime_error("empty or null training pairs");
+        if (n > (1LL << 31) - 1) throw std::runtime_error("too many training pairs for one call");
+        const double l = get_surr(s).net->train_epoch(n, y0, P, y1, batch, lr, nullptr);
+        if (loss) *loss = l;
+        return 0;
+    });
+}
+
+int32_t cg_surrogate_loss_grad(int32_t s, int64_t n, const double* y0, const double* P, const double* y1,
+                               double* loss, float* grad) {
+    return guarded([&] {
+        if (n <= 0 || n > (1 << 20) || !y0 || !P || !y1) throw std::runtime_error("1..2^20 pairs required");
+        const double l = get_surr(s).net->loss_grad(n, y0, P, y1, grad, nullptr);
+        if (loss) *loss = l;
+        return 0;
+    });
+}
+
+int32_t cg_surrogate_predict(int32_t s, int64_t n, const double* y0, const double* P, double* y1) {
+    return guarded([&] {
+        if (n <= 0) return 0;
+        if (!y0 || !P || !y1) throw std::runtime_error("null array");
+        get_surr(s).net->predict(n, y0, P, y1, nullptr);
+        check(hipDeviceSynchronize(), "surrogate predict");
+        return 0;
+    });
+}
+
+int32_t cg_surrogate_get_params(int32_t s, float* p) {
+    return guarded([&] { get_surr(s).net->get_params(p); return 0; });
+}
+int32_t cg_surrogate_set_params(int32_t s, const float* p) {
+    return guarded([&] { get_surr(s).net->set_params(p); return 0; });
+}
+int32_t cg_surrogate_get_normalization(int32_t s, double* v) {
+    return guarded([&] { get_surr(s).net->get_normalization(v); return 0; });
+}
+int32_t cg_surrogate_set_normalization(int32_t s, const double* v) {
+    return guarded([&] { get_surr(s).net->set_normalization(v); return 0; });
+}
+
+int32_t cg_reactors_advance_surrogate(int32_t bh, int32_t sh, const double* t_out, double dT_tol, double rtol_g,
+                                      int64_t* counts) {
+    return guarded([&] {
+        Batch& b = get_batch(bh);
+        SurrogateObj& so = get_surr(sh);
+        if (so.mech != b.mech) throw std::runtime_error("surrogate and reactor batch use different mechanisms");
+        if (!t_out) throw std::runtime_error("null t_out");
+        if (!(dT_tol > 0) || !(rtol_g > 0)) throw std::runtime_error("guard tolerances must be positive");
+        const int n = b.n, N = b.N, K = b.K, NP = 64 * b.ns;
+        const double dt = so.net->dt();
+        check(hipStreamSynchronize(b.stream), "hipStreamSynchronize");
+        std::vector<double> tu(n), st(n);
+        check(hipMemcpy2D(tu.data(), 8, b.rec + 7 * NP + C_TUSER, b.rec_stride * 8, 8, n, hipMemcpyDeviceToHost),
+              "hipMemcpy2D(time)");
+        check(hipMemcpy2D(st.data(), 8, b.rec + 7 * NP + C_STATUS, b.rec_stride * 8, 8, n, hipMemcpyDeviceToHost),
+              "hipMemcpy2D(status)");
+        std::vector<int> list, nsub, bdf;
+        std::vector<double> t0;
+        for (int r = 0; r < n; r++) {
+            if (st[r] < 0) continue;
+            const double span = t_out[r] - tu[r];
+            if (span < 0) throw std::runtime_error("t_out before the current time");
+            const long long k = std::llround(span / dt);
+            if (k >= 1 && std::fabs(k * dt - span) <= 1e-9 * span) {
+                list.push_back(r);
+                nsub.push_back(static_cast<int>(k));
+                t0.push_back(tu[r]);
+            } else if (span > 0) {
+                bdf.push_back(r);
+            }
+        }
+        const int na = static_cast<int>(list.size());
+        int64_t accepted = 0, whole = 0;
+        if (na > 0) {
+            const size_t rows = static_cast<size_t>(na);
+            std::vector<void*> mem;
+            auto dalloc = [&](size_t bytes) {
+                void* p = nullptr;
+                check(hipMalloc(&p, bytes), "hipMalloc(surrogate step)");
+                mem.push_back(p);
+                return p;
+            };
+            struct Free {
+                std::vector<void*>& m;
+                ~Free() { for (void* p : m) (void)hipFree(p); }
+            } fr{mem};
+            int* d_list = static_cast<int*>(dalloc(rows * 4));
+            int* d_nsub = static_cast<int*>(dalloc(rows * 4));
+            int* d_done = static_cast<int*>(dalloc(rows * 4));
+            int* d_rej = static_cast<int*>(dalloc(rows * 4));
+            double* d_t0 = static_cast<double*>(dalloc(rows * 8));
+            double* d_P = static_cast<double*>(dalloc(rows * 8));
+            double* d_Y = static_cast<double*>(dalloc(rows * N * 8));
+            double* d_Yn = static_cast<double*>(dalloc(rows * N * 8));
+            check(hipMemcpy(d_list, list.data(), rows * 4, hipMemcpyHostToDevice), "hipMemcpy");
+            check(hipMemcpy(d_nsub, nsub.data(), rows * 4, hipMemcpyHostToDevice), "hipMemcpy");
+            check(hipMemcpy(d_t0, t0.data(), rows * 8, hipMemcpyHostToDevice), "hipMemcpy");
+            check(hipMemset(d_done, 0, rows * 4), "hipMemset");
+            check(hipMemset(d_rej, 0, rows * 4), "hipMemset");
+            check(surrogate_gather(na, d_list, b.rec, b.rec_stride, NP, N, K, d_Y, d_P, b.stream), "surrogate gather");
+            const DevMech& m = b.mech->dm->view();
+            const size_t lds = guard_lds_bytes(m, b.ns);
+            const void* gk = guard_kernel_ptr(b.ns);
+            set_lds_limit(gk, lds);
+            const int grid = grid_for(gk, lds, 0, na);
+            const int maxsub = *std::max_element(nsub.begin(), nsub.end());
+            check(hipEventRecord(b.e0, b.stream), "hipEventRecord");
+            for (int k = 0; k < maxsub; k++) {
+                so.net->predict(na, d_Y, d_P, d_Yn, b.stream);
+                check(surrogate_guard(m, b.ns, na, d_nsub, d_done, d_rej, d_Y, d_Yn, d_P, dt, dT_tol, rtol_g, grid, lds,
+                                      b.stream),
+                      "surrogate guard");
+            }
+            check(surrogate_restart(na, d_list, d_done, d_t0, dt, d_Y, b.rec, b.rec_stride, NP, N, K, b.stream),
+                  "surrogate restart");
+            check(hipEventRecord(b.e1, b.stream), "hipEventRecord");
+            std::vector<int> done(na);
+            check(hipMemcpyAsync(done.data(), d_done, rows * 4, hipMemcpyDeviceToHost, b.stream), "hipMemcpyAsync");
+            check(hipStreamSynchronize(b.stream), "hipStreamSynchronize");
+            for (int i = 0; i < na; i++) {
+                accepted += done[i];
+                if (done[i] == nsub[i]) whole++;
+                else bdf.push_back(list[i]);
+            }
+        }
+        if (!bdf.empty()) {
+            int* d_bdf = nullptr;
+            check(hipMalloc(&d_bdf, bdf.size() * 4), "hipMalloc(fallback list)");
+            std::unique_ptr<int, void (*)(int*)> g(d_bdf, [](int* p) { (void)hipFree(p); });
+            check(hipMemcpyAsync(b.tout, t_out, static_cast<size_t>(n) * 8, hipMemcpyHostToDevice, b.stream),
+                  "hipMemcpyAsync(tout)");
+            check(hipMemcpyAsync(d_bdf, bdf.data(), bdf.size() * 4, hipMemcpyHostToDevice, b.stream),
+                  "hipMemcpyAsync(list)");
+            launch_advance(b, d_bdf, static_cast<int>(bdf.size()));
+            check(hipStreamSynchronize(b.stream), "hipStreamSynchronize");
+        }
+        if (counts) {
+            counts[0] = accepted;
+            counts[1] = whole;
+            counts[2] = static_cast<int64_t>(bdf.size());
+        }
+        int bad = cg_reactors_status(bh, nullptr);
+        if (bad < 0) throw std::runtime_error(g_error);
+        if (bad > 0) throw std::runtime_error(std::to_string(bad) + " reactor(s) failed to integrate; see cg_reactors_status");
+        return 0;
+    });
 }
 
 }  // extern "C"

@@ -5,33 +5,32 @@ namespace cg {
 
 const void* bdf_kernel_r16(bool);
 const void* bdf_kernel_r32(bool);
-const void* bdf_kernel_r56(bool);
-const void* bdf_kernel_r64(bool);
-const void* bdf_kernel_lds(bool);
+const void* bdf_kernel_l1(bool);
+const void* bdf_kernel_l2(bool);
 hipError_t bdf_launch_r16(const BdfArgs&, bool, int, size_t, hipStream_t);
 hipError_t bdf_launch_r32(const BdfArgs&, bool, int, size_t, hipStream_t);
-hipError_t bdf_launch_r56(const BdfArgs&, bool, int, size_t, hipStream_t);
-hipError_t bdf_launch_r64(const BdfArgs&, bool, int, size_t, hipStream_t);
-hipError_t bdf_launch_lds(const BdfArgs&, bool, int, size_t, hipStream_t);
+hipError_t bdf_launch_l1(const BdfArgs&, bool, int, size_t, hipStream_t);
+hipError_t bdf_launch_l2(const BdfArgs&, bool, int, size_t, hipStream_t);
 
 namespace {
 __global__ void reset_queue_kernel(int* q) { *q = 0; }
 }  // namespace
 
+// 16, 32: iteration matrix in registers (one row per lane).
+// 1, 2: iteration matrix in LDS, state spread over 1 or 2 slots per lane.
 int bdf_variant(int N) {
     if (N <= 16) return 16;
     if (N <= 32) return 32;
-    if (N <= 56) return 56;
-    if (N <= 64) return 64;
-    if (N <= 128) return 0;
+    if (N <= 64) return 1;
+    if (N <= 128) return 2;
     return -1;
 }
 
 size_t bdf_lds_bytes(const DevMech& m, int ld, bool fp32) {
     const int v = bdf_variant(m.N);
-    const int ns = (v == 0) ? 2 : 1;
+    const int ns = (v == 2) ? 2 : 1;
     size_t bytes = bdf_scratch_doubles(m.K, m.I, m.nsub, ns) * sizeof(double);
-    if (v == 0) {
+    if (v == 1 || v == 2) {
         bytes += static_cast<size_t>(kWave) * ns * (sizeof(double) + sizeof(int));
         bytes += static_cast<size_t>(m.N) * ld * (fp32 ? 4 : 8);
     }
@@ -42,9 +41,8 @@ const void* bdf_kernel_ptr(int N, bool fp32) {
     switch (bdf_variant(N)) {
         case 16: return bdf_kernel_r16(fp32);
         case 32: return bdf_kernel_r32(fp32);
-        case 56: return bdf_kernel_r56(fp32);
-        case 64: return bdf_kernel_r64(fp32);
-        case 0: return bdf_kernel_lds(fp32);
+        case 1: return bdf_kernel_l1(fp32);
+        case 2: return bdf_kernel_l2(fp32);
         default: return nullptr;
     }
 }
@@ -54,9 +52,8 @@ hipError_t bdf_launch(const BdfArgs& a, bool fp32, int grid, size_t lds, hipStre
     switch (bdf_variant(a.mech.N)) {
         case 16: return bdf_launch_r16(a, fp32, grid, lds, stream);
         case 32: return bdf_launch_r32(a, fp32, grid, lds, stream);
-        case 56: return bdf_launch_r56(a, fp32, grid, lds, stream);
-        case 64: return bdf_launch_r64(a, fp32, grid, lds, stream);
-        case 0: return bdf_launch_lds(a, fp32, grid, lds, stream);
+        case 1: return bdf_launch_l1(a, fp32, grid, lds, stream);
+        case 2: return bdf_launch_l2(a, fp32, grid, lds, stream);
         default: return hipErrorInvalidValue;
     }
 }

@@ -40,8 +40,9 @@ struct BdfArgs {
     template <class T> __host__ __device__ T* lu() const { return static_cast<T*>(lu_save_raw); }
 };
 
-// kernel variant for a state size N: 16/32/56/64 = matrix in registers with
-// that many columns, 0 = matrix in LDS (N <= 128), -1 = unsupported
+// kernel variant for a state size N: 16/32 = matrix in registers with that many
+// columns, 1/2 = matrix in LDS with 1 or 2 state slots per lane (N <= 64 / 128),
+// -1 = unsupported
 int bdf_variant(int N);
 size_t bdf_lds_bytes(const DevMech& m, int ld, bool fp32);
 hipError_t bdf_launch(const BdfArgs& a, bool fp32, int grid, size_t lds, hipStream_t stream);
@@ -52,5 +53,28 @@ size_t rates_lds_bytes(const DevMech& m, int ns);
 hipError_t rates_launch(const DevMech& m, int ns, long n, const double* T, const double* P, const double* Y,
                         double* wdot, int grid, size_t lds, hipStream_t stream);
 const void* rates_kernel_ptr(int ns);
+
+// longest-first processing order for the integrator's work queue
+size_t order_temp_bytes(int n);
+hipError_t order_update(const double* rec, size_t rec_stride, int ctl_off, int n, int* prev_nst, int* keys_in,
+                        int* keys_out, int* idx_in, int* order_out, void* temp, size_t temp_bytes,
+                        hipStream_t stream);
+
+// guarded surrogate steps for reactor batches
+size_t guard_lds_bytes(const DevMech& m, int ns);
+const void* guard_kernel_ptr(int ns);
+hipError_t surrogate_gather(int n, const int* list, const double* rec, size_t stride, int NP, int N, int K, double* Y,
+                            double* P, hipStream_t s);
+hipError_t surrogate_guard(const DevMech& m, int ns, int n, const int* nsub, int* done, int* rejected, double* Ycur,
+                           const double* Ynext, const double* Pa, double dt, double dT_tol, double rtol_g, int grid,
+                           size_t lds, hipStream_t s);
+hipError_t surrogate_restart(int n, const int* list, const int* done, const double* t0, double dt, const double* Y,
+                             double* rec, size_t stride, int NP, int N, int K, hipStream_t s);
+
+// batched reactor Jacobians (inputs as the rates, output N x N per state)
+size_t jac_lds_bytes(const DevMech& m, int ns);
+hipError_t jac_launch(const DevMech& m, int ns, long n, const double* T, const double* P, const double* Y, double* J,
+                      int grid, size_t lds, hipStream_t stream);
+const void* jac_kernel_ptr(int ns);
 
 }  // namespace cg

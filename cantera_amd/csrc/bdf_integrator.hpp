@@ -74,6 +74,7 @@ struct Integrator {
     const DevMech& m;
     const BdfArgs& a;
     WaveScratch S;
+    ScratchLayout L;
     Ctl& c;
     LS ls;
     int N, lane;
@@ -149,14 +150,18 @@ struct Integrator {
 
     __device__ __forceinline__ int f_eval(const double (&yy)[NS], double (&ff)[NS]) {
         if (lane == 0) c.nfe += 1;
-        GasState g;
-        SpeciesRegs<NS> sr;
-        RhsScalars rs;
-        reactor_rhs<NS>(m, S, uni(c.P), yy, ff, g, sr, rs);
-        int bad = 0;
 #pragma unroll
         for (int s = 0; s < NS; s++)
+            if (comp(s) < N) S.yb[comp(s)] = yy[s];
+        wave_sync();
+        rhs_call<NS>(m, L, uni(c.P));
+        int bad = 0;
+#pragma unroll
+        for (int s = 0; s < NS; s++) {
+            ff[s] = (comp(s) < N) ? S.F[comp(s)] : 0.0;
             if (comp(s) < N && !isfinite(ff[s])) bad = 1;
+        }
+        wave_sync();
         return __any(bad) ? 1 : 0;
     }
 
@@ -184,7 +189,14 @@ struct Integrator {
                 c.jcur = 1;
                 c.nfe_dq += 2;
             }
-            reactor_jacobian<NS>(m, S, uni(c.P), zn[0], ftemp, Js);
+#pragma unroll
+            for (int s = 0; s < NS; s++)
+                if (comp(s) < N) {
+                    S.Y0[comp(s)] = zn[0][s];
+                    S.F0[comp(s)] = ftemp[s];
+                }
+            wave_sync();
+            jac_call<NS>(m, L, uni(c.P), Js);
         }
         ls.form(Js, N, gamma);
         const bool ok = ls.factor(N);
@@ -854,30 +866,34 @@ struct Integrator {
 // LDS per wavefront: state broadcast, species/reaction scratch, control block,
 // then the linear-system area (LDS policy only).
 __host__ __device__ inline size_t bdf_scratch_doubles(int K, int I, int nsub, int ns) {
-    return static_cast<size_t>(kWave) * ns + K + K + I + nsub + 2 * K + kCtlDoubles;
+    return static_cast<size_t>(kWave) * ns * 4 + K + K + I + nsub + 2 * K + kCtlDoubles;
 }
 
 template <int NS, class LS, typename MT>
 __device__ __forceinline__ void bdf_advance_body(const BdfArgs& a) {
-    extern __shared__ double lds_raw[];
     const DevMech& m = a.mech;
-    double* base = lds_raw;
-    WaveScratch S;
-    S.yb = base;  base += kWave * NS;
-    S.C = base;   base += m.K;
-    S.MU = base;  base += m.K;
-    S.ROP = base; base += m.I;
-    S.SUB = base; base += m.nsub;
-    S.W2 = base;  base += 2 * m.K;
+    ScratchLayout L;
+    int off = 0;
+    L.yb = off;  off += kWave * NS;
+    L.F = off;   off += kWave * NS;
+    L.Y0 = off;  off += kWave * NS;
+    L.F0 = off;  off += kWave * NS;
+    L.C = off;   off += m.K;
+    L.MU = off;  off += m.K;
+    L.ROP = off; off += m.I;
+    L.SUB = off; off += m.nsub;
+    L.W2 = off;  off += 2 * m.K;
+    double* base = lds_base() + off;
     Ctl& c = *reinterpret_cast<Ctl*>(base);
     base += kCtlDoubles;
     Integrator<NS, LS, MT> it(m, a, c);
-    it.S = S;
+    it.S = scratch_view(L);
+    it.L = L;
     it.lane = lane_id();
     it.N = m.N;
     it.rtol = a.rtol;
     it.atol = a.atol;
-    it.ls.init(base);
+    it.ls.init(base, a.ld);
     for (;;) {
         int r = 0;
         if (it.lane == 0) r = atomicAdd(a.queue, 1);

@@ -1,0 +1,86 @@
+// Batched analytic reactor Jacobians d(ydot)/dy of IdealGasConstPressureReactor
+// states, the quantity Reactor::finiteDifferenceJacobian approximates
+// (src/zeroD/Reactor.cpp:233-276 builds it column by column from eval()).
+//
+// Input layout as for the batched rates: T[n], P[n], Y[k*n + s] (species-major).
+// Output: J[s*N*N + r*N + c], row-major per state, in the reactor's state order
+// [m, T, Y_0..Y_{K-1}].  ydot does not depend on the reactor mass (dY/dt and
+// dT/dt scale with V/m = 1/rho), so the state is taken with m = 1 kg and the m
+// row and column are zero.
+//
+// One wavefront per state: RHS at the state (out-of-line kinetics), then the
+// Jacobian assembly writes straight into the state's output block.
+#include <hip/hip_runtime.h>
+
+#include "bdf.hpp"
+#include "wave_jac.hpp"
+
+namespace cg {
+
+namespace {
+
+template <int NS>
+__global__ __launch_bounds__(64) void jac_batch_kernel(DevMech m, long n, const double* __restrict__ T,
+                                                       const double* __restrict__ P, const double* __restrict__ Y,
+                                                       double* __restrict__ J) {
+    const int lane = lane_id();
+    ScratchLayout L;
+    int off = 0;
+    L.yb = off;  off += kWave * NS;
+    L.F = off;   off += kWave * NS;
+    L.Y0 = off;  off += kWave * NS;
+    L.F0 = off;  off += kWave * NS;
+    L.C = off;   off += m.K;
+    L.MU = off;  off += m.K;
+    L.ROP = off; off += m.I;
+    L.SUB = off; off += m.nsub;
+    L.W2 = off;
+    const WaveScratch S = scratch_view(L);
+    const size_t NN = static_cast<size_t>(m.N) * m.N;
+    for (long st = blockIdx.x; st < n; st += gridDim.x) {
+        const double Ts = T[st];
+        const double Ps = uni(P[st]);
+        double y[NS], f[NS];
+#pragma unroll
+        for (int s = 0; s < NS; s++) {
+            const int i = lane + kWave * s;
+            double v = 0.0;
+            if (i < m.K) v = Y[static_cast<size_t>(i) * n + st];
+            else if (i == m.K) v = Ts;
+            else if (i == m.K + 1) v = 1.0;
+            y[s] = v;
+            if (i < m.N) S.yb[i] = v;
+        }
+        wave_sync();
+        rhs_call<NS>(m, L, Ps);
+#pragma unroll
+        for (int s = 0; s < NS; s++) {
+            const int i = lane + kWave * s;
+            f[s] = (i < m.N) ? S.F[i] : 0.0;
+        }
+        wave_sync();
+        reactor_jacobian<NS, true>(m, S, Ps, y, f, J + static_cast<size_t>(st) * NN);
+    }
+}
+
+}  // namespace
+
+size_t jac_lds_bytes(const DevMech& m, int ns) {
+    const size_t d = static_cast<size_t>(kWave) * ns * 4 + 2 * m.K + m.I + m.nsub + 2 * m.K;
+    return d * sizeof(double);
+}
+
+hipError_t jac_launch(const DevMech& m, int ns, long n, const double* T, const double* P, const double* Y, double* J,
+                      int grid, size_t lds, hipStream_t stream) {
+    if (ns == 1) hipLaunchKernelGGL((jac_batch_kernel<1>), dim3(grid), dim3(64), lds, stream, m, n, T, P, Y, J);
+    else if (ns == 2) hipLaunchKernelGGL((jac_batch_kernel<2>), dim3(grid), dim3(64), lds, stream, m, n, T, P, Y, J);
+    else return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+const void* jac_kernel_ptr(int ns) {
+    return ns == 1 ? reinterpret_cast<const void*>(&jac_batch_kernel<1>)
+                   : reinterpret_cast<const void*>(&jac_batch_kernel<2>);
+}
+
+}  // namespace cg

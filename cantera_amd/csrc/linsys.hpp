@@ -35,7 +35,7 @@ struct RegLinSys {
     int step;      // pivot step of this lane's row (NMAX if not a pivot row yet)
     int pivlane;   // lane k: row (lane) chosen as pivot at step k
 
-    __device__ __forceinline__ void init(void*) {}
+    __device__ __forceinline__ void init(void*, int) {}
 
     __device__ __forceinline__ void form(const double* __restrict__ J, int N, double gamma) {
         const int lane = lane_id();
@@ -148,7 +148,8 @@ struct LdsLinSys {
     int* pmap;    // LDS
     MT* bv;       // LDS
 
-    __device__ __forceinline__ void init(void* lds) {
+    __device__ __forceinline__ void init(void* lds, int ld) {
+        LD = ld;
         char* p = static_cast<char*>(lds);
         bv = reinterpret_cast<MT*>(p);
         p += kWave * NS * sizeof(double);

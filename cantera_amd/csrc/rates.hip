@@ -33,6 +33,7 @@ __global__ __launch_bounds__(64) void rates_tile_kernel(DevMech m, long n, const
     S.ROP = base; base += m.I;
     S.SUB = base; base += m.nsub;
     S.W2 = nullptr;
+    S.F = S.Y0 = S.F0 = nullptr;
     const long ntiles = (n + kTile - 1) / kTile;
     for (long t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const long s0 = t * kTile;

@@ -137,19 +137,28 @@ __device__ __forceinline__ void scatter_reaction_derivs(const DevMech& m, int p,
 }
 
 // Build J = d(ydot)/dy into the FP64 matrix J (global memory, N x N row-major).
-// y/f0: state and RHS at the expansion point (lane-distributed).  The
-// per-reaction derivatives are accumulated with FP64 global atomics, so their
-// summation order is not fixed; J only enters the Newton iteration matrix.
-template <int NS>
+// y/f0: state and RHS at the expansion point (lane-distributed, internal order).
+// kRefOrder = false: J in the internal order [Y_0..Y_{K-1}, T, m];
+// kRefOrder = true:  J in the reactor's own order [m, T, Y_0..Y_{K-1}]
+// (IdealGasConstPressureReactor::getState, src/zeroD/IdealGasConstPressureReactor.cpp:16-26).
+// The per-reaction derivatives are accumulated with FP64 global atomics, so
+// their summation order is not fixed.
+template <int NS, bool kRefOrder = false>
 __device__ void reactor_jacobian(const DevMech& m, const WaveScratch& S, double P, const double (&y)[NS],
                                  const double (&f0)[NS], double* __restrict__ J) {
     const int lane = lane_id();
     const int K = m.K, N = m.N;
+    // species block, temperature/mass rows and columns
+    double* const JC = kRefOrder ? J + 2 * N + 2 : J;
+    double* const rowT = kRefOrder ? J + N + 2 : J + K * N;
+    double* const rowM = kRefOrder ? J + 2 : J + (K + 1) * N;
+    double* const colT = kRefOrder ? J + 1 : J + K;
+    double* const colM = kRefOrder ? J : J + K + 1;
     GasState g;
     SpeciesRegs<NS> sr;
     RhsScalars rs;
     // temperature column by a one-sided difference
-    double y1[NS], f1[NS], colT[NS];
+    double y1[NS], f1[NS], dfdT[NS];
 #pragma unroll
     for (int s = 0; s < NS; s++) y1[s] = y[s];
     const double Tval = bcast<NS>(y, K);
@@ -159,7 +168,7 @@ __device__ void reactor_jacobian(const DevMech& m, const WaveScratch& S, double 
         if (lane + kWave * s == K) y1[s] = Tval + dT;
     reactor_rhs<NS>(m, S, P, y1, f1, g, sr, rs);
 #pragma unroll
-    for (int s = 0; s < NS; s++) colT[s] = (f1[s] - f0[s]) / dT;
+    for (int s = 0; s < NS; s++) dfdT[s] = (f1[s] - f0[s]) / dT;
     // base state (species scratch, wdot)
     reactor_rhs<NS>(m, S, P, y, f1, g, sr, rs);
 #pragma unroll
@@ -170,7 +179,7 @@ __device__ void reactor_jacobian(const DevMech& m, const WaveScratch& S, double 
     }
     __threadfence();
     wave_sync();
-    for (int p = lane; p < m.I; p += kWave) scatter_reaction_derivs(m, p, g, S.C, S.MU, J, N);
+    for (int p = lane; p < m.I; p += kWave) scatter_reaction_derivs(m, p, g, S.C, S.MU, JC, N);
     __threadfence();
     wave_sync();
     // D_k = sum_l JC[k][l] C_l ; keep wdot and D in W2, h_k in MU
@@ -180,7 +189,7 @@ __device__ void reactor_jacobian(const DevMech& m, const WaveScratch& S, double 
         int k = lane + kWave * s;
         if (k < K) {
             double d = 0.0;
-            for (int l = 0; l < K; l++) d += J[k * N + l] * S.C[l];
+            for (int l = 0; l < K; l++) d += JC[k * N + l] * S.C[l];
             const double hk = sr.hrt[s] * g.RT;
             S.W2[k] = sr.wdot[s];
             S.W2[K + k] = d;
@@ -203,27 +212,43 @@ __device__ void reactor_jacobian(const DevMech& m, const WaveScratch& S, double 
         int j = lane + kWave * s;
         if (j < K) {
             double hJ = 0.0;
-            for (int k = 0; k < K; k++) hJ += S.MU[k] * J[k * N + j];
+            for (int k = 0; k < K; k++) hJ += S.MU[k] * JC[k * N + j];
             const double wj = m.mw[j];
             for (int k = 0; k < K; k++) {
-                const double jc = J[k * N + j];
-                J[k * N + j] = m.mw[k] / wj * (jc + (S.W2[k] - S.W2[K + k]) / Ct);
+                const double jc = JC[k * N + j];
+                JC[k * N + j] = m.mw[k] / wj * (jc + (S.W2[k] - S.W2[K + k]) / Ct);
             }
             const double cpj = kGasConstant * sr.cpr[s] / wj;
-            J[K * N + j] = -1.0 / (rho * cp) * ((rho * hJ - Wbar * hD + Wbar * H) / wj - H * cpj / cp);
-            J[(K + 1) * N + j] = 0.0;
+            rowT[j] = -1.0 / (rho * cp) * ((rho * hJ - Wbar * hD + Wbar * H) / wj - H * cpj / cp);
+            rowM[j] = 0.0;
         }
     }
 #pragma unroll
     for (int s = 0; s < NS; s++) {
         int i = lane + kWave * s;
         if (i < N) {
-            J[i * N + K] = (i == K + 1) ? 0.0 : colT[s];
-            J[i * N + K + 1] = 0.0;
+            const int r = kRefOrder ? (i < K ? i + 2 : (i == K ? 1 : 0)) : i;
+            colT[r * N] = (i == K + 1) ? 0.0 : dfdT[s];
+            colM[r * N] = 0.0;
         }
     }
     __threadfence();
     wave_sync();
+}
+
+// Out-of-line Jacobian: expansion point in S.Y0, its RHS in S.F0.
+template <int NS>
+__device__ __noinline__ void jac_call(const DevMech& m, ScratchLayout L, double P, double* J) {
+    const WaveScratch S = scratch_view(L);
+    const int lane = lane_id();
+    double y[NS], f0[NS];
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        const int i = lane + kWave * s;
+        y[s] = (i < m.N) ? S.Y0[i] : 0.0;
+        f0[s] = (i < m.N) ? S.F0[i] : 0.0;
+    }
+    reactor_jacobian<NS>(m, S, P, y, f0, J);
 }
 
 }  // namespace cg

@@ -90,7 +90,30 @@ struct WaveScratch {
     double* ROP;   // I net rates of progress
     double* SUB;   // nsub gather partial sums
     double* W2;    // 2K (Jacobian: wdot, D)
+    double* F;     // N (right-hand side out)
+    double* Y0;    // N (Jacobian expansion point)
+    double* F0;    // N (right-hand side at Y0)
 };
+
+// The same scratch as offsets (in doubles) from the start of dynamic LDS, for
+// passing to out-of-line device functions: pointers rebuilt from the LDS base
+// inside the callee keep the LDS address space (ds_* instructions).
+struct ScratchLayout {
+    int yb, C, MU, ROP, SUB, W2, F, Y0, F0;
+};
+
+__device__ __forceinline__ double* lds_base() {
+    extern __shared__ double lds_raw[];
+    return lds_raw;
+}
+
+__device__ __forceinline__ WaveScratch scratch_view(const ScratchLayout& L) {
+    double* b = lds_base();
+    WaveScratch S;
+    S.yb = b + L.yb; S.C = b + L.C; S.MU = b + L.MU; S.ROP = b + L.ROP; S.SUB = b + L.SUB;
+    S.W2 = b + L.W2; S.F = b + L.F; S.Y0 = b + L.Y0; S.F0 = b + L.F0;
+    return S;
+}
 
 // Scalar thermodynamic state derived from (T, P, Y): setMassFractions_NoNorm +
 // setState_TP (Phase.cpp:362-372; ThermoPhase.cpp:187-198; IdealGasPhase.h:338-354).
@@ -156,8 +179,25 @@ __device__ __forceinline__ void eval_reaction(const DevMech& m, int p, const Gas
     const bool rev = x.flags & 1;
     double M = 1.0;
     if (x.flags & 32) {
+        // efficiencies in groups of 4: the table loads of a group, then its
+        // concentration loads, are in flight together
         double sum = 0.0;
-        for (int j = 0; j < x.tbn; j++) sum += m.tbe[x.tbo + j] * C[m.tbk[x.tbo + j]];
+        for (int j0 = 0; j0 < x.tbn; j0 += 4) {
+            int kk[4];
+            double ee[4];
+#pragma unroll
+            for (int b = 0; b < 4; b++) {
+                const bool ok = j0 + b < x.tbn;
+                kk[b] = ok ? m.tbk[x.tbo + j0 + b] : 0;
+                ee[b] = ok ? m.tbe[x.tbo + j0 + b] : 0.0;
+            }
+            double cc[4];
+#pragma unroll
+            for (int b = 0; b < 4; b++) cc[b] = C[kk[b]];
+#pragma unroll
+            for (int b = 0; b < 4; b++)
+                if (j0 + b < x.tbn) sum += ee[b] * cc[b];
+        }
         M = m.tbdef[p] * g.ctot + sum;
     }
     double kf;
@@ -340,8 +380,23 @@ __device__ __forceinline__ void production_rates(const DevMech& m, const WaveScr
     wave_sync();
     for (int sb = lane; sb < m.nsub; sb += kWave) {
         double acc = 0.0;
-        const int e1 = m.sub_start[sb + 1];
-        for (int e = m.sub_start[sb]; e < e1; e++) acc += m.ent_nu[e] * S.ROP[m.ent_r[e]];
+        const int e0 = m.sub_start[sb], e1 = m.sub_start[sb + 1];
+        for (int e = e0; e < e1; e += 4) {
+            int rr[4];
+            double nn[4];
+#pragma unroll
+            for (int b = 0; b < 4; b++) {
+                const bool ok = e + b < e1;
+                rr[b] = ok ? m.ent_r[e + b] : 0;
+                nn[b] = ok ? m.ent_nu[e + b] : 0.0;
+            }
+            double rp[4];
+#pragma unroll
+            for (int b = 0; b < 4; b++) rp[b] = S.ROP[rr[b]];
+#pragma unroll
+            for (int b = 0; b < 4; b++)
+                if (e + b < e1) acc += nn[b] * rp[b];
+        }
         S.SUB[sb] = acc;
     }
     wave_sync();
@@ -404,6 +459,31 @@ __device__ __forceinline__ void reactor_rhs(const DevMech& m, const WaveScratch&
         else if (i == m.K) v = mcpdTdt / (mass * cp_mass);
         f[s] = v;
     }
+}
+
+// Out-of-line reactor RHS: state taken from S.yb (written by the caller),
+// ydot written to S.F.  One copy of the kinetics code serves every call site of
+// the integrator (smaller code, and the caller keeps little live across it).
+template <int NS>
+__device__ __noinline__ void rhs_call(const DevMech& m, ScratchLayout L, double P) {
+    const WaveScratch S = scratch_view(L);
+    const int lane = lane_id();
+    double y[NS], f[NS];
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        const int i = lane + kWave * s;
+        y[s] = (i < m.N) ? S.yb[i] : 0.0;
+    }
+    GasState g;
+    SpeciesRegs<NS> sr;
+    RhsScalars rs;
+    reactor_rhs<NS>(m, S, P, y, f, g, sr, rs);
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        const int i = lane + kWave * s;
+        if (i < m.N) S.F[i] = f[s];
+    }
+    wave_sync();
 }
 
 }  // namespace cg

@@ -63,13 +63,28 @@ __device__ bool lu_factor(T* A, int LD, int N, int* pmap) {
             if (i > k && i < N) A[i * LD + k] *= mult;
         }
         wave_sync();
+        // Trailing update, lane j owns column j: A[i][j] -= A[i][k] * A[k][j]
+        // for i > k.  Rows are taken in blocks of kB so that the LDS loads of
+        // a block (A[i][k] broadcast, A[i][j] per lane) are in flight together.
+        constexpr int kB = 8;
 #pragma unroll
         for (int s = 0; s < NS; s++) {
             int j = lane + kWave * s;
             if (j > k && j < N) {
                 const T akj = A[k * LD + j];
                 if (akj != T(0)) {
-                    for (int i = k + 1; i < N; i++) A[i * LD + j] -= A[i * LD + k] * akj;
+                    int i = k + 1;
+                    for (; i + kB <= N; i += kB) {
+                        T lik[kB], aij[kB];
+#pragma unroll
+                        for (int b = 0; b < kB; b++) {
+                            lik[b] = A[(i + b) * LD + k];
+                            aij[b] = A[(i + b) * LD + j];
+                        }
+#pragma unroll
+                        for (int b = 0; b < kB; b++) A[(i + b) * LD + j] = aij[b] - lik[b] * akj;
+                    }
+                    for (; i < N; i++) A[i * LD + j] -= A[i * LD + k] * akj;
                 }
             }
         }
@@ -95,26 +110,56 @@ __device__ void lu_solve(const T* A, int LD, int N, const int* pmap, V* bv, V (&
         b[s] = (i < N) ? bv[pmap[i]] : V(0);
     }
     wave_sync();
-    // forward substitution with unit lower factor
-    for (int k = 0; k < N - 1; k++) {
-        const V bk = bcast<NS>(b, k);
+    // Forward substitution with the unit lower factor.  Lane i needs column
+    // k of its row at step k; the loads of kB consecutive columns are issued
+    // together ahead of the sequential readlane/FMA chain.
+    constexpr int kB = 4;
+    for (int k0 = 0; k0 < N - 1; k0 += kB) {
+        T lv[kB][NS];
 #pragma unroll
-        for (int s = 0; s < NS; s++) {
-            int i = lane + kWave * s;
-            if (i > k && i < N) b[s] -= static_cast<V>(A[i * LD + k]) * bk;
+        for (int b2 = 0; b2 < kB; b2++)
+#pragma unroll
+            for (int s = 0; s < NS; s++) {
+                const int i = lane + kWave * s;
+                const int k = k0 + b2;
+                lv[b2][s] = (i > k && i < N && k < N - 1) ? A[i * LD + k] : T(0);
+            }
+#pragma unroll
+        for (int b2 = 0; b2 < kB; b2++) {
+            const int k = k0 + b2;
+            if (k < N - 1) {
+                const V bk = bcast<NS>(b, k);
+#pragma unroll
+                for (int s = 0; s < NS; s++) b[s] -= static_cast<V>(lv[b2][s]) * bk;
+            }
         }
     }
-    // back substitution
-    for (int k = N - 1; k >= 0; k--) {
-        const int s_k = k >> 6;
+    // Back substitution with the upper factor.
+    for (int k0 = N - 1; k0 >= 0; k0 -= kB) {
+        T uv[kB][NS];
 #pragma unroll
-        for (int s = 0; s < NS; s++)
-            if (s == s_k && lane == (k & 63)) b[s] /= static_cast<V>(A[k * LD + k]);
-        const V bk = bcast<NS>(b, k);
+        for (int b2 = 0; b2 < kB; b2++)
 #pragma unroll
-        for (int s = 0; s < NS; s++) {
-            int i = lane + kWave * s;
-            if (i < k) b[s] -= static_cast<V>(A[i * LD + k]) * bk;
+            for (int s = 0; s < NS; s++) {
+                const int i = lane + kWave * s;
+                const int k = k0 - b2;
+                uv[b2][s] = (k >= 0 && i <= k) ? A[i * LD + k] : T(1);
+            }
+#pragma unroll
+        for (int b2 = 0; b2 < kB; b2++) {
+            const int k = k0 - b2;
+            if (k >= 0) {
+                const int s_k = k >> 6;
+#pragma unroll
+                for (int s = 0; s < NS; s++)
+                    if (s == s_k && lane == (k & 63)) b[s] /= static_cast<V>(uv[b2][s]);
+                const V bk = bcast<NS>(b, k);
+#pragma unroll
+                for (int s = 0; s < NS; s++) {
+                    const int i = lane + kWave * s;
+                    if (i < k) b[s] -= static_cast<V>(uv[b2][s]) * bk;
+                }
+            }
         }
     }
 }

@@ -44,6 +44,7 @@ void* cg_dev_alloc(int64_t bytes);
 int32_t cg_dev_free(void* p);
 int32_t cg_memcpy_h2d(void* dst, const void* src, int64_t bytes);
 int32_t cg_memcpy_d2h(void* dst, const void* src, int64_t bytes);
+int32_t cg_memcpy_d2d(void* dst, const void* src, int64_t bytes);
 int32_t cg_device_synchronize(void);
 
 /* ---- mechanisms ----------------------------------------------------------- */
@@ -79,6 +80,24 @@ int32_t cg_rates_batch(int32_t mech, int64_t n, const double* T, const double* P
  * NULL for the default stream). */
 int32_t cg_rates_batch_device(int32_t mech, int64_t n, const double* T, const double* P,
                               const double* Y, double* wdot, int32_t precision, void* stream);
+
+/* ---- batched reactor Jacobians ---------------------------------------------- */
+/* J = d(ydot)/dy of an IdealGasConstPressureReactor at each of n states, the
+ * matrix Reactor::jacobian() returns for it by finite differences
+ * (Reactor::finiteDifferenceJacobian, src/zeroD/Reactor.cpp:233-276; eval
+ * src/zeroD/IdealGasConstPressureReactor.cpp:46-110), here analytic in the
+ * species columns (the chain rule of BulkKinetics::netProductionRates_ddCi,
+ * src/kinetics/BulkKinetics.cpp:761-811) and a one-sided difference in T.
+ * T[n], P[n], Y[K*n] species-major as cg_rates_batch; J[n*N*N] with N = K+2,
+ * row-major per state, state order [m, T, Y_0..Y_{K-1}].  ydot is independent
+ * of the mass, so the m row and column are zero.  precision: CG_FP64. */
+int32_t cg_jac_batch(int32_t mech, int64_t n, const double* T, const double* P,
+                     const double* Y, double* J, int32_t precision);
+/* Same on device pointers, asynchronous on `stream`. */
+int32_t cg_jac_batch_device(int32_t mech, int64_t n, const double* T, const double* P,
+                            const double* Y, double* J, int32_t precision, void* stream);
+/* Duration [ms] of the last batched-Jacobian kernel on `mech`. */
+double cg_jac_last_kernel_ms(int32_t mech);
 
 /* ---- batched reactors ------------------------------------------------------- */
 /* Create n IdealGasConstPressureReactor + ReactorNet pairs (one per state) with
@@ -120,11 +139,58 @@ int32_t cg_reactors_status(int32_t batch, int32_t* status);
  * CVodesIntegrator.h:120-132) and the persistent grid size in wavefronts per
  * CU (0 = from occupancy). */
 int32_t cg_reactors_set_options(int32_t batch, int32_t max_steps, int32_t waves_per_cu);
+/* Work-queue schedule (no counterpart in the reference, whose reactors run one
+ * per thread): 1 (default) = each advance call takes reactors in decreasing
+ * order of the internal steps they needed in the previous call; 0 = index
+ * order.  Results do not depend on it. */
+int32_t cg_reactors_set_schedule(int32_t batch, int32_t longest_first);
 /* Duration [ms] of the last integrator kernel of this batch, from HIP events
  * on the batch stream. */
 double cg_reactors_last_kernel_ms(int32_t batch);
 /* Duration [ms] of the last batched-rates kernel on `mech`. */
 double cg_rates_last_kernel_ms(int32_t mech);
+
+/* ---- learned surrogate (no counterpart in the reference) ---------------------
+ * An MLP (2+K inputs: T, ln P, ln max(Y_k, 1e-30); `depth` GELU layers of
+ * `hidden` units; 1+K outputs: the increments of T and Y_k over one step dt)
+ * with BF16 weights/activations and FP32 accumulation on the matrix cores,
+ * trained with Adam on state pairs (y0 -> y1 after dt) that the FP64
+ * integrator produced.  Rows of y0/y1 are reactor states [m, T, Y_0..Y_{K-1}]
+ * (device arrays, n x (K+2)); P per row.  It serves callers that advance many
+ * reactors by the same short step and accept a guarded approximation. */
+int32_t cg_surrogate_create(int32_t mech, double dt, int32_t hidden, int32_t depth, uint64_t seed);
+int32_t cg_surrogate_free(int32_t surrogate);
+int64_t cg_surrogate_nparams(int32_t surrogate);
+/* Feature/target means and deviations from (up to 2^20 of) the pairs. */
+int32_t cg_surrogate_fit_normalization(int32_t surrogate, int64_t n, const double* y0, const double* P,
+                                       const double* y1);
+/* One epoch of minibatch Adam (learning rate lr) over the pairs; mean loss out. */
+int32_t cg_surrogate_train_epoch(int32_t surrogate, int64_t n, const double* y0, const double* P,
+                                 const double* y1, int32_t batch, double lr, double* loss);
+/* Loss and parameter gradient of the pairs as one batch, no update (grad:
+ * host array of cg_surrogate_nparams floats, may be NULL). */
+int32_t cg_surrogate_loss_grad(int32_t surrogate, int64_t n, const double* y0, const double* P,
+                               const double* y1, double* loss, float* grad);
+/* y1 = y0 + net(y0) (device arrays); m carried over. */
+int32_t cg_surrogate_predict(int32_t surrogate, int64_t n, const double* y0, const double* P, double* y1);
+/* Parameters (host arrays): FP32 weights and biases layer by layer, weights
+ * row-major [out][in] with inputs/outputs padded to multiples of 64;
+ * normalisation = feature means, feature deviations, target means, target
+ * deviations (padded lengths). */
+int32_t cg_surrogate_get_params(int32_t surrogate, float* params);
+int32_t cg_surrogate_set_params(int32_t surrogate, const float* params);
+int32_t cg_surrogate_get_normalization(int32_t surrogate, double* norm);
+int32_t cg_surrogate_set_normalization(int32_t surrogate, const double* norm);
+/* cg_reactors_advance with surrogate steps where they pass an FP64 guard:
+ * reactors whose t_out is a whole number of surrogate steps away take them,
+ * each checked (physical range, element conservation, trapezoidal defect of
+ * the reactor ODE: |r_T| <= dT_tol K, |r_k| <= rtol_g max(|Y0_k|, |Y1_k|) +
+ * 1e-12); at the first rejected step the reactor continues with the BDF
+ * integrator from its last kept state.  counts[3] (may be NULL): surrogate
+ * steps kept, reactors advanced by the surrogate alone, reactors that ran
+ * the integrator. */
+int32_t cg_reactors_advance_surrogate(int32_t batch, int32_t surrogate, const double* t_out, double dT_tol,
+                                      double rtol_g, int64_t* counts);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,92 @@
+"""Batched reactor Jacobians on the GPU against the CPU oracle.
+
+The GPU kernel and the oracle both build d(ydot)/dy of the constant-pressure
+reactor (state order [m, T, Y], as Reactor::finiteDifferenceJacobian orders it,
+src/zeroD/Reactor.cpp:233-276): species columns analytic, the T column a
+one-sided difference with increment sqrt(eps) T.  The oracle's analytic
+Jacobian is itself pinned against central differences of its RHS
+(test_oracle.py).
+
+Bars (written per entry):
+  species columns: |dJ_ij| <= 1e-10 * max_j |J_ij|  (row scale; the GPU sums the
+      per-reaction terms with atomics, so only the rounding of the sum differs);
+  T column: |dJ_i1| <= 1e-5 * max(|J_i1|, |ydot_i| / T)  (a difference quotient
+      over an increment of 1.5e-8 T amplifies the last-bit differences of the two
+      RHS evaluations by ~1/1.5e-8);
+  m row and column exactly zero; species rows sum to zero column by column
+  (sum_k dY_k/dt = 0), to 1e-9 of the column's absolute sum in the species
+  columns (each entry is itself a sum of per-reaction terms that cancel through
+  sum_k nu_ki W_k = 0, so rounding is relative to those terms, not to the
+  entries) and 1e-6 in the T column (a difference quotient carries rounding of
+  order sqrt(eps) relative to its terms).
+"""
+import numpy as np
+import pytest
+
+from conftest import mech_file
+
+pytestmark = pytest.mark.gpu
+
+
+def states(info, n, seed):
+    g = np.random.default_rng(seed)
+    K = info.n_species
+    T = g.uniform(600.0, 2500.0, n)
+    P = 101325.0 * np.exp(g.uniform(np.log(0.1), np.log(50.0), n))
+    X = 10.0 ** g.uniform(-10, 0, (n, K))
+    X /= X.sum(axis=1, keepdims=True)
+    return T, P, info.mass_fractions_from_moles(X)
+
+
+def check_against_oracle(om, T, P, Y, J):
+    K = Y.shape[1]
+    worst_c = worst_t = 0.0
+    for s in range(len(T)):
+        y = np.concatenate(([1.0, T[s]], Y[s]))
+        Jo = om.jacobian(P[s], y, analytic=True)
+        f = om.rhs(P[s], y)
+        Jg = J[s]
+        assert np.all(Jg[0, :] == 0.0) and np.all(Jg[:, 0] == 0.0)
+        rs = np.abs(Jo[:, 2:]).max(axis=1, keepdims=True) + 1e-300
+        ec = float((np.abs(Jg[:, 2:] - Jo[:, 2:]) / rs).max())
+        st = np.maximum(np.abs(Jo[:, 1]), np.abs(f) / T[s]) + 1e-300
+        et = float((np.abs(Jg[:, 1] - Jo[:, 1]) / st).max())
+        worst_c, worst_t = max(worst_c, ec), max(worst_t, et)
+        colsum = np.abs(Jg[2:, :].sum(axis=0))
+        colabs = np.abs(Jg[2:, :]).sum(axis=0) + 1e-300
+        assert float((colsum / colabs)[2:].max()) <= 1e-9
+        assert float(colsum[1] / colabs[1]) <= 1e-6
+    return worst_c, worst_t
+
+
+@pytest.mark.parametrize("mech,n", [("h2o2", 70), ("gri30", 40), ("nDodecane", 6)])
+def test_jacobian_matches_oracle(mech, n, oracle_mod):
+    import cantera_amd as ca
+    from cantera_amd.workloads import MechInfo
+    info = MechInfo(mech_file(mech))
+    om = oracle_mod.Mech(mech_file(mech))
+    sol = ca.Solution(mech)
+    T, P, Y = states(info, n, 11)
+    J = sol.reactor_jacobians(T, P, Y.T)
+    assert J.shape == (n, info.n_species + 2, info.n_species + 2)
+    assert np.all(np.isfinite(J))
+    ec, et = check_against_oracle(om, T, P, Y, J)
+    print(f"{mech} n={n}: species columns {ec:.2e}, T column {et:.2e}", flush=True)
+    assert ec <= 1e-10
+    assert et <= 1e-5
+
+
+def test_jacobian_gri30_config_states(oracle_mod):
+    """The gri30 state distribution of the rates/Jacobian workload (CH4/O2/N2
+    majors at phi in [0.3, 3], minors 1e-12..1e-2), sampled at both ends of
+    the batch so the grid-stride loop's later iterations are covered."""
+    import cantera_amd as ca
+    from cantera_amd.workloads import config2_states
+    sol = ca.Solution("gri30")
+    om = oracle_mod.Mech(mech_file("gri30"))
+    T, P, Y = config2_states(sol, n=20000)
+    J = sol.reactor_jacobians(T, P, Y)
+    pick = np.r_[0:8, 19990:20000]
+    ec, et = check_against_oracle(om, T[pick], P[pick], Y[:, pick].T, J[pick])
+    assert ec <= 1e-10
+    assert et <= 1e-5

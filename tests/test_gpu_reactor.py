@@ -99,6 +99,41 @@ def test_gri30_batch_matches_oracle(oracle_mod):
     _compare("gri30", wl, 10, True, 1e-6, 1e-9, 1e-6, oracle_mod)
 
 
+def test_ndodecane_batch_matches_oracle(oracle_mod):
+    """N = 102: two state slots per lane, iteration matrix in LDS."""
+    from cantera_amd.workloads import MechInfo, config5
+    wl = config5(MechInfo(mech_file("nDodecane")), n=8, nsteps=5)
+    _compare("nDodecane", wl, 5, True, 1e-6, 1e-9, 1e-6, oracle_mod)
+
+
+def test_gri30_mixed_precision_within_tolerance(oracle_mod):
+    """FP32 iteration matrix (FP64 state, residual and error test): the Newton
+    corrections converge to the same FP64 solution, so trajectories stay within
+    the integrator tolerance bar of the CVODES-mode comparison."""
+    from cantera_amd.workloads import MechInfo, config3
+    wl = config3(MechInfo(mech_file("gri30")), n=48, nsteps=10)
+    _compare("gri30", wl, 10, True, 1e-4, 1e-7, 1e-3, oracle_mod, precision=1)
+
+
+def test_schedule_does_not_change_results():
+    """Longest-first and index-order work queues integrate the same reactors;
+    only the Jacobian's atomic summation order can differ between runs."""
+    import cantera_amd as ca
+    from cantera_amd.workloads import MechInfo, config3
+    wl = config3(MechInfo(mech_file("gri30")), n=96, nsteps=6)
+    sol = ca.Solution("gri30")
+    out = []
+    for lpt in (True, False):
+        rb = ca.ReactorBatch(sol, wl.T0, wl.P, wl.Y0, rtol=wl.rtol, atol=wl.atol)
+        rb.set_schedule(lpt)
+        for k in range(1, 7):
+            rb.advance(k * wl.dt)
+        assert np.all(rb.status == 0)
+        out.append(rb.state)
+    dT = np.abs(out[0][:, 1] - out[1][:, 1]) / out[1][:, 1]
+    assert dT.max() <= 1e-6
+
+
 def test_reactor_invariants():
     """Mass fractions keep summing to 1 and enthalpy is conserved (adiabatic,
     isobaric): checked through the elements and the state after 50 outputs."""

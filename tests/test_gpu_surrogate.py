@@ -1,0 +1,184 @@
+"""Learned surrogate: BF16 MFMA network, training, and the guarded reactor path.
+
+* Loss and gradient of the GPU network against a float64 numpy restatement of
+  the same network that rounds to BF16 exactly where the GPU does (features,
+  weights, activations, gradient operands): this pins the three GEMM operand
+  layouts (forward, weight gradient, input gradient) and the MFMA lane layout.
+  Bars: loss relative error <= 1e-4; gradient relative L2 error <= 2e-2 per
+  layer (an activation that lands on a BF16 rounding boundary may round the
+  other way, the only source of differences beyond FP32 summation order).
+* Training on integrator pairs lowers the loss.
+* The guarded advance keeps every reactor within 1 K of the plain FP64 BDF
+  result, and an untrained network is rejected by the guard so the results
+  are those of the integrator.
+"""
+import numpy as np
+import pytest
+
+from conftest import mech_file
+
+pytestmark = pytest.mark.gpu
+
+
+def bf16(x):
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    u = x.view(np.uint32).astype(np.uint64)
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    return (r & 0xFFFFFFFF).astype(np.uint32).view(np.float32)
+
+
+C_G, A_G = np.float32(0.7978845608028654), np.float32(0.044715)
+
+
+def gelu(z):
+    t = np.tanh(C_G * (z + A_G * z * z * z))
+    return np.float32(0.5) * z * (np.float32(1) + t)
+
+
+def gelu_grad(z):
+    t = np.tanh(C_G * (z + A_G * z * z * z))
+    return (np.float32(0.5) * (1 + t) + np.float32(0.5) * z * (1 - t * t) * C_G * (1 + 3 * A_G * z * z)).astype(np.float32)
+
+
+def pad(x):
+    return (x + 63) // 64 * 64
+
+
+def reference_loss_grad(params, norm, K, hidden, depth, y0, P, y1):
+    n_in, n_out = K + 2, K + 1
+    ip, op = pad(n_in), pad(n_out)
+    fi = [ip] + [hidden] * depth
+    fo = [hidden] * depth + [op]
+    W, b, off = [], [], 0
+    for l in range(depth + 1):
+        W.append(params[off:off + fo[l] * fi[l]].reshape(fo[l], fi[l]))
+        off += fo[l] * fi[l]
+        b.append(params[off:off + fo[l]])
+        off += fo[l]
+    Wb = [bf16(w).astype(np.float64) for w in W]
+    mu_i, sd_i, mu_o, sd_o = norm[:ip], norm[ip:2 * ip], norm[2 * ip:2 * ip + op], norm[2 * ip + op:]
+    rows = len(y0)
+    f = np.zeros((rows, ip))
+    f[:, 0] = y0[:, 1]
+    f[:, 1] = np.log(P)
+    f[:, 2:n_in] = np.log(np.maximum(y0[:, 2:], 1e-30))
+    z = np.clip((f - mu_i) / sd_i, -8, 8)
+    z[:, n_in:] = 0
+    X = bf16(z.astype(np.float32)).astype(np.float64)
+    T = np.zeros((rows, op), dtype=np.float32)
+    T[:, :n_out] = ((y1[:, 1:] - y0[:, 1:] - mu_o[:n_out]) / sd_o[:n_out]).astype(np.float32)
+    H, Z = [X], []
+    for l in range(depth):
+        z = (H[-1] @ Wb[l].T).astype(np.float32) + b[l]
+        Z.append(z)
+        H.append(bf16(gelu(z)).astype(np.float64))
+    out = (H[-1] @ Wb[depth].T).astype(np.float32) + b[depth]
+    d = (out - T)[:, :n_out].astype(np.float64)
+    scale = 1.0 / (rows * n_out)
+    loss = float((d * d).sum() * scale)
+    G = np.zeros((rows, op), dtype=np.float32)
+    G[:, :n_out] = np.float32(2 * scale) * (out - T)[:, :n_out]
+    G = bf16(G).astype(np.float64)
+    grads = [None] * (depth + 1)
+    for l in range(depth, -1, -1):
+        dW = (G.T @ H[l]).astype(np.float32)
+        db = G.sum(axis=0).astype(np.float32)
+        grads[l] = (dW, db)
+        if l > 0:
+            G = bf16((G @ Wb[l]).astype(np.float32) * gelu_grad(Z[l - 1])).astype(np.float64)
+    flat = np.concatenate([np.concatenate([g[0].ravel(), g[1]]) for g in grads])
+    return loss, flat, [fo[l] * fi[l] + fo[l] for l in range(depth + 1)]
+
+
+def synthetic_pairs(info, n, seed):
+    g = np.random.default_rng(seed)
+    K = info.n_species
+    X = 10.0 ** g.uniform(-8, 0, (n, K))
+    X /= X.sum(axis=1, keepdims=True)
+    Y = info.mass_fractions_from_moles(X)
+    y0 = np.concatenate([np.ones((n, 1)), g.uniform(900, 2200, (n, 1)), Y], axis=1)
+    P = 101325.0 * g.uniform(1, 20, n)
+    y1 = y0.copy()
+    y1[:, 1] += g.normal(0, 5.0, n)
+    y1[:, 2:] += 1e-4 * g.normal(0, 1, (n, K)) * Y
+    return y0, P, y1
+
+
+@pytest.mark.parametrize("mech,hidden,depth,n", [("h2o2", 64, 2, 300), ("gri30", 256, 4, 1000)])
+def test_loss_and_gradient_match_reference(mech, hidden, depth, n):
+    import cantera_amd as ca
+    from cantera_amd.surrogate import Surrogate
+    from cantera_amd.workloads import MechInfo
+    info = MechInfo(mech_file(mech))
+    sol = ca.Solution(mech)
+    sur = Surrogate(sol, dt=1e-6, hidden=hidden, depth=depth, seed=3)
+    y0, P, y1 = synthetic_pairs(info, n, 5)
+    sur.fit_normalization(y0, P, y1)
+    loss, g = sur.loss_grad(y0, P, y1)
+    rl, rg, sizes = reference_loss_grad(sur.params, sur.normalization, sol.n_species, hidden, depth, y0, P, y1)
+    assert abs(loss - rl) <= 1e-4 * rl
+    off = 0
+    for l, sz in enumerate(sizes):
+        a, b = g[off:off + sz].astype(np.float64), rg[off:off + sz].astype(np.float64)
+        err = np.linalg.norm(a - b) / (np.linalg.norm(b) + 1e-30)
+        print(f"{mech} layer {l}: gradient relative error {err:.2e}", flush=True)
+        assert err <= 2e-2
+        off += sz
+
+
+def test_training_lowers_loss():
+    import cantera_amd as ca
+    from cantera_amd.surrogate import Surrogate, generate_pairs
+    from cantera_amd.workloads import MechInfo, config1
+    info = MechInfo(mech_file("h2o2"))
+    sol = ca.Solution("h2o2")
+    wl = config1(info, n=256)
+    y0, P, y1 = generate_pairs(sol, wl.T0, wl.P, wl.Y0, dt=1e-6, nsteps=60)
+    sur = Surrogate(sol, dt=1e-6, hidden=128, depth=3, seed=1)
+    losses = sur.fit(y0, P, y1, epochs=25, batch=512, lr=1e-3)
+    print("losses", losses[0], losses[-1], flush=True)
+    assert losses[-1] < 0.2 * losses[0]
+
+
+def _plain_and_surrogate(sol, wl, sur, nsteps, dT_tol, rtol_g):
+    import cantera_amd as ca
+    a = ca.ReactorBatch(sol, wl.T0, wl.P, wl.Y0, rtol=wl.rtol, atol=wl.atol)
+    b = ca.ReactorBatch(sol, wl.T0, wl.P, wl.Y0, rtol=wl.rtol, atol=wl.atol)
+    tot = np.zeros(3, dtype=np.int64)
+    for k in range(1, nsteps + 1):
+        a.advance(k * 1e-6)
+        tot += b.advance_surrogate(k * 1e-6, sur, dT_tol=dT_tol, rtol_g=rtol_g)
+    return a.state, b.state, tot
+
+
+def test_guarded_advance_within_1K():
+    import cantera_amd as ca
+    from cantera_amd.surrogate import Surrogate, generate_pairs
+    from cantera_amd.workloads import MechInfo, config1
+    info = MechInfo(mech_file("h2o2"))
+    sol = ca.Solution("h2o2")
+    train = config1(info, n=512)
+    y0, P, y1 = generate_pairs(sol, train.T0, train.P, train.Y0, dt=1e-6, nsteps=80)
+    sur = Surrogate(sol, dt=1e-6, hidden=128, depth=3, seed=2)
+    sur.fit(y0, P, y1, epochs=30, batch=512, lr=1e-3)
+    wl = config1(info, n=96)
+    ya, yb, tot = _plain_and_surrogate(sol, wl, sur, 40, dT_tol=0.05, rtol_g=1e-3)
+    dT = np.abs(ya[:, 1] - yb[:, 1])
+    print(f"kept surrogate steps {tot[0]}, reactor-steps by surrogate alone {tot[1]}, by BDF {tot[2]}, "
+          f"max |dT| {dT.max():.3e} K", flush=True)
+    assert dT.max() <= 1.0
+    assert np.all(np.abs(yb[:, 2:].sum(axis=1) - 1) < 1e-5)
+
+
+def test_untrained_network_falls_back_to_integrator():
+    import cantera_amd as ca
+    from cantera_amd.surrogate import Surrogate
+    from cantera_amd.workloads import MechInfo, config3
+    info = MechInfo(mech_file("gri30"))
+    sol = ca.Solution("gri30")
+    wl = config3(info, n=64, nsteps=5)
+    sur = Surrogate(sol, dt=1e-6, hidden=256, depth=4, seed=9)
+    ya, yb, tot = _plain_and_surrogate(sol, wl, sur, 5, dT_tol=0.01, rtol_g=1e-4)
+    print("counts", tot, flush=True)
+    assert tot[1] == 0
+    assert np.max(np.abs(ya[:, 1] - yb[:, 1]) / ya[:, 1]) <= 1e-6
