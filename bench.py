@@ -1,14 +1,20 @@
 """Benchmark: GRI-3.0 constant-pressure reactor integration throughput.
 
-Workload: 65,536 independent gri30 IdealGasConstPressureReactors, T0 ~ U[900,
-2000] K, CH4/air at phi ~ U[0.5, 3.0], P ~ 1 atm x exp(U[ln 1, ln 40]), each
-advanced 100 times by its own dt ~ exp(U[ln 1e-6, ln 1e-4]) s, rtol 1e-8,
-atol 1e-15, FP64.  A "step" is one ReactorNet::advance of every reactor by its
-dt (65,536 reactor-steps).
+Headline workload (default, ``--workload gri30-reactors``): 65,536 independent
+gri30 IdealGasConstPressureReactors, T0 ~ U[900, 2000] K, CH4/air at
+phi ~ U[0.5, 3.0], P ~ 1 atm x exp(U[ln 1, ln 40]), each advanced 100 times by
+its own dt ~ exp(U[ln 1e-6, ln 1e-4]) s, rtol 1e-8, atol 1e-15, FP64.  A "step"
+is one ReactorNet::advance of every reactor by its dt (65,536 reactor-steps).
 
 Warm-up steps run on a separate batch with the same inputs, so the timed
 region covers steps 1..K of a fresh batch from t = 0, with all inputs already
 resident in HBM.  One JSON line is printed.
+
+Other workloads of the same path, one JSON line each:
+  gri30-rates        1,048,576 gri30 states, net production rates (states/s)
+  gri30-jacobian     1,048,576 gri30 states, reactor Jacobians (states/s)
+  ndodecane-reactors 262,144 n-dodecane reactors x 100 steps of 1e-6 s
+  h2o2-reactors      1,024 h2o2 reactors x 100 outputs every 1e-5 s
 """
 from __future__ import annotations
 
@@ -27,6 +33,7 @@ sys.path.insert(0, ROOT)
 FP64_PEAK_TFLOPS = 78.6      # MI355X FP64 (vector and matrix)
 HBM_PEAK_GBS = 8000.0
 METRIC = "GRI30 reactor-steps/s: plain FP64 BDF vs mixed+surrogate at ≤1 K T error"
+PIPE = "FP64 (MI355X vector FMA and MFMA share the 78.6 TFLOP/s FP64 peak)"
 
 # Algorithmic cost convention: every FP64 add/sub/mul/div counts 1, an FMA 2,
 # and each transcendental (exp, log, log10, exp10, pow) counts TRANS_FLOPS.
@@ -34,8 +41,8 @@ TRANS_FLOPS = 20.0
 
 
 def rhs_cost(mech_path):
-    """(flops, transcendentals) of one reactor RHS evaluation, from the tables."""
-    K = I = 0
+    """(flops, transcendentals, K) of one reactor RHS evaluation, from the tables."""
+    K = 0
     fl = 0.0
     tr = 0.0
     nnz = 0
@@ -64,7 +71,7 @@ def rhs_cost(mech_path):
                     tr += 1
                     if typ == 3:
                         fl += 20                  # Fcent, f1, lgf
-                        tr += 5                   # 3 exp (Fcent) + 2 log10 + exp10 - counted 5 (T2 term optional)
+                        tr += 5                   # exps of Fcent, 2 log10, exp10
                 fl += nr + (np_ if rev else 0) + 3  # concentration products, net
                 nnz += nr + np_
     fl += 2 * nnz                                 # wdot gather
@@ -88,83 +95,90 @@ def flop_model(mech_path, stats):
     return float(W), F_rhs
 
 
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "bdf_advance_gri30_65536_traffic.md")
-
-
-def measured_traffic(n):
-    """HBM bytes per integrator launch from the committed rocprofv3 --pmc summary.
+def measured_traffic(name, n):
+    """HBM bytes per launch of the dominant kernel from a committed rocprofv3
+    --pmc summary (profiles/<name>_traffic.md).
 
     The summary holds FETCH_SIZE and WRITE_SIZE (KiB per dispatch) of the
-    integrator kernel and of a plain copy kernel of known size; each counter is
-    scaled by (copy bytes / copy counter).  None when the summary is absent or
-    was taken at another batch size.
+    kernel and of a plain copy kernel of known size; each counter is scaled by
+    (copy bytes / copy counter).  None when the summary is absent or was taken
+    at another batch size.
     """
     try:
         kv = {}
-        with open(TRAFFIC_FILE) as f:
+        with open(os.path.join(ROOT, "profiles", f"{name}_traffic.md")) as f:
             for line in f:
                 if line.startswith("traffic_") and ":" in line:
                     k, v = line.split(":", 1)
                     kv[k.strip()] = float(v.split()[0])
-        if int(kv["traffic_n_reactors"]) != n:
+        if int(kv["traffic_n"]) != n:
             return None
         return kv["traffic_bytes_per_launch"]
     except (OSError, KeyError, ValueError):
         return None
 
 
-def cpu_baseline(wl, steps, sample, threads):
+def cpu_threads():
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return max(1, min(threads, 16))
+
+
+def cpu_reactors(mech_path, wl, steps, sample, threads):
     from oracle.oracle import Mech
-    om = Mech(os.path.join(ROOT, "cantera_amd", "mechanisms", "gri30.mech"))
+    om = Mech(mech_path)
     S = min(sample, len(wl.T0))
     t0 = time.perf_counter()
-    _, st = om.run_reactors(wl.T0[:S], wl.P[:S], wl.Y0[:S], wl.dt[:S], steps, wl.rtol, wl.atol,
-                            analytic=False, nthreads=threads, keep_states=False)
+    om.run_reactors(wl.T0[:S], wl.P[:S], wl.Y0[:S], wl.dt[:S], steps, wl.rtol, wl.atol,
+                    analytic=False, nthreads=threads, keep_states=False)
     el = time.perf_counter() - t0
     return {"value": S * steps / el, "unit": "reactor-steps/s", "cores": threads, "kind": "port",
             "sample": f"{S} of the {len(wl.T0)} reactors x {steps} steps, CVODES-method BDF with "
                       f"difference-quotient Jacobian (oracle), OpenMP schedule(static,1), {el:.1f} s"}
 
 
-def main(argv=None):
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=100)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--n", type=int, default=65536)
-    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
-    ap.add_argument("--waves-per-cu", type=int, default=0)
-    ap.add_argument("--schedule", choices=["longest-first", "index"], default="longest-first")
-    ap.add_argument("--cpu-sample", type=int, default=16384)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
-                    help="after the timed region, run one device-to-device copy (blit kernel) of MIB MiB "
-                         "(known HBM bytes, to scale FETCH_SIZE/WRITE_SIZE counters)")
-    a = ap.parse_args(argv)
-    if a.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (the reactors are independent, "
-              "so N GPUs would each run their own shard).", file=sys.stderr)
-        return 2
-
+def reactors(a):
     import cantera_amd as ca
     from cantera_amd._lib import check, lib
-    from cantera_amd.workloads import config3
+    from cantera_amd.workloads import config1, config3, config5
 
     L = lib()
-    sol = ca.Solution("gri30")
-    wl = config3(sol, n=a.n, nsteps=a.steps)
+    mech = {"gri30-reactors": "gri30", "ndodecane-reactors": "nDodecane", "h2o2-reactors": "h2o2"}[a.workload]
+    sol = ca.Solution(mech)
+    steps = a.steps or 100
+    if mech == "gri30":
+        n = a.n or 65536
+        wl = config3(sol, n=n, nsteps=steps)
+        metric = METRIC
+        desc = (f"gri30 ({sol.n_species} species, {sol.n_reactions} reactions) IdealGasConstPressureReactor, {n} "
+                f"reactors x {steps} advance "
+                f"steps, per-reactor dt log-uniform in [1e-6, 1e-4] s, T0 U[900,2000] K, CH4/air phi U[0.5,3], "
+                f"P 1-40 atm, rtol 1e-8 atol 1e-15")
+    elif mech == "nDodecane":
+        n = a.n or 262144
+        wl = config5(sol, n=n, nsteps=steps)
+        metric = "nDodecane reactor-steps/s (FP64 BDF)"
+        desc = (f"nDodecane ({sol.n_species} species, {sol.n_reactions} reactions) IdealGasConstPressureReactor, "
+                f"{n} reactors x {steps} "
+                f"advance steps of 1e-6 s, T0 U[700,2200] K, c12h26/air phi U[0.3,3], P 20-60 atm, rtol 1e-8 "
+                f"atol 1e-15")
+    else:
+        n = a.n or 1024
+        wl = config1(sol, n=n)
+        metric = "h2o2 reactor-steps/s (FP64 BDF)"
+        desc = (f"h2o2 ({sol.n_species} species, {sol.n_reactions} reactions) IdealGasConstPressureReactor, {n} "
+                f"reactors x {steps} outputs "
+                f"every 1e-5 s, T0 U[1000,1800] K, H2/air phi U[0.5,2], 1 atm, rtol 1e-8 atol 1e-15")
     prec = ca.CG_FP64 if a.precision == "fp64" else ca.CG_MIXED
-    n = a.n
 
     # all output times resident in HBM before the timed region
-    touts = np.ascontiguousarray(np.outer(np.arange(1, a.steps + 1, dtype=np.float64), wl.dt))
+    touts = np.ascontiguousarray(np.outer(np.arange(1, steps + 1, dtype=np.float64), wl.dt))
     d_tout = L.cg_dev_alloc(touts.nbytes)
     check(L.cg_memcpy_h2d(d_tout, touts.ctypes.data_as(C.c_void_p), touts.nbytes))
 
     if a.warmup > 0:
         wb = ca.ReactorBatch(sol, wl.T0, wl.P, wl.Y0, rtol=wl.rtol, atol=wl.atol, precision=prec)
         wb.set_options(0, a.waves_per_cu)
-        for k in range(a.warmup):
+        for k in range(min(a.warmup, steps)):
             check(L.cg_reactors_advance_device(wb.handle, C.c_void_p(d_tout + k * n * 8)))
         check(L.cg_device_synchronize())
         del wb
@@ -172,13 +186,14 @@ def main(argv=None):
     rb = ca.ReactorBatch(sol, wl.T0, wl.P, wl.Y0, rtol=wl.rtol, atol=wl.atol, precision=prec)
     rb.set_options(0, a.waves_per_cu)
     rb.set_schedule(a.schedule == "longest-first")
+    rb.set_matrix_storage(a.matrix_storage)
     check(L.cg_device_synchronize())
     kernel_ms = []
     t0 = time.perf_counter()
-    for k in range(a.steps):
+    for k in range(steps):
         check(L.cg_reactors_advance_device(rb.handle, C.c_void_p(d_tout + k * n * 8)))
         kernel_ms.append(rb.last_kernel_ms())
-        print(f"step {k + 1}/{a.steps}: kernel {kernel_ms[-1]:.2f} ms", file=sys.stderr, flush=True)
+        print(f"step {k + 1}/{steps}: kernel {kernel_ms[-1]:.2f} ms", file=sys.stderr, flush=True)
     check(L.cg_device_synchronize())
     elapsed = time.perf_counter() - t0
     bad = int((rb.status != 0).sum())
@@ -188,49 +203,134 @@ def main(argv=None):
     stats = rb.stats
     W, F_rhs = flop_model(sol.path, stats)
     avg_launch_s = float(np.mean(kernel_ms)) / 1e3
-    achieved = W / a.steps / avg_launch_s / 1e12
-    # HBM bytes: state records, saved Jacobian and factors per call
-    value = n * a.steps / elapsed
+    achieved = W / steps / avg_launch_s / 1e12
     out = {
-        "metric": METRIC,
-        "value": value,
+        "metric": metric,
+        "value": n * steps / elapsed,
         "unit": "reactor-steps/s",
         "n_gpus": 1,
-        "steps": a.steps,
+        "steps": steps,
         "warmup": a.warmup,
-        "ms_per_step": elapsed / a.steps * 1e3,
+        "ms_per_step": elapsed / steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64" if a.precision == "fp64" else "f64 state/residual, f32 iteration matrix",
         "data": "synthetic",
         "config": {
-            "workload": f"gri30 (53 species, 325 reactions) IdealGasConstPressureReactor, {n} reactors x "
-                        f"{a.steps} advance steps, per-reactor dt log-uniform in [1e-6, 1e-4] s, "
-                        f"T0 U[900,2000] K, CH4/air phi U[0.5,3], P 1-40 atm, rtol 1e-8 atol 1e-15",
+            "workload": desc,
             "precision": a.precision,
             "parallelism": "1 GPU, one wavefront per reactor, persistent work-queue kernel",
             "schedule": a.schedule,
+            "matrix_storage": a.matrix_storage,
         },
         "roofline": {
             "bound": "mfma",
-            "pipe": "FP64 (MI355X vector FMA and MFMA share the 78.6 TFLOP/s FP64 peak)",
+            "pipe": PIPE,
             "achieved": achieved,
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / FP64_PEAK_TFLOPS,
-            "traffic": measured_traffic(n),
-            "kernel": "bdf_advance_kernel",
+            "traffic": measured_traffic(f"bdf_advance_{mech}", n),
+            "kernel": "bdf_advance",
             "avg_launch_ms": avg_launch_s * 1e3,
-            "flops_per_launch": W / a.steps,
+            "flops_per_launch": W / steps,
         },
         "integrator": {k: float(np.mean(v)) for k, v in stats.items() if k != "order"},
     }
     if not a.no_cpu:
-        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        threads = max(1, min(threads, 16))
-        out["cpu_baseline"] = cpu_baseline(wl, a.steps, a.cpu_sample, threads)
+        sample = a.cpu_sample or {"gri30": 16384, "nDodecane": 512, "h2o2": 1024}[mech]
+        out["cpu_baseline"] = cpu_reactors(sol.path, wl, steps, sample, cpu_threads())
     L.cg_dev_free(d_tout)
+    copy_calib(a, L, check)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def batch_kernels(a):
+    """Batched rates or Jacobians over gri30 states resident in HBM."""
+    import cantera_amd as ca
+    from cantera_amd._lib import check, lib
+    from cantera_amd.workloads import config2_states
+
+    L = lib()
+    sol = ca.Solution("gri30")
+    n = a.n or 1_048_576
+    steps = a.steps or 20
+    K = sol.n_species
+    N = K + 2
+    T, P, Y = config2_states(sol, n=n)
+    jac = a.workload == "gri30-jacobian"
+    out_doubles = n * N * N if jac else n * K
+    d_in = L.cg_dev_alloc((2 + K) * n * 8)
+    d_out = L.cg_dev_alloc(out_doubles * 8)
+    for off, arr in ((0, T), (n, P), (2 * n, Y)):
+        arr = np.ascontiguousarray(arr)
+        check(L.cg_memcpy_h2d(d_in + off * 8, arr.ctypes.data_as(C.c_void_p), arr.nbytes))
+    dT, dP, dY = d_in, d_in + n * 8, d_in + 2 * n * 8
+    call = L.cg_jac_batch_device if jac else L.cg_rates_batch_device
+    last = L.cg_jac_last_kernel_ms if jac else L.cg_rates_last_kernel_ms
+    for _ in range(a.warmup):
+        check(call(sol.handle, n, dT, dP, dY, d_out, ca.CG_FP64, None))
+    check(L.cg_device_synchronize())
+    kernel_ms = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        check(call(sol.handle, n, dT, dP, dY, d_out, ca.CG_FP64, None))
+        kernel_ms.append(last(sol.handle))
+    check(L.cg_device_synchronize())
+    elapsed = time.perf_counter() - t0
+    fl, tr, _ = rhs_cost(sol.path)
+    F_rates = fl + TRANS_FLOPS * tr
+    avg_s = float(np.mean(kernel_ms)) / 1e3
+    name = "jac_batch_gri30" if jac else "rates_tile_gri30"
+    if jac:
+        # HBM-bound: T, P, Y in; the N x N Jacobian out
+        alg_bytes = n * (8 * (2 + K) + 8 * N * N)
+        roof = {"bound": "hbm", "achieved": alg_bytes / avg_s / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": alg_bytes / avg_s / 1e9 / HBM_PEAK_GBS, "traffic": measured_traffic(name, n),
+                "kernel": "jac_batch_kernel", "avg_launch_ms": avg_s * 1e3, "bytes_per_launch": alg_bytes}
+        metric = "gri30 reactor Jacobians/s (FP64, 55x55 dense per state)"
+    else:
+        flops = n * F_rates
+        roof = {"bound": "mfma", "pipe": PIPE, "achieved": flops / avg_s / 1e12, "peak": FP64_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": flops / avg_s / 1e12 / FP64_PEAK_TFLOPS,
+                "traffic": measured_traffic(name, n), "kernel": "rates_tile_kernel", "avg_launch_ms": avg_s * 1e3,
+                "flops_per_launch": flops, "hbm_bytes_algorithmic": n * 8 * (2 + 2 * K)}
+        metric = "gri30 net production rates, states/s (FP64)"
+    out = {
+        "metric": metric, "value": n * steps / elapsed, "unit": "states/s", "n_gpus": 1, "steps": steps,
+        "warmup": a.warmup, "ms_per_step": elapsed / steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"gri30 states, {n} per launch: T U[300,2500] K, P 0.1-50 atm, CH4/O2/N2 majors at "
+                               f"phi U[0.3,3], minors X 1e-12..1e-2; species-major SoA in HBM"},
+        "roofline": roof,
+    }
+    if not a.no_cpu:
+        from oracle.oracle import Mech
+        om = Mech(sol.path)
+        th = cpu_threads()
+        S = min(n, a.cpu_sample or (4096 if jac else 262144))
+        Ys = np.ascontiguousarray(Y[:, :S])
+        t1 = time.perf_counter()
+        if jac:
+            for s in range(S):
+                om.jacobian(P[s], np.concatenate(([1.0, T[s]], Ys[:, s])), analytic=True)
+            th = 1
+        else:
+            om.rates_batch(T[:S], P[:S], Ys, nthreads=th)
+        el = time.perf_counter() - t1
+        out["cpu_baseline"] = {"value": S / el, "unit": "states/s", "cores": th, "kind": "port",
+                               "sample": f"first {S} of the {n} states, oracle "
+                                         f"{'analytic Jacobian, one thread' if jac else 'rates, OpenMP'}, {el:.1f} s"}
+    L.cg_dev_free(d_in)
+    L.cg_dev_free(d_out)
+    copy_calib(a, L, check)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def copy_calib(a, L, check):
     if a.copy_calib > 0:
         nb = a.copy_calib << 20
         src, dst = L.cg_dev_alloc(nb), L.cg_dev_alloc(nb)
@@ -238,8 +338,34 @@ def main(argv=None):
         check(L.cg_device_synchronize())
         L.cg_dev_free(src)
         L.cg_dev_free(dst)
-    print(json.dumps(out), flush=True)
-    return 0
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--workload", default="gri30-reactors",
+                    choices=["gri30-reactors", "gri30-rates", "gri30-jacobian", "ndodecane-reactors", "h2o2-reactors"],
+                    help="gri30-reactors is the headline line; the others measure the other workloads of the path")
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=None)
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--waves-per-cu", type=int, default=0)
+    ap.add_argument("--schedule", choices=["longest-first", "index"], default="longest-first")
+    ap.add_argument("--matrix-storage", choices=["default", "lds", "global"], default="default")
+    ap.add_argument("--cpu-sample", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
+                    help="at the end, run one device-to-device copy (blit kernel) of MIB MiB "
+                         "(known HBM bytes, to scale FETCH_SIZE/WRITE_SIZE counters)")
+    a = ap.parse_args(argv)
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (the reactors are independent, "
+              "so N GPUs would each run their own shard).", file=sys.stderr)
+        return 2
+    if a.workload in ("gri30-rates", "gri30-jacobian"):
+        return batch_kernels(a)
+    return reactors(a)
 
 
 if __name__ == "__main__":

@@ -152,6 +152,10 @@ class ReactorBatch:
         self.time = t_out.copy()
         return np.array(counts[:], dtype=np.int64)
 
+    def set_matrix_storage(self, storage):
+        """'lds' or 'global' (or 'default') iteration-matrix storage for 32 < N <= 64."""
+        check(lib().cg_reactors_set_matrix_storage(self._h, {"default": 0, "lds": 1, "global": 2}[storage]))
+
     def set_schedule(self, longest_first=True):
         check(lib().cg_reactors_set_schedule(self._h, 1 if longest_first else 0))
 

@@ -56,6 +56,7 @@ SIGNATURES = [
     ("cg_reactors_get_control", C.c_int32, [C.c_int32, _dp]),
     ("cg_reactors_set_options", C.c_int32, [C.c_int32, C.c_int32, C.c_int32]),
     ("cg_reactors_set_schedule", C.c_int32, [C.c_int32, C.c_int32]),
+    ("cg_reactors_set_matrix_storage", C.c_int32, [C.c_int32, C.c_int32]),
     ("cg_reactors_last_kernel_ms", C.c_double, [C.c_int32]),
     ("cg_rates_last_kernel_ms", C.c_double, [C.c_int32]),
     ("cg_memcpy_d2d", C.c_int32, [C.c_void_p, C.c_void_p, C.c_int64]),

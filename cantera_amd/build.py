@@ -16,7 +16,8 @@ OUT = os.path.join(HERE, "libcantera_amd.so")
 OBJDIR = os.path.join(HERE, "_build")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["mech.cpp", "api.cpp", "rates.hip", "jacobian.hip", "order.hip", "bdf.hip", "bdf_r16.hip", "bdf_r32.hip", "bdf_l1.hip",
+SOURCES = ["mech.cpp", "api.cpp", "rates.hip", "jacobian.hip", "order.hip", "bdf.hip", "bdf_r16.hip", "bdf_r32.hip", "bdf_lr56.hip", "bdf_lr64.hip",
+           "bdf_gr56.hip", "bdf_gr64.hip",
            "bdf_l2.hip", "surrogate.hip", "surrogate_reactors.hip"]
 
 COMMON = [

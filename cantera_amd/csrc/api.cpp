@@ -65,6 +65,7 @@ struct Batch {
     void* sort_tmp = nullptr;
     size_t sort_tmp_bytes = 0;
     bool lpt_on = true, have_order = false;
+    bool global_matrix = true;   // 32 < N <= 64: iteration matrix in the global factor slot (else LDS)
     size_t rec_stride = 0, j_stride = 0, lu_stride = 0, piv_stride = 0;
     hipStream_t stream = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -256,7 +257,7 @@ int32_t cg_rates_batch_device(int32_t h, int64_t n, const double* T, const doubl
         const size_t lds = rates_lds_bytes(m, mo.ns);
         const void* kp = rates_kernel_ptr(mo.ns);
         set_lds_limit(kp, lds);
-        const int grid = grid_for(kp, lds, 0, (n + 63) / 64);
+        const int grid = grid_for(kp, lds, 0, n);
         hipStream_t s = static_cast<hipStream_t>(stream);
         check(hipEventRecord(mo.e0, s), "hipEventRecord");
         check(rates_launch(m, mo.ns, n, T, P, Y, wdot, grid, lds, s), "rates kernel launch");
@@ -382,7 +383,7 @@ int32_t cg_reactors_create(int32_t h, int64_t n, const double* T0, const double*
         const int NP = 64 * b->ns;
         b->rec_stride = 7 * static_cast<size_t>(NP) + C_COUNT;
         b->j_stride = static_cast<size_t>(m.N) * m.N;
-        b->lu_stride = static_cast<size_t>(m.N) * b->ld;
+        b->lu_stride = bdf_lu_elems(m.N, b->ld);
         b->piv_stride = 2 * static_cast<size_t>(NP);
         if (bdf_variant(m.N) < 0) throw std::runtime_error("mechanisms with more than 126 species are not supported");
         const size_t es = b->fp32 ? 4 : 8;
@@ -418,8 +419,8 @@ int32_t cg_reactors_create(int32_t h, int64_t n, const double* T0, const double*
             c[C_ETA] = 1.0;
         }
         check(hipMemcpy(b->rec, rec.data(), rec.size() * 8, hipMemcpyHostToDevice), "hipMemcpy(records)");
-        b->lds = bdf_lds_bytes(m, b->ld, b->fp32);
-        set_lds_limit(bdf_kernel_ptr(m.N, b->fp32), b->lds);
+        b->lds = bdf_lds_bytes(m, b->ld, b->fp32, b->global_matrix);
+        set_lds_limit(bdf_kernel_ptr(m.N, b->fp32, b->global_matrix), b->lds);
         std::lock_guard<std::mutex> lk(g_mu);
         g_batches.push_back(std::move(b));
         return static_cast<int32_t>(g_batches.size() - 1);
@@ -441,6 +442,18 @@ int32_t cg_reactors_set_options(int32_t h, int32_t max_steps, int32_t waves_per_
         Batch& b = get_batch(h);
         if (max_steps > 0) b.max_steps = max_steps;
         if (waves_per_cu >= 0) b.waves_per_cu = waves_per_cu;
+        return 0;
+    });
+}
+
+int32_t cg_reactors_set_matrix_storage(int32_t h, int32_t storage) {
+    return guarded([&] {
+        Batch& b = get_batch(h);
+        if (storage < 0 || storage > 2) throw std::runtime_error("storage must be 0 (default), 1 (LDS) or 2 (global)");
+        b.global_matrix = storage != 1;
+        const DevMech& m = b.mech->dm->view();
+        b.lds = bdf_lds_bytes(m, b.ld, b.fp32, b.global_matrix);
+        set_lds_limit(bdf_kernel_ptr(m.N, b.fp32, b.global_matrix), b.lds);
         return 0;
     });
 }
@@ -474,9 +487,9 @@ static void launch_advance(Batch& b, const int* list = nullptr, int count = 0) {
     int* order = b.lpt + 4 * static_cast<size_t>(b.n);
     a.order = list ? list : ((b.lpt_on && b.have_order) ? order : nullptr);
     a.queue = b.queue;
-    const int grid = grid_for(bdf_kernel_ptr(b.N, b.fp32), b.lds, b.waves_per_cu, b.n);
+    const int grid = grid_for(bdf_kernel_ptr(b.N, b.fp32, b.global_matrix), b.lds, b.waves_per_cu, b.n);
     check(hipEventRecord(b.e0, b.stream), "hipEventRecord");
-    check(bdf_launch(a, b.fp32, grid, b.lds, b.stream), "integrator kernel launch");
+    check(bdf_launch(a, b.fp32, b.global_matrix, grid, b.lds, b.stream), "integrator kernel launch");
     check(hipEventRecord(b.e1, b.stream), "hipEventRecord");
     // cost keys of this call -> processing order of the next
     const size_t n = static_cast<size_t>(b.n);
@@ -609,6 +622,7 @@ int32_t cg_surrogate_create(int32_t mech, double dt, int32_t hidden, int32_t dep
         auto obj = std::make_unique<SurrogateObj>();
         obj->mech = &mo;
         obj->net = std::make_unique<SurrogateNet>(mo.dm->view().K, dt, hidden, depth, seed);
+        obj->net->set_elements(mo.dm->view().comp, mo.dm->view().rmw, mo.dm->view().E);
         std::lock_guard<std::mutex> lk(g_mu);
         g_surr.push_back(std::move(obj));
         return static_cast<int32_t>(g_surr.size() - 1);

@@ -41,12 +41,16 @@ struct BdfArgs {
 };
 
 // kernel variant for a state size N: 16/32 = matrix in registers with that many
-// columns, 1/2 = matrix in LDS with 1 or 2 state slots per lane (N <= 64 / 128),
-// -1 = unsupported
+// columns, 56/64 = matrix in LDS factored/solved in registers, 2 = matrix in
+// LDS with 2 state slots per lane (N <= 128), -1 = unsupported
 int bdf_variant(int N);
-size_t bdf_lds_bytes(const DevMech& m, int ld, bool fp32);
-hipError_t bdf_launch(const BdfArgs& a, bool fp32, int grid, size_t lds, hipStream_t stream);
-const void* bdf_kernel_ptr(int N, bool fp32);
+// per-reactor storage (elements) of the saved factored matrix
+size_t bdf_lu_elems(int N, int ld);
+// global: for 32 < N <= 64, keep the iteration matrix in the reactor's
+// global-memory factor slot instead of LDS
+size_t bdf_lds_bytes(const DevMech& m, int ld, bool fp32, bool global);
+hipError_t bdf_launch(const BdfArgs& a, bool fp32, bool global, int grid, size_t lds, hipStream_t stream);
+const void* bdf_kernel_ptr(int N, bool fp32, bool global);
 
 // batched rate evaluation (species-major SoA in and out)
 size_t rates_lds_bytes(const DevMech& m, int ns);

@@ -117,6 +117,7 @@ struct Integrator {
             c.lu_dirty = 0;
         }
         const bool have_lu = uni(x[C_HAVE_LU]) != 0.0;
+        ls.bind(a.template lu<MT>() + r * a.lu_stride, a.piv_save + r * a.piv_stride);
         wave_sync();
         if (have_lu)
             ls.restore(a.template lu<MT>() + r * a.lu_stride, a.piv_save + r * a.piv_stride, N);
@@ -865,8 +866,8 @@ struct Integrator {
 
 // LDS per wavefront: state broadcast, species/reaction scratch, control block,
 // then the linear-system area (LDS policy only).
-__host__ __device__ inline size_t bdf_scratch_doubles(int K, int I, int nsub, int ns) {
-    return static_cast<size_t>(kWave) * ns * 4 + K + K + I + nsub + 2 * K + kCtlDoubles;
+__host__ __device__ inline size_t bdf_scratch_doubles(int K, int I, int ns) {
+    return static_cast<size_t>(kWave) * ns * 4 + K + K + I + K + 2 * K + kCtlDoubles;
 }
 
 template <int NS, class LS, typename MT>
@@ -881,7 +882,7 @@ __device__ __forceinline__ void bdf_advance_body(const BdfArgs& a) {
     L.C = off;   off += m.K;
     L.MU = off;  off += m.K;
     L.ROP = off; off += m.I;
-    L.SUB = off; off += m.nsub;
+    L.WD = off;  off += m.K;
     L.W2 = off;  off += 2 * m.K;
     double* base = lds_base() + off;
     Ctl& c = *reinterpret_cast<Ctl*>(base);

@@ -1,0 +1,22 @@
+// Integrator kernels with the iteration matrix kept in LDS, factored and solved in registers (56 < N <= 64).
+#include "bdf_integrator.hpp"
+
+namespace cg {
+
+template <typename MT>
+__global__ __launch_bounds__(64) void bdf_advance_lr64(BdfArgs a) {
+    bdf_advance_body<1, LdsRegLinSys<64, MT>, MT>(a);
+}
+
+const void* bdf_kernel_lr64(bool fp32) {
+    return fp32 ? reinterpret_cast<const void*>(&bdf_advance_lr64<float>)
+                : reinterpret_cast<const void*>(&bdf_advance_lr64<double>);
+}
+
+hipError_t bdf_launch_lr64(const BdfArgs& a, bool fp32, int grid, size_t lds, hipStream_t stream) {
+    if (fp32) hipLaunchKernelGGL(bdf_advance_lr64<float>, dim3(grid), dim3(64), lds, stream, a);
+    else hipLaunchKernelGGL(bdf_advance_lr64<double>, dim3(grid), dim3(64), lds, stream, a);
+    return hipGetLastError();
+}
+
+}  // namespace cg

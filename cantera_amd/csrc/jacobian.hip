@@ -9,7 +9,7 @@
 // row and column are zero.
 //
 // One wavefront per state: RHS at the state (out-of-line kinetics), then the
-// Jacobian assembly writes straight into the state's output block.
+// Jacobian assembly (in LDS when it fits, else in the state's output block).
 #include <hip/hip_runtime.h>
 
 #include "bdf.hpp"
@@ -19,7 +19,10 @@ namespace cg {
 
 namespace {
 
-template <int NS>
+// kLdsJ: assemble J in LDS (FP64 LDS atomics for the per-reaction terms), then
+// write it out with coalesced stores; otherwise assemble it in place in HBM
+// (L2 atomics) -- used when N x N doubles do not fit next to the scratch.
+template <int NS, bool kLdsJ>
 __global__ __launch_bounds__(64) void jac_batch_kernel(DevMech m, long n, const double* __restrict__ T,
                                                        const double* __restrict__ P, const double* __restrict__ Y,
                                                        double* __restrict__ J) {
@@ -33,10 +36,11 @@ __global__ __launch_bounds__(64) void jac_batch_kernel(DevMech m, long n, const 
     L.C = off;   off += m.K;
     L.MU = off;  off += m.K;
     L.ROP = off; off += m.I;
-    L.SUB = off; off += m.nsub;
-    L.W2 = off;
+    L.WD = off;  off += m.K;
+    L.W2 = off;  off += 2 * m.K;
     const WaveScratch S = scratch_view(L);
-    const size_t NN = static_cast<size_t>(m.N) * m.N;
+    double* Jl = lds_base() + off;
+    const int NN = m.N * m.N;
     for (long st = blockIdx.x; st < n; st += gridDim.x) {
         const double Ts = T[st];
         const double Ps = uni(P[st]);
@@ -59,28 +63,39 @@ __global__ __launch_bounds__(64) void jac_batch_kernel(DevMech m, long n, const 
             f[s] = (i < m.N) ? S.F[i] : 0.0;
         }
         wave_sync();
-        reactor_jacobian<NS, true>(m, S, Ps, y, f, J + static_cast<size_t>(st) * NN);
+        double* Jg = J + static_cast<size_t>(st) * NN;
+        if (kLdsJ) {
+            reactor_jacobian<NS, true>(m, S, Ps, y, f, Jl);
+            for (int e = lane; e < NN; e += kWave) Jg[e] = Jl[e];
+            wave_sync();
+        } else {
+            reactor_jacobian<NS, true>(m, S, Ps, y, f, Jg);
+        }
     }
 }
 
 }  // namespace
 
+// J goes to LDS for one state slot per lane (N <= 64); larger systems assemble in HBM
+static bool jac_in_lds(int ns) { return ns == 1; }
+
 size_t jac_lds_bytes(const DevMech& m, int ns) {
-    const size_t d = static_cast<size_t>(kWave) * ns * 4 + 2 * m.K + m.I + m.nsub + 2 * m.K;
+    size_t d = static_cast<size_t>(kWave) * ns * 4 + 2 * m.K + m.I + m.K + 2 * m.K;
+    if (jac_in_lds(ns)) d += static_cast<size_t>(m.N) * m.N;
     return d * sizeof(double);
 }
 
 hipError_t jac_launch(const DevMech& m, int ns, long n, const double* T, const double* P, const double* Y, double* J,
                       int grid, size_t lds, hipStream_t stream) {
-    if (ns == 1) hipLaunchKernelGGL((jac_batch_kernel<1>), dim3(grid), dim3(64), lds, stream, m, n, T, P, Y, J);
-    else if (ns == 2) hipLaunchKernelGGL((jac_batch_kernel<2>), dim3(grid), dim3(64), lds, stream, m, n, T, P, Y, J);
+    if (ns == 1) hipLaunchKernelGGL((jac_batch_kernel<1, true>), dim3(grid), dim3(64), lds, stream, m, n, T, P, Y, J);
+    else if (ns == 2) hipLaunchKernelGGL((jac_batch_kernel<2, false>), dim3(grid), dim3(64), lds, stream, m, n, T, P, Y, J);
     else return hipErrorInvalidValue;
     return hipGetLastError();
 }
 
 const void* jac_kernel_ptr(int ns) {
-    return ns == 1 ? reinterpret_cast<const void*>(&jac_batch_kernel<1>)
-                   : reinterpret_cast<const void*>(&jac_batch_kernel<2>);
+    return ns == 1 ? reinterpret_cast<const void*>(&jac_batch_kernel<1, true>)
+                   : reinterpret_cast<const void*>(&jac_batch_kernel<2, false>);
 }
 
 }  // namespace cg

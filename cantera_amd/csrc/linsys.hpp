@@ -6,6 +6,11 @@
 //   Every register index is static (all loops over columns are unrolled), so
 //   nothing spills to scratch.  The arithmetic is the same as a row-swapping
 //   LU (same multipliers, same updates), only the storage differs.
+// LdsRegLinSys: the same register LU and solves, but the matrix is kept in LDS
+//   between calls (row i in lane i, column-major so a lane's row loads are
+//   conflict-free) and the factorisation / solves are out-of-line functions
+//   that load it into registers: the registers are held only inside them, not
+//   across the whole integrator.  Used for 32 < N <= 64.
 // LdsLinSys: the matrix lives in LDS (row-major, leading dimension LD), used
 //   for N > 64 (two lanes' worth of rows per lane).
 //
@@ -28,6 +33,37 @@ __device__ __forceinline__ T readlane_t(T v, int l) {
     }
 }
 
+// 64-bit DPP move (both halves) with the given control.
+template <int kCtrl>
+__device__ __forceinline__ unsigned long long dpp_u64(unsigned long long x) {
+    const int lo = __builtin_amdgcn_update_dpp(0, static_cast<int>(x & 0xffffffffull), kCtrl, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, static_cast<int>(x >> 32), kCtrl, 0xf, 0xf, false);
+    return (static_cast<unsigned long long>(static_cast<unsigned int>(hi)) << 32) | static_cast<unsigned int>(lo);
+}
+
+__device__ __forceinline__ unsigned long long readlane_u64(unsigned long long x, int l) {
+    const unsigned int lo = __builtin_amdgcn_readlane(static_cast<int>(x & 0xffffffffull), l);
+    const unsigned int hi = __builtin_amdgcn_readlane(static_cast<int>(x >> 32), l);
+    return (static_cast<unsigned long long>(hi) << 32) | lo;
+}
+
+// Wavefront max of an unsigned 64-bit key: within each row of 16 lanes by
+// quad_perm [1,0,3,2], quad_perm [2,3,0,1], row_ror:4, row_ror:8; then over
+// the four rows through lanes 0, 16, 32, 48.  Returns the (uniform) maximum.
+__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long x) {
+    unsigned long long y;
+    y = dpp_u64<0xB1>(x);  x = y > x ? y : x;
+    y = dpp_u64<0x4E>(x);  x = y > x ? y : x;
+    y = dpp_u64<0x124>(x); x = y > x ? y : x;
+    y = dpp_u64<0x128>(x); x = y > x ? y : x;
+    unsigned long long m = readlane_u64(x, 0);
+    const unsigned long long m1 = readlane_u64(x, 16), m2 = readlane_u64(x, 32), m3 = readlane_u64(x, 48);
+    m = m1 > m ? m1 : m;
+    m = m2 > m ? m2 : m;
+    m = m3 > m ? m3 : m;
+    return m;
+}
+
 template <int NMAX, typename MT>
 struct RegLinSys {
     static constexpr int NS = 1;
@@ -36,6 +72,7 @@ struct RegLinSys {
     int pivlane;   // lane k: row (lane) chosen as pivot at step k
 
     __device__ __forceinline__ void init(void*, int) {}
+    __device__ __forceinline__ void bind(MT*, int*) {}
 
     __device__ __forceinline__ void form(const double* __restrict__ J, int N, double gamma) {
         const int lane = lane_id();
@@ -51,38 +88,42 @@ struct RegLinSys {
         }
     }
 
+    // Partial pivoting without row moves: at step k the row (lane) with the
+    // largest |entry| in column k among rows not yet used becomes the k-th
+    // pivot row.  The search packs |entry| (upper 58 bits) and the row into one
+    // 64-bit key and takes a wavefront max with DPP moves, so ties and
+    // near-ties (equal leading 58 bits) go to the lowest row.  The update of
+    // the columns right of k is branch-free: every lane of a column block that
+    // reaches k applies the same FMA and keeps the old value by selection.
     __device__ __forceinline__ bool factor(int N) {
         const int lane = lane_id();
-        const bool act = lane < N;
-        bool pivoted = !act;
+        bool pivoted = lane >= N;
         step = NMAX;
         pivlane = lane;
         MT colk = a[0];
         for (int k = 0; k < N; k++) {
-            double best = pivoted ? -1.0 : fabs(static_cast<double>(colk));
-            int bi = lane;
-#pragma unroll
-            for (int off = 32; off >= 1; off >>= 1) {
-                double ob = __shfl_xor(best, off, kWave);
-                int oi = __shfl_xor(bi, off, kWave);
-                if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
-            }
-            const int p = uni(bi);
+            const unsigned long long key =
+                pivoted ? 0ull
+                        : ((static_cast<unsigned long long>(__double_as_longlong(fabs(static_cast<double>(colk)))) &
+                            ~63ull) |
+                           static_cast<unsigned long long>(63 - lane));
+            const int p = 63 - static_cast<int>(wave_max_u64(key) & 63ull);
             const MT piv = readlane_t(colk, p);
             if (piv == MT(0)) return false;
-            const bool was = pivoted;
+            const bool keep = pivoted || lane == p;   // rows that are (now) pivot rows
             if (lane == p) { pivoted = true; step = k; }
             if (lane == k) pivlane = p;
-            const MT mult = MT(1) / piv;
-            const MT l = colk * mult;
+            const MT l = colk * (MT(1) / piv);
+            const MT lm = keep ? MT(0) : l;
 #pragma unroll
-            for (int j = 0; j < NMAX; j++) {
-                if (j < N) {
-                    if (j == k) {
-                        if (!was && lane != p) a[j] = l;
-                    } else if (j > k) {
+            for (int j0 = 0; j0 < NMAX; j0 += 8) {
+                if (j0 + 7 >= k) {
+#pragma unroll
+                    for (int j = j0; j < j0 + 8 && j < NMAX; j++) {
                         const MT pj = readlane_t(a[j], p);
-                        if (!was && lane != p) a[j] -= l * pj;
+                        const MT upd = a[j] - lm * pj;
+                        const MT atk = keep ? a[j] : l;
+                        a[j] = (j > k) ? upd : ((j == k) ? atk : a[j]);
                         if (j == k + 1) colk = a[j];
                     }
                 }
@@ -100,16 +141,17 @@ struct RegLinSys {
             if (k < N) {
                 const int pk = __builtin_amdgcn_readlane(pivlane, k);
                 const V yk = readlane_t(b, pk);
-                if (step > k) b -= static_cast<V>(a[k]) * yk;
+                const V nb = b - static_cast<V>(a[k]) * yk;
+                b = (step > k) ? nb : b;
             }
         }
 #pragma unroll
         for (int k = NMAX - 1; k >= 0; k--) {
             if (k < N) {
                 const int pk = __builtin_amdgcn_readlane(pivlane, k);
-                if (lane == pk) b = b / static_cast<V>(a[k]);
-                const V xk = readlane_t(b, pk);
-                if (step < k) b -= static_cast<V>(a[k]) * xk;
+                const V xk = readlane_t(b, pk) / readlane_t(static_cast<V>(a[k]), pk);
+                const V nb = b - static_cast<V>(a[k]) * xk;
+                b = (lane == pk) ? xk : ((step < k) ? nb : b);
             }
         }
         return __shfl(b, pivlane, kWave);
@@ -140,6 +182,164 @@ struct RegLinSys {
     }
 };
 
+// Out-of-line register LU of the LDS matrix at byte offset offA from the start
+// of dynamic LDS: A[j * 64 + i] = M(i, j); step[64] and pivlane[64] follow A.
+template <int NMAX, typename MT>
+__device__ __noinline__ int lr_factor(int offA, int N) {
+    MT* A = reinterpret_cast<MT*>(reinterpret_cast<char*>(lds_base()) + offA);
+    int* stp = reinterpret_cast<int*>(A + NMAX * kWave);
+    int* pvl = stp + kWave;
+    RegLinSys<NMAX, MT> r;
+    const int lane = lane_id();
+#pragma unroll
+    for (int j = 0; j < NMAX; j++) r.a[j] = A[j * kWave + lane];
+    const bool ok = r.factor(N);
+#pragma unroll
+    for (int j = 0; j < NMAX; j++) A[j * kWave + lane] = r.a[j];
+    stp[lane] = r.step;
+    pvl[lane] = r.pivlane;
+    wave_sync();
+    return ok ? 1 : 0;
+}
+
+template <int NMAX, typename MT, typename V>
+__device__ __noinline__ V lr_solve(int offA, V b, int N) {
+    const MT* A = reinterpret_cast<const MT*>(reinterpret_cast<const char*>(lds_base()) + offA);
+    const int* stp = reinterpret_cast<const int*>(A + NMAX * kWave);
+    const int* pvl = stp + kWave;
+    RegLinSys<NMAX, MT> r;
+    const int lane = lane_id();
+#pragma unroll
+    for (int j = 0; j < NMAX; j++) r.a[j] = A[j * kWave + lane];
+    r.step = stp[lane];
+    r.pivlane = pvl[lane];
+    return r.solve(b, N);
+}
+
+template <int NMAX, typename MT>
+struct LdsRegLinSys {
+    static constexpr int NS = 1;
+    MT* A;        // LDS, NMAX x 64, column j at A + 64 j
+    int* piv;     // LDS, step[64] then pivlane[64]
+    int offA;     // byte offset of A from the start of dynamic LDS
+
+    __device__ __forceinline__ void init(void* lds, int) {
+        A = static_cast<MT*>(lds);
+        piv = reinterpret_cast<int*>(A + NMAX * kWave);
+        offA = static_cast<int>(static_cast<char*>(lds) - reinterpret_cast<char*>(lds_base()));
+    }
+    __device__ __forceinline__ void bind(MT*, int*) {}
+
+    __device__ __forceinline__ void form(const double* __restrict__ J, int N, double gamma) {
+        const int lane = lane_id();
+        const bool act = lane < N;
+#pragma unroll
+        for (int j = 0; j < NMAX; j++) {
+            double v = 0.0;
+            if (act && j < N) {
+                v = -gamma * J[lane * N + j];
+                if (lane == j) v += 1.0;
+            }
+            A[j * kWave + lane] = static_cast<MT>(v);
+        }
+        wave_sync();
+    }
+
+    __device__ __forceinline__ bool factor(int N) { return lr_factor<NMAX, MT>(offA, N) != 0; }
+
+    template <typename V, int S>
+    __device__ __forceinline__ void solve(V (&b)[S], int N) const {
+        b[0] = lr_solve<NMAX, MT, V>(offA, b[0], N);
+    }
+
+    __device__ __forceinline__ void save(MT* __restrict__ dst, int* __restrict__ pdst, int) const {
+        const int lane = lane_id();
+#pragma unroll
+        for (int j = 0; j < NMAX; j++) dst[j * kWave + lane] = A[j * kWave + lane];
+        pdst[lane] = piv[lane];
+        pdst[kWave + lane] = piv[kWave + lane];
+    }
+
+    __device__ __forceinline__ void restore(const MT* __restrict__ src, const int* __restrict__ psrc, int) {
+        const int lane = lane_id();
+#pragma unroll
+        for (int j = 0; j < NMAX; j++) A[j * kWave + lane] = src[j * kWave + lane];
+        piv[lane] = psrc[lane];
+        piv[kWave + lane] = psrc[kWave + lane];
+        wave_sync();
+    }
+};
+
+// Out-of-line register LU / solves of a matrix kept in global memory (the
+// reactor's own saved-factor slot), layout A[j * 64 + i] = M(i, j), followed by
+// step[64] and pivlane[64] in the pivot slot.  Each lane reads and writes only
+// its own row, so no fence is needed between the factorisation and the solves.
+template <int NMAX, typename MT>
+__device__ __noinline__ int gr_factor(MT* __restrict__ A, int* __restrict__ piv, int N) {
+    RegLinSys<NMAX, MT> r;
+    const int lane = lane_id();
+#pragma unroll
+    for (int j = 0; j < NMAX; j++) r.a[j] = A[j * kWave + lane];
+    const bool ok = r.factor(N);
+#pragma unroll
+    for (int j = 0; j < NMAX; j++) A[j * kWave + lane] = r.a[j];
+    piv[lane] = r.step;
+    piv[kWave + lane] = r.pivlane;
+    return ok ? 1 : 0;
+}
+
+template <int NMAX, typename MT, typename V>
+__device__ __noinline__ V gr_solve(const MT* __restrict__ A, const int* __restrict__ piv, V b, int N) {
+    RegLinSys<NMAX, MT> r;
+    const int lane = lane_id();
+#pragma unroll
+    for (int j = 0; j < NMAX; j++) r.a[j] = A[j * kWave + lane];
+    r.step = piv[lane];
+    r.pivlane = piv[kWave + lane];
+    return r.solve(b, N);
+}
+
+// M = I - gamma J into the global slot (column j of M at A + 64 j)
+template <int NMAX, typename MT>
+__device__ __noinline__ void gr_form(const double* __restrict__ J, MT* __restrict__ A, int N, double gamma) {
+    const int lane = lane_id();
+    const bool act = lane < N;
+#pragma unroll
+    for (int j = 0; j < NMAX; j++) {
+        double v = 0.0;
+        if (act && j < N) {
+            v = -gamma * J[lane * N + j];
+            if (lane == j) v += 1.0;
+        }
+        A[j * kWave + lane] = static_cast<MT>(v);
+    }
+}
+
+template <int NMAX, typename MT>
+struct GlobalRegLinSys {
+    static constexpr int NS = 1;
+    MT* A = nullptr;     // the reactor's factor slot (NMAX x 64)
+    int* piv = nullptr;  // the reactor's pivot slot (128 ints)
+
+    __device__ __forceinline__ void init(void*, int) {}
+    __device__ __forceinline__ void bind(MT* a, int* p) { A = a; piv = p; }
+
+    __device__ __forceinline__ void form(const double* __restrict__ J, int N, double gamma) {
+        gr_form<NMAX, MT>(J, A, N, gamma);
+    }
+
+    __device__ __forceinline__ bool factor(int N) { return gr_factor<NMAX, MT>(A, piv, N) != 0; }
+
+    template <typename V, int S>
+    __device__ __forceinline__ void solve(V (&b)[S], int N) const {
+        b[0] = gr_solve<NMAX, MT, V>(A, piv, b[0], N);
+    }
+
+    // the factors already live in the reactor's slot
+    __device__ __forceinline__ void save(MT*, int*, int) const {}
+    __device__ __forceinline__ void restore(const MT*, const int*, int) {}
+};
+
 template <int NS_, typename MT>
 struct LdsLinSys {
     static constexpr int NS = NS_;
@@ -148,6 +348,7 @@ struct LdsLinSys {
     int* pmap;    // LDS
     MT* bv;       // LDS
 
+    __device__ __forceinline__ void bind(MT*, int*) {}
     __device__ __forceinline__ void init(void* lds, int ld) {
         LD = ld;
         char* p = static_cast<char*>(lds);

@@ -1,11 +1,13 @@
 // Batched net production rates (Kinetics::getNetProductionRates,
 // src/kinetics/Kinetics.cpp:447-457) over many independent states.
 //
-// Layout: species-major SoA in HBM, Y[k*n + s] and wdot[k*n + s].  A block is
-// one wavefront that owns a tile of 64 states: the tile is read with coalesced
-// loads into LDS, each state is then evaluated by the whole wavefront (lanes
-// over species, then over reactions, then over gather sub-rows), its wdot is
-// written over its Y column, and the tile is written back coalesced.
+// Layout: species-major SoA in HBM, Y[k*n + s] and wdot[k*n + s].  One
+// wavefront evaluates one state at a time (grid-stride over states): lane k
+// loads Y_k, the wavefront runs the species, reaction and production passes
+// with its scratch in LDS, and lane k stores wdot_k.  Consecutive states go to
+// neighbouring wavefronts, so the 64-byte lines of Y and wdot (8 states of one
+// species) are shared through L2.  The LDS scratch is small (a few KB), so many
+// wavefronts per CU hide the latency of the table loads and transcendentals.
 #include <hip/hip_runtime.h>
 
 #include "bdf.hpp"
@@ -15,78 +17,59 @@ namespace cg {
 
 namespace {
 
-constexpr int kTile = 64;
-constexpr int kTilePad = kTile + 1;
-
 template <int NS>
-__global__ __launch_bounds__(64) void rates_tile_kernel(DevMech m, long n, const double* __restrict__ T,
-                                                        const double* __restrict__ P, const double* __restrict__ Y,
-                                                        double* __restrict__ wdot) {
+__global__ __launch_bounds__(64) void rates_kernel(DevMech m, long n, const double* __restrict__ T,
+                                                   const double* __restrict__ P, const double* __restrict__ Y,
+                                                   double* __restrict__ wdot) {
     extern __shared__ double lds_raw[];
     const int lane = lane_id();
-    double* tile = lds_raw;                               // K x kTilePad
-    double* base = tile + static_cast<size_t>(m.K) * kTilePad;
-    WaveScratch S;
+    double* base = lds_raw;
+    WaveScratch S{};
     S.yb = base;  base += kWave * NS;
     S.C = base;   base += m.K;
     S.MU = base;  base += m.K;
     S.ROP = base; base += m.I;
-    S.SUB = base; base += m.nsub;
-    S.W2 = nullptr;
-    S.F = S.Y0 = S.F0 = nullptr;
-    const long ntiles = (n + kTile - 1) / kTile;
-    for (long t = blockIdx.x; t < ntiles; t += gridDim.x) {
-        const long s0 = t * kTile;
-        const int cnt = static_cast<int>(min(static_cast<long>(kTile), n - s0));
-        const bool have = lane < cnt;
-        const double Tl = have ? T[s0 + lane] : 300.0;
-        const double Pl = have ? P[s0 + lane] : 101325.0;
-        for (int k = 0; k < m.K; k++) tile[k * kTilePad + lane] = have ? Y[static_cast<size_t>(k) * n + s0 + lane] : 0.0;
-        __syncthreads();
-        for (int j = 0; j < cnt; j++) {
-            const double Tj = uni(__shfl(Tl, j, kWave));
-            const double Pj = uni(__shfl(Pl, j, kWave));
-            double yk[NS];
+    S.WD = base;
+    for (long st = blockIdx.x; st < n; st += gridDim.x) {
+        const double Ts = uni(T[st]);
+        const double Ps = uni(P[st]);
+        double yk[NS];
 #pragma unroll
-            for (int s = 0; s < NS; s++) {
-                const int k = lane + kWave * s;
-                yk[s] = (k < m.K) ? tile[k * kTilePad + j] : 0.0;
-            }
-            GasState g;
-            SpeciesRegs<NS> sr;
-            species_pass<NS>(m, S, Tj, Pj, yk, g, sr);
-            production_rates<NS>(m, S, g, sr);
-#pragma unroll
-            for (int s = 0; s < NS; s++) {
-                const int k = lane + kWave * s;
-                if (k < m.K) tile[k * kTilePad + j] = sr.wdot[s];
-            }
+        for (int s = 0; s < NS; s++) {
+            const int k = lane + kWave * s;
+            yk[s] = (k < m.K) ? Y[static_cast<size_t>(k) * n + st] : 0.0;
         }
-        __syncthreads();
-        if (have)
-            for (int k = 0; k < m.K; k++) wdot[static_cast<size_t>(k) * n + s0 + lane] = tile[k * kTilePad + lane];
-        __syncthreads();
+        GasState g;
+        SpeciesRegs<NS> sr;
+        species_pass<NS>(m, S, Ts, Ps, yk, g, sr);
+        production_rates<NS>(m, S, g, sr);
+#pragma unroll
+        for (int s = 0; s < NS; s++) {
+            const int k = lane + kWave * s;
+            if (k < m.K) wdot[static_cast<size_t>(k) * n + st] = sr.wdot[s];
+        }
+        wave_sync();
     }
 }
 
 }  // namespace
 
 size_t rates_lds_bytes(const DevMech& m, int ns) {
-    size_t d = static_cast<size_t>(m.K) * kTilePad + kWave * ns + 2 * m.K + m.I + m.nsub;
+    const size_t d = static_cast<size_t>(kWave) * ns + 2 * m.K + m.I + m.K;
     return d * sizeof(double);
 }
 
 hipError_t rates_launch(const DevMech& m, int ns, long n, const double* T, const double* P, const double* Y,
                         double* wdot, int grid, size_t lds, hipStream_t stream) {
-    if (ns == 1) hipLaunchKernelGGL((rates_tile_kernel<1>), dim3(grid), dim3(64), lds, stream, m, n, T, P, Y, wdot);
-    else if (ns == 2) hipLaunchKernelGGL((rates_tile_kernel<2>), dim3(grid), dim3(64), lds, stream, m, n, T, P, Y, wdot);
+    if (ns == 1) hipLaunchKernelGGL((rates_kernel<1>), dim3(grid), dim3(64), lds, stream, m, n, T, P, Y, wdot);
+    else if (ns == 2) hipLaunchKernelGGL((rates_kernel<2>), dim3(grid), dim3(64), lds, stream, m, n, T, P, Y, wdot);
     else return hipErrorInvalidValue;
     return hipGetLastError();
 }
 
 const void* rates_kernel_ptr(int ns) {
-    return ns == 1 ? reinterpret_cast<const void*>(&rates_tile_kernel<1>)
-                   : reinterpret_cast<const void*>(&rates_tile_kernel<2>);
+    return ns == 1 ? reinterpret_cast<const void*>(&rates_kernel<1>)
+                   : reinterpret_cast<const void*>(&rates_kernel<2>);
 }
 
 }  // namespace cg

@@ -247,17 +247,72 @@ __global__ void to_bf16_kernel(size_t n, const float* __restrict__ w, uint16_t* 
         wb[i] = f2bf(w[i]);
 }
 
-// y1 = y0 + denormalised output; m carried over.
+// y1 = y0 + denormalised output; m carried over.  With the element table
+// (comp != nullptr) the mass-fraction increments are projected onto the
+// element-conserving subspace: with B_ek = a_ek / W_k and weights
+// D_k = max(|Y0_k|, |Y0_k + dY_k|) + 1e-12, dY <- dY - D B^T (B D B^T)^-1 B dY,
+// the correction of least D^-1-weighted norm, so species near zero hardly move
+// and sum_k dY_k = 0 follows from element conservation.
+constexpr int kMaxK = 128, kMaxE = 8;
 __global__ void apply_kernel(int rows, int N, int K, const double* __restrict__ y0, const float* __restrict__ out,
-                             const double* __restrict__ norm, int in_p, int out_p, double* __restrict__ y1) {
+                             const double* __restrict__ norm, int in_p, int out_p, const double* __restrict__ comp,
+                             const double* __restrict__ rmw, int E, double* __restrict__ y1) {
+    __shared__ double dY[kMaxK], D[kMaxK];
+    __shared__ double lam[kMaxE];
     const int r = blockIdx.x;
     if (r >= rows) return;
     const double* mu = norm + 2 * in_p;
     const double* sd = mu + out_p;
+    const double* y0r = y0 + static_cast<size_t>(r) * N;
     for (int j = threadIdx.x; j < N; j += blockDim.x) {
-        double v = y0[static_cast<size_t>(r) * N + j];
-        if (j >= 1) v += static_cast<double>(out[static_cast<size_t>(r) * out_p + (j - 1)]) * sd[j - 1] + mu[j - 1];
-        y1[static_cast<size_t>(r) * N + j] = v;
+        double d = 0.0;
+        if (j >= 1) d = static_cast<double>(out[static_cast<size_t>(r) * out_p + (j - 1)]) * sd[j - 1] + mu[j - 1];
+        if (j >= 2) {
+            dY[j - 2] = d;
+            D[j - 2] = fmax(fabs(y0r[j]), fabs(y0r[j] + d)) + 1e-12;
+        }
+        y1[static_cast<size_t>(r) * N + j] = y0r[j] + d;
+    }
+    __syncthreads();
+    if (!comp || E < 1 || E > kMaxE) return;
+    if (threadIdx.x == 0) {
+        double G[kMaxE][kMaxE + 1];
+        for (int a = 0; a < E; a++) {
+            for (int b = 0; b < E; b++) {
+                double g = 0.0;
+                for (int k = 0; k < K; k++) g += comp[k * E + a] * rmw[k] * D[k] * comp[k * E + b] * rmw[k];
+                G[a][b] = g;
+            }
+            double v = 0.0;
+            for (int k = 0; k < K; k++) v += comp[k * E + a] * rmw[k] * dY[k];
+            G[a][E] = v;
+        }
+        // Gaussian elimination with partial pivoting; elements absent from the
+        // mechanism's species give zero rows and get lambda = 0
+        int used[kMaxE];
+        for (int a = 0; a < E; a++) used[a] = 0;
+        for (int c = 0; c < E; c++) {
+            int p = -1;
+            double best = 0.0;
+            for (int a = 0; a < E; a++)
+                if (!used[a] && fabs(G[a][c]) > best) { best = fabs(G[a][c]); p = a; }
+            lam[c] = 0.0;
+            if (p < 0 || best < 1e-300) continue;
+            used[p] = c + 1;
+            for (int a = 0; a < E; a++) {
+                if (a == p) continue;
+                const double f = G[a][c] / G[p][c];
+                for (int b = c; b <= E; b++) G[a][b] -= f * G[p][b];
+            }
+        }
+        for (int a = 0; a < E; a++)
+            if (used[a]) lam[used[a] - 1] = G[a][E] / G[a][used[a] - 1];
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < K; k += blockDim.x) {
+        double c = 0.0;
+        for (int a = 0; a < E; a++) c += comp[k * E + a] * rmw[k] * lam[a];
+        y1[static_cast<size_t>(r) * N + 2 + k] = y0r[2 + k] + (dY[k] - D[k] * c);
     }
 }
 
@@ -511,7 +566,7 @@ void SurrogateNet::predict(long n, const double* y0, const double* P, double* y1
         check(hipGetLastError(), "surrogate features");
         forward(rows, st);
         hipLaunchKernelGGL(apply_kernel, dim3(rows), dim3(64), 0, st, rows, N, s_.K, y0 + c0 * N, out_, norm_,
-                           s_.in_p, s_.out_p, y1 + c0 * N);
+                           s_.in_p, s_.out_p, comp_, rmw_, E_, y1 + c0 * N);
         check(hipGetLastError(), "surrogate apply");
     }
 }

@@ -54,6 +54,10 @@ public:
     // y1 = y0 + net(y0) on device rows; m is carried over.
     void predict(long n, const double* y0, const double* P, double* y1, hipStream_t stream);
 
+    // element table (K x E) and 1/W_k of the mechanism (device pointers): with
+    // them, predictions conserve elements exactly (projection)
+    void set_elements(const double* comp, const double* rmw, int E) { comp_ = comp; rmw_ = rmw; E_ = E; }
+
     void get_params(float* host) const;
     void set_params(const float* host);
     void get_normalization(double* host) const;   // mu_in, sd_in, mu_out, sd_out (in_p, in_p, out_p, out_p)
@@ -76,6 +80,9 @@ private:
     uint16_t* wb_ = nullptr;        // BF16 weights (same offsets as par_)
     double* norm_ = nullptr;        // mu_in, sd_in, mu_out, sd_out
     int adam_t_ = 0;
+    const double* comp_ = nullptr;
+    const double* rmw_ = nullptr;
+    int E_ = 0;
     uint64_t rng_ = 0;
     // batch workspace
     int cap_ = 0;

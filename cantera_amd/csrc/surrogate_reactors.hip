@@ -51,7 +51,7 @@ __global__ __launch_bounds__(64) void guard_kernel(DevMech m, int n, const int* 
     L.C = off;   off += m.K;
     L.MU = off;  off += m.K;
     L.ROP = off; off += m.I;
-    L.SUB = off; off += m.nsub;
+    L.WD = off;  off += m.K;
     L.W2 = off;
     const WaveScratch S = scratch_view(L);
     const int N = m.N, K = m.K;
@@ -186,7 +186,7 @@ __global__ void restart_kernel(int n, const int* __restrict__ list, const int* _
 }  // namespace
 
 size_t guard_lds_bytes(const DevMech& m, int ns) {
-    const size_t d = static_cast<size_t>(kWave) * ns * 4 + 2 * m.K + m.I + m.nsub + 2 * m.K;
+    const size_t d = static_cast<size_t>(kWave) * ns * 4 + 2 * m.K + m.I + m.K + 2 * m.K;
     return d * sizeof(double);
 }
 

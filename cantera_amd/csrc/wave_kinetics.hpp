@@ -88,7 +88,7 @@ struct WaveScratch {
     double* C;     // K concentrations
     double* MU;    // K standard chemical potentials [J/kmol]
     double* ROP;   // I net rates of progress
-    double* SUB;   // nsub gather partial sums
+    double* WD;    // K production-rate accumulators
     double* W2;    // 2K (Jacobian: wdot, D)
     double* F;     // N (right-hand side out)
     double* Y0;    // N (Jacobian expansion point)
@@ -99,7 +99,7 @@ struct WaveScratch {
 // passing to out-of-line device functions: pointers rebuilt from the LDS base
 // inside the callee keep the LDS address space (ds_* instructions).
 struct ScratchLayout {
-    int yb, C, MU, ROP, SUB, W2, F, Y0, F0;
+    int yb, C, MU, ROP, WD, W2, F, Y0, F0;
 };
 
 __device__ __forceinline__ double* lds_base() {
@@ -110,7 +110,7 @@ __device__ __forceinline__ double* lds_base() {
 __device__ __forceinline__ WaveScratch scratch_view(const ScratchLayout& L) {
     double* b = lds_base();
     WaveScratch S;
-    S.yb = b + L.yb; S.C = b + L.C; S.MU = b + L.MU; S.ROP = b + L.ROP; S.SUB = b + L.SUB;
+    S.yb = b + L.yb; S.C = b + L.C; S.MU = b + L.MU; S.ROP = b + L.ROP; S.WD = b + L.WD;
     S.W2 = b + L.W2; S.F = b + L.F; S.Y0 = b + L.Y0; S.F0 = b + L.F0;
     return S;
 }
@@ -365,50 +365,41 @@ __device__ __forceinline__ void species_pass(const DevMech& m, const WaveScratch
 }
 
 // Net production rates for the species each lane owns (after species_pass).
+// Each reaction lane adds -nu' ropnet to its reactants and +nu'' ropnet to its
+// products with LDS FP64 atomics (the species lists are already in registers),
+// so no second pass over a gather table is needed; the summation order of a
+// species' terms is the hardware's, not the reference's reaction order.
 template <int NS>
 __device__ __forceinline__ void production_rates(const DevMech& m, const WaveScratch& S, const GasState& g,
                                                  SpeciesRegs<NS>& sr) {
     const int lane = lane_id();
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        const int k = lane + kWave * s;
+        if (k < m.K) S.WD[k] = 0.0;
+    }
     wave_sync();
     for (int p = lane; p < m.I; p += kWave) {
         RxnVals v;
         RxnRegs x;
         load_rxn(m, p, x);
         eval_reaction(m, p, g, S.C, S.MU, x, v);
-        S.ROP[p] = v.ropf - v.ropr;
-    }
-    wave_sync();
-    for (int sb = lane; sb < m.nsub; sb += kWave) {
-        double acc = 0.0;
-        const int e0 = m.sub_start[sb], e1 = m.sub_start[sb + 1];
-        for (int e = e0; e < e1; e += 4) {
-            int rr[4];
-            double nn[4];
+        const double r = v.ropf - v.ropr;
+        S.ROP[p] = r;
+        const double* gs = (x.gen >= 0) ? m.gend + 12 * x.gen : m.gend;
+        const bool rgen = x.flags & 8, pgen = x.flags & 16;
 #pragma unroll
-            for (int b = 0; b < 4; b++) {
-                const bool ok = e + b < e1;
-                rr[b] = ok ? m.ent_r[e + b] : 0;
-                nn[b] = ok ? m.ent_nu[e + b] : 0.0;
-            }
-            double rp[4];
+        for (int j = 0; j < kMaxSide; j++)
+            if (j < x.nr) atomicAdd(&S.WD[x.rk[j]], -(rgen ? gs[4 + j] : 1.0) * r);
 #pragma unroll
-            for (int b = 0; b < 4; b++) rp[b] = S.ROP[rr[b]];
-#pragma unroll
-            for (int b = 0; b < 4; b++)
-                if (e + b < e1) acc += nn[b] * rp[b];
-        }
-        S.SUB[sb] = acc;
+        for (int j = 0; j < kMaxSide; j++)
+            if (j < x.np) atomicAdd(&S.WD[x.pk[j]], (pgen ? gs[8 + j] : 1.0) * r);
     }
     wave_sync();
 #pragma unroll
     for (int s = 0; s < NS; s++) {
-        int k = lane + kWave * s;
-        double w = 0.0;
-        if (k < m.K) {
-            const int b1 = m.sp_sub[k + 1];
-            for (int b = m.sp_sub[k]; b < b1; b++) w += S.SUB[b];
-        }
-        sr.wdot[s] = w;
+        const int k = lane + kWave * s;
+        sr.wdot[s] = (k < m.K) ? S.WD[k] : 0.0;
     }
 }
 

@@ -144,6 +144,11 @@ int32_t cg_reactors_set_options(int32_t batch, int32_t max_steps, int32_t waves_
  * order of the internal steps they needed in the previous call; 0 = index
  * order.  Results do not depend on it. */
 int32_t cg_reactors_set_schedule(int32_t batch, int32_t longest_first);
+/* Where the iteration matrix lives for 32 < N <= 64 (no reference
+ * counterpart): 0 default (= 2), 1 LDS, 2 the reactor's slot in global
+ * memory; factorisation and solves run in registers either way.  Takes effect
+ * at the next advance; results do not depend on it. */
+int32_t cg_reactors_set_matrix_storage(int32_t batch, int32_t storage);
 /* Duration [ms] of the last integrator kernel of this batch, from HIP events
  * on the batch stream. */
 double cg_reactors_last_kernel_ms(int32_t batch);

@@ -134,6 +134,23 @@ def test_schedule_does_not_change_results():
     assert dT.max() <= 1e-6
 
 
+def test_matrix_storage_does_not_change_results():
+    """Iteration matrix in LDS or in the reactor's global slot: same arithmetic."""
+    import cantera_amd as ca
+    from cantera_amd.workloads import MechInfo, config3
+    wl = config3(MechInfo(mech_file("gri30")), n=64, nsteps=6)
+    sol = ca.Solution("gri30")
+    out = []
+    for storage in ("lds", "global"):
+        rb = ca.ReactorBatch(sol, wl.T0, wl.P, wl.Y0, rtol=wl.rtol, atol=wl.atol)
+        rb.set_matrix_storage(storage)
+        for k in range(1, 7):
+            rb.advance(k * wl.dt)
+        assert np.all(rb.status == 0)
+        out.append(rb.state)
+    assert np.max(np.abs(out[0][:, 1] - out[1][:, 1]) / out[1][:, 1]) <= 1e-6
+
+
 def test_reactor_invariants():
     """Mass fractions keep summing to 1 and enthalpy is conserved (adiabatic,
     isobaric): checked through the elements and the state after 50 outputs."""
