@@ -283,7 +283,7 @@ def batch_kernels(a):
     fl, tr, _ = rhs_cost(sol.path)
     F_rates = fl + TRANS_FLOPS * tr
     avg_s = float(np.mean(kernel_ms)) / 1e3
-    name = "jac_batch_gri30" if jac else "rates_tile_gri30"
+    name = "jac_batch_gri30" if jac else "rates_gri30"
     if jac:
         # HBM-bound: T, P, Y in; the N x N Jacobian out
         alg_bytes = n * (8 * (2 + K) + 8 * N * N)
@@ -295,7 +295,7 @@ def batch_kernels(a):
         flops = n * F_rates
         roof = {"bound": "mfma", "pipe": PIPE, "achieved": flops / avg_s / 1e12, "peak": FP64_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": flops / avg_s / 1e12 / FP64_PEAK_TFLOPS,
-                "traffic": measured_traffic(name, n), "kernel": "rates_tile_kernel", "avg_launch_ms": avg_s * 1e3,
+                "traffic": measured_traffic(name, n), "kernel": "rates_kernel", "avg_launch_ms": avg_s * 1e3,
                 "flops_per_launch": flops, "hbm_bytes_algorithmic": n * 8 * (2 + 2 * K)}
         metric = "gri30 net production rates, states/s (FP64)"
     out = {

@@ -141,14 +141,14 @@ class ReactorBatch:
     def set_options(self, max_steps=0, waves_per_cu=-1):
         check(lib().cg_reactors_set_options(self._h, int(max_steps), int(waves_per_cu)))
 
-    def advance_surrogate(self, t, surrogate, dT_tol=0.05, rtol_g=1e-3) -> np.ndarray:
+    def advance_surrogate(self, t, surrogate, dT_tol=0.05, rtol_g=1e-3, atol_g=1e-9) -> np.ndarray:
         """advance() with guarded surrogate steps where t is a whole number of
         surrogate steps away; returns [surrogate steps kept, reactors advanced
         by the surrogate alone, reactors that ran the integrator]."""
         t_out = np.ascontiguousarray(np.broadcast_to(np.asarray(t, dtype=np.float64), (self.n,)))
         counts = (C.c_int64 * 3)()
         check(lib().cg_reactors_advance_surrogate(self._h, surrogate.handle, t_out, float(dT_tol), float(rtol_g),
-                                                  counts), "cg_reactors_advance_surrogate")
+                                                  float(atol_g), counts), "cg_reactors_advance_surrogate")
         self.time = t_out.copy()
         return np.array(counts[:], dtype=np.int64)
 

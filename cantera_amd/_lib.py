@@ -73,7 +73,7 @@ SIGNATURES = [
     ("cg_surrogate_set_params", C.c_int32, [C.c_int32, C.POINTER(C.c_float)]),
     ("cg_surrogate_get_normalization", C.c_int32, [C.c_int32, C.POINTER(C.c_double)]),
     ("cg_surrogate_set_normalization", C.c_int32, [C.c_int32, C.POINTER(C.c_double)]),
-    ("cg_reactors_advance_surrogate", C.c_int32, [C.c_int32, C.c_int32, _dp, C.c_double, C.c_double,
+    ("cg_reactors_advance_surrogate", C.c_int32, [C.c_int32, C.c_int32, _dp, C.c_double, C.c_double, C.c_double,
                                                   C.POINTER(C.c_int64)]),
 ]
 

@@ -65,7 +65,7 @@ struct Batch {
     void* sort_tmp = nullptr;
     size_t sort_tmp_bytes = 0;
     bool lpt_on = true, have_order = false;
-    bool global_matrix = true;   // 32 < N <= 64: iteration matrix in the global factor slot (else LDS)
+    bool global_matrix = false;  // 32 < N <= 64: iteration matrix in the global factor slot (else LDS)
     size_t rec_stride = 0, j_stride = 0, lu_stride = 0, piv_stride = 0;
     hipStream_t stream = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -450,7 +450,7 @@ int32_t cg_reactors_set_matrix_storage(int32_t h, int32_t storage) {
     return guarded([&] {
         Batch& b = get_batch(h);
         if (storage < 0 || storage > 2) throw std::runtime_error("storage must be 0 (default), 1 (LDS) or 2 (global)");
-        b.global_matrix = storage != 1;
+        b.global_matrix = storage == 2;
         const DevMech& m = b.mech->dm->view();
         b.lds = bdf_lds_bytes(m, b.ld, b.fp32, b.global_matrix);
         set_lds_limit(bdf_kernel_ptr(m.N, b.fp32, b.global_matrix), b.lds);
@@ -700,13 +700,13 @@ int32_t cg_surrogate_set_normalization(int32_t s, const double* v) {
 }
 
 int32_t cg_reactors_advance_surrogate(int32_t bh, int32_t sh, const double* t_out, double dT_tol, double rtol_g,
-                                      int64_t* counts) {
+                                      double atol_g, int64_t* counts) {
     return guarded([&] {
         Batch& b = get_batch(bh);
         SurrogateObj& so = get_surr(sh);
         if (so.mech != b.mech) throw std::runtime_error("surrogate and reactor batch use different mechanisms");
         if (!t_out) throw std::runtime_error("null t_out");
-        if (!(dT_tol > 0) || !(rtol_g > 0)) throw std::runtime_error("guard tolerances must be positive");
+        if (!(dT_tol > 0) || !(rtol_g > 0) || !(atol_g > 0)) throw std::runtime_error("guard tolerances must be positive");
         const int n = b.n, N = b.N, K = b.K, NP = 64 * b.ns;
         const double dt = so.net->dt();
         check(hipStreamSynchronize(b.stream), "hipStreamSynchronize");
@@ -753,6 +753,7 @@ int32_t cg_reactors_advance_surrogate(int32_t bh, int32_t sh, const double* t_ou
             double* d_P = static_cast<double*>(dalloc(rows * 8));
             double* d_Y = static_cast<double*>(dalloc(rows * N * 8));
             double* d_Yn = static_cast<double*>(dalloc(rows * N * 8));
+            double* d_F = static_cast<double*>(dalloc(rows * N * 8));
             check(hipMemcpy(d_list, list.data(), rows * 4, hipMemcpyHostToDevice), "hipMemcpy");
             check(hipMemcpy(d_nsub, nsub.data(), rows * 4, hipMemcpyHostToDevice), "hipMemcpy");
             check(hipMemcpy(d_t0, t0.data(), rows * 8, hipMemcpyHostToDevice), "hipMemcpy");
@@ -768,8 +769,8 @@ int32_t cg_reactors_advance_surrogate(int32_t bh, int32_t sh, const double* t_ou
             check(hipEventRecord(b.e0, b.stream), "hipEventRecord");
             for (int k = 0; k < maxsub; k++) {
                 so.net->predict(na, d_Y, d_P, d_Yn, b.stream);
-                check(surrogate_guard(m, b.ns, na, d_nsub, d_done, d_rej, d_Y, d_Yn, d_P, dt, dT_tol, rtol_g, grid, lds,
-                                      b.stream),
+                check(surrogate_guard(m, b.ns, na, d_nsub, d_done, d_rej, d_Y, d_Yn, d_F, d_P, dt, dT_tol, rtol_g, atol_g,
+                                      grid, lds, b.stream),
                       "surrogate guard");
             }
             check(surrogate_restart(na, d_list, d_done, d_t0, dt, d_Y, b.rec, b.rec_stride, NP, N, K, b.stream),

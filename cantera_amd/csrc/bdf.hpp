@@ -70,8 +70,8 @@ const void* guard_kernel_ptr(int ns);
 hipError_t surrogate_gather(int n, const int* list, const double* rec, size_t stride, int NP, int N, int K, double* Y,
                             double* P, hipStream_t s);
 hipError_t surrogate_guard(const DevMech& m, int ns, int n, const int* nsub, int* done, int* rejected, double* Ycur,
-                           const double* Ynext, const double* Pa, double dt, double dT_tol, double rtol_g, int grid,
-                           size_t lds, hipStream_t s);
+                           const double* Ynext, double* Fcur, const double* Pa, double dt, double dT_tol,
+                           double rtol_g, double atol_g, int grid, size_t lds, hipStream_t s);
 hipError_t surrogate_restart(int n, const int* list, const int* done, const double* t0, double dt, const double* Y,
                              double* rec, size_t stride, int NP, int N, int K, hipStream_t s);
 

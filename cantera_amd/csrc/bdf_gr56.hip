@@ -4,7 +4,7 @@
 namespace cg {
 
 template <typename MT>
-__global__ __launch_bounds__(64) void bdf_advance_gr56(BdfArgs a) {
+__global__ __launch_bounds__(64, 2) void bdf_advance_gr56(BdfArgs a) {
     bdf_advance_body<1, GlobalRegLinSys<56, MT>, MT>(a);
 }
 

@@ -4,7 +4,7 @@
 namespace cg {
 
 template <typename MT>
-__global__ __launch_bounds__(64) void bdf_advance_gr64(BdfArgs a) {
+__global__ __launch_bounds__(64, 2) void bdf_advance_gr64(BdfArgs a) {
     bdf_advance_body<1, GlobalRegLinSys<64, MT>, MT>(a);
 }
 

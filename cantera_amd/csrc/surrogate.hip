@@ -172,7 +172,9 @@ __global__ void features_kernel(int rows, const int* __restrict__ idx, int N, in
     }
 }
 
-// Normalised targets: (T1 - T0, Y1_k - Y0_k) scaled by the output statistics.
+// Targets: the increments d = (T1 - T0, Y1_k - Y0_k), compressed as asinh(d / s)
+// (linear below the scale s, logarithmic above, so small and large increments
+// get comparable relative accuracy), then standardised.
 __global__ void targets_kernel(int rows, const int* __restrict__ idx, int N, int K, const double* __restrict__ y0,
                                const double* __restrict__ y1, const double* __restrict__ norm, int in_p, int out_p,
                                float* __restrict__ T) {
@@ -181,11 +183,12 @@ __global__ void targets_kernel(int rows, const int* __restrict__ idx, int N, int
     const long s = idx ? idx[r] : r;
     const double* mu = norm + 2 * in_p;
     const double* sd = mu + out_p;
+    const double* sc = sd + out_p;
     for (int j = threadIdx.x; j < out_p; j += blockDim.x) {
         float v = 0.f;
         if (j <= K) {
             const double d = y1[s * N + 1 + j] - y0[s * N + 1 + j];
-            v = static_cast<float>((d - mu[j]) / sd[j]);
+            v = static_cast<float>((asinh(d / sc[j]) - mu[j]) / sd[j]);
         }
         T[static_cast<size_t>(r) * out_p + j] = v;
     }
@@ -263,10 +266,12 @@ __global__ void apply_kernel(int rows, int N, int K, const double* __restrict__ 
     if (r >= rows) return;
     const double* mu = norm + 2 * in_p;
     const double* sd = mu + out_p;
+    const double* sc = sd + out_p;
     const double* y0r = y0 + static_cast<size_t>(r) * N;
     for (int j = threadIdx.x; j < N; j += blockDim.x) {
         double d = 0.0;
-        if (j >= 1) d = static_cast<double>(out[static_cast<size_t>(r) * out_p + (j - 1)]) * sd[j - 1] + mu[j - 1];
+        if (j >= 1)
+            d = sc[j - 1] * sinh(static_cast<double>(out[static_cast<size_t>(r) * out_p + (j - 1)]) * sd[j - 1] + mu[j - 1]);
         if (j >= 2) {
             dY[j - 2] = d;
             D[j - 2] = fmax(fabs(y0r[j]), fabs(y0r[j] + d)) + 1e-12;
@@ -370,14 +375,15 @@ SurrogateNet::SurrogateNet(int K, double dt, int hidden, int depth, uint64_t see
     check(hipMalloc(&m1_, n_par_ * 4), "hipMalloc(surrogate)");
     check(hipMalloc(&m2_, n_par_ * 4), "hipMalloc(surrogate)");
     check(hipMalloc(&wb_, n_par_ * 2), "hipMalloc(surrogate)");
-    check(hipMalloc(&norm_, (2 * s_.in_p + 2 * s_.out_p) * sizeof(double)), "hipMalloc(surrogate)");
+    check(hipMalloc(&norm_, norm_len() * sizeof(double)), "hipMalloc(surrogate)");
     check(hipMalloc(&loss_, sizeof(double)), "hipMalloc(surrogate)");
     check(hipMemcpy(par_, h.data(), n_par_ * 4, hipMemcpyHostToDevice), "hipMemcpy");
     check(hipMemset(m1_, 0, n_par_ * 4), "hipMemset");
     check(hipMemset(m2_, 0, n_par_ * 4), "hipMemset");
-    std::vector<double> nm(2 * s_.in_p + 2 * s_.out_p, 0.0);
+    std::vector<double> nm(norm_len(), 0.0);
     for (int j = 0; j < s_.in_p; j++) nm[s_.in_p + j] = 1.0;
     for (int j = 0; j < s_.out_p; j++) nm[2 * s_.in_p + s_.out_p + j] = 1.0;
+    for (int j = 0; j < s_.out_p; j++) nm[2 * s_.in_p + 2 * s_.out_p + j] = 1.0;
     set_normalization(nm.data());
     sync_bf16(nullptr);
     check(hipDeviceSynchronize(), "surrogate init");
@@ -485,7 +491,15 @@ void SurrogateNet::fit_normalization(long n, const double* y0, const double* P, 
     check(hipMemcpy(b.data(), y1, b.size() * 8, hipMemcpyDeviceToHost), "hipMemcpy");
     check(hipMemcpy(p.data(), P, p.size() * 8, hipMemcpyDeviceToHost), "hipMemcpy");
     const int ip = s_.in_p, op = s_.out_p;
-    std::vector<double> nm(2 * ip + 2 * op, 0.0), s1(ip, 0.0), s2(ip, 0.0), t1(op, 0.0), t2(op, 0.0);
+    std::vector<double> nm(norm_len(), 0.0), s1(ip, 0.0), s2(ip, 0.0), t1(op, 0.0), t2(op, 0.0), sq(op, 0.0);
+    // scale of each increment: 1e-3 of its RMS over the sample
+    for (long r = 0; r < S; r++)
+        for (int j = 0; j < s_.n_out; j++) {
+            const double d = b[static_cast<size_t>(r) * N + 1 + j] - a[static_cast<size_t>(r) * N + 1 + j];
+            sq[j] += d * d;
+        }
+    std::vector<double> sc(op, 1.0);
+    for (int j = 0; j < s_.n_out; j++) sc[j] = std::max(1e-3 * std::sqrt(sq[j] / S), 1e-300);
     for (long r = 0; r < S; r++) {
         const double* y = &a[static_cast<size_t>(r) * N];
         const double* z = &b[static_cast<size_t>(r) * N];
@@ -495,7 +509,7 @@ void SurrogateNet::fit_normalization(long n, const double* y0, const double* P, 
             s2[j] += f * f;
         }
         for (int j = 0; j < s_.n_out; j++) {
-            const double d = z[1 + j] - y[1 + j];
+            const double d = std::asinh((z[1 + j] - y[1 + j]) / sc[j]);
             t1[j] += d;
             t2[j] += d * d;
         }
@@ -508,7 +522,8 @@ void SurrogateNet::fit_normalization(long n, const double* y0, const double* P, 
     for (int j = 0; j < op; j++) {
         const double mu = t1[j] / S, var = t2[j] / S - mu * mu;
         nm[2 * ip + j] = (j < s_.n_out) ? mu : 0.0;
-        nm[2 * ip + op + j] = (j < s_.n_out && var > 1e-60) ? std::sqrt(var) : 1.0;
+        nm[2 * ip + op + j] = (j < s_.n_out && var > 1e-24) ? std::sqrt(var) : 1.0;
+        nm[2 * ip + 2 * op + j] = sc[j];
     }
     set_normalization(nm.data());
 }
@@ -580,10 +595,10 @@ void SurrogateNet::set_params(const float* host) {
     check(hipDeviceSynchronize(), "surrogate set_params");
 }
 void SurrogateNet::get_normalization(double* host) const {
-    check(hipMemcpy(host, norm_, (2 * s_.in_p + 2 * s_.out_p) * 8, hipMemcpyDeviceToHost), "hipMemcpy(norm)");
+    check(hipMemcpy(host, norm_, norm_len() * 8, hipMemcpyDeviceToHost), "hipMemcpy(norm)");
 }
 void SurrogateNet::set_normalization(const double* host) {
-    check(hipMemcpy(norm_, host, (2 * s_.in_p + 2 * s_.out_p) * 8, hipMemcpyHostToDevice), "hipMemcpy(norm)");
+    check(hipMemcpy(norm_, host, norm_len() * 8, hipMemcpyHostToDevice), "hipMemcpy(norm)");
 }
 
 }  // namespace cg

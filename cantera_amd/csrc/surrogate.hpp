@@ -60,7 +60,9 @@ public:
 
     void get_params(float* host) const;
     void set_params(const float* host);
-    void get_normalization(double* host) const;   // mu_in, sd_in, mu_out, sd_out (in_p, in_p, out_p, out_p)
+    // mu_in, sd_in (in_p each), mu_out, sd_out, scale_out (out_p each)
+    size_t norm_len() const { return 2 * static_cast<size_t>(s_.in_p) + 3 * static_cast<size_t>(s_.out_p); }
+    void get_normalization(double* host) const;
     void set_normalization(const double* host);
 
 private:
@@ -78,7 +80,7 @@ private:
     float* m1_ = nullptr;
     float* m2_ = nullptr;
     uint16_t* wb_ = nullptr;        // BF16 weights (same offsets as par_)
-    double* norm_ = nullptr;        // mu_in, sd_in, mu_out, sd_out
+    double* norm_ = nullptr;        // mu_in, sd_in, mu_out, sd_out, scale_out
     int adam_t_ = 0;
     const double* comp_ = nullptr;
     const double* rmw_ = nullptr;

@@ -8,7 +8,7 @@
 //     sum_k a_ek Y0_k / W_k for every element e;
 //   * the trapezoidal defect r = y1 - y0 - dt/2 (f(y0) + f(y1)) of the
 //     reactor ODE (f from the same FP64 kinetics as the integrator) is small:
-//     |r_T| <= dT_tol and |r_k| <= rtol_g * max(|Y0_k|, |Y1_k|) + 1e-12.
+//     |r_T| <= dT_tol and |r_k| <= rtol_g * max(|Y0_k|, |Y1_k|) + atol_g.
 // The first rejected step hands the reactor back to the BDF integrator,
 // restarted from the last kept state; a reactor whose steps were all kept is
 // restarted at its output time, as CVodeReInit would be after a jump in state.
@@ -35,12 +35,14 @@ __global__ void gather_states_kernel(int n, const int* __restrict__ list, const 
 }
 
 // One guard per row still in play (done[i] < nsub[i], not rejected).
+// f(y0) of the first step is computed here; each kept step leaves f(y1) in
+// Fcur (internal order) for the next, so a step costs one RHS evaluation.
 template <int NS>
 __global__ __launch_bounds__(64) void guard_kernel(DevMech m, int n, const int* __restrict__ nsub,
                                                    int* __restrict__ done, int* __restrict__ rejected,
                                                    double* __restrict__ Ycur, const double* __restrict__ Ynext,
-                                                   const double* __restrict__ Pa, double dt, double dT_tol,
-                                                   double rtol_g) {
+                                                   double* __restrict__ Fcur, const double* __restrict__ Pa,
+                                                   double dt, double dT_tol, double rtol_g, double atol_g) {
     const int lane = lane_id();
     ScratchLayout L;
     int off = 0;
@@ -101,19 +103,28 @@ __global__ __launch_bounds__(64) void guard_kernel(DevMech m, int n, const int* 
         bad = __any(bad);
         if (!bad) {
             // trapezoidal defect with the FP64 right-hand side
+            double* fr = Fcur + static_cast<size_t>(i) * N;
+            if (uni(done[i]) == 0) {
 #pragma unroll
-            for (int s = 0; s < NS; s++) {
-                const int c = lane + kWave * s;
-                if (c < N) S.yb[c] = y0[s];
-            }
-            wave_sync();
-            rhs_call<NS>(m, L, P);
+                for (int s = 0; s < NS; s++) {
+                    const int c = lane + kWave * s;
+                    if (c < N) S.yb[c] = y0[s];
+                }
+                wave_sync();
+                rhs_call<NS>(m, L, P);
 #pragma unroll
-            for (int s = 0; s < NS; s++) {
-                const int c = lane + kWave * s;
-                f0[s] = (c < N) ? S.F[c] : 0.0;
+                for (int s = 0; s < NS; s++) {
+                    const int c = lane + kWave * s;
+                    f0[s] = (c < N) ? S.F[c] : 0.0;
+                }
+                wave_sync();
+            } else {
+#pragma unroll
+                for (int s = 0; s < NS; s++) {
+                    const int c = lane + kWave * s;
+                    f0[s] = (c < N) ? fr[c] : 0.0;
+                }
             }
-            wave_sync();
 #pragma unroll
             for (int s = 0; s < NS; s++) {
                 const int c = lane + kWave * s;
@@ -132,7 +143,7 @@ __global__ __launch_bounds__(64) void guard_kernel(DevMech m, int n, const int* 
                 const int c = lane + kWave * s;
                 if (c <= K) {
                     const double r = y1[s] - y0[s] - 0.5 * dt * (f0[s] + f1[s]);
-                    const double tol = (c == K) ? dT_tol : rtol_g * fmax(fabs(y0[s]), fabs(y1[s])) + 1e-12;
+                    const double tol = (c == K) ? dT_tol : rtol_g * fmax(fabs(y0[s]), fabs(y1[s])) + atol_g;
                     if (!(fabs(r) <= tol)) bad = 1;
                 }
             }
@@ -143,6 +154,12 @@ __global__ __launch_bounds__(64) void guard_kernel(DevMech m, int n, const int* 
         } else {
             double* yc = Ycur + static_cast<size_t>(i) * N;
             for (int c = lane; c < N; c += kWave) yc[c] = y1r[c];
+            double* fr = Fcur + static_cast<size_t>(i) * N;
+#pragma unroll
+            for (int s = 0; s < NS; s++) {
+                const int c = lane + kWave * s;
+                if (c < N) fr[c] = f1[s];
+            }
             if (lane == 0) done[i] += 1;
         }
     }
@@ -201,14 +218,14 @@ hipError_t surrogate_gather(int n, const int* list, const double* rec, size_t st
 }
 
 hipError_t surrogate_guard(const DevMech& m, int ns, int n, const int* nsub, int* done, int* rejected, double* Ycur,
-                           const double* Ynext, const double* Pa, double dt, double dT_tol, double rtol_g, int grid,
-                           size_t lds, hipStream_t s) {
+                           const double* Ynext, double* Fcur, const double* Pa, double dt, double dT_tol,
+                           double rtol_g, double atol_g, int grid, size_t lds, hipStream_t s) {
     if (ns == 1)
-        hipLaunchKernelGGL((guard_kernel<1>), dim3(grid), dim3(64), lds, s, m, n, nsub, done, rejected, Ycur, Ynext, Pa,
-                           dt, dT_tol, rtol_g);
+        hipLaunchKernelGGL((guard_kernel<1>), dim3(grid), dim3(64), lds, s, m, n, nsub, done, rejected, Ycur, Ynext,
+                           Fcur, Pa, dt, dT_tol, rtol_g, atol_g);
     else
-        hipLaunchKernelGGL((guard_kernel<2>), dim3(grid), dim3(64), lds, s, m, n, nsub, done, rejected, Ycur, Ynext, Pa,
-                           dt, dT_tol, rtol_g);
+        hipLaunchKernelGGL((guard_kernel<2>), dim3(grid), dim3(64), lds, s, m, n, nsub, done, rejected, Ycur, Ynext,
+                           Fcur, Pa, dt, dT_tol, rtol_g, atol_g);
     return hipGetLastError();
 }
 

@@ -116,7 +116,7 @@ class Surrogate:
 
     def _norm_len(self):
         pad = lambda x: (x + 63) // 64 * 64
-        return 2 * pad(self.K + 2) + 2 * pad(self.K + 1)
+        return 2 * pad(self.K + 2) + 3 * pad(self.K + 1)
 
     @property
     def normalization(self) -> np.ndarray:

@@ -145,7 +145,7 @@ int32_t cg_reactors_set_options(int32_t batch, int32_t max_steps, int32_t waves_
  * order.  Results do not depend on it. */
 int32_t cg_reactors_set_schedule(int32_t batch, int32_t longest_first);
 /* Where the iteration matrix lives for 32 < N <= 64 (no reference
- * counterpart): 0 default (= 2), 1 LDS, 2 the reactor's slot in global
+ * counterpart): 0 default (= 1), 1 LDS, 2 the reactor's slot in global
  * memory; factorisation and solves run in registers either way.  Takes effect
  * at the next advance; results do not depend on it. */
 int32_t cg_reactors_set_matrix_storage(int32_t batch, int32_t storage);
@@ -181,7 +181,8 @@ int32_t cg_surrogate_predict(int32_t surrogate, int64_t n, const double* y0, con
 /* Parameters (host arrays): FP32 weights and biases layer by layer, weights
  * row-major [out][in] with inputs/outputs padded to multiples of 64;
  * normalisation = feature means, feature deviations, target means, target
- * deviations (padded lengths). */
+ * deviations, target scales (padded lengths; a target is asinh(increment /
+ * scale), standardised). */
 int32_t cg_surrogate_get_params(int32_t surrogate, float* params);
 int32_t cg_surrogate_set_params(int32_t surrogate, const float* params);
 int32_t cg_surrogate_get_normalization(int32_t surrogate, double* norm);
@@ -190,12 +191,12 @@ int32_t cg_surrogate_set_normalization(int32_t surrogate, const double* norm);
  * reactors whose t_out is a whole number of surrogate steps away take them,
  * each checked (physical range, element conservation, trapezoidal defect of
  * the reactor ODE: |r_T| <= dT_tol K, |r_k| <= rtol_g max(|Y0_k|, |Y1_k|) +
- * 1e-12); at the first rejected step the reactor continues with the BDF
+ * atol_g); at the first rejected step the reactor continues with the BDF
  * integrator from its last kept state.  counts[3] (may be NULL): surrogate
  * steps kept, reactors advanced by the surrogate alone, reactors that ran
  * the integrator. */
 int32_t cg_reactors_advance_surrogate(int32_t batch, int32_t surrogate, const double* t_out, double dT_tol,
-                                      double rtol_g, int64_t* counts);
+                                      double rtol_g, double atol_g, int64_t* counts);
 
 #ifdef __cplusplus
 }

@@ -56,7 +56,8 @@ def reference_loss_grad(params, norm, K, hidden, depth, y0, P, y1):
         b.append(params[off:off + fo[l]])
         off += fo[l]
     Wb = [bf16(w).astype(np.float64) for w in W]
-    mu_i, sd_i, mu_o, sd_o = norm[:ip], norm[ip:2 * ip], norm[2 * ip:2 * ip + op], norm[2 * ip + op:]
+    mu_i, sd_i = norm[:ip], norm[ip:2 * ip]
+    mu_o, sd_o, sc_o = norm[2 * ip:2 * ip + op], norm[2 * ip + op:2 * ip + 2 * op], norm[2 * ip + 2 * op:]
     rows = len(y0)
     f = np.zeros((rows, ip))
     f[:, 0] = y0[:, 1]
@@ -66,7 +67,7 @@ def reference_loss_grad(params, norm, K, hidden, depth, y0, P, y1):
     z[:, n_in:] = 0
     X = bf16(z.astype(np.float32)).astype(np.float64)
     T = np.zeros((rows, op), dtype=np.float32)
-    T[:, :n_out] = ((y1[:, 1:] - y0[:, 1:] - mu_o[:n_out]) / sd_o[:n_out]).astype(np.float32)
+    T[:, :n_out] = ((np.arcsinh((y1[:, 1:] - y0[:, 1:]) / sc_o[:n_out]) - mu_o[:n_out]) / sd_o[:n_out]).astype(np.float32)
     H, Z = [X], []
     for l in range(depth):
         z = (H[-1] @ Wb[l].T).astype(np.float32) + b[l]
