@@ -34,9 +34,11 @@ int bdf_variant(int N) {
     return -1;
 }
 
+// register policies keep 64 reciprocal pivots after the factors
 size_t bdf_lu_elems(int N, int ld) {
     const int v = bdf_variant(N);
-    if (v == 56 || v == 64) return static_cast<size_t>(v) * kWave;
+    if (v == 56 || v == 64) return static_cast<size_t>(v + 1) * kWave;
+    if (v == 16 || v == 32) return static_cast<size_t>(N) * ld + kWave;
     return static_cast<size_t>(N) * ld;
 }
 
@@ -48,7 +50,7 @@ size_t bdf_lds_bytes(const DevMech& m, int ld, bool fp32, bool global) {
         bytes += static_cast<size_t>(kWave) * ns * (sizeof(double) + sizeof(int));
         bytes += static_cast<size_t>(m.N) * ld * (fp32 ? 4 : 8);
     } else if ((v == 56 || v == 64) && !global) {
-        bytes += static_cast<size_t>(v) * kWave * (fp32 ? 4 : 8) + 2 * kWave * sizeof(int);
+        bytes += static_cast<size_t>(v + 1) * kWave * (fp32 ? 4 : 8) + 2 * kWave * sizeof(int);
     }
     return bytes;
 }

@@ -70,6 +70,7 @@ struct RegLinSys {
     MT a[NMAX];
     int step;      // pivot step of this lane's row (NMAX if not a pivot row yet)
     int pivlane;   // lane k: row (lane) chosen as pivot at step k
+    MT rdiag;      // pivot rows: 1 / U_kk of their step
 
     __device__ __forceinline__ void init(void*, int) {}
     __device__ __forceinline__ void bind(MT*, int*) {}
@@ -92,14 +93,16 @@ struct RegLinSys {
     // largest |entry| in column k among rows not yet used becomes the k-th
     // pivot row.  The search packs |entry| (upper 58 bits) and the row into one
     // 64-bit key and takes a wavefront max with DPP moves, so ties and
-    // near-ties (equal leading 58 bits) go to the lowest row.  The update of
-    // the columns right of k is branch-free: every lane of a column block that
-    // reaches k applies the same FMA and keeps the old value by selection.
+    // near-ties (equal leading 58 bits) go to the lowest row.  The pivot row is
+    // broadcast with v_readlane in blocks of 8 columns; blocks entirely right of
+    // column k are updated without selects, only the block holding column k
+    // selects per column.  Each pivot row keeps 1/U_kk for the back substitution.
     __device__ __forceinline__ bool factor(int N) {
         const int lane = lane_id();
         bool pivoted = lane >= N;
         step = NMAX;
         pivlane = lane;
+        rdiag = MT(1);
         MT colk = a[0];
         for (int k = 0; k < N; k++) {
             const unsigned long long key =
@@ -111,21 +114,44 @@ struct RegLinSys {
             const MT piv = readlane_t(colk, p);
             if (piv == MT(0)) return false;
             const bool keep = pivoted || lane == p;   // rows that are (now) pivot rows
-            if (lane == p) { pivoted = true; step = k; }
+            const MT rp = MT(1) / piv;
+            if (lane == p) { pivoted = true; step = k; rdiag = rp; }
             if (lane == k) pivlane = p;
-            const MT l = colk * (MT(1) / piv);
+            const MT l = colk * rp;
             const MT lm = keep ? MT(0) : l;
 #pragma unroll
             for (int j0 = 0; j0 < NMAX; j0 += 8) {
-                if (j0 + 7 >= k) {
+                constexpr int B = 8;
+                if (j0 > k) {
+                    MT pr[B];
 #pragma unroll
-                    for (int j = j0; j < j0 + 8 && j < NMAX; j++) {
-                        const MT pj = readlane_t(a[j], p);
-                        const MT upd = a[j] - lm * pj;
-                        const MT atk = keep ? a[j] : l;
-                        a[j] = (j > k) ? upd : ((j == k) ? atk : a[j]);
-                        if (j == k + 1) colk = a[j];
+                    for (int b = 0; b < B; b++)
+                        if (j0 + b < NMAX) pr[b] = readlane_t(a[j0 + b], p);
+#pragma unroll
+                    for (int b = 0; b < B; b++)
+                        if (j0 + b < NMAX) a[j0 + b] = a[j0 + b] - lm * pr[b];
+                } else if (j0 + B > k) {
+                    MT pr[B];
+#pragma unroll
+                    for (int b = 0; b < B; b++)
+                        if (j0 + b < NMAX) pr[b] = readlane_t(a[j0 + b], p);
+#pragma unroll
+                    for (int b = 0; b < B; b++) {
+                        const int j = j0 + b;
+                        if (j < NMAX) {
+                            const MT upd = a[j] - lm * pr[b];
+                            const MT atk = keep ? a[j] : l;
+                            a[j] = (j > k) ? upd : ((j == k) ? atk : a[j]);
+                        }
                     }
+                }
+            }
+#pragma unroll
+            for (int j0 = 0; j0 < NMAX; j0 += 8) {
+                if (j0 <= k + 1 && k + 1 < j0 + 8) {
+#pragma unroll
+                    for (int b = 0; b < 8; b++)
+                        if (j0 + b < NMAX && j0 + b == k + 1) colk = a[j0 + b];
                 }
             }
         }
@@ -145,11 +171,12 @@ struct RegLinSys {
                 b = (step > k) ? nb : b;
             }
         }
+        const V rd = static_cast<V>(rdiag);
 #pragma unroll
         for (int k = NMAX - 1; k >= 0; k--) {
             if (k < N) {
                 const int pk = __builtin_amdgcn_readlane(pivlane, k);
-                const V xk = readlane_t(b, pk) / readlane_t(static_cast<V>(a[k]), pk);
+                const V xk = readlane_t(b * rd, pk);
                 const V nb = b - static_cast<V>(a[k]) * xk;
                 b = (lane == pk) ? xk : ((step < k) ? nb : b);
             }
@@ -169,6 +196,7 @@ struct RegLinSys {
             for (int j = 0; j < NMAX; j++)
                 if (j < N) dst[lane * N + j] = a[j];
         }
+        dst[static_cast<size_t>(N) * N + lane] = rdiag;
         pdst[lane] = step;
         pdst[kWave + lane] = pivlane;
     }
@@ -177,6 +205,7 @@ struct RegLinSys {
         const int lane = lane_id();
 #pragma unroll
         for (int j = 0; j < NMAX; j++) a[j] = (lane < N && j < N) ? src[lane * N + j] : MT(0);
+        rdiag = src[static_cast<size_t>(N) * N + lane];
         step = psrc[lane];
         pivlane = psrc[kWave + lane];
     }
@@ -187,7 +216,7 @@ struct RegLinSys {
 template <int NMAX, typename MT>
 __device__ __noinline__ int lr_factor(int offA, int N) {
     MT* A = reinterpret_cast<MT*>(reinterpret_cast<char*>(lds_base()) + offA);
-    int* stp = reinterpret_cast<int*>(A + NMAX * kWave);
+    int* stp = reinterpret_cast<int*>(A + (NMAX + 1) * kWave);
     int* pvl = stp + kWave;
     RegLinSys<NMAX, MT> r;
     const int lane = lane_id();
@@ -196,6 +225,7 @@ __device__ __noinline__ int lr_factor(int offA, int N) {
     const bool ok = r.factor(N);
 #pragma unroll
     for (int j = 0; j < NMAX; j++) A[j * kWave + lane] = r.a[j];
+    A[NMAX * kWave + lane] = r.rdiag;
     stp[lane] = r.step;
     pvl[lane] = r.pivlane;
     wave_sync();
@@ -205,12 +235,13 @@ __device__ __noinline__ int lr_factor(int offA, int N) {
 template <int NMAX, typename MT, typename V>
 __device__ __noinline__ V lr_solve(int offA, V b, int N) {
     const MT* A = reinterpret_cast<const MT*>(reinterpret_cast<const char*>(lds_base()) + offA);
-    const int* stp = reinterpret_cast<const int*>(A + NMAX * kWave);
+    const int* stp = reinterpret_cast<const int*>(A + (NMAX + 1) * kWave);
     const int* pvl = stp + kWave;
     RegLinSys<NMAX, MT> r;
     const int lane = lane_id();
 #pragma unroll
     for (int j = 0; j < NMAX; j++) r.a[j] = A[j * kWave + lane];
+    r.rdiag = A[NMAX * kWave + lane];
     r.step = stp[lane];
     r.pivlane = pvl[lane];
     return r.solve(b, N);
@@ -219,13 +250,13 @@ __device__ __noinline__ V lr_solve(int offA, V b, int N) {
 template <int NMAX, typename MT>
 struct LdsRegLinSys {
     static constexpr int NS = 1;
-    MT* A;        // LDS, NMAX x 64, column j at A + 64 j
+    MT* A;        // LDS, (NMAX + 1) x 64: column j at A + 64 j, then 1/U_kk
     int* piv;     // LDS, step[64] then pivlane[64]
     int offA;     // byte offset of A from the start of dynamic LDS
 
     __device__ __forceinline__ void init(void* lds, int) {
         A = static_cast<MT*>(lds);
-        piv = reinterpret_cast<int*>(A + NMAX * kWave);
+        piv = reinterpret_cast<int*>(A + (NMAX + 1) * kWave);
         offA = static_cast<int>(static_cast<char*>(lds) - reinterpret_cast<char*>(lds_base()));
     }
     __device__ __forceinline__ void bind(MT*, int*) {}
@@ -255,7 +286,7 @@ struct LdsRegLinSys {
     __device__ __forceinline__ void save(MT* __restrict__ dst, int* __restrict__ pdst, int) const {
         const int lane = lane_id();
 #pragma unroll
-        for (int j = 0; j < NMAX; j++) dst[j * kWave + lane] = A[j * kWave + lane];
+        for (int j = 0; j <= NMAX; j++) dst[j * kWave + lane] = A[j * kWave + lane];
         pdst[lane] = piv[lane];
         pdst[kWave + lane] = piv[kWave + lane];
     }
@@ -263,7 +294,7 @@ struct LdsRegLinSys {
     __device__ __forceinline__ void restore(const MT* __restrict__ src, const int* __restrict__ psrc, int) {
         const int lane = lane_id();
 #pragma unroll
-        for (int j = 0; j < NMAX; j++) A[j * kWave + lane] = src[j * kWave + lane];
+        for (int j = 0; j <= NMAX; j++) A[j * kWave + lane] = src[j * kWave + lane];
         piv[lane] = psrc[lane];
         piv[kWave + lane] = psrc[kWave + lane];
         wave_sync();
@@ -283,6 +314,7 @@ __device__ __noinline__ int gr_factor(MT* __restrict__ A, int* __restrict__ piv,
     const bool ok = r.factor(N);
 #pragma unroll
     for (int j = 0; j < NMAX; j++) A[j * kWave + lane] = r.a[j];
+    A[NMAX * kWave + lane] = r.rdiag;
     piv[lane] = r.step;
     piv[kWave + lane] = r.pivlane;
     return ok ? 1 : 0;
@@ -294,6 +326,7 @@ __device__ __noinline__ V gr_solve(const MT* __restrict__ A, const int* __restri
     const int lane = lane_id();
 #pragma unroll
     for (int j = 0; j < NMAX; j++) r.a[j] = A[j * kWave + lane];
+    r.rdiag = A[NMAX * kWave + lane];
     r.step = piv[lane];
     r.pivlane = piv[kWave + lane];
     return r.solve(b, N);
@@ -318,7 +351,7 @@ __device__ __noinline__ void gr_form(const double* __restrict__ J, MT* __restric
 template <int NMAX, typename MT>
 struct GlobalRegLinSys {
     static constexpr int NS = 1;
-    MT* A = nullptr;     // the reactor's factor slot (NMAX x 64)
+    MT* A = nullptr;     // the reactor's factor slot ((NMAX + 1) x 64, last column 1/U_kk)
     int* piv = nullptr;  // the reactor's pivot slot (128 ints)
 
     __device__ __forceinline__ void init(void*, int) {}
