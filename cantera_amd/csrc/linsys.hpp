@@ -33,37 +33,6 @@ __device__ __forceinline__ T readlane_t(T v, int l) {
     }
 }
 
-// 64-bit DPP move (both halves) with the given control.
-template <int kCtrl>
-__device__ __forceinline__ unsigned long long dpp_u64(unsigned long long x) {
-    const int lo = __builtin_amdgcn_update_dpp(0, static_cast<int>(x & 0xffffffffull), kCtrl, 0xf, 0xf, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, static_cast<int>(x >> 32), kCtrl, 0xf, 0xf, false);
-    return (static_cast<unsigned long long>(static_cast<unsigned int>(hi)) << 32) | static_cast<unsigned int>(lo);
-}
-
-__device__ __forceinline__ unsigned long long readlane_u64(unsigned long long x, int l) {
-    const unsigned int lo = __builtin_amdgcn_readlane(static_cast<int>(x & 0xffffffffull), l);
-    const unsigned int hi = __builtin_amdgcn_readlane(static_cast<int>(x >> 32), l);
-    return (static_cast<unsigned long long>(hi) << 32) | lo;
-}
-
-// Wavefront max of an unsigned 64-bit key: within each row of 16 lanes by
-// quad_perm [1,0,3,2], quad_perm [2,3,0,1], row_ror:4, row_ror:8; then over
-// the four rows through lanes 0, 16, 32, 48.  Returns the (uniform) maximum.
-__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long x) {
-    unsigned long long y;
-    y = dpp_u64<0xB1>(x);  x = y > x ? y : x;
-    y = dpp_u64<0x4E>(x);  x = y > x ? y : x;
-    y = dpp_u64<0x124>(x); x = y > x ? y : x;
-    y = dpp_u64<0x128>(x); x = y > x ? y : x;
-    unsigned long long m = readlane_u64(x, 0);
-    const unsigned long long m1 = readlane_u64(x, 16), m2 = readlane_u64(x, 32), m3 = readlane_u64(x, 48);
-    m = m1 > m ? m1 : m;
-    m = m2 > m ? m2 : m;
-    m = m3 > m ? m3 : m;
-    return m;
-}
-
 template <int NMAX, typename MT>
 struct RegLinSys {
     static constexpr int NS = 1;

@@ -32,10 +32,64 @@ constexpr int kWave = 64;
 
 __device__ __forceinline__ int lane_id() { return static_cast<int>(threadIdx.x) & (kWave - 1); }
 
+// 64-bit DPP move (both halves) with the given control; all lanes valid for
+// quad_perm and row_ror controls.
+template <int kCtrl>
+__device__ __forceinline__ double dpp_f64(double x) {
+    const long long u = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_update_dpp(0, static_cast<int>(u & 0xffffffffll), kCtrl, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, static_cast<int>(u >> 32), kCtrl, 0xf, 0xf, false);
+    return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+}
+
+__device__ __forceinline__ double readlane_f64(double v, int l) {
+    const long long x = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane(static_cast<int>(x & 0xffffffffll), l);
+    const int hi = __builtin_amdgcn_readlane(static_cast<int>(x >> 32), l);
+    return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+}
+
+// 64-bit DPP move (both halves) with the given control.
+template <int kCtrl>
+__device__ __forceinline__ unsigned long long dpp_u64(unsigned long long x) {
+    const int lo = __builtin_amdgcn_update_dpp(0, static_cast<int>(x & 0xffffffffull), kCtrl, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, static_cast<int>(x >> 32), kCtrl, 0xf, 0xf, false);
+    return (static_cast<unsigned long long>(static_cast<unsigned int>(hi)) << 32) | static_cast<unsigned int>(lo);
+}
+
+__device__ __forceinline__ unsigned long long readlane_u64(unsigned long long x, int l) {
+    const unsigned int lo = __builtin_amdgcn_readlane(static_cast<int>(x & 0xffffffffull), l);
+    const unsigned int hi = __builtin_amdgcn_readlane(static_cast<int>(x >> 32), l);
+    return (static_cast<unsigned long long>(hi) << 32) | lo;
+}
+
+// Wavefront max of an unsigned 64-bit key: within each row of 16 lanes by
+// quad_perm [1,0,3,2], quad_perm [2,3,0,1], row_ror:4, row_ror:8; then over
+// the four rows through lanes 0, 16, 32, 48.  Returns the (uniform) maximum.
+__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long x) {
+    unsigned long long y;
+    y = dpp_u64<0xB1>(x);  x = y > x ? y : x;
+    y = dpp_u64<0x4E>(x);  x = y > x ? y : x;
+    y = dpp_u64<0x124>(x); x = y > x ? y : x;
+    y = dpp_u64<0x128>(x); x = y > x ? y : x;
+    unsigned long long m = readlane_u64(x, 0);
+    const unsigned long long m1 = readlane_u64(x, 16), m2 = readlane_u64(x, 32), m3 = readlane_u64(x, 48);
+    m = m1 > m ? m1 : m;
+    m = m2 > m ? m2 : m;
+    m = m3 > m ? m3 : m;
+    return m;
+}
+
+// Wavefront sum, the same value in every lane: within each row of 16 lanes by
+// DPP (quad_perm [1,0,3,2], quad_perm [2,3,0,1], row_ror:4, row_ror:8), then the
+// four row sums through lanes 0, 16, 32, 48 (fixed order, so every lane and
+// every call with the same inputs gives the same result).
 __device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, kWave);
-    return v;
+    v += dpp_f64<0xB1>(v);
+    v += dpp_f64<0x4E>(v);
+    v += dpp_f64<0x124>(v);
+    v += dpp_f64<0x128>(v);
+    return (readlane_f64(v, 0) + readlane_f64(v, 16)) + (readlane_f64(v, 32) + readlane_f64(v, 48));
 }
 __device__ __forceinline__ float wave_sumf(float v) {
 #pragma unroll
@@ -43,9 +97,11 @@ __device__ __forceinline__ float wave_sumf(float v) {
     return v;
 }
 __device__ __forceinline__ double wave_max(double v) {
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) v = fmax(v, __shfl_xor(v, off, kWave));
-    return v;
+    v = fmax(v, dpp_f64<0xB1>(v));
+    v = fmax(v, dpp_f64<0x4E>(v));
+    v = fmax(v, dpp_f64<0x124>(v));
+    v = fmax(v, dpp_f64<0x128>(v));
+    return fmax(fmax(readlane_f64(v, 0), readlane_f64(v, 16)), fmax(readlane_f64(v, 32), readlane_f64(v, 48)));
 }
 
 // Order LDS traffic between the lanes of one wavefront.  Every kernel here

@@ -20,24 +20,22 @@ __device__ bool lu_factor(T* A, int LD, int N, int* pmap) {
     }
     wave_sync();
     for (int k = 0; k < N; k++) {
-        double best = -1.0;
-        int bi = N;
+        // arg-max over rows i >= k as one 64-bit key: |A[i][k]| in the upper
+        // 57 bits, 127 - i in the low 7 (ties and near-ties go to the lowest row)
+        unsigned long long key = 0ull;
 #pragma unroll
         for (int s = 0; s < NS; s++) {
             int i = lane + kWave * s;
             if (i >= k && i < N) {
-                double v = fabs(static_cast<double>(A[i * LD + k]));
-                if (v > best) { best = v; bi = i; }
+                const unsigned long long kk =
+                    (static_cast<unsigned long long>(__double_as_longlong(fabs(static_cast<double>(A[i * LD + k])))) &
+                     ~127ull) |
+                    static_cast<unsigned long long>(127 - i);
+                key = kk > key ? kk : key;
             }
         }
-#pragma unroll
-        for (int off = 32; off >= 1; off >>= 1) {
-            double ob = __shfl_xor(best, off, kWave);
-            int oi = __shfl_xor(bi, off, kWave);
-            if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
-        }
-        const int p = uni(bi);
-        if (p >= N) return false;
+        const int p = 127 - static_cast<int>(wave_max_u64(key) & 127ull);
+        if (p < k || p >= N) return false;
         if (uni(A[p * LD + k]) == T(0)) return false;
         if (p != k) {
 #pragma unroll
