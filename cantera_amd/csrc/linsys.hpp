@@ -33,6 +33,18 @@ __device__ __forceinline__ T readlane_t(T v, int l) {
     }
 }
 
+template <typename T>
+__device__ __forceinline__ T shfl_t(T v, int src) {
+    if constexpr (sizeof(T) == 8) {
+        const long long x = __double_as_longlong(v);
+        const int lo = __shfl(static_cast<int>(x & 0xffffffffLL), src, kWave);
+        const int hi = __shfl(static_cast<int>(x >> 32), src, kWave);
+        return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+    } else {
+        return __shfl(v, src, kWave);
+    }
+}
+
 template <int NMAX, typename MT>
 struct RegLinSys {
     static constexpr int NS = 1;
@@ -124,33 +136,40 @@ struct RegLinSys {
                 }
             }
         }
+        // rows into pivot order (lane k holds the k-th pivot row), so the
+        // triangular solves index lanes by step with no lookups
+#pragma unroll
+        for (int j = 0; j < NMAX; j++) a[j] = shfl_t(a[j], pivlane);
+        rdiag = shfl_t(rdiag, pivlane);
+        step = lane;
         return true;
     }
 
     // b: this lane's component of the right-hand side; returns the solution
+    // component of the same lane.  With the rows in pivot order, P b is one
+    // shuffle and step k of both substitutions reads lane k.
     template <typename V>
     __device__ __forceinline__ V solve(V b, int N) const {
         const int lane = lane_id();
+        b = shfl_t(b, pivlane);
 #pragma unroll
         for (int k = 0; k < NMAX; k++) {
             if (k < N) {
-                const int pk = __builtin_amdgcn_readlane(pivlane, k);
-                const V yk = readlane_t(b, pk);
+                const V yk = readlane_t(b, k);
                 const V nb = b - static_cast<V>(a[k]) * yk;
-                b = (step > k) ? nb : b;
+                b = (lane > k) ? nb : b;
             }
         }
         const V rd = static_cast<V>(rdiag);
 #pragma unroll
         for (int k = NMAX - 1; k >= 0; k--) {
             if (k < N) {
-                const int pk = __builtin_amdgcn_readlane(pivlane, k);
-                const V xk = readlane_t(b * rd, pk);
+                const V xk = readlane_t(b * rd, k);
                 const V nb = b - static_cast<V>(a[k]) * xk;
-                b = (lane == pk) ? xk : ((step < k) ? nb : b);
+                b = (lane == k) ? xk : ((lane < k) ? nb : b);
             }
         }
-        return __shfl(b, pivlane, kWave);
+        return b;
     }
 
     template <typename V, int S>
