@@ -97,25 +97,32 @@ def flop_model(mech_path, stats):
 
 def measured_traffic(name, n):
     """HBM bytes per launch of the dominant kernel from a committed rocprofv3
-    --pmc summary (profiles/<name>_traffic.md).
+    --pmc summary (profiles/<name>_*.md holding a `traffic_n: <n>` line).
 
     The summary holds FETCH_SIZE and WRITE_SIZE (KiB per dispatch) of the
     kernel and of a plain copy kernel of known size; each counter is scaled by
-    (copy bytes / copy counter).  None when the summary is absent or was taken
-    at another batch size.
+    (copy bytes / copy counter).  None when no summary was taken at this batch
+    size.
     """
+    pdir = os.path.join(ROOT, "profiles")
     try:
-        kv = {}
-        with open(os.path.join(ROOT, "profiles", f"{name}_traffic.md")) as f:
-            for line in f:
-                if line.startswith("traffic_") and ":" in line:
-                    k, v = line.split(":", 1)
-                    kv[k.strip()] = float(v.split()[0])
-        if int(kv["traffic_n"]) != n:
-            return None
-        return kv["traffic_bytes_per_launch"]
-    except (OSError, KeyError, ValueError):
+        files = sorted(f for f in os.listdir(pdir) if f.startswith(name + "_") and f.endswith(".md"))
+    except OSError:
         return None
+    for fn in files:
+        try:
+            kv = {}
+            with open(os.path.join(pdir, fn)) as f:
+                for line in f:
+                    if line.startswith("traffic_") and ":" in line:
+                        k, v = line.split(":", 1)
+                        kv[k.strip()] = float(v.split()[0])
+            if int(kv["traffic_n"]) != n:
+                continue
+            return kv["traffic_bytes_per_launch"]
+        except (OSError, KeyError, ValueError):
+            continue
+    return None
 
 
 def cpu_threads():

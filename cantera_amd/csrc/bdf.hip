@@ -45,7 +45,7 @@ size_t bdf_lu_elems(int N, int ld) {
 size_t bdf_lds_bytes(const DevMech& m, int ld, bool fp32, bool global) {
     const int v = bdf_variant(m.N);
     const int ns = (v == 2) ? 2 : 1;
-    size_t bytes = bdf_scratch_doubles(m.K, m.I, ns) * sizeof(double);
+    size_t bytes = bdf_scratch_doubles(m.K, ns) * sizeof(double);
     if (v == 2) {
         bytes += static_cast<size_t>(kWave) * ns * (sizeof(double) + sizeof(int));
         bytes += static_cast<size_t>(m.N) * ld * (fp32 ? 4 : 8);

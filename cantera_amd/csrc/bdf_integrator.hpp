@@ -21,11 +21,17 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <cstddef>
+
 #include "bdf.hpp"
 #include "linsys.hpp"
 #include "wave_jac.hpp"
 
 namespace cg {
+
+// rhs_call and jac_call read the mechanism through kernarg_mech(): the
+// integrator kernels take one BdfArgs parameter, which must begin with it.
+static_assert(offsetof(BdfArgs, mech) == 0, "BdfArgs must begin with the mechanism");
 
 namespace bdfc {
 constexpr int QMAX = 5;
@@ -155,7 +161,7 @@ struct Integrator {
         for (int s = 0; s < NS; s++)
             if (comp(s) < N) S.yb[comp(s)] = yy[s];
         wave_sync();
-        rhs_call<NS>(m, L, uni(c.P));
+        rhs_call<NS>(kernarg_mech(), L, uni(c.P));
         int bad = 0;
 #pragma unroll
         for (int s = 0; s < NS; s++) {
@@ -197,7 +203,7 @@ struct Integrator {
                     S.F0[comp(s)] = ftemp[s];
                 }
             wave_sync();
-            jac_call<NS>(m, L, uni(c.P), Js);
+            jac_call<NS>(kernarg_mech(), L, uni(c.P), Js);
         }
         ls.form(Js, N, gamma);
         const bool ok = ls.factor(N);
@@ -331,14 +337,17 @@ struct Integrator {
         wave_sync();
     }
 
+    // zn[j] for a run-time j.  Every candidate is read and the choice is made
+    // on values: a branch per candidate lets the optimiser merge the reads into
+    // one load through a pointer into zn, which keeps the whole history array
+    // in scratch memory instead of registers.
     __device__ __forceinline__ void pick(int j, double (&out)[NS]) {
 #pragma unroll
-        for (int s = 0; s < NS; s++) out[s] = zn[0][s];
+        for (int s = 0; s < NS; s++) out[s] = opaque(zn[0][s]);
 #pragma unroll
         for (int t = 1; t <= bdfc::QMAX; t++)
-            if (t == j)
 #pragma unroll
-                for (int s = 0; s < NS; s++) out[s] = zn[t][s];
+            for (int s = 0; s < NS; s++) out[s] = (t == j) ? opaque(zn[t][s]) : out[s];
     }
 
     __device__ __forceinline__ void increase_bdf() {
@@ -373,9 +382,8 @@ struct Integrator {
         for (int s = 0; s < NS; s++) zL[s] = A1 * zn[QMAX][s];
 #pragma unroll
         for (int j = 2; j <= QMAX; j++)
-            if (j == q + 1)
 #pragma unroll
-                for (int s = 0; s < NS; s++) zn[j][s] = zL[s];
+            for (int s = 0; s < NS; s++) zn[j][s] = (j == q + 1) ? zL[s] : opaque(zn[j][s]);
 #pragma unroll
         for (int j = 2; j <= QMAX; j++)
             if (j <= q)
@@ -866,8 +874,8 @@ struct Integrator {
 
 // LDS per wavefront: state broadcast, species/reaction scratch, control block,
 // then the linear-system area (LDS policy only).
-__host__ __device__ inline size_t bdf_scratch_doubles(int K, int I, int ns) {
-    return static_cast<size_t>(kWave) * ns * 4 + K + K + I + K + 2 * K + kCtlDoubles;
+__host__ __device__ inline size_t bdf_scratch_doubles(int K, int ns) {
+    return static_cast<size_t>(kWave) * ns * 4 + K + K + K + 2 * K + kCtlDoubles;
 }
 
 template <int NS, class LS, typename MT>
@@ -881,7 +889,6 @@ __device__ __forceinline__ void bdf_advance_body(const BdfArgs& a) {
     L.F0 = off;  off += kWave * NS;
     L.C = off;   off += m.K;
     L.MU = off;  off += m.K;
-    L.ROP = off; off += m.I;
     L.WD = off;  off += m.K;
     L.W2 = off;  off += 2 * m.K;
     double* base = lds_base() + off;

@@ -22,6 +22,8 @@ namespace {
 // kLdsJ: assemble J in LDS (FP64 LDS atomics for the per-reaction terms), then
 // write it out with coalesced stores; otherwise assemble it in place in HBM
 // (L2 atomics) -- used when N x N doubles do not fit next to the scratch.
+// The mechanism is the first parameter: rhs_call reads it through
+// kernarg_mech().
 template <int NS, bool kLdsJ>
 __global__ __launch_bounds__(64) void jac_batch_kernel(DevMech m, long n, const double* __restrict__ T,
                                                        const double* __restrict__ P, const double* __restrict__ Y,
@@ -35,7 +37,6 @@ __global__ __launch_bounds__(64) void jac_batch_kernel(DevMech m, long n, const 
     L.F0 = off;  off += kWave * NS;
     L.C = off;   off += m.K;
     L.MU = off;  off += m.K;
-    L.ROP = off; off += m.I;
     L.WD = off;  off += m.K;
     L.W2 = off;  off += 2 * m.K;
     const WaveScratch S = scratch_view(L);
@@ -56,7 +57,7 @@ __global__ __launch_bounds__(64) void jac_batch_kernel(DevMech m, long n, const 
             if (i < m.N) S.yb[i] = v;
         }
         wave_sync();
-        rhs_call<NS>(m, L, Ps);
+        rhs_call<NS>(kernarg_mech(), L, Ps);
 #pragma unroll
         for (int s = 0; s < NS; s++) {
             const int i = lane + kWave * s;
@@ -80,7 +81,7 @@ __global__ __launch_bounds__(64) void jac_batch_kernel(DevMech m, long n, const 
 static bool jac_in_lds(int ns) { return ns == 1; }
 
 size_t jac_lds_bytes(const DevMech& m, int ns) {
-    size_t d = static_cast<size_t>(kWave) * ns * 4 + 2 * m.K + m.I + m.K + 2 * m.K;
+    size_t d = static_cast<size_t>(kWave) * ns * 4 + 2 * m.K + m.K + 2 * m.K;
     if (jac_in_lds(ns)) d += static_cast<size_t>(m.N) * m.N;
     return d * sizeof(double);
 }

@@ -28,7 +28,6 @@ __global__ __launch_bounds__(64) void rates_kernel(DevMech m, long n, const doub
     S.yb = base;  base += kWave * NS;
     S.C = base;   base += m.K;
     S.MU = base;  base += m.K;
-    S.ROP = base; base += m.I;
     S.WD = base;
     for (long st = blockIdx.x; st < n; st += gridDim.x) {
         const double Ts = uni(T[st]);
@@ -55,7 +54,7 @@ __global__ __launch_bounds__(64) void rates_kernel(DevMech m, long n, const doub
 }  // namespace
 
 size_t rates_lds_bytes(const DevMech& m, int ns) {
-    const size_t d = static_cast<size_t>(kWave) * ns + 2 * m.K + m.I + m.K;
+    const size_t d = static_cast<size_t>(kWave) * ns + 2 * m.K + m.K;
     return d * sizeof(double);
 }
 

@@ -37,6 +37,8 @@ __global__ void gather_states_kernel(int n, const int* __restrict__ list, const 
 // One guard per row still in play (done[i] < nsub[i], not rejected).
 // f(y0) of the first step is computed here; each kept step leaves f(y1) in
 // Fcur (internal order) for the next, so a step costs one RHS evaluation.
+// The mechanism is the first parameter: rhs_call reads it through
+// kernarg_mech().
 template <int NS>
 __global__ __launch_bounds__(64) void guard_kernel(DevMech m, int n, const int* __restrict__ nsub,
                                                    int* __restrict__ done, int* __restrict__ rejected,
@@ -52,7 +54,6 @@ __global__ __launch_bounds__(64) void guard_kernel(DevMech m, int n, const int* 
     L.F0 = off;  off += kWave * NS;
     L.C = off;   off += m.K;
     L.MU = off;  off += m.K;
-    L.ROP = off; off += m.I;
     L.WD = off;  off += m.K;
     L.W2 = off;
     const WaveScratch S = scratch_view(L);
@@ -111,7 +112,7 @@ __global__ __launch_bounds__(64) void guard_kernel(DevMech m, int n, const int* 
                     if (c < N) S.yb[c] = y0[s];
                 }
                 wave_sync();
-                rhs_call<NS>(m, L, P);
+                rhs_call<NS>(kernarg_mech(), L, P);
 #pragma unroll
                 for (int s = 0; s < NS; s++) {
                     const int c = lane + kWave * s;
@@ -131,7 +132,7 @@ __global__ __launch_bounds__(64) void guard_kernel(DevMech m, int n, const int* 
                 if (c < N) S.yb[c] = y1[s];
             }
             wave_sync();
-            rhs_call<NS>(m, L, P);
+            rhs_call<NS>(kernarg_mech(), L, P);
 #pragma unroll
             for (int s = 0; s < NS; s++) {
                 const int c = lane + kWave * s;
@@ -203,7 +204,7 @@ __global__ void restart_kernel(int n, const int* __restrict__ list, const int* _
 }  // namespace
 
 size_t guard_lds_bytes(const DevMech& m, int ns) {
-    const size_t d = static_cast<size_t>(kWave) * ns * 4 + 2 * m.K + m.I + m.K + 2 * m.K;
+    const size_t d = static_cast<size_t>(kWave) * ns * 4 + 2 * m.K + m.K + 2 * m.K;
     return d * sizeof(double);
 }
 

@@ -238,7 +238,8 @@ __device__ void reactor_jacobian(const DevMech& m, const WaveScratch& S, double 
 
 // Out-of-line Jacobian: expansion point in S.Y0, its RHS in S.F0.
 template <int NS>
-__device__ __noinline__ void jac_call(const DevMech& m, ScratchLayout L, double P, double* J) {
+__device__ __noinline__ void jac_call(KernargMech* mech, ScratchLayout L, double P, double* J) {
+    const DevMech m = *(const DevMech*)mech;
     const WaveScratch S = scratch_view(L);
     const int lane = lane_id();
     double y[NS], f0[NS];

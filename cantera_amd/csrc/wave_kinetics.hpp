@@ -42,6 +42,14 @@ __device__ __forceinline__ double dpp_f64(double x) {
     return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
 }
 
+// The value unchanged, but opaque to the optimiser: it cannot trace it back to
+// the load that produced it (used to stop selects between array elements from
+// being turned into a load through a selected pointer).
+__device__ __forceinline__ double opaque(double x) {
+    asm("" : "+v"(x));
+    return x;
+}
+
 __device__ __forceinline__ double readlane_f64(double v, int l) {
     const long long x = __double_as_longlong(v);
     const int lo = __builtin_amdgcn_readlane(static_cast<int>(x & 0xffffffffll), l);
@@ -143,7 +151,6 @@ struct WaveScratch {
     double* yb;    // N (state broadcast)
     double* C;     // K concentrations
     double* MU;    // K standard chemical potentials [J/kmol]
-    double* ROP;   // I net rates of progress
     double* WD;    // K production-rate accumulators
     double* W2;    // 2K (Jacobian: wdot, D)
     double* F;     // N (right-hand side out)
@@ -155,7 +162,7 @@ struct WaveScratch {
 // passing to out-of-line device functions: pointers rebuilt from the LDS base
 // inside the callee keep the LDS address space (ds_* instructions).
 struct ScratchLayout {
-    int yb, C, MU, ROP, WD, W2, F, Y0, F0;
+    int yb, C, MU, WD, W2, F, Y0, F0;
 };
 
 __device__ __forceinline__ double* lds_base() {
@@ -166,7 +173,7 @@ __device__ __forceinline__ double* lds_base() {
 __device__ __forceinline__ WaveScratch scratch_view(const ScratchLayout& L) {
     double* b = lds_base();
     WaveScratch S;
-    S.yb = b + L.yb; S.C = b + L.C; S.MU = b + L.MU; S.ROP = b + L.ROP; S.WD = b + L.WD;
+    S.yb = b + L.yb; S.C = b + L.C; S.MU = b + L.MU; S.WD = b + L.WD;
     S.W2 = b + L.W2; S.F = b + L.F; S.Y0 = b + L.Y0; S.F0 = b + L.F0;
     return S;
 }
@@ -441,7 +448,6 @@ __device__ __forceinline__ void production_rates(const DevMech& m, const WaveScr
         load_rxn(m, p, x);
         eval_reaction(m, p, g, S.C, S.MU, x, v);
         const double r = v.ropf - v.ropr;
-        S.ROP[p] = r;
         const double* gs = (x.gen >= 0) ? m.gend + 12 * x.gen : m.gend;
         const bool rgen = x.flags & 8, pgen = x.flags & 16;
 #pragma unroll
@@ -508,11 +514,23 @@ __device__ __forceinline__ void reactor_rhs(const DevMech& m, const WaveScratch&
     }
 }
 
+// The mechanism of a kernel whose first parameter is a DevMech, or a struct
+// that begins with one, addressed in the kernel-argument segment.  Out-of-line
+// functions take this pointer rather than a reference to the kernel's copy of
+// the argument: a reference would make the kernel keep that copy in scratch
+// memory and read every field through it, while the kernel-argument segment is
+// read with scalar loads.
+using KernargMech = const __attribute__((address_space(4))) DevMech;
+__device__ __forceinline__ KernargMech* kernarg_mech() {
+    return (KernargMech*)__builtin_amdgcn_kernarg_segment_ptr();
+}
+
 // Out-of-line reactor RHS: state taken from S.yb (written by the caller),
 // ydot written to S.F.  One copy of the kinetics code serves every call site of
 // the integrator (smaller code, and the caller keeps little live across it).
 template <int NS>
-__device__ __noinline__ void rhs_call(const DevMech& m, ScratchLayout L, double P) {
+__device__ __noinline__ void rhs_call(KernargMech* mech, ScratchLayout L, double P) {
+    const DevMech m = *(const DevMech*)mech;
     const WaveScratch S = scratch_view(L);
     const int lane = lane_id();
     double y[NS], f[NS];
