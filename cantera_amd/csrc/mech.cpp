@@ -159,7 +159,11 @@ DeviceMech::DeviceMech(const HostMech& h) : h_(h) {
     if (K > 126) throw std::runtime_error("mechanisms with more than 126 species are not supported");
     perm_.resize(I);
     std::iota(perm_.begin(), perm_.end(), 0);
-    std::stable_sort(perm_.begin(), perm_.end(), [&](int a, int b) { return h.rxn[a].type < h.rxn[b].type; });
+    // Reactions of the costliest class first: a wavefront evaluates 64
+    // consecutive reactions, so the falloff reactions share one pass (with the
+    // three-body ones and the first elementary ones) instead of leaving a last
+    // pass that runs the Troe code for a few lanes.
+    std::stable_sort(perm_.begin(), perm_.end(), [&](int a, int b) { return h.rxn[a].type > h.rxn[b].type; });
 
     std::vector<double> rd(4 * static_cast<size_t>(I)), tbdef(I), fod, gend;
     std::vector<RxnI> ri(I);

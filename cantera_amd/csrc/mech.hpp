@@ -63,7 +63,7 @@ static_assert(sizeof(RxnI) == 48, "RxnI layout");
 struct DevMech {
     int K, I, N, E;          // species, reactions, state size (K+2), elements
     int nsub;                // wdot gather sub-rows
-    int n_el, n_tb, n_fo;    // reactions of each class, in permuted order: [elementary | three-body | falloff]
+    int n_el, n_tb, n_fo;    // reactions of each class, in permuted order: [Troe | Lindemann | three-body | elementary]
     double pref;
     const double* mw;        // K
     const double* rmw;       // K
