@@ -11,7 +11,9 @@ region covers steps 1..K of a fresh batch from t = 0, with all inputs already
 resident in HBM.  One JSON line is printed.
 
 Other workloads of the same path, one JSON line each:
-  gri30-rates        1,048,576 gri30 states, net production rates (states/s)
+  gri30-rates        1,048,576 gri30 states, net production rates (states/s);
+                     --precision fp32 adds the FP32 rate-constant option and its
+                     error against the FP64 kernel
   gri30-jacobian     1,048,576 gri30 states, reactor Jacobians (states/s)
   ndodecane-reactors 262,144 n-dodecane reactors x 100 steps of 1e-6 s
   h2o2-reactors      1,024 h2o2 reactors x 100 outputs every 1e-5 s
@@ -175,6 +177,8 @@ def reactors(a):
         desc = (f"h2o2 ({sol.n_species} species, {sol.n_reactions} reactions) IdealGasConstPressureReactor, {n} "
                 f"reactors x {steps} outputs "
                 f"every 1e-5 s, T0 U[1000,1800] K, H2/air phi U[0.5,2], 1 atm, rtol 1e-8 atol 1e-15")
+    if a.precision == "fp32":
+        raise SystemExit("--precision fp32 applies to gri30-rates only (reactors: fp64 or mixed)")
     prec = ca.CG_FP64 if a.precision == "fp64" else ca.CG_MIXED
 
     # all output times resident in HBM before the timed region
@@ -277,16 +281,34 @@ def batch_kernels(a):
     dT, dP, dY = d_in, d_in + n * 8, d_in + 2 * n * 8
     call = L.cg_jac_batch_device if jac else L.cg_rates_batch_device
     last = L.cg_jac_last_kernel_ms if jac else L.cg_rates_last_kernel_ms
+    f32 = a.precision == "fp32"
+    if f32 and jac:
+        raise SystemExit("--precision fp32 applies to gri30-rates only")
+    prec = ca.CG_FP32 if f32 else ca.CG_FP64
     for _ in range(a.warmup):
-        check(call(sol.handle, n, dT, dP, dY, d_out, ca.CG_FP64, None))
+        check(call(sol.handle, n, dT, dP, dY, d_out, prec, None))
     check(L.cg_device_synchronize())
     kernel_ms = []
     t0 = time.perf_counter()
     for _ in range(steps):
-        check(call(sol.handle, n, dT, dP, dY, d_out, ca.CG_FP64, None))
+        check(call(sol.handle, n, dT, dP, dY, d_out, prec, None))
         kernel_ms.append(last(sol.handle))
     check(L.cg_device_synchronize())
     elapsed = time.perf_counter() - t0
+    fp32_error = None
+    if f32:
+        # error of the FP32 option against the FP64 kernel on the same states
+        # (untimed): per state, max_k |wdot32 - wdot64| / max_k |wdot64|
+        w32 = np.empty((K, n))
+        check(L.cg_memcpy_d2h(w32.ctypes.data_as(C.c_void_p), d_out, w32.nbytes))
+        check(call(sol.handle, n, dT, dP, dY, d_out, ca.CG_FP64, None))
+        check(L.cg_device_synchronize())
+        w64 = np.empty((K, n))
+        check(L.cg_memcpy_d2h(w64.ctypes.data_as(C.c_void_p), d_out, w64.nbytes))
+        e = np.abs(w32 - w64).max(axis=0) / (np.abs(w64).max(axis=0) + 1e-300)
+        fp32_error = {"per_state_max_rel_to_largest_wdot": {"max": float(e.max()), "p99": float(np.percentile(e, 99)),
+                                                             "median": float(np.median(e))}}
+        del w32, w64
     fl, tr, _ = rhs_cost(sol.path)
     F_rates = fl + TRANS_FLOPS * tr
     avg_s = float(np.mean(kernel_ms)) / 1e3
@@ -304,7 +326,7 @@ def batch_kernels(a):
                 "unit": "TFLOP/s", "frac": flops / avg_s / 1e12 / FP64_PEAK_TFLOPS,
                 "traffic": measured_traffic(name, n), "kernel": "rates_kernel", "avg_launch_ms": avg_s * 1e3,
                 "flops_per_launch": flops, "hbm_bytes_algorithmic": n * 8 * (2 + 2 * K)}
-        metric = "gri30 net production rates, states/s (FP64)"
+        metric = "gri30 net production rates, states/s (" + ("FP32 rate constants" if f32 else "FP64") + ")"
     out = {
         "metric": metric, "value": n * steps / elapsed, "unit": "states/s", "n_gpus": 1, "steps": steps,
         "warmup": a.warmup, "ms_per_step": elapsed / steps * 1e3, "higher_is_better": True, "scaling": "weak",
@@ -313,6 +335,9 @@ def batch_kernels(a):
                                f"phi U[0.3,3], minors X 1e-12..1e-2; species-major SoA in HBM"},
         "roofline": roof,
     }
+    if f32:
+        out["dtype"] = "f32 rate-constant transcendentals, f64 otherwise"
+        out["fp32_error_vs_fp64"] = fp32_error
     if not a.no_cpu:
         from oracle.oracle import Mech
         om = Mech(sol.path)
@@ -356,7 +381,8 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=None)
-    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--precision", choices=["fp64", "mixed", "fp32"], default="fp64",
+                    help="reactors: fp64 or mixed (FP32 LU); gri30-rates: fp64 or fp32 (FP32 rate constants)")
     ap.add_argument("--waves-per-cu", type=int, default=0)
     ap.add_argument("--schedule", choices=["longest-first", "index"], default="longest-first")
     ap.add_argument("--matrix-storage", choices=["default", "lds", "global"], default="default")

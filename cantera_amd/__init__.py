@@ -21,11 +21,11 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._lib import CG_FP64, CG_MIXED, CanteraGpuError, check, lib
+from ._lib import CG_FP32, CG_FP64, CG_MIXED, CanteraGpuError, check, lib
 
 MECH_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "mechanisms")
 
-__all__ = ["Solution", "ReactorBatch", "CanteraGpuError", "CG_FP64", "CG_MIXED", "mech_path", "lib"]
+__all__ = ["Solution", "ReactorBatch", "CanteraGpuError", "CG_FP64", "CG_MIXED", "CG_FP32", "mech_path", "lib"]
 
 
 def mech_path(name: str) -> str:
@@ -85,14 +85,15 @@ class Solution:
         Y = X * self.molecular_weights
         return Y / Y.sum(axis=-1, keepdims=True)
 
-    def net_production_rates(self, T, P, Y) -> np.ndarray:
-        """wdot [kmol/m^3/s] for n states.  Y is species-major (K, n); returns (K, n)."""
+    def net_production_rates(self, T, P, Y, precision=CG_FP64) -> np.ndarray:
+        """wdot [kmol/m^3/s] for n states.  Y is species-major (K, n); returns (K, n).
+        precision CG_FP32 evaluates the rate constants' transcendentals in FP32."""
         T = np.ascontiguousarray(np.atleast_1d(T), dtype=np.float64)
         n = T.shape[0]
         P = np.ascontiguousarray(np.broadcast_to(np.asarray(P, dtype=np.float64), (n,)))
         Y = np.ascontiguousarray(np.asarray(Y, dtype=np.float64).reshape(self.n_species, n))
         wdot = np.zeros((self.n_species, n))
-        check(lib().cg_rates_batch(self._h, n, T, P, Y, wdot, CG_FP64), "cg_rates_batch")
+        check(lib().cg_rates_batch(self._h, n, T, P, Y, wdot, int(precision)), "cg_rates_batch")
         return wdot
 
     def last_rates_kernel_ms(self) -> float:

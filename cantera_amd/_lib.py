@@ -14,6 +14,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libcantera_amd.so")
 CG_FP64 = 0
 CG_MIXED = 1
+CG_FP32 = 2
 
 _lib = None
 

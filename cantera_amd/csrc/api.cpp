@@ -250,17 +250,19 @@ int32_t cg_rates_batch_device(int32_t h, int64_t n, const double* T, const doubl
                               double* wdot, int32_t precision, void* stream) {
     return guarded([&] {
         MechObj& mo = get_mech(h);
-        if (precision != CG_FP64) throw std::runtime_error("cg_rates_batch: only CG_FP64 is available");
+        if (precision != CG_FP64 && precision != CG_FP32)
+            throw std::runtime_error("cg_rates_batch: precision must be CG_FP64 or CG_FP32");
+        const bool f32 = precision == CG_FP32;
         if (n <= 0) return 0;
         if (!T || !P || !Y || !wdot) throw std::runtime_error("null array");
         const DevMech& m = mo.dm->view();
         const size_t lds = rates_lds_bytes(m, mo.ns);
-        const void* kp = rates_kernel_ptr(mo.ns);
+        const void* kp = rates_kernel_ptr(mo.ns, f32);
         set_lds_limit(kp, lds);
         const int grid = grid_for(kp, lds, 0, n);
         hipStream_t s = static_cast<hipStream_t>(stream);
         check(hipEventRecord(mo.e0, s), "hipEventRecord");
-        check(rates_launch(m, mo.ns, n, T, P, Y, wdot, grid, lds, s), "rates kernel launch");
+        check(rates_launch(m, mo.ns, f32, n, T, P, Y, wdot, grid, lds, s), "rates kernel launch");
         check(hipEventRecord(mo.e1, s), "hipEventRecord");
         return 0;
     });

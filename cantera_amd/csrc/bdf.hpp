@@ -54,9 +54,10 @@ const void* bdf_kernel_ptr(int N, bool fp32, bool global);
 
 // batched rate evaluation (species-major SoA in and out)
 size_t rates_lds_bytes(const DevMech& m, int ns);
-hipError_t rates_launch(const DevMech& m, int ns, long n, const double* T, const double* P, const double* Y,
-                        double* wdot, int grid, size_t lds, hipStream_t stream);
-const void* rates_kernel_ptr(int ns);
+// f32: rate constants through FP32 transcendentals (CG_FP32)
+hipError_t rates_launch(const DevMech& m, int ns, bool f32, long n, const double* T, const double* P,
+                        const double* Y, double* wdot, int grid, size_t lds, hipStream_t stream);
+const void* rates_kernel_ptr(int ns, bool f32);
 
 // longest-first processing order for the integrator's work queue
 size_t order_temp_bytes(int n);

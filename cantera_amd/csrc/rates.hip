@@ -8,6 +8,7 @@
 // neighbouring wavefronts, so the 64-byte lines of Y and wdot (8 states of one
 // species) are shared through L2.  The LDS scratch is small (a few KB), so many
 // wavefronts per CU hide the latency of the table loads and transcendentals.
+// kF32: the rate constants' transcendentals in FP32 (see eval_reaction).
 #include <hip/hip_runtime.h>
 
 #include "bdf.hpp"
@@ -17,7 +18,7 @@ namespace cg {
 
 namespace {
 
-template <int NS>
+template <int NS, bool kF32>
 __global__ __launch_bounds__(64) void rates_kernel(DevMech m, long n, const double* __restrict__ T,
                                                    const double* __restrict__ P, const double* __restrict__ Y,
                                                    double* __restrict__ wdot) {
@@ -41,7 +42,7 @@ __global__ __launch_bounds__(64) void rates_kernel(DevMech m, long n, const doub
         GasState g;
         SpeciesRegs<NS> sr;
         species_pass<NS>(m, S, Ts, Ps, yk, g, sr);
-        production_rates<NS>(m, S, g, sr);
+        production_rates<NS, kF32>(m, S, g, sr);
 #pragma unroll
         for (int s = 0; s < NS; s++) {
             const int k = lane + kWave * s;
@@ -58,17 +59,25 @@ size_t rates_lds_bytes(const DevMech& m, int ns) {
     return d * sizeof(double);
 }
 
-hipError_t rates_launch(const DevMech& m, int ns, long n, const double* T, const double* P, const double* Y,
-                        double* wdot, int grid, size_t lds, hipStream_t stream) {
-    if (ns == 1) hipLaunchKernelGGL((rates_kernel<1>), dim3(grid), dim3(64), lds, stream, m, n, T, P, Y, wdot);
-    else if (ns == 2) hipLaunchKernelGGL((rates_kernel<2>), dim3(grid), dim3(64), lds, stream, m, n, T, P, Y, wdot);
+hipError_t rates_launch(const DevMech& m, int ns, bool f32, long n, const double* T, const double* P,
+                        const double* Y, double* wdot, int grid, size_t lds, hipStream_t stream) {
+    if (ns == 1 && !f32)
+        hipLaunchKernelGGL((rates_kernel<1, false>), dim3(grid), dim3(64), lds, stream, m, n, T, P, Y, wdot);
+    else if (ns == 1)
+        hipLaunchKernelGGL((rates_kernel<1, true>), dim3(grid), dim3(64), lds, stream, m, n, T, P, Y, wdot);
+    else if (ns == 2 && !f32)
+        hipLaunchKernelGGL((rates_kernel<2, false>), dim3(grid), dim3(64), lds, stream, m, n, T, P, Y, wdot);
+    else if (ns == 2)
+        hipLaunchKernelGGL((rates_kernel<2, true>), dim3(grid), dim3(64), lds, stream, m, n, T, P, Y, wdot);
     else return hipErrorInvalidValue;
     return hipGetLastError();
 }
 
-const void* rates_kernel_ptr(int ns) {
-    return ns == 1 ? reinterpret_cast<const void*>(&rates_kernel<1>)
-                   : reinterpret_cast<const void*>(&rates_kernel<2>);
+const void* rates_kernel_ptr(int ns, bool f32) {
+    if (ns == 1) return f32 ? reinterpret_cast<const void*>(&rates_kernel<1, true>)
+                            : reinterpret_cast<const void*>(&rates_kernel<1, false>);
+    return f32 ? reinterpret_cast<const void*>(&rates_kernel<2, true>)
+               : reinterpret_cast<const void*>(&rates_kernel<2, false>);
 }
 
 }  // namespace cg

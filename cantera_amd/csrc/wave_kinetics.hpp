@@ -233,7 +233,36 @@ __device__ __forceinline__ void load_rxn(const DevMech& m, int p, RxnRegs& x) {
     x.pk[2] = static_cast<short>(c.w & 0xffff); x.pk[3] = static_cast<short>(c.w >> 16);
 }
 
+// Transcendentals of the rate constants.  kF32: evaluated in FP32 (hardware
+// exp2/log2 based) on FP64 arguments rounded to FP32, widened back to FP64;
+// arguments whose result would leave the FP32 range take the FP64 function.
+// The equilibrium constants never use these (their exp stays FP64).
+template <bool kF32>
+__device__ __forceinline__ double rate_exp(double arg) {
+    if constexpr (kF32) {
+        if (fabs(arg) < 80.0) return static_cast<double>(expf(static_cast<float>(arg)));
+    }
+    return exp(arg);
+}
+template <bool kF32>
+__device__ __forceinline__ double rate_log10(double x) {
+    if constexpr (kF32) {
+        if (x > 1e-35 && x < 1e35) return static_cast<double>(log10f(static_cast<float>(x)));
+    }
+    return log10(x);
+}
+template <bool kF32>
+__device__ __forceinline__ double rate_exp10(double x) {
+    if constexpr (kF32) {
+        if (fabs(x) < 35.0) return static_cast<double>(exp10f(static_cast<float>(x)));
+    }
+    return exp10(x);
+}
+
 // Evaluate reaction p (permuted index) at the gas state; C and MU live in LDS.
+// kF32: rate constants (Arrhenius, falloff centre and blending) through the
+// FP32 transcendentals above; everything else FP64.
+template <bool kF32 = false>
 __device__ __forceinline__ void eval_reaction(const DevMech& m, int p, const GasState& g,
                                               const double* __restrict__ C, const double* __restrict__ MU,
                                               const RxnRegs& x, RxnVals& o) {
@@ -265,26 +294,26 @@ __device__ __forceinline__ void eval_reaction(const DevMech& m, int p, const Gas
     }
     double kf;
     if (type <= kThreeBody) {
-        kf = rd.x * exp(rd.y * g.logT - rd.z * g.recipT);
+        kf = rd.x * rate_exp<kF32>(rd.y * g.logT - rd.z * g.recipT);
     } else {
         const double* fo = m.fod + 8 * x.fo;
         double work = 0.0;
         if (type == kTroe) {
-            double Fcent = (1.0 - fo[3]) * exp(-g.T * fo[4]) + fo[3] * exp(-g.T * fo[5]);
-            if (fo[6] != 0.0) Fcent += exp(-fo[6] / g.T);
-            work = log10(fmax(Fcent, kSmallNumber));
+            double Fcent = (1.0 - fo[3]) * rate_exp<kF32>(-g.T * fo[4]) + fo[3] * rate_exp<kF32>(-g.T * fo[5]);
+            if (fo[6] != 0.0) Fcent += rate_exp<kF32>(-fo[6] / g.T);
+            work = rate_log10<kF32>(fmax(Fcent, kSmallNumber));
         }
-        double rc_low = fo[0] * exp(fo[1] * g.logT - fo[2] * g.recipT);
-        double rc_high = rd.x * exp(rd.y * g.logT - rd.z * g.recipT);
+        double rc_low = fo[0] * rate_exp<kF32>(fo[1] * g.logT - fo[2] * g.recipT);
+        double rc_high = rd.x * rate_exp<kF32>(rd.y * g.logT - rd.z * g.recipT);
         double pr = M * rc_low / (rc_high + kSmallNumber);
         double F = 1.0;
         if (type == kTroe) {
-            double lpr = log10(fmax(pr, kSmallNumber));
+            double lpr = rate_log10<kF32>(fmax(pr, kSmallNumber));
             double cc = -0.4 - 0.67 * work;
             double nn = 0.75 - 1.27 * work;
             double f1 = (lpr + cc) / (nn - 0.14 * (lpr + cc));
             double lgf = work / (1.0 + f1 * f1);
-            F = exp10(lgf);
+            F = rate_exp10<kF32>(lgf);
         }
         pr *= F / (1.0 + pr);
         kf = pr * rc_high;
@@ -432,7 +461,7 @@ __device__ __forceinline__ void species_pass(const DevMech& m, const WaveScratch
 // products with LDS FP64 atomics (the species lists are already in registers),
 // so no second pass over a gather table is needed; the summation order of a
 // species' terms is the hardware's, not the reference's reaction order.
-template <int NS>
+template <int NS, bool kF32 = false>
 __device__ __forceinline__ void production_rates(const DevMech& m, const WaveScratch& S, const GasState& g,
                                                  SpeciesRegs<NS>& sr) {
     const int lane = lane_id();
@@ -446,7 +475,7 @@ __device__ __forceinline__ void production_rates(const DevMech& m, const WaveScr
         RxnVals v;
         RxnRegs x;
         load_rxn(m, p, x);
-        eval_reaction(m, p, g, S.C, S.MU, x, v);
+        eval_reaction<kF32>(m, p, g, S.C, S.MU, x, v);
         const double r = v.ropf - v.ropr;
         const double* gs = (x.gen >= 0) ? m.gend + 12 * x.gen : m.gend;
         const bool rgen = x.flags & 8, pgen = x.flags & 16;

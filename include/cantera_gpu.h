@@ -30,7 +30,9 @@ extern "C" {
 
 enum cg_precision {
     CG_FP64 = 0,   /* plain: everything in FP64 */
-    CG_MIXED = 1   /* FP32 iteration matrix / LU / solves, FP64 residual, state and step control */
+    CG_MIXED = 1,  /* FP32 iteration matrix / LU / solves, FP64 residual, state and step control */
+    CG_FP32 = 2    /* rates only: the rate constants' exp/log10/exp10 (Arrhenius, falloff) in FP32;
+                      thermo, equilibrium constants, mass action and wdot in FP64; FP64 I/O */
 };
 
 /* ---- library ------------------------------------------------------------ */
@@ -73,7 +75,7 @@ int32_t cg_mech_molecular_weights(int32_t mech, int32_t len, double* mw);
  * interfaces/sourcegen/headers/ctkin.yaml:46) with the state set as
  * IdealGasConstPressureReactor::updateState does
  * (src/zeroD/IdealGasConstPressureReactor.cpp:34-44).
- * Host pointers; copies in and out.  precision: CG_FP64. */
+ * Host pointers; copies in and out.  precision: CG_FP64 or CG_FP32. */
 int32_t cg_rates_batch(int32_t mech, int64_t n, const double* T, const double* P,
                        const double* Y, double* wdot, int32_t precision);
 /* Same on device pointers, no copies, asynchronous on `stream` (hipStream_t,
