@@ -186,25 +186,43 @@ def reactors(a):
     d_tout = L.cg_dev_alloc(touts.nbytes)
     check(L.cg_memcpy_h2d(d_tout, touts.ctypes.data_as(C.c_void_p), touts.nbytes))
 
-    if a.warmup > 0:
+    tuned = None
+    if a.warmup > 0 or a.autotune:
         wb = ca.ReactorBatch(sol, wl.T0, wl.P, wl.Y0, rtol=wl.rtol, atol=wl.atol, precision=prec)
         wb.set_options(0, a.waves_per_cu)
-        for k in range(min(a.warmup, steps)):
+        done = min(max(a.warmup, 1 if a.autotune else 0), steps)
+        for k in range(done):
             check(L.cg_reactors_advance_device(wb.handle, C.c_void_p(d_tout + k * n * 8)))
         check(L.cg_device_synchronize())
+        if a.autotune:
+            # measured launch choice on the warm-up batch's next step (the
+            # first step from t = 0 is a start-up transient, not typical)
+            probe = touts[done] if done < steps else touts[-1] + wl.dt
+            choice, times = wb.autotune(probe)
+            tuned = {"matrix_storage": ["default", "lds", "global"][choice[0]], "waves_per_cu": choice[1],
+                     "candidate_ms": [round(float(x), 3) for x in times]}
         del wb
 
     rb = ca.ReactorBatch(sol, wl.T0, wl.P, wl.Y0, rtol=wl.rtol, atol=wl.atol, precision=prec)
     rb.set_options(0, a.waves_per_cu)
     rb.set_schedule(a.schedule == "longest-first")
     rb.set_matrix_storage(a.matrix_storage)
+    if tuned:
+        rb.set_matrix_storage(tuned["matrix_storage"])
+        rb.set_options(0, tuned["waves_per_cu"])
     check(L.cg_device_synchronize())
     kernel_ms = []
     t0 = time.perf_counter()
-    for k in range(steps):
-        check(L.cg_reactors_advance_device(rb.handle, C.c_void_p(d_tout + k * n * 8)))
-        kernel_ms.append(rb.last_kernel_ms())
-        print(f"step {k + 1}/{steps}: kernel {kernel_ms[-1]:.2f} ms", file=sys.stderr, flush=True)
+    if a.graph:
+        # all K advances captured as one HIP graph (capture included in the timing)
+        check(L.cg_reactors_advance_graph(rb.handle, steps, C.c_void_p(d_tout)))
+        kernel_ms = [rb.last_kernel_ms() / steps]
+        print(f"{steps} steps in one graph: {kernel_ms[0] * steps:.2f} ms", file=sys.stderr, flush=True)
+    else:
+        for k in range(steps):
+            check(L.cg_reactors_advance_device(rb.handle, C.c_void_p(d_tout + k * n * 8)))
+            kernel_ms.append(rb.last_kernel_ms())
+            print(f"step {k + 1}/{steps}: kernel {kernel_ms[-1]:.2f} ms", file=sys.stderr, flush=True)
     check(L.cg_device_synchronize())
     elapsed = time.perf_counter() - t0
     bad = int((rb.status != 0).sum())
@@ -234,6 +252,8 @@ def reactors(a):
             "parallelism": "1 GPU, one wavefront per reactor, persistent work-queue kernel",
             "schedule": a.schedule,
             "matrix_storage": a.matrix_storage,
+            "launch": "one HIP graph of all steps" if a.graph else "one advance call per step",
+            "autotune": tuned,
         },
         "roofline": {
             "bound": "mfma",
@@ -386,6 +406,10 @@ def main(argv=None):
     ap.add_argument("--waves-per-cu", type=int, default=0)
     ap.add_argument("--schedule", choices=["longest-first", "index"], default="longest-first")
     ap.add_argument("--matrix-storage", choices=["default", "lds", "global"], default="default")
+    ap.add_argument("--graph", action="store_true",
+                    help="reactor workloads: run the K steps as one captured HIP graph")
+    ap.add_argument("--autotune", action="store_true",
+                    help="reactor workloads: pick the integrator launch by timing the candidates first")
     ap.add_argument("--cpu-sample", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",

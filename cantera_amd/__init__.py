@@ -132,6 +132,7 @@ class ReactorBatch:
     def __del__(self):
         h = getattr(self, "_h", None)
         if h is not None and _lib._lib is not None:
+            self._free_tbuf()
             _lib._lib.cg_reactors_free(h)
             self._h = None
 
@@ -164,6 +165,55 @@ class ReactorBatch:
         t = np.ascontiguousarray(np.broadcast_to(np.asarray(t, dtype=np.float64), (self.n,)))
         check(lib().cg_reactors_advance(self._h, t), "cg_reactors_advance")
         self.time = t.copy()
+
+    def advance_steps(self, t_steps):
+        """Advance through the rows of t_steps (n_steps x n output times, or
+        n_steps scalars) with one replayed HIP graph of n_steps integrator
+        launches (cg_reactors_advance_graph).  The device copy of the times is
+        kept, so later calls with the same number of steps replay the graph."""
+        t = np.asarray(t_steps, dtype=np.float64)
+        if t.ndim == 1:
+            t = np.repeat(t[:, None], self.n, axis=1)
+        t = np.ascontiguousarray(t.reshape(-1, self.n))
+        L = lib()
+        if getattr(self, "_tbuf", None) is None or self._tbuf[1] != t.nbytes:
+            self._free_tbuf()
+            p = L.cg_dev_alloc(t.nbytes)
+            if not p:
+                check(-1, "cg_dev_alloc")
+            self._tbuf = (p, t.nbytes)
+        check(L.cg_memcpy_h2d(self._tbuf[0], t.ctypes.data_as(C.c_void_p), t.nbytes), "cg_memcpy_h2d")
+        check(L.cg_reactors_advance_graph(self._h, t.shape[0], self._tbuf[0]), "cg_reactors_advance_graph")
+        st = np.zeros(self.n, dtype=np.int32)
+        bad = check(L.cg_reactors_status(self._h, st), "cg_reactors_status")
+        if bad:
+            raise CanteraGpuError(f"{bad} reactor(s) failed to integrate; see ReactorBatch.status")
+        self.time = t[-1].copy()
+
+    def autotune(self, t_probe):
+        """Time each integrator launch candidate on one advance to t_probe
+        (scalar or per reactor) from the current state, restore the state and
+        keep the fastest (cg_reactors_autotune).  Returns (choice, times_ms):
+        choice = (matrix storage, waves per CU)."""
+        t = np.ascontiguousarray(np.broadcast_to(np.asarray(t_probe, dtype=np.float64), (self.n,)))
+        L = lib()
+        p = L.cg_dev_alloc(t.nbytes)
+        if not p:
+            check(-1, "cg_dev_alloc")
+        try:
+            check(L.cg_memcpy_h2d(p, t.ctypes.data_as(C.c_void_p), t.nbytes), "cg_memcpy_h2d")
+            choice = np.zeros(2, dtype=np.int32)
+            times = np.full(8, -1.0)
+            nc = check(L.cg_reactors_autotune(self._h, p, choice.ctypes.data_as(C.c_void_p),
+                                              times.ctypes.data_as(C.c_void_p), 8), "cg_reactors_autotune")
+        finally:
+            L.cg_dev_free(p)
+        return (int(choice[0]), int(choice[1])), times[:nc]
+
+    def _free_tbuf(self):
+        if getattr(self, "_tbuf", None) is not None:
+            lib().cg_dev_free(self._tbuf[0])
+            self._tbuf = None
 
     @property
     def state(self) -> np.ndarray:

@@ -51,6 +51,8 @@ SIGNATURES = [
     ("cg_reactors_free", C.c_int32, [C.c_int32]),
     ("cg_reactors_advance", C.c_int32, [C.c_int32, _dp]),
     ("cg_reactors_advance_device", C.c_int32, [C.c_int32, C.c_void_p]),
+    ("cg_reactors_advance_graph", C.c_int32, [C.c_int32, C.c_int32, C.c_void_p]),
+    ("cg_reactors_autotune", C.c_int32, [C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32]),
     ("cg_reactors_get_state", C.c_int32, [C.c_int32, _dp]),
     ("cg_reactors_stats", C.c_int32, [C.c_int32, _i64p]),
     ("cg_reactors_status", C.c_int32, [C.c_int32, _i32p]),

@@ -61,7 +61,7 @@ struct Batch {
     int* piv = nullptr;
     double* tout = nullptr;
     int* queue = nullptr;
-    int* lpt = nullptr;          // 5n ints: previous step counts, keys in/out, indices, order
+    int* lpt = nullptr;          // 5n ints: previous step counts, bucket, rank in bucket, (spare), order
     void* sort_tmp = nullptr;
     size_t sort_tmp_bytes = 0;
     bool lpt_on = true, have_order = false;
@@ -73,7 +73,21 @@ struct Batch {
     int max_steps = 20000;
     int waves_per_cu = 0;
     size_t lds = 0;
+    // multi-step advance replayed from one captured graph (cg_reactors_advance_graph)
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    int graph_steps = 0;
+    const double* graph_tout = nullptr;
+    const int* graph_order = nullptr;
+    void drop_graph() {
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        graph_exec = nullptr;
+        graph = nullptr;
+        graph_steps = 0;
+    }
     ~Batch() {
+        drop_graph();
         for (void* p : {static_cast<void*>(rec), static_cast<void*>(jsave), lusave, static_cast<void*>(piv),
                         static_cast<void*>(tout), static_cast<void*>(queue), static_cast<void*>(lpt), sort_tmp})
             if (p) (void)hipFree(p);
@@ -444,6 +458,7 @@ int32_t cg_reactors_set_options(int32_t h, int32_t max_steps, int32_t waves_per_
         Batch& b = get_batch(h);
         if (max_steps > 0) b.max_steps = max_steps;
         if (waves_per_cu >= 0) b.waves_per_cu = waves_per_cu;
+        b.drop_graph();
         return 0;
     });
 }
@@ -453,6 +468,7 @@ int32_t cg_reactors_set_matrix_storage(int32_t h, int32_t storage) {
         Batch& b = get_batch(h);
         if (storage < 0 || storage > 2) throw std::runtime_error("storage must be 0 (default), 1 (LDS) or 2 (global)");
         b.global_matrix = storage == 2;
+        b.drop_graph();
         const DevMech& m = b.mech->dm->view();
         b.lds = bdf_lds_bytes(m, b.ld, b.fp32, b.global_matrix);
         set_lds_limit(bdf_kernel_ptr(m.N, b.fp32, b.global_matrix), b.lds);
@@ -468,7 +484,7 @@ int32_t cg_reactors_set_schedule(int32_t h, int32_t longest_first) {
     });
 }
 
-static void launch_advance(Batch& b, const int* list = nullptr, int count = 0) {
+static BdfArgs advance_args(Batch& b, const int* list, int count) {
     BdfArgs a{};
     a.mech = b.mech->dm->view();
     a.n = list ? count : b.n;
@@ -489,16 +505,30 @@ static void launch_advance(Batch& b, const int* list = nullptr, int count = 0) {
     int* order = b.lpt + 4 * static_cast<size_t>(b.n);
     a.order = list ? list : ((b.lpt_on && b.have_order) ? order : nullptr);
     a.queue = b.queue;
-    const int grid = grid_for(bdf_kernel_ptr(b.N, b.fp32, b.global_matrix), b.lds, b.waves_per_cu, b.n);
+    return a;
+}
+
+static int advance_grid(const Batch& b) {
+    return grid_for(bdf_kernel_ptr(b.N, b.fp32, b.global_matrix), b.lds, b.waves_per_cu, b.n);
+}
+
+// cost keys of the call just launched -> processing order of the next
+static void update_order(Batch& b) {
+    const size_t n = static_cast<size_t>(b.n);
+    int* order = b.lpt + 4 * n;
+    check(order_update(b.rec, b.rec_stride, 7 * 64 * b.ns, b.n, b.lpt, b.lpt + n, b.lpt + 2 * n, order, b.sort_tmp,
+                       b.stream),
+          "work-queue order update");
+    b.have_order = true;
+}
+
+static void launch_advance(Batch& b, const int* list = nullptr, int count = 0) {
+    const BdfArgs a = advance_args(b, list, count);
+    const int grid = advance_grid(b);
     check(hipEventRecord(b.e0, b.stream), "hipEventRecord");
     check(bdf_launch(a, b.fp32, b.global_matrix, grid, b.lds, b.stream), "integrator kernel launch");
     check(hipEventRecord(b.e1, b.stream), "hipEventRecord");
-    // cost keys of this call -> processing order of the next
-    const size_t n = static_cast<size_t>(b.n);
-    check(order_update(b.rec, b.rec_stride, 7 * 64 * b.ns, b.n, b.lpt, b.lpt + n, b.lpt + 2 * n, b.lpt + 3 * n, order,
-                       b.sort_tmp, b.sort_tmp_bytes, b.stream),
-          "work-queue order update");
-    b.have_order = true;
+    update_order(b);
 }
 
 int32_t cg_reactors_advance_device(int32_t h, const double* t_out_dev) {
@@ -508,6 +538,126 @@ int32_t cg_reactors_advance_device(int32_t h, const double* t_out_dev) {
               "hipMemcpyAsync(tout)");
         launch_advance(b);
         return 0;
+    });
+}
+
+int32_t cg_reactors_advance_graph(int32_t h, int32_t n_steps, const double* t_out_dev) {
+    return guarded([&] {
+        Batch& b = get_batch(h);
+        if (n_steps < 1) throw std::runtime_error("n_steps must be >= 1");
+        if (!t_out_dev) throw std::runtime_error("null t_out");
+        const int* order = (b.lpt_on && b.have_order) ? b.lpt + 4 * static_cast<size_t>(b.n) : nullptr;
+        if (!b.graph_exec || b.graph_steps != n_steps || b.graph_tout != t_out_dev || b.graph_order != order) {
+            b.drop_graph();
+            // kernel arguments (including each step's t_out row) are fixed at
+            // capture; the values behind the pointers are read at run time
+            const int grid = advance_grid(b);
+            const size_t nn = static_cast<size_t>(b.n);
+            int* order_buf = b.lpt + 4 * nn;
+            check(hipStreamBeginCapture(b.stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+            hipError_t err = hipSuccess;
+            for (int k = 0; k < n_steps && err == hipSuccess; k++) {
+                BdfArgs a = advance_args(b, nullptr, 0);
+                // the first launch uses the order from before the call; each
+                // later one the order its predecessor's step counts give
+                a.order = (k == 0) ? order : (b.lpt_on ? order_buf : nullptr);
+                a.tout = t_out_dev + static_cast<size_t>(k) * nn;
+                err = bdf_launch(a, b.fp32, b.global_matrix, grid, b.lds, b.stream);
+                if (err == hipSuccess && b.lpt_on)
+                    err = order_update(b.rec, b.rec_stride, 7 * 64 * b.ns, b.n, b.lpt, b.lpt + nn, b.lpt + 2 * nn,
+                                       order_buf, b.sort_tmp, b.stream);
+            }
+            hipGraph_t g = nullptr;
+            const hipError_t end = hipStreamEndCapture(b.stream, &g);
+            check(err, "integrator kernel capture");
+            check(end, "hipStreamEndCapture");
+            b.graph = g;
+            check(hipGraphInstantiate(&b.graph_exec, b.graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+            b.graph_steps = n_steps;
+            b.graph_tout = t_out_dev;
+            b.graph_order = order;
+        }
+        check(hipEventRecord(b.e0, b.stream), "hipEventRecord");
+        check(hipGraphLaunch(b.graph_exec, b.stream), "hipGraphLaunch");
+        check(hipEventRecord(b.e1, b.stream), "hipEventRecord");
+        if (b.lpt_on) b.have_order = true;
+        return 0;
+    });
+}
+
+int32_t cg_reactors_autotune(int32_t h, const double* t_probe_dev, int32_t* choice, double* times_ms,
+                             int32_t max_candidates) {
+    return guarded([&] {
+        Batch& b = get_batch(h);
+        if (!t_probe_dev) throw std::runtime_error("null t_probe");
+        const DevMech& m = b.mech->dm->view();
+        const int v = bdf_variant(m.N);
+        // (matrix storage, waves per CU) candidates; storage 1 = LDS, 2 = global
+        std::vector<std::pair<int, int>> cand;
+        if (v == 56 || v == 64) cand = {{1, 0}, {2, 0}, {2, 4}};
+        else cand = {{0, 0}, {0, 4}, {0, 8}};
+        // snapshot of everything an advance changes, restored before each trial
+        const size_t n = static_cast<size_t>(b.n);
+        const size_t es = b.fp32 ? 4 : 8;
+        const size_t sizes[5] = {b.rec_stride * n * 8, b.j_stride * n * 8, b.lu_stride * n * es, b.piv_stride * n * 4,
+                                 5 * n * sizeof(int)};
+        void* live[5] = {b.rec, b.jsave, b.lusave, b.piv, b.lpt};
+        void* snap[5] = {};
+        struct Free {
+            void** p;
+            ~Free() { for (int i = 0; i < 5; i++) if (p[i]) (void)hipFree(p[i]); }
+        } guard_snap{snap};
+        check(hipStreamSynchronize(b.stream), "hipStreamSynchronize");
+        for (int i = 0; i < 5; i++) {
+            check(hipMalloc(&snap[i], sizes[i]), "hipMalloc(autotune snapshot)");
+            check(hipMemcpyAsync(snap[i], live[i], sizes[i], hipMemcpyDeviceToDevice, b.stream), "hipMemcpyAsync");
+        }
+        const bool had_order = b.have_order;
+        auto restore = [&] {
+            for (int i = 0; i < 5; i++)
+                check(hipMemcpyAsync(live[i], snap[i], sizes[i], hipMemcpyDeviceToDevice, b.stream), "hipMemcpyAsync");
+            b.have_order = had_order;
+        };
+        const bool global0 = b.global_matrix;
+        const int wpc0 = b.waves_per_cu;
+        int best = -1;
+        double best_ms = 0.0;
+        std::vector<std::pair<const void*, int>> seen;
+        for (size_t c = 0; c < cand.size(); c++) {
+            b.global_matrix = cand[c].first == 2;
+            b.waves_per_cu = cand[c].second;
+            b.lds = bdf_lds_bytes(m, b.ld, b.fp32, b.global_matrix);
+            const void* kp = bdf_kernel_ptr(m.N, b.fp32, b.global_matrix);
+            set_lds_limit(kp, b.lds);
+            const int grid = advance_grid(b);
+            double ms = -1.0;
+            if (std::find(seen.begin(), seen.end(), std::make_pair(kp, grid)) == seen.end()) {
+                seen.emplace_back(kp, grid);
+                restore();
+                check(hipMemcpyAsync(b.tout, t_probe_dev, n * 8, hipMemcpyDeviceToDevice, b.stream), "hipMemcpyAsync");
+                launch_advance(b);
+                check(hipStreamSynchronize(b.stream), "hipStreamSynchronize");
+                float f = 0.f;
+                check(hipEventElapsedTime(&f, b.e0, b.e1), "hipEventElapsedTime");
+                ms = f;
+                // the first candidate is the default launch; another replaces
+                // it only when at least 3% faster (one trial's timing noise)
+                if (best < 0 || ms < (best == 0 ? 0.97 : 1.0) * best_ms) { best = static_cast<int>(c); best_ms = ms; }
+            }
+            if (times_ms && static_cast<int>(c) < max_candidates) times_ms[c] = ms;
+        }
+        restore();
+        check(hipStreamSynchronize(b.stream), "hipStreamSynchronize");
+        if (best < 0) { b.global_matrix = global0; b.waves_per_cu = wpc0; }
+        else { b.global_matrix = cand[best].first == 2; b.waves_per_cu = cand[best].second; }
+        b.lds = bdf_lds_bytes(m, b.ld, b.fp32, b.global_matrix);
+        set_lds_limit(bdf_kernel_ptr(m.N, b.fp32, b.global_matrix), b.lds);
+        b.drop_graph();
+        if (choice) {
+            choice[0] = b.global_matrix ? 2 : ((v == 56 || v == 64) ? 1 : 0);
+            choice[1] = b.waves_per_cu;
+        }
+        return static_cast<int32_t>(cand.size());
     });
 }
 

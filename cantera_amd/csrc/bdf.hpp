@@ -59,11 +59,12 @@ hipError_t rates_launch(const DevMech& m, int ns, bool f32, long n, const double
                         const double* Y, double* wdot, int grid, size_t lds, hipStream_t stream);
 const void* rates_kernel_ptr(int ns, bool f32);
 
-// longest-first processing order for the integrator's work queue
+// longest-first processing order for the integrator's work queue (kernels
+// only, so it can be captured in a HIP graph); prev_nst, bucket, rank and
+// order_out hold n ints, temp order_temp_bytes(n) bytes
 size_t order_temp_bytes(int n);
-hipError_t order_update(const double* rec, size_t rec_stride, int ctl_off, int n, int* prev_nst, int* keys_in,
-                        int* keys_out, int* idx_in, int* order_out, void* temp, size_t temp_bytes,
-                        hipStream_t stream);
+hipError_t order_update(const double* rec, size_t rec_stride, int ctl_off, int n, int* prev_nst, int* bucket,
+                        int* rank, int* order_out, void* temp, hipStream_t stream);
 
 // guarded surrogate steps for reactor batches
 size_t guard_lds_bytes(const DevMech& m, int ns);

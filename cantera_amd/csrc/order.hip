@@ -1,15 +1,16 @@
-// Processing order of the persistent integrator's work queue: reactors sorted
+// Processing order of the persistent integrator's work queue: reactors ordered
 // by the number of internal steps they took in the previous advance call, most
 // expensive first (longest-processing-time-first list scheduling).  An advance
 // call ends when its slowest wavefront finishes, so starting the stiff reactors
 // first keeps the cheap ones for the end, where they fill the gaps.
 //
-// Runs on the batch stream after each integrator launch: one small kernel
-// turns the step counters of the control blocks into keys, then a device radix
-// sort (hipCUB) produces the order the next launch reads.  The order changes
-// only which wavefront integrates which reactor, never a reactor's result.
+// Runs on the batch stream after each integrator launch as a counting sort on
+// the step count (clamped to kBuckets - 1): clear the histogram, count and rank
+// each reactor in its bucket, scan the histogram, scatter.  Four kernels and no
+// memset or library call, so the same sequence can be captured in a HIP graph.
+// Reactors with equal counts land in atomic order; the order changes only
+// which wavefront integrates which reactor, never a reactor's result.
 #include <hip/hip_runtime.h>
-#include <hipcub/device/device_radix_sort.hpp>
 
 #include "bdf.hpp"
 
@@ -17,39 +18,65 @@ namespace cg {
 
 namespace {
 
-constexpr int kKeyBits = 20;                       // step counts are capped below 2^20
-constexpr int kKeyMax = (1 << kKeyBits) - 1;
+constexpr int kBuckets = 1024;
 
-__global__ void order_keys_kernel(const double* __restrict__ rec, size_t rec_stride, int ctl_off, int n,
-                                  int* __restrict__ prev_nst, int* __restrict__ keys, int* __restrict__ idx) {
+__global__ void clear_hist_kernel(int* __restrict__ hist) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < kBuckets) hist[i] = 0;
+}
+
+// bucket 0 holds the most expensive reactors
+__global__ void bucket_kernel(const double* __restrict__ rec, size_t rec_stride, int ctl_off, int n,
+                              int* __restrict__ prev_nst, int* __restrict__ bucket, int* __restrict__ rank,
+                              int* __restrict__ hist) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n) return;
     const int nst = static_cast<int>(rec[static_cast<size_t>(r) * rec_stride + ctl_off + C_NST]);
-    const int d = nst - prev_nst[r];
+    int d = nst - prev_nst[r];
     prev_nst[r] = nst;
-    keys[r] = d < 0 ? 0 : (d > kKeyMax ? kKeyMax : d);
-    idx[r] = r;
+    d = d < 0 ? 0 : (d > kBuckets - 1 ? kBuckets - 1 : d);
+    const int b = kBuckets - 1 - d;
+    bucket[r] = b;
+    rank[r] = atomicAdd(&hist[b], 1);
+}
+
+// exclusive scan of the histogram: one workgroup of kBuckets threads
+__global__ __launch_bounds__(kBuckets) void scan_hist_kernel(const int* __restrict__ hist, int* __restrict__ offs) {
+    __shared__ int s[kBuckets];
+    const int t = threadIdx.x;
+    const int v = hist[t];
+    s[t] = v;
+    __syncthreads();
+    for (int d = 1; d < kBuckets; d <<= 1) {
+        const int add = t >= d ? s[t - d] : 0;
+        __syncthreads();
+        s[t] += add;
+        __syncthreads();
+    }
+    offs[t] = s[t] - v;
+}
+
+__global__ void scatter_kernel(int n, const int* __restrict__ bucket, const int* __restrict__ rank,
+                               const int* __restrict__ offs, int* __restrict__ order) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    order[offs[bucket[r]] + rank[r]] = r;
 }
 
 }  // namespace
 
-size_t order_temp_bytes(int n) {
-    size_t bytes = 0;
-    (void)hipcub::DeviceRadixSort::SortPairsDescending(nullptr, bytes, static_cast<const int*>(nullptr),
-                                                       static_cast<int*>(nullptr), static_cast<const int*>(nullptr),
-                                                       static_cast<int*>(nullptr), n, 0, kKeyBits);
-    return bytes;
-}
+size_t order_temp_bytes(int) { return 2 * kBuckets * sizeof(int); }
 
-hipError_t order_update(const double* rec, size_t rec_stride, int ctl_off, int n, int* prev_nst, int* keys_in,
-                        int* keys_out, int* idx_in, int* order_out, void* temp, size_t temp_bytes,
-                        hipStream_t stream) {
-    hipLaunchKernelGGL(order_keys_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, rec, rec_stride, ctl_off, n,
-                       prev_nst, keys_in, idx_in);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    return hipcub::DeviceRadixSort::SortPairsDescending(temp, temp_bytes, keys_in, keys_out, idx_in, order_out, n, 0,
-                                                        kKeyBits, stream);
+hipError_t order_update(const double* rec, size_t rec_stride, int ctl_off, int n, int* prev_nst, int* bucket,
+                        int* rank, int* order_out, void* temp, hipStream_t stream) {
+    int* hist = static_cast<int*>(temp);
+    int* offs = hist + kBuckets;
+    const dim3 g((n + 255) / 256), b(256);
+    hipLaunchKernelGGL(clear_hist_kernel, dim3(kBuckets / 256), b, 0, stream, hist);
+    hipLaunchKernelGGL(bucket_kernel, g, b, 0, stream, rec, rec_stride, ctl_off, n, prev_nst, bucket, rank, hist);
+    hipLaunchKernelGGL(scan_hist_kernel, dim3(1), dim3(kBuckets), 0, stream, hist, offs);
+    hipLaunchKernelGGL(scatter_kernel, g, b, 0, stream, n, bucket, rank, offs, order_out);
+    return hipGetLastError();
 }
 
 }  // namespace cg

@@ -120,6 +120,31 @@ int32_t cg_reactors_advance(int32_t batch, const double* t_out);
 /* Same with t_out a device pointer; asynchronous (no error check on reactor
  * status until cg_reactors_status / cg_reactors_get_state). */
 int32_t cg_reactors_advance_device(int32_t batch, const double* t_out_dev);
+/* n_steps successive advances, to t_out_dev[k*n + r] for k = 0..n_steps-1, as
+ * repeated ReactorNet::advance calls (src/zeroD/ReactorNet.cpp:351-357) would
+ * do them.  The n_steps integrator launches are captured once as a HIP graph
+ * and replayed by later calls with the same n_steps and t_out_dev pointer (the
+ * times behind it may change between calls; they are read when the graph
+ * runs).  The longest-first order is updated between the captured launches as
+ * between separate calls.  Asynchronous on the batch stream like
+ * cg_reactors_advance_device. */
+int32_t cg_reactors_advance_graph(int32_t batch, int32_t n_steps, const double* t_out_dev);
+/* Measured choice of the integrator launch (no counterpart in the reference,
+ * whose integrator has no device launch to tune): for 32 < N <= 64 the
+ * iteration matrix in LDS (one wavefront per SIMD) or in the reactor's global
+ * factor slot (two per SIMD), with the natural or a capped number of
+ * wavefronts per CU; otherwise the cap alone.  Each candidate advances the
+ * whole batch from its current state to t_probe_dev[n] once; the state is
+ * restored after every trial and at the end, so the call leaves the
+ * integration where it was and keeps the fastest setting for later advances
+ * (the first candidate is the default launch; another must be at least 3%
+ * faster to replace it).
+ * choice[2] (may be NULL): matrix storage as cg_reactors_set_matrix_storage,
+ * waves per CU (0 = as many as fit).  times_ms[max_candidates] (may be NULL):
+ * each candidate's time, -1 for a candidate identical to an earlier one.
+ * Returns the number of candidates. */
+int32_t cg_reactors_autotune(int32_t batch, const double* t_probe_dev, int32_t* choice, double* times_ms,
+                             int32_t max_candidates);
 /* State y = [m, T, Y_1..Y_K] per reactor, AoS n x (K+2)
  * (ReactorNet::getState, src/zeroD/ReactorNet.cpp:754-759). */
 int32_t cg_reactors_get_state(int32_t batch, double* y);

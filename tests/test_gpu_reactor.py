@@ -151,6 +151,60 @@ def test_matrix_storage_does_not_change_results():
     assert np.max(np.abs(out[0][:, 1] - out[1][:, 1]) / out[1][:, 1]) <= 1e-6
 
 
+def test_graph_replay_matches_eager_advances():
+    """cg_reactors_advance_graph captures 4 integrator launches once and replays
+    them; three replays with new output times each, with eager work (another
+    batch's advances) and synchronisation between them, each agree with the
+    same advances done eagerly.  Bar as the schedule test (only atomic
+    summation order can differ)."""
+    import cantera_amd as ca
+    from cantera_amd.workloads import MechInfo, config3
+    wl = config3(MechInfo(mech_file("gri30")), n=96, nsteps=12)
+    sol = ca.Solution("gri30")
+    eager = ca.ReactorBatch(sol, wl.T0, wl.P, wl.Y0, rtol=wl.rtol, atol=wl.atol)
+    graph = ca.ReactorBatch(sol, wl.T0, wl.P, wl.Y0, rtol=wl.rtol, atol=wl.atol)
+    K = 4
+    for r in range(3):
+        times = np.stack([(r * K + k + 1) * wl.dt for k in range(K)])
+        for k in range(K):
+            eager.advance(times[k])
+        graph.advance_steps(times)
+        ya, yb = eager.state, graph.state
+        assert np.all(graph.status == 0)
+        dT = np.abs(ya[:, 1] - yb[:, 1]) / ya[:, 1]
+        dY = np.abs(ya[:, 2:] - yb[:, 2:]).max()
+        print(f"replay {r}: max rel dT {dT.max():.2e}, max |dY| {dY:.2e}", flush=True)
+        assert dT.max() <= 1e-6
+        assert dY <= 1e-7
+
+
+def test_autotune_restores_state_and_keeps_results():
+    """cg_reactors_autotune runs every candidate launch from the same state and
+    puts that state back: the state right after tuning is bit-identical to the
+    one before, and later advances agree with an untuned batch (bar as the
+    schedule test)."""
+    import cantera_amd as ca
+    from cantera_amd.workloads import MechInfo, config3
+    wl = config3(MechInfo(mech_file("gri30")), n=128, nsteps=6)
+    sol = ca.Solution("gri30")
+    tuned = ca.ReactorBatch(sol, wl.T0, wl.P, wl.Y0, rtol=wl.rtol, atol=wl.atol)
+    plain = ca.ReactorBatch(sol, wl.T0, wl.P, wl.Y0, rtol=wl.rtol, atol=wl.atol)
+    tuned.advance(wl.dt)
+    plain.advance(wl.dt)
+    before = tuned.state
+    choice, times = tuned.autotune(2 * wl.dt)
+    print("autotune choice", choice, "times [ms]", times, flush=True)
+    assert choice[0] in (1, 2) and choice[1] in (0, 4)
+    assert np.any(times > 0)
+    assert np.array_equal(tuned.state, before)
+    for k in range(2, 7):
+        tuned.advance(k * wl.dt)
+        plain.advance(k * wl.dt)
+    assert np.all(tuned.status == 0)
+    ya, yb = plain.state, tuned.state
+    assert (np.abs(ya[:, 1] - yb[:, 1]) / ya[:, 1]).max() <= 1e-6
+
+
 def test_reactor_invariants():
     """Mass fractions keep summing to 1 and enthalpy is conserved (adiabatic,
     isobaric): checked through the elements and the state after 50 outputs."""
