@@ -17,6 +17,8 @@ Other workloads of the same path, one JSON line each:
   gri30-jacobian     1,048,576 gri30 states, reactor Jacobians (states/s)
   ndodecane-reactors 262,144 n-dodecane reactors x 100 steps of 1e-6 s
   h2o2-reactors      1,024 h2o2 reactors x 100 outputs every 1e-5 s
+  gri30-surrogate    BF16 MLP surrogate: training on integrator pairs,
+                     inference on the matrix cores, guarded steps vs plain BDF
 """
 from __future__ import annotations
 
@@ -382,6 +384,95 @@ def batch_kernels(a):
     return 0
 
 
+def surrogate(a):
+    """Learned surrogate on gri30: train on integrator pairs, time inference on
+    the matrix cores, and run guarded surrogate steps against the plain BDF
+    integrator on the same reactors."""
+    import cantera_amd as ca
+    from cantera_amd._lib import check, lib
+    from cantera_amd.surrogate import DeviceArray, Surrogate, generate_pairs
+    from cantera_amd.workloads import config3
+
+    L = lib()
+    sol = ca.Solution("gri30")
+    K = sol.n_species
+    n = a.n or 65536
+    steps = a.steps or 10
+    dt = 1e-6
+    hidden, depth = 256, 4
+    # training pairs along FP64 integrator trajectories from the evaluation
+    # distribution (another seed), dt = 1e-6 s
+    ntr = max(1024, a.pairs // 16)
+    tw = config3(sol, n=ntr, nsteps=16, seed=0x5EED0104)
+    t0 = time.perf_counter()
+    y0, P, y1 = generate_pairs(sol, tw.T0, tw.P, tw.Y0, dt=dt, nsteps=16, rtol=tw.rtol, atol=tw.atol)
+    gen_s = time.perf_counter() - t0
+    sur = Surrogate(sol, dt=dt, hidden=hidden, depth=depth, seed=1)
+    t0 = time.perf_counter()
+    losses = sur.fit(y0, P, y1, epochs=a.epochs, batch=4096, lr=1e-3)
+    check(L.cg_device_synchronize())
+    train_s = time.perf_counter() - t0
+    n_in, n_out = K + 2, K + 1
+    flop_fwd = 2.0 * (n_in * hidden + (depth - 1) * hidden * hidden + hidden * n_out)
+    train_samples = len(y0) * a.epochs
+    # inference on the evaluation states (device arrays, GEMMs on MFMA)
+    wl = config3(sol, n=n, nsteps=steps)
+    st0 = np.concatenate([np.ones((n, 1)), wl.T0[:, None], wl.Y0], axis=1)
+    d0, dP, d1 = DeviceArray(st0), DeviceArray(np.asarray(wl.P, np.float64)), DeviceArray.empty(st0.shape)
+    for _ in range(2):
+        check(L.cg_surrogate_predict(sur.handle, n, d0.ptr, dP.ptr, d1.ptr))
+    reps = 20
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        check(L.cg_surrogate_predict(sur.handle, n, d0.ptr, dP.ptr, d1.ptr))
+    inf_s = (time.perf_counter() - t0) / reps
+    # guarded advance vs the plain integrator, same reactors and output times
+    plain = ca.ReactorBatch(sol, wl.T0, wl.P, wl.Y0, rtol=wl.rtol, atol=wl.atol)
+    guarded = ca.ReactorBatch(sol, wl.T0, wl.P, wl.Y0, rtol=wl.rtol, atol=wl.atol)
+    t0 = time.perf_counter()
+    for k in range(1, steps + 1):
+        plain.advance(k * dt)
+    plain_s = time.perf_counter() - t0
+    counts = np.zeros(3, dtype=np.int64)
+    t0 = time.perf_counter()
+    for k in range(1, steps + 1):
+        counts += guarded.advance_surrogate(k * dt, sur, dT_tol=a.dT_tol, rtol_g=a.rtol_g, atol_g=a.atol_g)
+    guarded_s = time.perf_counter() - t0
+    yp, yg = plain.state, guarded.state
+    dT = np.abs(yp[:, 1] - yg[:, 1])
+    dY = np.abs(yp[:, 2:] - yg[:, 2:]).max()
+    achieved = n * flop_fwd / inf_s / 1e12
+    out = {
+        "metric": "gri30 reactor-steps/s at dt = 1e-6 s: guarded surrogate with BDF fallback",
+        "value": n * steps / guarded_s, "unit": "reactor-steps/s", "n_gpus": 1, "steps": steps, "warmup": 0,
+        "ms_per_step": guarded_s / steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16 network (fp32 accumulate), f64 guard and fallback", "data": "synthetic",
+        "config": {"workload": f"gri30 MLP {n_in} -> {depth} x {hidden} GELU -> {n_out}, dt 1e-6 s; training on "
+                               f"{len(y0)} integrator pairs ({ntr} reactors x 16 steps, T0 U[900,2000] K, CH4/air "
+                               f"phi U[0.5,3], 1-40 atm), {a.epochs} epochs, batch 4096, Adam lr 1e-3; evaluation "
+                               f"{n} reactors x {steps} steps of the same distribution (another seed)",
+                   "guard": {"dT_tol_K": a.dT_tol, "rtol_g": a.rtol_g, "atol_g": a.atol_g}},
+        "roofline": {"bound": "mfma", "pipe": "BF16 MFMA (dense 2.5 PFLOP/s)", "achieved": achieved, "peak": 2500.0,
+                     "unit": "TFLOP/s", "frac": achieved / 2500.0, "traffic": None,
+                     "kernel": "gemm_bf16 (surrogate inference: features, 5 GEMMs, decode)",
+                     "avg_launch_ms": inf_s * 1e3, "flops_per_launch": n * flop_fwd},
+        "surrogate": {
+            "plain_fp64_reactor_steps_per_s": n * steps / plain_s,
+            "speedup_vs_plain": plain_s / guarded_s,
+            "kept_surrogate_steps": int(counts[0]), "reactor_steps_by_surrogate_alone": int(counts[1]),
+            "reactor_steps_by_bdf": int(counts[2]), "fallback_fraction": float(counts[2] / max(1, n * steps)),
+            "max_abs_dT_vs_plain_K": float(dT.max()), "max_abs_dY_vs_plain": float(dY),
+            "training_pairs": len(y0), "pair_generation_s": gen_s, "training_s": train_s,
+            "training_samples_per_s": train_samples / train_s,
+            "training_tflops": train_samples * 3 * flop_fwd / train_s / 1e12,
+            "loss_first_last_epoch": [losses[0], losses[-1]],
+            "inference_samples_per_s": n / inf_s,
+        },
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def copy_calib(a, L, check):
     if a.copy_calib > 0:
         nb = a.copy_calib << 20
@@ -396,7 +487,8 @@ def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--workload", default="gri30-reactors",
-                    choices=["gri30-reactors", "gri30-rates", "gri30-jacobian", "ndodecane-reactors", "h2o2-reactors"],
+                    choices=["gri30-reactors", "gri30-rates", "gri30-jacobian", "ndodecane-reactors", "h2o2-reactors",
+                             "gri30-surrogate"],
                     help="gri30-reactors is the headline line; the others measure the other workloads of the path")
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=2)
@@ -410,6 +502,11 @@ def main(argv=None):
                     help="reactor workloads: run the K steps as one captured HIP graph")
     ap.add_argument("--autotune", action="store_true",
                     help="reactor workloads: pick the integrator launch by timing the candidates first")
+    ap.add_argument("--pairs", type=int, default=1 << 20, help="gri30-surrogate: training pairs")
+    ap.add_argument("--epochs", type=int, default=4, help="gri30-surrogate: training epochs")
+    ap.add_argument("--dT-tol", type=float, default=0.05, help="gri30-surrogate: guard, K")
+    ap.add_argument("--rtol-g", type=float, default=1e-3, help="gri30-surrogate: guard, relative")
+    ap.add_argument("--atol-g", type=float, default=1e-9, help="gri30-surrogate: guard, absolute Y")
     ap.add_argument("--cpu-sample", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
@@ -422,6 +519,8 @@ def main(argv=None):
         return 2
     if a.workload in ("gri30-rates", "gri30-jacobian"):
         return batch_kernels(a)
+    if a.workload == "gri30-surrogate":
+        return surrogate(a)
     return reactors(a)
 
 

@@ -2,6 +2,8 @@
 // targets from FP64 reactor states, loss, Adam.  See surrogate.hpp.
 #include "surrogate.hpp"
 
+#include "wave_kinetics.hpp"
+
 #include <algorithm>
 #include <cmath>
 #include <random>
@@ -52,9 +54,11 @@ constexpr int kTM = 64, kTN = 64, kTK = 32, kLdsK = kTK + 8;
 // 256 threads: a 64 x 64 tile of C per workgroup, 32 x 32 per wavefront as
 // 2 x 2 MFMA 16x16x16 tiles; K staged through LDS in steps of 32.  Out-of-range
 // rows / columns / k read as zero and are not stored.
+// kStoreF32 may split K over gridDim.z (chunks of kchunk, a multiple of kTK):
+// each split adds its partial sums to the cleared output with FP32 atomics.
 template <bool TA, bool TB, int EPI>
 __global__ __launch_bounds__(256) void gemm_bf16(int M, int N, int K, const uint16_t* __restrict__ A, int lda,
-                                                 const uint16_t* __restrict__ B, int ldb, EpiArgs e) {
+                                                 const uint16_t* __restrict__ B, int ldb, EpiArgs e, int kchunk) {
     __shared__ uint16_t As[kTM][kLdsK];
     __shared__ uint16_t Bs[kTN][kLdsK];
     const int t = threadIdx.x;
@@ -66,7 +70,8 @@ __global__ __launch_bounds__(256) void gemm_bf16(int M, int N, int K, const uint
     for (int i = 0; i < 2; i++)
 #pragma unroll
         for (int j = 0; j < 2; j++) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-    for (int k0 = 0; k0 < K; k0 += kTK) {
+    const int kbeg = blockIdx.z * kchunk, kend = min(K, kbeg + kchunk);
+    for (int k0 = kbeg; k0 < kend; k0 += kTK) {
         // stage A (64 x 32) and B (64 x 32), 8 elements per thread each
         if (!TA) {
             const int r = t >> 2, kk = (t & 3) * 8;
@@ -142,7 +147,8 @@ __global__ __launch_bounds__(256) void gemm_bf16(int M, int N, int K, const uint
                 } else if (EPI == kFwdLinear) {
                     e.zf[o] = v + e.bias[n];
                 } else if (EPI == kStoreF32) {
-                    e.zf[o] = v;
+                    if (gridDim.z > 1) atomicAdd(&e.zf[o], v);
+                    else e.zf[o] = v;
                 } else {
                     e.hb[o] = f2bf(v * gelu_grad(e.zin[o]));
                 }
@@ -221,12 +227,22 @@ __global__ void loss_kernel(int rows, int n_out, int out_p, const float* __restr
 }
 
 // Column sums of a BF16 gradient operand: bias gradient.
-__global__ void colsum_kernel(int rows, int cols, const uint16_t* __restrict__ g, float* __restrict__ db) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= cols) return;
+// db[c] += sum over a block of 64 rows of g[r][c] (db cleared by the caller):
+// 256 threads as 4 row groups x 64 columns, coalesced 128-byte row segments,
+// partial sums combined in LDS and added with one atomic per column and block.
+constexpr int kColsumRows = 64;
+__global__ __launch_bounds__(256) void colsum_kernel(int rows, int cols, const uint16_t* __restrict__ g,
+                                                     float* __restrict__ db) {
+    __shared__ float part[4][64];
+    const int c = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int rg = threadIdx.x >> 6;
+    const int r0 = blockIdx.y * kColsumRows;
     float s = 0.f;
-    for (int r = 0; r < rows; r++) s += bf2f(g[static_cast<size_t>(r) * cols + c]);
-    db[c] = s;
+    if (c < cols)
+        for (int r = r0 + rg; r < min(rows, r0 + kColsumRows); r += 4) s += bf2f(g[static_cast<size_t>(r) * cols + c]);
+    part[rg][threadIdx.x & 63] = s;
+    __syncthreads();
+    if (rg == 0 && c < cols) atomicAdd(&db[c], (part[0][threadIdx.x] + part[1][threadIdx.x]) + (part[2][threadIdx.x] + part[3][threadIdx.x]));
 }
 
 // Adam (Kingma & Ba 2015) with bias correction, then the BF16 weight copy.
@@ -256,69 +272,193 @@ __global__ void to_bf16_kernel(size_t n, const float* __restrict__ w, uint16_t* 
 // D_k = max(|Y0_k|, |Y0_k + dY_k|) + 1e-12, dY <- dY - D B^T (B D B^T)^-1 B dY,
 // the correction of least D^-1-weighted norm, so species near zero hardly move
 // and sum_k dY_k = 0 follows from element conservation.
-constexpr int kMaxK = 128, kMaxE = 8;
-__global__ void apply_kernel(int rows, int N, int K, const double* __restrict__ y0, const float* __restrict__ out,
-                             const double* __restrict__ norm, int in_p, int out_p, const double* __restrict__ comp,
-                             const double* __restrict__ rmw, int E, double* __restrict__ y1) {
-    __shared__ double dY[kMaxK], D[kMaxK];
-    __shared__ double lam[kMaxE];
+// One wavefront per row, lanes over species: the E x E Gram matrix B D B^T and
+// B dY are wavefront sums; every lane then solves the (symmetric positive
+// definite) system by elimination; an element absent from the mechanism's
+// species has a zero row and gets lambda = 0.
+constexpr int kMaxK = 128, kMaxE = 8, kSlots = kMaxK / 64;
+__global__ __launch_bounds__(64) void apply_kernel(int rows, int N, int K, const double* __restrict__ y0,
+                                                   const float* __restrict__ out, const double* __restrict__ norm,
+                                                   int in_p, int out_p, const double* __restrict__ comp,
+                                                   const double* __restrict__ rmw, int E, double* __restrict__ y1) {
     const int r = blockIdx.x;
     if (r >= rows) return;
+    const int lane = threadIdx.x;
     const double* mu = norm + 2 * in_p;
     const double* sd = mu + out_p;
     const double* sc = sd + out_p;
     const double* y0r = y0 + static_cast<size_t>(r) * N;
-    for (int j = threadIdx.x; j < N; j += blockDim.x) {
-        double d = 0.0;
-        if (j >= 1)
-            d = sc[j - 1] * sinh(static_cast<double>(out[static_cast<size_t>(r) * out_p + (j - 1)]) * sd[j - 1] + mu[j - 1]);
-        if (j >= 2) {
-            dY[j - 2] = d;
-            D[j - 2] = fmax(fabs(y0r[j]), fabs(y0r[j] + d)) + 1e-12;
+    double* y1r = y1 + static_cast<size_t>(r) * N;
+    const float* o = out + static_cast<size_t>(r) * out_p;
+    // j = 0 (mass) and j = 1 (T)
+    if (lane == 0) y1r[0] = y0r[0];
+    // the network output is FP32, so the decode is too (sinhf); the increments
+    // and the projection below are FP64
+    if (lane == 1) y1r[1] = y0r[1] + sc[0] * static_cast<double>(sinhf(static_cast<float>(o[0] * sd[0] + mu[0])));
+    double dY[kSlots], D[kSlots];
+#pragma unroll
+    for (int t = 0; t < kSlots; t++) {
+        const int k = lane + 64 * t;
+        dY[t] = 0.0;
+        D[t] = 0.0;
+        if (k < K) {
+            dY[t] = sc[1 + k] * static_cast<double>(sinhf(static_cast<float>(o[1 + k] * sd[1 + k] + mu[1 + k])));
+            D[t] = fmax(fabs(y0r[2 + k]), fabs(y0r[2 + k] + dY[t])) + 1e-12;
         }
-        y1[static_cast<size_t>(r) * N + j] = y0r[j] + d;
     }
-    __syncthreads();
-    if (!comp || E < 1 || E > kMaxE) return;
-    if (threadIdx.x == 0) {
+    const bool project = comp && E >= 1 && E <= kMaxE;
+    double lam[kMaxE];
+#pragma unroll
+    for (int a = 0; a < kMaxE; a++) lam[a] = 0.0;
+    // B_ek of the lane's species
+    double Bk[kSlots][kMaxE];
+#pragma unroll
+    for (int t = 0; t < kSlots; t++) {
+        const int k = lane + 64 * t;
+#pragma unroll
+        for (int a = 0; a < kMaxE; a++) Bk[t][a] = (project && k < K && a < E) ? comp[k * E + a] * rmw[k] : 0.0;
+    }
+    if (project) {
         double G[kMaxE][kMaxE + 1];
-        for (int a = 0; a < E; a++) {
-            for (int b = 0; b < E; b++) {
-                double g = 0.0;
-                for (int k = 0; k < K; k++) g += comp[k * E + a] * rmw[k] * D[k] * comp[k * E + b] * rmw[k];
-                G[a][b] = g;
+#pragma unroll
+        for (int a = 0; a < kMaxE; a++) {
+#pragma unroll
+            for (int b = 0; b <= kMaxE; b++) G[a][b] = 0.0;
+        }
+#pragma unroll
+        for (int a = 0; a < kMaxE; a++) {
+            if (a >= E) continue;
+#pragma unroll
+            for (int b = a; b < kMaxE; b++) {
+                if (b >= E) continue;
+                double part = 0.0;
+#pragma unroll
+                for (int t = 0; t < kSlots; t++) part += Bk[t][a] * D[t] * Bk[t][b];
+                G[a][b] = G[b][a] = wave_sum(part);
             }
             double v = 0.0;
-            for (int k = 0; k < K; k++) v += comp[k * E + a] * rmw[k] * dY[k];
-            G[a][E] = v;
+#pragma unroll
+            for (int t = 0; t < kSlots; t++) v += Bk[t][a] * dY[t];
+            G[a][kMaxE] = wave_sum(v);
         }
-        // Gaussian elimination with partial pivoting; elements absent from the
-        // mechanism's species give zero rows and get lambda = 0
-        int used[kMaxE];
-        for (int a = 0; a < E; a++) used[a] = 0;
-        for (int c = 0; c < E; c++) {
-            int p = -1;
-            double best = 0.0;
-            for (int a = 0; a < E; a++)
-                if (!used[a] && fabs(G[a][c]) > best) { best = fabs(G[a][c]); p = a; }
-            lam[c] = 0.0;
-            if (p < 0 || best < 1e-300) continue;
-            used[p] = c + 1;
-            for (int a = 0; a < E; a++) {
-                if (a == p) continue;
-                const double f = G[a][c] / G[p][c];
-                for (int b = c; b <= E; b++) G[a][b] -= f * G[p][b];
+        // elimination without pivoting (SPD); zero rows give lambda = 0
+        bool live[kMaxE];
+        double rp[kMaxE];
+#pragma unroll
+        for (int c = 0; c < kMaxE; c++) {
+            live[c] = c < E && G[c][c] > 1e-300;
+            rp[c] = live[c] ? 1.0 / G[c][c] : 0.0;
+            if (!live[c]) continue;
+#pragma unroll
+            for (int a = c + 1; a < kMaxE; a++) {
+                if (a >= E) continue;
+                const double f = G[a][c] * rp[c];
+#pragma unroll
+                for (int b = c; b <= kMaxE; b++)
+                    if (b < E || b == kMaxE) G[a][b] -= f * G[c][b];
             }
         }
-        for (int a = 0; a < E; a++)
-            if (used[a]) lam[used[a] - 1] = G[a][E] / G[a][used[a] - 1];
+#pragma unroll
+        for (int c = kMaxE - 1; c >= 0; c--) {
+            if (!live[c]) continue;
+            double v = G[c][kMaxE];
+#pragma unroll
+            for (int b = c + 1; b < kMaxE; b++)
+                if (b < E) v -= G[c][b] * lam[b];
+            lam[c] = v * rp[c];
+        }
     }
-    __syncthreads();
-    for (int k = threadIdx.x; k < K; k += blockDim.x) {
+#pragma unroll
+    for (int t = 0; t < kSlots; t++) {
+        const int k = lane + 64 * t;
+        if (k >= K) continue;
         double c = 0.0;
-        for (int a = 0; a < E; a++) c += comp[k * E + a] * rmw[k] * lam[a];
-        y1[static_cast<size_t>(r) * N + 2 + k] = y0r[2 + k] + (dY[k] - D[k] * c);
+#pragma unroll
+        for (int a = 0; a < kMaxE; a++) c += Bk[t][a] * lam[a];
+        y1r[2 + k] = y0r[2 + k] + (dY[t] - D[t] * c);
     }
+}
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+// Forward-layer GEMM, C = A B^T with A (M x K) and B (N x K) both row-major,
+// on the gfx950 v_mfma_f32_16x16x32_bf16 (A/B lane layout: row lane % 16,
+// k = 8 (lane / 16) .. +7; D: row 4 (lane / 16) + r, column lane % 16).
+// 256 threads, a 128 x BN tile per workgroup, (64 x BN/2) per wavefront; K in
+// steps of 32 through LDS with 16-byte global loads and LDS accesses, the next
+// step's tile loaded into registers while the current one is multiplied.
+// Requires K % 32 == 0, N % BN == 0, lda, ldb % 8 == 0 and 16-byte aligned A,
+// B (checked by the launcher); rows >= M read zero and are not stored.
+template <int BN, int EPI>
+__global__ __launch_bounds__(256) void gemm_nt_bf16(int M, int N, int K, const uint16_t* __restrict__ A, int lda,
+                                                    const uint16_t* __restrict__ B, int ldb, EpiArgs e) {
+    constexpr int BM = 128, BK = 32, LK = BK + 8, WN = BN / 2, TJ = WN / 16, QB = BN / 64;
+    __shared__ __attribute__((aligned(16))) uint16_t As[BM * LK];
+    __shared__ __attribute__((aligned(16))) uint16_t Bs[BN * LK];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
+    const int wm = (w >> 1) * 64, wn = (w & 1) * WN;
+    f32x4 acc[4][TJ];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < TJ; j++) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    // chunk c = t + 256 q of a tile: row c / 4, k offset 8 (c % 4)
+    const int kc = (t & 3) * 8, r0 = t >> 2, r1 = (t + 256) >> 2;
+    const bool a0ok = m0 + r0 < M, a1ok = m0 + r1 < M;
+    const uint16_t* pa0 = A + static_cast<size_t>(a0ok ? m0 + r0 : 0) * lda + kc;
+    const uint16_t* pa1 = A + static_cast<size_t>(a1ok ? m0 + r1 : 0) * lda + kc;
+    const uint16_t* pb0 = B + static_cast<size_t>(n0 + r0) * ldb + kc;
+    const uint16_t* pb1 = B + static_cast<size_t>(n0 + (QB == 2 ? r1 : r0)) * ldb + kc;
+    const uint4 zero{0u, 0u, 0u, 0u};
+    uint4 ra0 = a0ok ? *reinterpret_cast<const uint4*>(pa0) : zero;
+    uint4 ra1 = a1ok ? *reinterpret_cast<const uint4*>(pa1) : zero;
+    uint4 rb0 = *reinterpret_cast<const uint4*>(pb0);
+    uint4 rb1 = (QB == 2) ? *reinterpret_cast<const uint4*>(pb1) : zero;
+    for (int k0 = 0; k0 < K; k0 += BK) {
+        *reinterpret_cast<uint4*>(&As[r0 * LK + kc]) = ra0;
+        *reinterpret_cast<uint4*>(&As[r1 * LK + kc]) = ra1;
+        *reinterpret_cast<uint4*>(&Bs[r0 * LK + kc]) = rb0;
+        if (QB == 2) *reinterpret_cast<uint4*>(&Bs[r1 * LK + kc]) = rb1;
+        __syncthreads();
+        if (k0 + BK < K) {
+            const int k1 = k0 + BK;
+            ra0 = a0ok ? *reinterpret_cast<const uint4*>(pa0 + k1) : zero;
+            ra1 = a1ok ? *reinterpret_cast<const uint4*>(pa1 + k1) : zero;
+            rb0 = *reinterpret_cast<const uint4*>(pb0 + k1);
+            if (QB == 2) rb1 = *reinterpret_cast<const uint4*>(pb1 + k1);
+        }
+        bf16x8 a[4], b[TJ];
+        const int ko = 8 * (lane >> 4);
+#pragma unroll
+        for (int i = 0; i < 4; i++) a[i] = *reinterpret_cast<const bf16x8*>(&As[(wm + 16 * i + (lane & 15)) * LK + ko]);
+#pragma unroll
+        for (int j = 0; j < TJ; j++) b[j] = *reinterpret_cast<const bf16x8*>(&Bs[(wn + 16 * j + (lane & 15)) * LK + ko]);
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int j = 0; j < TJ; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < TJ; j++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int m = m0 + wm + 16 * i + 4 * (lane >> 4) + r;
+                const int n = n0 + wn + 16 * j + (lane & 15);
+                if (m >= M) continue;
+                const float v = acc[i][j][r];
+                const size_t o = static_cast<size_t>(m) * e.ldc + n;
+                if (EPI == kFwdGelu) {
+                    const float z = v + e.bias[n];
+                    e.zf[o] = z;
+                    e.hb[o] = f2bf(gelu(z));
+                } else {
+                    e.zf[o] = v + e.bias[n];
+                }
+            }
 }
 
 void check(hipError_t e, const char* what) {
@@ -328,8 +468,36 @@ void check(hipError_t e, const char* what) {
 template <bool TA, bool TB, int EPI>
 void gemm(int M, int N, int K, const uint16_t* A, int lda, const uint16_t* B, int ldb, const EpiArgs& e,
           hipStream_t s) {
+    if constexpr (!TA && !TB && (EPI == kFwdGelu || EPI == kFwdLinear)) {
+        const bool aligned = (reinterpret_cast<uintptr_t>(A) % 16 == 0) && (reinterpret_cast<uintptr_t>(B) % 16 == 0) &&
+                             lda % 8 == 0 && ldb % 8 == 0 && K % 32 == 0;
+        if (aligned && N % 128 == 0) {
+            hipLaunchKernelGGL((gemm_nt_bf16<128, EPI>), dim3((M + 127) / 128, N / 128), dim3(256), 0, s, M, N, K, A,
+                               lda, B, ldb, e);
+            check(hipGetLastError(), "surrogate GEMM launch");
+            return;
+        }
+        if (aligned && N % 64 == 0) {
+            hipLaunchKernelGGL((gemm_nt_bf16<64, EPI>), dim3((M + 127) / 128, N / 64), dim3(256), 0, s, M, N, K, A,
+                               lda, B, ldb, e);
+            check(hipGetLastError(), "surrogate GEMM launch");
+            return;
+        }
+    }
     dim3 grid((M + kTM - 1) / kTM, (N + kTN - 1) / kTN);
-    hipLaunchKernelGGL((gemm_bf16<TA, TB, EPI>), grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, e);
+    int kchunk = K;
+    if (EPI == kStoreF32) {
+        // weight gradients: few output tiles and a long K (the batch rows);
+        // split K so that the grid has about 256 workgroups
+        const int tiles = static_cast<int>(grid.x * grid.y);
+        const int splits = std::max(1, std::min(256 / std::max(tiles, 1), K / 256));
+        if (splits > 1) {
+            kchunk = ((K + splits - 1) / splits + kTK - 1) / kTK * kTK;
+            grid.z = (K + kchunk - 1) / kchunk;
+            check(hipMemsetAsync(e.zf, 0, static_cast<size_t>(M) * e.ldc * sizeof(float), s), "hipMemsetAsync");
+        }
+    }
+    hipLaunchKernelGGL((gemm_bf16<TA, TB, EPI>), grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, e, kchunk);
     check(hipGetLastError(), "surrogate GEMM launch");
 }
 
@@ -340,6 +508,7 @@ int pad64(int x) { return (x + 63) / 64 * 64; }
 SurrogateNet::SurrogateNet(int K, double dt, int hidden, int depth, uint64_t seed) {
     if (K < 1 || hidden < 16 || hidden % 16 || depth < 1 || depth > 16 || !(dt > 0))
         throw std::runtime_error("surrogate: bad shape (hidden must be a positive multiple of 16, 1 <= depth <= 16)");
+    if (K > kMaxK) throw std::runtime_error("surrogate: at most 128 species");
     s_.K = K;
     s_.n_in = 2 + K;
     s_.n_out = 1 + K;
@@ -457,7 +626,9 @@ double SurrogateNet::backward_and_loss(int rows, bool update, double lr, hipStre
         ew.zf = grad_ + oW_[l];
         ew.ldc = fi;
         gemm<true, true, kStoreF32>(fo, fi, rows, G, fo, H, fi, ew, st);
-        hipLaunchKernelGGL(colsum_kernel, dim3((fo + 255) / 256), dim3(256), 0, st, rows, fo, G, grad_ + oB_[l]);
+        check(hipMemsetAsync(grad_ + oB_[l], 0, static_cast<size_t>(fo) * sizeof(float), st), "hipMemsetAsync");
+        hipLaunchKernelGGL(colsum_kernel, dim3((fo + 63) / 64, (rows + kColsumRows - 1) / kColsumRows), dim3(256), 0, st,
+                           rows, fo, G, grad_ + oB_[l]);
         check(hipGetLastError(), "surrogate bias gradient");
         if (l > 0) {
             // G_{l-1} = (G W_l) * gelu'(Z_{l-1})  (rows x fi)
@@ -584,6 +755,13 @@ void SurrogateNet::predict(long n, const double* y0, const double* P, double* y1
                            s_.in_p, s_.out_p, comp_, rmw_, E_, y1 + c0 * N);
         check(hipGetLastError(), "surrogate apply");
     }
+}
+
+void SurrogateNet::set_elements(const double* comp, const double* rmw, int E) {
+    if (E < 1 || E > kMaxE) throw std::runtime_error("surrogate: element projection supports 1 to 8 elements");
+    comp_ = comp;
+    rmw_ = rmw;
+    E_ = E;
 }
 
 void SurrogateNet::get_params(float* host) const {

@@ -56,7 +56,8 @@ public:
 
     // element table (K x E) and 1/W_k of the mechanism (device pointers): with
     // them, predictions conserve elements exactly (projection)
-    void set_elements(const double* comp, const double* rmw, int E) { comp_ = comp; rmw_ = rmw; E_ = E; }
+    // (at most 8 elements)
+    void set_elements(const double* comp, const double* rmw, int E);
 
     void get_params(float* host) const;
     void set_params(const float* host);

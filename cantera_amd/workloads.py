@@ -135,9 +135,11 @@ def config2_states(sol, n=1_048_576, config=2):
     return T, P, np.ascontiguousarray(Y.T)
 
 
-def config3(sol, n=65536, nsteps=100) -> ReactorWorkload:
-    """gri30, constant-pressure reactors of uneven stiffness, per-reactor dt."""
-    g = rng(3)
+def config3(sol, n=65536, nsteps=100, seed=None) -> ReactorWorkload:
+    """gri30, constant-pressure reactors of uneven stiffness, per-reactor dt.
+    `seed` (default: this workload's own) draws another set of reactors from
+    the same distribution."""
+    g = rng(3) if seed is None else np.random.Generator(np.random.Philox(seed))
     T0 = g.uniform(900.0, 2000.0, n)
     phi = g.uniform(0.5, 3.0, n)
     P = ONE_ATM * np.exp(g.uniform(np.log(1.0), np.log(40.0), n))

@@ -183,3 +183,31 @@ def test_untrained_network_falls_back_to_integrator():
     print("counts", tot, flush=True)
     assert tot[1] == 0
     assert np.max(np.abs(ya[:, 1] - yb[:, 1]) / ya[:, 1]) <= 1e-6
+
+
+def test_prediction_conserves_elements():
+    """predict() projects the mass-fraction increments onto the element-
+    conserving subspace (one wavefront per row, Gram matrix by wavefront sums):
+    element totals sum_k a_ek Y_k / W_k and sum_k Y_k of every predicted state
+    equal those of its input to rounding, whatever the network outputs (here an
+    untrained network on a fitted normalisation)."""
+    import cantera_amd as ca
+    from cantera_amd.surrogate import Surrogate
+    from cantera_amd.workloads import MechInfo
+    for mech in ("h2o2", "gri30", "nDodecane"):
+        info = MechInfo(mech_file(mech))
+        sol = ca.Solution(mech)
+        sur = Surrogate(sol, dt=1e-6, hidden=64, depth=2, seed=4)
+        y0, P, y1 = synthetic_pairs(info, 500, 6)
+        sur.fit_normalization(y0, P, y1)
+        yp = sur.predict(y0, P)
+        A = info.composition / info.molecular_weights[:, None]     # K x E
+        e0, e1 = y0[:, 2:] @ A, yp[:, 2:] @ A
+        scale = (np.abs(y0[:, 2:]) + np.abs(yp[:, 2:])) @ np.abs(A) + 1e-300
+        err = float(np.max(np.abs(e1 - e0) / scale))
+        dsum = float(np.max(np.abs(yp[:, 2:].sum(axis=1) - y0[:, 2:].sum(axis=1))
+                            / (np.abs(y0[:, 2:]).sum(axis=1) + np.abs(yp[:, 2:]).sum(axis=1))))
+        print(f"{mech}: element error {err:.2e}, sum Y change {dsum:.2e}", flush=True)
+        assert err <= 1e-10
+        assert dsum <= 1e-10
+        assert np.all(yp[:, 0] == y0[:, 0])
