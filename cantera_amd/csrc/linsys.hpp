@@ -79,6 +79,9 @@ struct RegLinSys {
     // column k are updated without selects, only the block holding column k
     // selects per column.  Each pivot row keeps 1/U_kk for the back substitution.
     __device__ __forceinline__ bool factor(int N) {
+        // N arrives in a VGPR when this is called through an out-of-line
+        // function: made uniform, every "k < N" below is a scalar branch
+        N = uni(N);
         const int lane = lane_id();
         bool pivoted = lane >= N;
         step = NMAX;
@@ -150,26 +153,32 @@ struct RegLinSys {
     // shuffle and step k of both substitutions reads lane k.
     template <typename V>
     __device__ __forceinline__ V solve(V b, int N) const {
+        // All NMAX steps run, with no test against N: outside the leading
+        // N x N block the factors are exact zeros and rdiag is 1 (form,
+        // restore and factor keep them so), and b is zeroed on lanes >= N, so
+        // steps k >= N leave b unchanged.  (A branch on k < N per step made
+        // the compiler keep all NMAX conditions in VGPR lanes.)
         const int lane = lane_id();
+        b = (lane < N) ? b : V(0);
         b = shfl_t(b, pivlane);
+        // The multipliers are masked off the dependency chain (l_ik for rows
+        // below k, u_ik for rows above), so a step is readlane -> FMA.  Lane k
+        // holds the final numerator of x_k once step k is reached and is not
+        // updated afterwards, so x = b / U_kk for all lanes at the end.
 #pragma unroll
         for (int k = 0; k < NMAX; k++) {
-            if (k < N) {
-                const V yk = readlane_t(b, k);
-                const V nb = b - static_cast<V>(a[k]) * yk;
-                b = (lane > k) ? nb : b;
-            }
+            const V lk = (lane > k) ? static_cast<V>(a[k]) : V(0);
+            const V yk = readlane_t(b, k);
+            b = b - lk * yk;
         }
         const V rd = static_cast<V>(rdiag);
 #pragma unroll
         for (int k = NMAX - 1; k >= 0; k--) {
-            if (k < N) {
-                const V xk = readlane_t(b * rd, k);
-                const V nb = b - static_cast<V>(a[k]) * xk;
-                b = (lane == k) ? xk : ((lane < k) ? nb : b);
-            }
+            const V uk = (lane < k) ? static_cast<V>(a[k]) : V(0);
+            const V xk = readlane_t(b * rd, k);
+            b = b - uk * xk;
         }
-        return b;
+        return b * rd;
     }
 
     template <typename V, int S>
@@ -323,6 +332,7 @@ __device__ __noinline__ V gr_solve(const MT* __restrict__ A, const int* __restri
 // M = I - gamma J into the global slot (column j of M at A + 64 j)
 template <int NMAX, typename MT>
 __device__ __noinline__ void gr_form(const double* __restrict__ J, MT* __restrict__ A, int N, double gamma) {
+    N = uni(N);
     const int lane = lane_id();
     const bool act = lane < N;
 #pragma unroll

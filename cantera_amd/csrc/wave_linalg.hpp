@@ -12,6 +12,8 @@ namespace cg {
 // false on an exactly zero pivot.
 template <int NS, typename T>
 __device__ bool lu_factor(T* A, int LD, int N, int* pmap) {
+    N = uni(N);
+    LD = uni(LD);
     const int lane = lane_id();
 #pragma unroll
     for (int s = 0; s < NS; s++) {
@@ -95,6 +97,8 @@ __device__ bool lu_factor(T* A, int LD, int N, int* pmap) {
 // lanes and returns x.  bv is an LDS scratch vector of length N.
 template <int NS, typename T, typename V>
 __device__ void lu_solve(const T* A, int LD, int N, const int* pmap, V* bv, V (&b)[NS]) {
+    N = uni(N);
+    LD = uni(LD);
     const int lane = lane_id();
 #pragma unroll
     for (int s = 0; s < NS; s++) {
