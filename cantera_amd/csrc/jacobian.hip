@@ -42,7 +42,8 @@ __global__ __launch_bounds__(64) void jac_batch_kernel(DevMech m, long n, const 
     const WaveScratch S = scratch_view(L);
     double* Jl = lds_base() + off;
     const int NN = m.N * m.N;
-    for (long st = blockIdx.x; st < n; st += gridDim.x) {
+    const XcdStates xs(n);
+    for (long st = xs.lo; st < xs.hi; st += xs.step) {
         const double Ts = T[st];
         const double Ps = uni(P[st]);
         double y[NS], f[NS];

@@ -30,7 +30,8 @@ __global__ __launch_bounds__(64) void rates_kernel(DevMech m, long n, const doub
     S.C = base;   base += m.K;
     S.MU = base;  base += m.K;
     S.WD = base;
-    for (long st = blockIdx.x; st < n; st += gridDim.x) {
+    const XcdStates xs(n);
+    for (long st = xs.lo; st < xs.hi; st += xs.step) {
         const double Ts = uni(T[st]);
         const double Ps = uni(P[st]);
         double yk[NS];

@@ -146,6 +146,31 @@ __device__ __forceinline__ T bcast(const T (&v)[NS], int k) {
     }
 }
 
+// States handled by this workgroup when a batch of n states, stored
+// species-major (the 8 states of a 64-byte line are neighbours), is spread over
+// the grid.  Workgroups are dispatched to the 8 XCDs round-robin
+// (blockIdx % 8), and each XCD has its own L2: XCD x takes the contiguous range
+// [x n / 8, (x + 1) n / 8), its workgroups stride through it, so the states
+// sharing a line meet in one L2 instead of eight.  Every state is visited
+// exactly once for any grid of at least 8 workgroups (plain grid-stride order
+// otherwise).
+struct XcdStates {
+    long lo, hi, step;
+    __device__ __forceinline__ XcdStates(long n) {
+        const long G = gridDim.x, b = blockIdx.x;
+        if (G < 8) {
+            lo = b; hi = n; step = G;
+            return;
+        }
+        const long x = b % 8, j = b / 8;
+        const long gx = (G - x + 7) / 8;     // workgroups on XCD x
+        const long c0 = n * x / 8, c1 = n * (x + 1) / 8;
+        lo = c0 + j;
+        hi = c1;
+        step = gx;
+    }
+};
+
 // Per-wave LDS scratch for one state evaluation.
 struct WaveScratch {
     double* yb;    // N (state broadcast)
