@@ -22,18 +22,6 @@
 namespace cg {
 
 template <typename T>
-__device__ __forceinline__ T readlane_t(T v, int l) {
-    if constexpr (sizeof(T) == 8) {
-        long long x = __double_as_longlong(v);
-        int lo = __builtin_amdgcn_readlane(static_cast<int>(x & 0xffffffffLL), l);
-        int hi = __builtin_amdgcn_readlane(static_cast<int>(x >> 32), l);
-        return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
-    } else {
-        return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
-    }
-}
-
-template <typename T>
 __device__ __forceinline__ T shfl_t(T v, int src) {
     if constexpr (sizeof(T) == 8) {
         const long long x = __double_as_longlong(v);

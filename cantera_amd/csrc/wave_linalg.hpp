@@ -7,6 +7,19 @@
 
 namespace cg {
 
+// v of lane l (l wave-uniform), for FP64 or FP32 values
+template <typename T>
+__device__ __forceinline__ T readlane_t(T v, int l) {
+    if constexpr (sizeof(T) == 8) {
+        long long x = __double_as_longlong(v);
+        int lo = __builtin_amdgcn_readlane(static_cast<int>(x & 0xffffffffLL), l);
+        int hi = __builtin_amdgcn_readlane(static_cast<int>(x >> 32), l);
+        return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+    } else {
+        return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+    }
+}
+
 // Factor A (N x N, leading dimension LD, row-major) in place: PA = LU with unit
 // L below the diagonal.  pmap[i] = original row of permuted row i.  Returns
 // false on an exactly zero pivot.
@@ -112,58 +125,71 @@ __device__ void lu_solve(const T* A, int LD, int N, const int* pmap, V* bv, V (&
         b[s] = (i < N) ? bv[pmap[i]] : V(0);
     }
     wave_sync();
-    // Forward substitution with the unit lower factor.  Lane i needs column
-    // k of its row at step k; the loads of kB consecutive columns are issued
-    // together ahead of the sequential readlane/FMA chain.
+    // Forward substitution with the unit lower factor, then back substitution.
+    // Step k broadcasts b_k (lane k % 64, slot k / 64 -- the slot is a
+    // compile-time index in each loop, so no select sits on the chain) and
+    // updates the other rows with the column-k multipliers, loaded kB columns
+    // ahead and masked to zero outside the triangle: a step is readlane -> FMA.
     constexpr int kB = 4;
-    for (int k0 = 0; k0 < N - 1; k0 += kB) {
-        T lv[kB][NS];
 #pragma unroll
-        for (int b2 = 0; b2 < kB; b2++)
+    for (int t = 0; t < NS; t++) {
+        const int kend = min(N - 1, kWave * (t + 1));
+        for (int k0 = kWave * t; k0 < kend; k0 += kB) {
+            T lv[kB][NS];
 #pragma unroll
-            for (int s = 0; s < NS; s++) {
-                const int i = lane + kWave * s;
-                const int k = k0 + b2;
-                lv[b2][s] = (i > k && i < N && k < N - 1) ? A[i * LD + k] : T(0);
-            }
-#pragma unroll
-        for (int b2 = 0; b2 < kB; b2++) {
-            const int k = k0 + b2;
-            if (k < N - 1) {
-                const V bk = bcast<NS>(b, k);
-#pragma unroll
-                for (int s = 0; s < NS; s++) b[s] -= static_cast<V>(lv[b2][s]) * bk;
-            }
-        }
-    }
-    // Back substitution with the upper factor.
-    for (int k0 = N - 1; k0 >= 0; k0 -= kB) {
-        T uv[kB][NS];
-#pragma unroll
-        for (int b2 = 0; b2 < kB; b2++)
-#pragma unroll
-            for (int s = 0; s < NS; s++) {
-                const int i = lane + kWave * s;
-                const int k = k0 - b2;
-                uv[b2][s] = (k >= 0 && i <= k) ? A[i * LD + k] : T(1);
-            }
-#pragma unroll
-        for (int b2 = 0; b2 < kB; b2++) {
-            const int k = k0 - b2;
-            if (k >= 0) {
-                const int s_k = k >> 6;
-#pragma unroll
-                for (int s = 0; s < NS; s++)
-                    if (s == s_k && lane == (k & 63)) b[s] /= static_cast<V>(uv[b2][s]);
-                const V bk = bcast<NS>(b, k);
+            for (int b2 = 0; b2 < kB; b2++)
 #pragma unroll
                 for (int s = 0; s < NS; s++) {
                     const int i = lane + kWave * s;
-                    if (i < k) b[s] -= static_cast<V>(uv[b2][s]) * bk;
+                    const int k = k0 + b2;
+                    lv[b2][s] = (i > k && i < N && k < kend) ? A[i * LD + k] : T(0);
+                }
+#pragma unroll
+            for (int b2 = 0; b2 < kB; b2++) {
+                const int k = k0 + b2;
+                if (k < kend) {
+                    const V bk = readlane_t(b[t], k - kWave * t);
+#pragma unroll
+                    for (int s = 0; s < NS; s++) b[s] -= static_cast<V>(lv[b2][s]) * bk;
                 }
             }
         }
     }
+    // Back substitution: 1/U_ii for every row first (in parallel); lane k holds
+    // the final numerator of x_k when step k is reached and is not updated
+    // afterwards, so x = b / U_ii for all rows at the end.
+    V rd[NS];
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        const int i = lane + kWave * s;
+        rd[s] = (i < N) ? V(1) / static_cast<V>(A[i * LD + i]) : V(1);
+    }
+#pragma unroll
+    for (int t = NS - 1; t >= 0; t--) {
+        const int klo = kWave * t;
+        for (int k0 = min(N, kWave * (t + 1)) - 1; k0 >= klo; k0 -= kB) {
+            T uv[kB][NS];
+#pragma unroll
+            for (int b2 = 0; b2 < kB; b2++)
+#pragma unroll
+                for (int s = 0; s < NS; s++) {
+                    const int i = lane + kWave * s;
+                    const int k = k0 - b2;
+                    uv[b2][s] = (k >= klo && i < k) ? A[i * LD + k] : T(0);
+                }
+#pragma unroll
+            for (int b2 = 0; b2 < kB; b2++) {
+                const int k = k0 - b2;
+                if (k >= klo) {
+                    const V xk = readlane_t(b[t], k - klo) * readlane_t(rd[t], k - klo);
+#pragma unroll
+                    for (int s = 0; s < NS; s++) b[s] -= static_cast<V>(uv[b2][s]) * xk;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < NS; s++) b[s] *= rd[s];
 }
 
 }  // namespace cg
