@@ -6,8 +6,9 @@
 //
 // Runs on the batch stream after each integrator launch as a counting sort on
 // the step count (clamped to kBuckets - 1): clear the histogram, count and rank
-// each reactor in its bucket, scan the histogram, scatter.  Four kernels and no
-// memset or library call, so the same sequence can be captured in a HIP graph.
+// each reactor in its bucket (LDS then global atomics), scan, scatter.  Four
+// kernels and no memset or library call, so the same sequence can be captured
+// in a HIP graph.
 // Reactors with equal counts land in atomic order; the order changes only
 // which wavefront integrates which reactor, never a reactor's result.
 #include <hip/hip_runtime.h>
@@ -25,19 +26,38 @@ __global__ void clear_hist_kernel(int* __restrict__ hist) {
     if (i < kBuckets) hist[i] = 0;
 }
 
-// bucket 0 holds the most expensive reactors
-__global__ void bucket_kernel(const double* __restrict__ rec, size_t rec_stride, int ctl_off, int n,
-                              int* __restrict__ prev_nst, int* __restrict__ bucket, int* __restrict__ rank,
-                              int* __restrict__ hist) {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= n) return;
-    const int nst = static_cast<int>(rec[static_cast<size_t>(r) * rec_stride + ctl_off + C_NST]);
-    int d = nst - prev_nst[r];
-    prev_nst[r] = nst;
-    d = d < 0 ? 0 : (d > kBuckets - 1 ? kBuckets - 1 : d);
-    const int b = kBuckets - 1 - d;
-    bucket[r] = b;
-    rank[r] = atomicAdd(&hist[b], 1);
+// bucket 0 holds the most expensive reactors.  Ranks within a bucket are
+// taken in two levels: LDS atomics inside the workgroup, then one global
+// atomic per non-empty bucket and workgroup reserves the workgroup's range
+// (most reactors share a few step counts, so per-reactor global atomics
+// would serialise on those buckets).
+__global__ __launch_bounds__(256) void bucket_kernel(const double* __restrict__ rec, size_t rec_stride, int ctl_off,
+                                                     int n, int* __restrict__ prev_nst, int* __restrict__ bucket,
+                                                     int* __restrict__ rank, int* __restrict__ hist) {
+    __shared__ int lh[kBuckets];
+    const int t = threadIdx.x;
+    for (int i = t; i < kBuckets; i += blockDim.x) lh[i] = 0;
+    __syncthreads();
+    const int r = blockIdx.x * blockDim.x + t;
+    int b = 0, lr = 0;
+    if (r < n) {
+        const int nst = static_cast<int>(rec[static_cast<size_t>(r) * rec_stride + ctl_off + C_NST]);
+        int d = nst - prev_nst[r];
+        prev_nst[r] = nst;
+        d = d < 0 ? 0 : (d > kBuckets - 1 ? kBuckets - 1 : d);
+        b = kBuckets - 1 - d;
+        lr = atomicAdd(&lh[b], 1);
+    }
+    __syncthreads();
+    for (int i = t; i < kBuckets; i += blockDim.x) {
+        const int c = lh[i];
+        if (c > 0) lh[i] = atomicAdd(&hist[i], c);
+    }
+    __syncthreads();
+    if (r < n) {
+        bucket[r] = b;
+        rank[r] = lh[b] + lr;
+    }
 }
 
 // exclusive scan of the histogram: one workgroup of kBuckets threads
