@@ -5,7 +5,6 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
-#include <map>
 #include <numeric>
 #include <sstream>
 #include <stdexcept>
@@ -232,51 +231,6 @@ DeviceMech::DeviceMech(const HostMech& h) : h_(h) {
         }
     }
 
-    // wdot gather: net stoichiometric coefficient per (species, permuted reaction)
-    std::vector<std::vector<std::pair<int, double>>> rows(K);
-    for (int p = 0; p < I; p++) {
-        const HostReaction& r = h.rxn[perm_[p]];
-        std::map<int, double> net;
-        for (size_t j = 0; j < r.rk.size(); j++) net[r.rk[j]] -= r.rs[j];
-        for (size_t j = 0; j < r.pk.size(); j++) net[r.pk[j]] += r.ps[j];
-        for (auto& [k, v] : net)
-            if (v != 0.0) rows[k].push_back({p, v});
-    }
-    size_t total = 0, longest = 0;
-    for (auto& r : rows) { total += r.size(); longest = std::max(longest, r.size()); }
-    // choose the sub-row length L minimising (passes * L + longest combine)
-    int bestL = 1;
-    double bestc = 1e300;
-    for (int L = 1; L <= std::max<int>(1, static_cast<int>(longest)); L++) {
-        int nsub = 0, maxsub = 0;
-        for (auto& r : rows) {
-            int s = std::max<int>(1, static_cast<int>((r.size() + L - 1) / L));
-            nsub += s;
-            maxsub = std::max(maxsub, s);
-        }
-        double c = static_cast<double>((nsub + 63) / 64) * L + maxsub;
-        if (c < bestc) { bestc = c; bestL = L; }
-    }
-    sub_len_ = bestL;
-    std::vector<int> sub_start{0}, sp_sub{0}, ent_r;
-    std::vector<double> ent_nu;
-    std::vector<float> ent_nuf;
-    for (int k = 0; k < K; k++) {
-        const auto& r = rows[k];
-        size_t i = 0;
-        do {
-            size_t e = std::min(r.size(), i + bestL);
-            for (; i < e; i++) {
-                ent_r.push_back(r[i].first);
-                ent_nu.push_back(r[i].second);
-                ent_nuf.push_back(static_cast<float>(r[i].second));
-            }
-            sub_start.push_back(static_cast<int>(ent_r.size()));
-        } while (i < r.size());
-        sp_sub.push_back(static_cast<int>(sub_start.size()) - 1);
-    }
-    (void)total;
-
     std::vector<double> nasa(16 * static_cast<size_t>(K), 0.0), rmw(K);
     for (int k = 0; k < K; k++) {
         nasa[16 * k] = h.tmid[k];
@@ -285,7 +239,6 @@ DeviceMech::DeviceMech(const HostMech& h) : h_(h) {
     }
 
     d_.K = K; d_.I = I; d_.N = K + 2; d_.E = h.E;
-    d_.nsub = static_cast<int>(sub_start.size()) - 1;
     d_.n_el = n_el; d_.n_tb = n_tb; d_.n_fo = n_fo;
     d_.pref = h.pref.empty() ? 101325.0 : h.pref[0];
     d_.mw = upload(h.mw);
@@ -298,11 +251,6 @@ DeviceMech::DeviceMech(const HostMech& h) : h_(h) {
     d_.gend = upload(gend);
     d_.tbk = upload(tbk);
     d_.tbe = upload(tbe);
-    d_.sub_start = upload(sub_start);
-    d_.sp_sub = upload(sp_sub);
-    d_.ent_r = upload(ent_r);
-    d_.ent_nu = upload(ent_nu);
-    d_.ent_nuf = upload(ent_nuf);
     d_.comp = upload(h.comp);
 }
 

@@ -62,7 +62,6 @@ static_assert(sizeof(RxnI) == 48, "RxnI layout");
 // Device view of a mechanism (all pointers are device pointers).
 struct DevMech {
     int K, I, N, E;          // species, reactions, state size (K+2), elements
-    int nsub;                // wdot gather sub-rows
     int n_el, n_tb, n_fo;    // reactions of each class, in permuted order: [Troe | Lindemann | three-body | elementary]
     double pref;
     const double* mw;        // K
@@ -75,11 +74,6 @@ struct DevMech {
     const double* gend;      // n_gen x 12: reactant order[4], reactant stoich[4], product stoich[4]
     const int* tbk;          // efficiency species
     const double* tbe;       // efficiency - default
-    const int* sub_start;    // nsub+1 entry offsets
-    const int* sp_sub;       // K+1 sub-row offsets per species
-    const int* ent_r;        // permuted reaction index per entry
-    const double* ent_nu;    // net stoichiometric coefficient per entry
-    const float* ent_nuf;    // same, FP32
     const double* comp;      // K x E element composition
 };
 
@@ -93,7 +87,6 @@ public:
     const DevMech& view() const { return d_; }
     const HostMech& host() const { return h_; }
     const std::vector<int>& perm() const { return perm_; }   // permuted index -> mechanism index
-    int sub_len() const { return sub_len_; }
     size_t table_bytes() const { return bytes_; }
 
 private:
@@ -101,7 +94,6 @@ private:
     DevMech d_{};
     std::vector<void*> allocs_;
     std::vector<int> perm_;
-    int sub_len_ = 0;
     size_t bytes_ = 0;
     template <class T> const T* upload(const std::vector<T>& v);
 };
