@@ -271,6 +271,31 @@ def reactors(a):
         },
         "integrator": {k: float(np.mean(v)) for k, v in stats.items() if k != "order"},
     }
+    if a.workload == "gri30-reactors" and a.precision == "fp64" and not a.no_mixed:
+        # the mixed form (FP32 iteration matrix / LU / solves, FP64 residual)
+        # on a fresh batch, same inputs and steps, after the timed FP64 region;
+        # its error is stated against the FP64 result
+        y64 = rb.state
+        del rb
+        # warm the mixed kernels on a few reactors, then time all K steps
+        wm = ca.ReactorBatch(sol, wl.T0[:64], wl.P[:64], wl.Y0[:64], rtol=wl.rtol, atol=wl.atol, precision=ca.CG_MIXED)
+        wm.advance(wl.dt[:64])
+        del wm
+        rm = ca.ReactorBatch(sol, wl.T0, wl.P, wl.Y0, rtol=wl.rtol, atol=wl.atol, precision=ca.CG_MIXED)
+        check(L.cg_device_synchronize())
+        t1 = time.perf_counter()
+        for k in range(steps):
+            rm.advance_device(d_tout, k)
+        check(L.cg_device_synchronize())
+        el_m = time.perf_counter() - t1
+        if int((rm.status != 0).sum()) == 0:
+            ym = rm.state
+            out["mixed_precision"] = {
+                "value": n * steps / el_m, "unit": "reactor-steps/s", "steps": steps,
+                "speedup_vs_fp64": (n * steps / el_m) / (n * steps / elapsed),
+                "max_abs_dT_vs_fp64_K": float(np.max(np.abs(ym[:, 1] - y64[:, 1]))),
+                "max_abs_dY_vs_fp64": float(np.max(np.abs(ym[:, 2:] - y64[:, 2:]))),
+            }
     if not a.no_cpu:
         sample = a.cpu_sample or {"gri30": 16384, "nDodecane": 512, "h2o2": 1024}[mech]
         out["cpu_baseline"] = cpu_reactors(sol.path, wl, steps, sample, cpu_threads())
@@ -507,6 +532,8 @@ def main(argv=None):
     ap.add_argument("--dT-tol", type=float, default=0.05, help="gri30-surrogate: guard, K")
     ap.add_argument("--rtol-g", type=float, default=1e-3, help="gri30-surrogate: guard, relative")
     ap.add_argument("--atol-g", type=float, default=1e-9, help="gri30-surrogate: guard, absolute Y")
+    ap.add_argument("--no-mixed", action="store_true",
+                    help="gri30-reactors: skip the mixed-precision run after the FP64 line")
     ap.add_argument("--cpu-sample", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",

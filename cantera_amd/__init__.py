@@ -166,6 +166,12 @@ class ReactorBatch:
         check(lib().cg_reactors_advance(self._h, t), "cg_reactors_advance")
         self.time = t.copy()
 
+    def advance_device(self, t_dev, row=0):
+        """Asynchronous advance to the output times held in device memory at
+        t_dev (row `row` of an n_steps x n array); check .status afterwards."""
+        ptr = (t_dev if isinstance(t_dev, int) else t_dev.ptr) + row * self.n * 8
+        check(lib().cg_reactors_advance_device(self._h, C.c_void_p(ptr)), "cg_reactors_advance_device")
+
     def advance_steps(self, t_steps):
         """Advance through the rows of t_steps (n_steps x n output times, or
         n_steps scalars) with one replayed HIP graph of n_steps integrator
