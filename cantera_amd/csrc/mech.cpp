@@ -113,8 +113,23 @@ HostMech load_mech(const std::string& path) {
             for (int k : r.rk) if (k < 0 || k >= m.K) throw std::runtime_error("reactant index out of range");
             for (int k : r.pk) if (k < 0 || k >= m.K) throw std::runtime_error("product index out of range");
             for (int k : r.tbk) if (k < 0 || k >= m.K) throw std::runtime_error("efficiency index out of range");
-            if (r.type < 0 || r.type > 3) throw std::runtime_error("unknown reaction type");
+            if (r.type < 0 || r.type > 5) throw std::runtime_error("unknown reaction type");
             last_r = i;
+        } else if (key == "PLOG") {
+            if (last_r < 0 || m.rxn[last_r].type != kPlog) throw std::runtime_error("PLOG record without a PLOG reaction");
+            const int ne = to_i(next());
+            if (ne < 1) throw std::runtime_error("PLOG reaction without rate expressions");
+            for (int e = 0; e < 4 * ne; e++) m.rxn[last_r].plog.push_back(to_d(next()));
+        } else if (key == "CHEB") {
+            if (last_r < 0 || m.rxn[last_r].type != kChebyshev)
+                throw std::runtime_error("CHEB record without a Chebyshev reaction");
+            std::vector<double>& c = m.rxn[last_r].cheb;
+            for (int e = 0; e < 4; e++) c.push_back(to_d(next()));
+            const int nT = to_i(next()), nP = to_i(next());
+            if (nT < 1 || nP < 1) throw std::runtime_error("empty Chebyshev coefficient matrix");
+            c.push_back(nT);
+            c.push_back(nP);
+            for (int e = 0; e < nT * nP; e++) c.push_back(to_d(next()));
         } else if (key == "end") {
             ended = true;
         }
@@ -161,14 +176,16 @@ DeviceMech::DeviceMech(const HostMech& h) : h_(h) {
     // Reactions of the costliest class first: a wavefront evaluates 64
     // consecutive reactions, so the falloff reactions share one pass (with the
     // three-body ones and the first elementary ones) instead of leaving a last
-    // pass that runs the Troe code for a few lanes.
+    // pass that runs the Troe code for a few lanes.  PLOG and Chebyshev
+    // reactions (types 4, 5) lead: their rate constants are evaluated once per
+    // state by lanes 0..63 before the reaction loop.
     std::stable_sort(perm_.begin(), perm_.end(), [&](int a, int b) { return h.rxn[a].type > h.rxn[b].type; });
 
-    std::vector<double> rd(4 * static_cast<size_t>(I)), tbdef(I), fod, gend;
+    std::vector<double> rd(4 * static_cast<size_t>(I)), tbdef(I), fod, gend, pdd;
     std::vector<RxnI> ri(I);
     std::vector<int> tbk;
     std::vector<double> tbe;
-    int n_el = 0, n_tb = 0, n_fo = 0;
+    int n_el = 0, n_tb = 0, n_fo = 0, n_pd = 0;
     for (int p = 0; p < I; p++) {
         const HostReaction& r = h.rxn[perm_[p]];
         RxnI& x = ri[p];
@@ -177,9 +194,14 @@ DeviceMech::DeviceMech(const HostMech& h) : h_(h) {
         x.orig = perm_[p];
         x.fo = -1;
         x.gen = -1;
-        x.flags = (r.rev ? 1 : 0) | (r.type << 1) | (r.type != kElementary ? 32 : 0);
-        if (r.type == kElementary) n_el++;
-        else if (r.type == kThreeBody) n_tb++;
+        // PLOG and Chebyshev reactions are mass-action reactions (type 0 on
+        // the device) whose rate constant comes from their pdd record
+        const int dtype = (r.type >= kPlog) ? kElementary : r.type;
+        x.flags = (r.rev ? 1 : 0) | (dtype << 1) | (dtype != kElementary ? 32 : 0) | (r.type == kPlog ? 64 : 0) |
+                  (r.type == kChebyshev ? 128 : 0);
+        if (r.type >= kPlog) n_pd++;
+        if (dtype == kElementary) n_el++;
+        else if (dtype == kThreeBody) n_tb++;
         else n_fo++;
         rd[4 * p + 0] = r.A; rd[4 * p + 1] = r.b; rd[4 * p + 2] = r.EaR; rd[4 * p + 3] = r.dn;
         tbdef[p] = r.tbdef;
@@ -222,7 +244,41 @@ DeviceMech::DeviceMech(const HostMech& h) : h_(h) {
             double fo[8] = {r.A0, r.b0, r.EaR0, r.troe[0], r.troe[1], r.troe[2], r.troe[3], 0.0};
             fod.insert(fod.end(), fo, fo + 8);
         }
-        if (r.type != kElementary) {
+        if (r.type == kPlog) {
+            // PlogRate::setRates (PlogRate.cpp:92-123): groups of equal
+            // pressure, plus copies of the first and last group at
+            // ln P = -1000 and +1000.  Record: ng, ln P[ng], first[ng],
+            // end[ng] (entry indices), then A, b, Ea/R per entry.
+            const int ne = static_cast<int>(r.plog.size() / 4);
+            std::vector<double> lp, g0, g1;
+            for (int e = 0; e < ne; e++) {
+                const double l = std::log(r.plog[4 * e]);
+                if (lp.empty() || lp.back() != l) { lp.push_back(l); g0.push_back(e); g1.push_back(e + 1); }
+                else g1.back() = e + 1;
+            }
+            lp.insert(lp.begin(), -1000.0); g0.insert(g0.begin(), g0.front()); g1.insert(g1.begin(), g1.front());
+            lp.push_back(1000.0); g0.push_back(g0.back()); g1.push_back(g1.back());
+            x.fo = static_cast<int>(pdd.size());
+            pdd.push_back(static_cast<double>(lp.size()));
+            pdd.insert(pdd.end(), lp.begin(), lp.end());
+            pdd.insert(pdd.end(), g0.begin(), g0.end());
+            pdd.insert(pdd.end(), g1.begin(), g1.end());
+            for (int e = 0; e < ne; e++)
+                for (int j = 1; j < 4; j++) pdd.push_back(r.plog[4 * e + j]);
+        } else if (r.type == kChebyshev) {
+            // ChebyshevRate::setLimits (ChebyshevRate.cpp:96-112).  Record:
+            // TrNum, TrDen, PrNum, PrDen, nT, nP, coefficients (row-major).
+            const double Tmin = r.cheb[0], Tmax = r.cheb[1];
+            const double lPmin = std::log10(r.cheb[2]), lPmax = std::log10(r.cheb[3]);
+            const double TminInv = 1.0 / Tmin, TmaxInv = 1.0 / Tmax;
+            x.fo = static_cast<int>(pdd.size());
+            pdd.push_back(-TminInv - TmaxInv);
+            pdd.push_back(1.0 / (TmaxInv - TminInv));
+            pdd.push_back(-lPmin - lPmax);
+            pdd.push_back(1.0 / (lPmax - lPmin));
+            pdd.insert(pdd.end(), r.cheb.begin() + 4, r.cheb.end());
+        }
+        if (dtype != kElementary) {
             if (r.tbk.size() > kMaxTB) throw std::runtime_error("too many third-body efficiencies");
             x.tbo = static_cast<int>(tbk.size());
             x.tbn = static_cast<int>(r.tbk.size());
@@ -240,6 +296,8 @@ DeviceMech::DeviceMech(const HostMech& h) : h_(h) {
 
     d_.K = K; d_.I = I; d_.N = K + 2; d_.E = h.E;
     d_.n_el = n_el; d_.n_tb = n_tb; d_.n_fo = n_fo;
+    d_.n_pd = n_pd;
+    if (n_pd > 64) throw std::runtime_error("mechanisms with more than 64 PLOG and Chebyshev reactions are not supported");
     d_.pref = h.pref.empty() ? 101325.0 : h.pref[0];
     d_.mw = upload(h.mw);
     d_.rmw = upload(rmw);
@@ -249,6 +307,7 @@ DeviceMech::DeviceMech(const HostMech& h) : h_(h) {
     d_.ri = upload(ri);
     d_.fod = upload(fod);
     d_.gend = upload(gend);
+    d_.pdd = upload(pdd);
     d_.tbk = upload(tbk);
     d_.tbe = upload(tbe);
     d_.comp = upload(h.comp);

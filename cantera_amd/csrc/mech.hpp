@@ -13,7 +13,7 @@ namespace cg {
 constexpr int kMaxSide = 4;   // species entries per reaction side on the device
 constexpr int kMaxTB = 32;    // explicit efficiencies per reaction
 
-enum RxnType { kElementary = 0, kThreeBody = 1, kLindemann = 2, kTroe = 3 };
+enum RxnType { kElementary = 0, kThreeBody = 1, kLindemann = 2, kTroe = 3, kPlog = 4, kChebyshev = 5 };
 
 struct HostReaction {
     int type = 0;
@@ -27,6 +27,8 @@ struct HostReaction {
     double tbdef = 0;
     std::vector<int> tbk;
     std::vector<double> tbe;         // efficiency - default
+    std::vector<double> plog;        // kPlog: (P [Pa], A, b, Ea/R) per entry, sorted by P
+    std::vector<double> cheb;        // kChebyshev: Tmin, Tmax, Pmin, Pmax, nT, nP, nT x nP coefficients
     std::string equation;
 };
 
@@ -48,10 +50,11 @@ HostMech load_mech(const std::string& path);
 // three molecules are expanded into repeated species ("kRep", C1/C2/C3); other
 // sides are "general" (C_AnyN: species, order, stoich).
 struct RxnI {
-    int flags;      // bit0 reversible, bits1-2 type, bit3 reactant side general, bit4 product side general, bit5 third body
+    int flags;      // bit0 reversible, bits1-2 type (0..3), bit3 reactant side general, bit4 product side general,
+                    // bit5 third body, bit6 PLOG, bit7 Chebyshev (those two with type 0)
     int nr, np;     // entries in rk/pk
     int tbo, tbn;   // third-body efficiency list offset/count
-    int fo;         // falloff record index (-1 if none)
+    int fo;         // falloff record index, or PLOG / Chebyshev record offset in pdd (-1 if none)
     int gen;        // general-order record index (-1 if none)
     int orig;       // reaction index in mechanism order
     short rk[kMaxSide];
@@ -62,7 +65,9 @@ static_assert(sizeof(RxnI) == 48, "RxnI layout");
 // Device view of a mechanism (all pointers are device pointers).
 struct DevMech {
     int K, I, N, E;          // species, reactions, state size (K+2), elements
-    int n_el, n_tb, n_fo;    // reactions of each class, in permuted order: [Troe | Lindemann | three-body | elementary]
+    int n_el, n_tb, n_fo;    // reactions of each class, in permuted order:
+                             // [Chebyshev | PLOG | Troe | Lindemann | three-body | elementary]
+    int n_pd;                // PLOG + Chebyshev reactions (the first n_pd, at most 64)
     double pref;
     const double* mw;        // K
     const double* rmw;       // K
@@ -75,6 +80,7 @@ struct DevMech {
     const int* tbk;          // efficiency species
     const double* tbe;       // efficiency - default
     const double* comp;      // K x E element composition
+    const double* pdd;       // PLOG and Chebyshev records (see DeviceMech)
 };
 
 // Owns device memory for a DevMech.

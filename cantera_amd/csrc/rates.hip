@@ -18,10 +18,12 @@ namespace cg {
 
 namespace {
 
+// At most 168 VGPRs, so that three wavefronts share a SIMD (latency-bound
+// kernel; without the bound the allocator settles at 169 for NS = 1).
 template <int NS, bool kF32>
-__global__ __launch_bounds__(64) void rates_kernel(DevMech m, long n, const double* __restrict__ T,
-                                                   const double* __restrict__ P, const double* __restrict__ Y,
-                                                   double* __restrict__ wdot) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void rates_kernel(
+    DevMech m, long n, const double* __restrict__ T, const double* __restrict__ P, const double* __restrict__ Y,
+    double* __restrict__ wdot) {
     extern __shared__ double lds_raw[];
     const int lane = lane_id();
     double* base = lds_raw;

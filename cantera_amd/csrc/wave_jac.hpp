@@ -24,11 +24,11 @@ __device__ __forceinline__ void mat_add(double* p, double v) { unsafeAtomicAdd(p
 // of A).  All per-reaction lists use static indices (registers only).
 __device__ __forceinline__ void scatter_reaction_derivs(const DevMech& m, int p, const GasState& g,
                                                         const double* __restrict__ C, const double* __restrict__ MU,
-                                                        double* A, int LD) {
+                                                        double kpd, double* A, int LD) {
     RxnRegs x;
     load_rxn(m, p, x);
     RxnVals v;
-    eval_reaction(m, p, g, C, MU, x, v);
+    eval_reaction(m, p, g, C, MU, x, kpd, v);
     const int type = (x.flags >> 1) & 3;
     const bool rev = x.flags & 1;
     const bool rgen = x.flags & 8, pgen = x.flags & 16;
@@ -179,7 +179,8 @@ __device__ void reactor_jacobian(const DevMech& m, const WaveScratch& S, double 
     }
     __threadfence();
     wave_sync();
-    for (int p = lane; p < m.I; p += kWave) scatter_reaction_derivs(m, p, g, S.C, S.MU, JC, N);
+    const double kpd = pdep_rate_lane(m, g);
+    for (int p = lane; p < m.I; p += kWave) scatter_reaction_derivs(m, p, g, S.C, S.MU, kpd, JC, N);
     __threadfence();
     wave_sync();
     // D_k = sum_l JC[k][l] C_l ; keep wdot and D in W2, h_k in MU

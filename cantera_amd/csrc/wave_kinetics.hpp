@@ -284,13 +284,90 @@ __device__ __forceinline__ double rate_exp10(double x) {
     return exp10(x);
 }
 
+// PLOG rate constant (PlogRate::updateFromStruct / evalFromStruct,
+// include/cantera/kinetics/PlogRate.h:110-164).  Record r: ng, ln P[ng],
+// first[ng], end[ng], then (A, b, Ea/R) per entry; groups 0 and ng-1 are the
+// copies at ln P = -1000 and +1000.  The interval is [ln P[i-1], ln P[i]) with
+// i the first group above ln P; each end takes ln(1e-300 + sum of the group's
+// Arrhenius terms), and ln k is interpolated linearly in ln P.
+__device__ __forceinline__ double plog_rate(const double* __restrict__ r, double logP, double logT, double recipT) {
+    const int ng = static_cast<int>(r[0]);
+    const double* lp = r + 1;
+    const double* e = r + 1 + 3 * ng;
+    int i = 1;
+    while (i < ng - 1 && !(lp[i] > logP)) i++;
+    double lk[2];
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        const int q = i - 1 + h;
+        double k = 1e-300;
+        for (int j = static_cast<int>(r[1 + ng + q]); j < static_cast<int>(r[1 + 2 * ng + q]); j++)
+            k += e[3 * j] * exp(e[3 * j + 1] * logT - e[3 * j + 2] * recipT);
+        lk[h] = log(k);
+    }
+    const double rdp = 1.0 / (lp[i] - lp[i - 1]);
+    return exp(lk[0] + (lk[1] - lk[0]) * (logP - lp[i - 1]) * rdp);
+}
+
+// Chebyshev rate constant (ChebyshevRate::updateFromStruct / evalFromStruct,
+// include/cantera/kinetics/ChebyshevRate.h:133-171).  Record r: TrNum, TrDen,
+// PrNum, PrDen, nT, nP, coefficients (nT x nP, row-major).  Each row is first
+// reduced over the pressure polynomials, then the temperature series is summed
+// with the same three-term recurrence, row by row.
+__device__ __forceinline__ double cheb_rate(const double* __restrict__ r, double log10P, double recipT) {
+    const int nT = static_cast<int>(r[4]), nP = static_cast<int>(r[5]);
+    const double* c = r + 6;
+    const double Pr = (2 * log10P + r[2]) * r[3];
+    const double Tr = (2 * recipT + r[0]) * r[1];
+    double logk = 0.0, Tm1 = Tr, Tn = 1.0;
+    for (int i = 0; i < nT; i++) {
+        double dot = c[i * nP];
+        double Pm1 = Pr, Pn = 1.0;
+        for (int j = 1; j < nP; j++) {
+            const double Pp1 = 2 * Pr * Pn - Pm1;
+            dot += Pp1 * c[i * nP + j];
+            Pm1 = Pn;
+            Pn = Pp1;
+        }
+        if (i == 0) {
+            logk = dot;
+        } else {
+            const double Tp1 = 2 * Tr * Tn - Tm1;
+            logk += Tp1 * dot;
+            Tm1 = Tn;
+            Tn = Tp1;
+        }
+    }
+    return pow(10.0, logk);
+}
+
+// Rate constants of the PLOG and Chebyshev reactions at the gas state.  They
+// come first in permuted order and number at most 64, so lane j evaluates
+// reaction j once per state, before the reaction loop; the loop's first pass
+// (lane j, reaction j) takes the value from the register.  Other lanes, and
+// mechanisms without such reactions, return 0.
+__device__ __forceinline__ double pdep_rate_lane(const DevMech& m, const GasState& g) {
+    double k = 0.0;
+    if (m.n_pd > 0) {
+        const int lane = lane_id();
+        if (lane < m.n_pd) {
+            const int flags = m.ri[lane].flags;
+            const int fo = m.ri[lane].fo;
+            k = (flags & 64) ? plog_rate(m.pdd + fo, log(g.pressure), g.logT, g.recipT)
+                             : cheb_rate(m.pdd + fo, log10(g.pressure), g.recipT);
+        }
+    }
+    return k;
+}
+
 // Evaluate reaction p (permuted index) at the gas state; C and MU live in LDS.
+// kpd: this lane's pdep_rate_lane value (used for flags bit 6/7 only).
 // kF32: rate constants (Arrhenius, falloff centre and blending) through the
 // FP32 transcendentals above; everything else FP64.
 template <bool kF32 = false>
 __device__ __forceinline__ void eval_reaction(const DevMech& m, int p, const GasState& g,
                                               const double* __restrict__ C, const double* __restrict__ MU,
-                                              const RxnRegs& x, RxnVals& o) {
+                                              const RxnRegs& x, double kpd, RxnVals& o) {
     const double4 rd = reinterpret_cast<const double4*>(m.rd)[p];
     const int type = (x.flags >> 1) & 3;
     const bool rev = x.flags & 1;
@@ -319,7 +396,8 @@ __device__ __forceinline__ void eval_reaction(const DevMech& m, int p, const Gas
     }
     double kf;
     if (type <= kThreeBody) {
-        kf = rd.x * rate_exp<kF32>(rd.y * g.logT - rd.z * g.recipT);
+        // PLOG and Chebyshev constants (FP64 in both precisions) come from kpd
+        kf = (x.flags & 192) ? kpd : rd.x * rate_exp<kF32>(rd.y * g.logT - rd.z * g.recipT);
     } else {
         const double* fo = m.fod + 8 * x.fo;
         double work = 0.0;
@@ -496,11 +574,12 @@ __device__ __forceinline__ void production_rates(const DevMech& m, const WaveScr
         if (k < m.K) S.WD[k] = 0.0;
     }
     wave_sync();
+    const double kpd = pdep_rate_lane(m, g);
     for (int p = lane; p < m.I; p += kWave) {
         RxnVals v;
         RxnRegs x;
         load_rxn(m, p, x);
-        eval_reaction<kF32>(m, p, g, S.C, S.MU, x, v);
+        eval_reaction<kF32>(m, p, g, S.C, S.MU, x, kpd, v);
         const double r = v.ropf - v.ropr;
         const double* gs = (x.gen >= 0) ? m.gend + 12 * x.gen : m.gend;
         const bool rgen = x.flags & 8, pgen = x.flags & 16;

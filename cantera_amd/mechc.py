@@ -30,10 +30,18 @@ What is restated from the reference, and where:
 * third-body efficiencies are stored as (eff - default), sorted by species
   index -- ``ThirdBodyCalc::install`` ``ThirdBodyCalc.h:23-44``.
 * delta-n -- ``BulkKinetics::addReaction`` ``BulkKinetics.cpp:27-35``.
+* pressure-dependent Arrhenius (PLOG) -- ``PlogRate::setRates``
+  ``PlogRate.cpp:92-123``: entries sorted by pressure (equal pressures keep
+  their input order, as in the reference's multimap), each A converted with
+  the reaction's rate units; negative A allowed inside a group.
+* Chebyshev -- ``ChebyshevRate::setParameters`` ``ChebyshevRate.cpp:62-94``:
+  temperature and pressure ranges converted to K and Pa, and log10 of the
+  rate-unit conversion of 1.0 added to coefficient (0, 0); reaction-local
+  ``units`` are honoured.
 
 Supported rate types: elementary Arrhenius (incl. ``negative-A``), three-body,
-Lindemann and Troe falloff, reaction ``orders``.  Anything else raises
-``MechanismError`` naming the reaction.
+Lindemann and Troe falloff, PLOG, Chebyshev, reaction ``orders``.  Anything
+else raises ``MechanismError`` naming the reaction.
 """
 from __future__ import annotations
 
@@ -64,6 +72,8 @@ _LENGTH = {"m": 1.0, "cm": 0.01, "mm": 0.001}
 _QUANTITY = {"kmol": 1.0, "mol": 1e-3, "molec": 1.0 / AVOGADRO}
 _TIME = {"s": 1.0, "ms": 0.001, "min": 60.0, "hr": 3600.0}
 _ENERGY = {"J": 1.0, "kJ": 1000.0, "cal": 4.184, "kcal": 4184.0, "eV": 1.602176634e-19 * 1.0}
+_PRESSURE = {"Pa": 1.0, "kPa": 1e3, "MPa": 1e6, "bar": 1e5, "atm": ONE_ATM, "torr": ONE_ATM / 760.0,
+             "dyn/cm^2": 0.1}
 
 
 class MechanismError(ValueError):
@@ -158,15 +168,30 @@ class UnitSystem:
             us.ea_factor = energy / us.quantity
         return us
 
+    def merged(self, node: dict | None) -> "UnitSystem":
+        """This system with a reaction's local ``units`` overrides."""
+        if not node:
+            return self
+        us = UnitSystem(self.length, self.quantity, self.time, self.ea_factor)
+        if "length" in node:
+            us.length = _LENGTH[node["length"]]
+        if "quantity" in node:
+            us.quantity = _QUANTITY[node["quantity"]]
+        if "time" in node:
+            us.time = _TIME[node["time"]]
+        return us
+
     def rate_coeff(self, value, order: float) -> float:
         """A in the input system -> SI kmol/m^3/s for a reaction of total order n.
 
         ``UnitSystem::convertTo`` (``Units.cpp:562-571``) with destination
         units (kmol/m^3)^(1-n)/s: length dim 3(n-1), quantity dim 1-n, time -1.
+        A value with explicit units (``2.9e-17 cm^3/molec/s``) is converted by
+        those units alone (``Units(string)``, ``Units.cpp:134-192``).
         """
         v, u = _split_value_units(value)
         if u:
-            raise MechanismError(f"explicit units on a pre-exponential factor are not supported: {value!r}")
+            return v * _rate_units_factor(u)
         L = 3.0 * (order - 1.0)
         Q = 1.0 - order
         Tm = -1.0
@@ -181,6 +206,37 @@ class UnitSystem:
                 return v
             return v * f / (GAS_CONSTANT * 1.0)
         return v * self.ea_factor / GAS_CONSTANT
+
+
+def _rate_units_factor(u: str) -> float:
+    """SI factor of a rate-coefficient unit string such as ``cm^3/molec/s``."""
+    factor = 1.0
+    sign = 1.0
+    for g in re.split(r"([*/])", u.strip()):
+        g = g.strip()
+        if g in ("*", "/"):
+            sign = -1.0 if g == "/" else 1.0
+            continue
+        if not g:
+            continue
+        expo = 1.0
+        if "^" in g:
+            g, e = g.split("^")
+            expo = float(e)
+        table = _LENGTH if g in _LENGTH else _QUANTITY if g in _QUANTITY else _TIME if g in _TIME else None
+        if table is None:
+            raise MechanismError(f"unsupported rate-coefficient unit {u!r}")
+        factor *= table[g] ** (expo * sign)
+    return factor
+
+
+def _pressure(value) -> float:
+    v, u = _split_value_units(value)
+    if not u:
+        return v
+    if u not in _PRESSURE:
+        raise MechanismError(f"unsupported pressure unit {u!r}")
+    return v * _PRESSURE[u]
 
 
 @dataclass
@@ -213,9 +269,11 @@ class Reaction:
     tb_default: float = 0.0
     tb_eff: list = field(default_factory=list)  # [(k, eff - default)] sorted by k
     dn: float = 0.0
+    plog: list = field(default_factory=list)    # PLOG: [(P [Pa], A, b, Ea/R)] sorted by P
+    cheb: tuple = ()                            # Chebyshev: (Tmin, Tmax, Pmin, Pmax, rows)
 
 
-TYPE_ELEMENTARY, TYPE_THREE_BODY, TYPE_LINDEMANN, TYPE_TROE = 0, 1, 2, 3
+TYPE_ELEMENTARY, TYPE_THREE_BODY, TYPE_LINDEMANN, TYPE_TROE, TYPE_PLOG, TYPE_CHEBYSHEV = 0, 1, 2, 3, 4, 5
 
 
 def _parse_equation(eq: str):
@@ -459,10 +517,19 @@ def _compile_reaction(node, index, units: UnitSystem) -> Reaction:
     rate_type = node.get("type", "")
     if rate_type in ("", "reaction"):
         rate_type = ""
-    if rate_type not in ("", "elementary", "three-body", "falloff"):
+    if rate_type not in ("", "elementary", "three-body", "falloff", "pressure-dependent-Arrhenius", "Chebyshev"):
         raise MechanismError(f"reaction {eq!r}: rate type {rate_type!r} not supported")
+    pdep = rate_type in ("pressure-dependent-Arrhenius", "Chebyshev")
     reactants, products, reversible = _parse_equation(eq)
-    tb_name, mass_action = _detect_third_body(reactants, products, rate_type, node)
+    if pdep:
+        # a (+M) written around a Chebyshev equation is the deprecated form;
+        # PLOG and Chebyshev reactions have no third body
+        for side in (reactants, products):
+            for name in [n for n in side if n == "M" or n.startswith("(+")]:
+                raise MechanismError(f"reaction {eq!r}: {rate_type} reactions take no third body")
+        tb_name, mass_action = None, True
+    else:
+        tb_name, mass_action = _detect_third_body(reactants, products, rate_type, node)
     if rate_type == "falloff":
         if tb_name is None:
             raise MechanismError(f"falloff reaction {eq!r} has no (+M)")
@@ -517,6 +584,25 @@ def _compile_reaction(node, index, units: UnitSystem) -> Reaction:
             raise MechanismError(f"reaction {eq!r}: SRI/Tsang falloff not supported")
         else:
             r.rtype = TYPE_LINDEMANN
+    elif rate_type == "pressure-dependent-Arrhenius":
+        ents = []
+        for rc in node["rate-constants"]:
+            ents.append((_pressure(rc["P"]), units.rate_coeff(rc["A"], order), _num(rc["b"]),
+                         units.ea_over_r(rc["Ea"]) if "Ea" in rc else 0.0))
+        if not ents:
+            raise MechanismError(f"reaction {eq!r}: PLOG without rate constants")
+        r.plog = sorted(ents, key=lambda e: e[0])      # stable: equal pressures keep input order
+        r.rtype = TYPE_PLOG
+    elif rate_type == "Chebyshev":
+        lu = units.merged(node.get("units"))
+        Tr = [_num(_split_value_units(t)[0]) for t in node["temperature-range"]]
+        Pr = [_pressure(x) for x in node["pressure-range"]]
+        rows = [[_num(x) for x in row] for row in node["data"]]
+        if not rows or any(len(row) != len(rows[0]) for row in rows):
+            raise MechanismError(f"reaction {eq!r}: inconsistent Chebyshev coefficient matrix")
+        rows[0][0] += math.log10(lu.rate_coeff(1.0, order))
+        r.cheb = (Tr[0], Tr[1], Pr[0], Pr[1], rows)
+        r.rtype = TYPE_CHEBYSHEV
     else:
         r.A, r.b, r.EaR = arr("rate-constant", 0.0)
         r.rtype = TYPE_THREE_BODY if is_tb else TYPE_ELEMENTARY
@@ -595,6 +681,12 @@ def write_mech(mech, out_path):
         for k, e in r.tb_eff:
             toks += [str(k), _f(e)]
         lines.append(" ".join(toks))
+        if r.rtype == TYPE_PLOG:
+            lines.append(" ".join(["PLOG", str(len(r.plog))] + [_f(x) for e in r.plog for x in e]))
+        elif r.rtype == TYPE_CHEBYSHEV:
+            Tmin, Tmax, Pmin, Pmax, rows = r.cheb
+            lines.append(" ".join(["CHEB", _f(Tmin), _f(Tmax), _f(Pmin), _f(Pmax), str(len(rows)), str(len(rows[0]))]
+                                  + [_f(x) for row in rows for x in row]))
         lines.append("E " + r.equation)
     lines.append("end")
     with open(out_path, "w") as f:

@@ -164,6 +164,9 @@ struct or_mech {
     int (*rk)[MAXR], (*pk)[MAXR], (*tbk)[MAXTB];
     double (*rs)[MAXR], (*ro)[MAXR], (*ps)[MAXR], (*tbe)[MAXTB];
     double* tbdef;
+    int* pdep;   /* 4 PLOG, 5 Chebyshev (type[] is 0 for those), else 0 */
+    double** pd; /* PLOG: (P, A, b, Ea/R) per entry; Chebyshev: Tmin, Tmax, Pmin, Pmax, nT, nP, coefficients */
+    int* npd;    /* PLOG entries / Chebyshev record length */
     stoich_t reac, prod, revprod;
 };
 
@@ -198,7 +201,7 @@ or_mech* or_mech_load(const char* path) {
     size_t cap = 1 << 16;
     char* line = malloc(cap);
     char tok[256];
-    int ok = 0;
+    int ok = 0, last = -1;
     while (fgets(line, (int)cap, f)) {
         char* p = line;
         if (!next_tok(&p, tok, sizeof tok)) continue;
@@ -230,12 +233,16 @@ or_mech* or_mech_load(const char* path) {
             m->rk = calloc(I, sizeof *m->rk); m->pk = calloc(I, sizeof *m->pk); m->tbk = calloc(I, sizeof *m->tbk);
             m->rs = calloc(I, sizeof *m->rs); m->ro = calloc(I, sizeof *m->ro); m->ps = calloc(I, sizeof *m->ps);
             m->tbe = calloc(I, sizeof *m->tbe); m->tbdef = calloc(I, 8);
+            m->pdep = calloc(I, 4); m->pd = calloc(I, sizeof *m->pd); m->npd = calloc(I, 4);
             stoich_init(&m->reac, I + 1);
             stoich_init(&m->prod, I + 1);
             stoich_init(&m->revprod, I + 1);
         } else if (!strcmp(tok, "R")) {
             int i = toki(&p);
             m->type[i] = toki(&p); m->rev[i] = toki(&p); m->dn[i] = tokd(&p);
+            if (m->type[i] < 0 || m->type[i] > 5) { set_err("unknown reaction type"); goto fail; }
+            last = i;
+            if (m->type[i] >= 4) { m->pdep[i] = m->type[i]; m->type[i] = 0; }
             m->A[i] = tokd(&p); m->b[i] = tokd(&p); m->EaR[i] = tokd(&p);
             m->A0[i] = tokd(&p); m->b0[i] = tokd(&p); m->EaR0[i] = tokd(&p);
             m->ta[i] = tokd(&p); m->rt3[i] = tokd(&p); m->rt1[i] = tokd(&p); m->t2[i] = tokd(&p);
@@ -253,6 +260,23 @@ or_mech* or_mech_load(const char* path) {
             stoich_add(&m->reac, i, m->nr[i], m->rk[i], m->ro[i], m->rs[i]);
             stoich_add(&m->prod, i, m->np[i], m->pk[i], m->ps[i], m->ps[i]);
             if (m->rev[i]) stoich_add(&m->revprod, i, m->np[i], m->pk[i], m->ps[i], m->ps[i]);
+        } else if (!strcmp(tok, "PLOG")) {
+            if (last < 0 || m->pdep[last] != 4) { set_err("PLOG record without a PLOG reaction"); goto fail; }
+            int n = toki(&p);
+            if (n < 1) { set_err("PLOG reaction without rate expressions"); goto fail; }
+            m->npd[last] = n;
+            m->pd[last] = calloc(4 * (size_t)n, 8);
+            for (int j = 0; j < 4 * n; j++) m->pd[last][j] = tokd(&p);
+        } else if (!strcmp(tok, "CHEB")) {
+            if (last < 0 || m->pdep[last] != 5) { set_err("CHEB record without a Chebyshev reaction"); goto fail; }
+            double hdr[6];
+            for (int j = 0; j < 6; j++) hdr[j] = tokd(&p);
+            int nT = (int)hdr[4], nP = (int)hdr[5];
+            if (nT < 1 || nP < 1) { set_err("empty Chebyshev coefficient table"); goto fail; }
+            m->npd[last] = 6 + nT * nP;
+            m->pd[last] = calloc((size_t)m->npd[last], 8);
+            memcpy(m->pd[last], hdr, sizeof hdr);
+            for (int j = 0; j < nT * nP; j++) m->pd[last][6 + j] = tokd(&p);
         } else if (!strcmp(tok, "end")) {
             ok = 1;
         }
@@ -275,6 +299,9 @@ void or_mech_free(or_mech* m) {
     free(m->b); free(m->EaR); free(m->A0); free(m->b0); free(m->EaR0); free(m->ta); free(m->rt3);
     free(m->rt1); free(m->t2); free(m->nr); free(m->np); free(m->ntb); free(m->rk); free(m->pk);
     free(m->tbk); free(m->rs); free(m->ro); free(m->ps); free(m->tbe); free(m->tbdef);
+    if (m->pd)
+        for (int i = 0; i < m->I; i++) free(m->pd[i]);
+    free(m->pdep); free(m->pd); free(m->npd);
     if (m->reac.cap) { stoich_free(&m->reac); stoich_free(&m->prod); stoich_free(&m->revprod); }
     free(m);
 }
@@ -352,6 +379,70 @@ static void set_state(const or_mech* m, work_t* w, double T, double P, const dou
     w->P = P;
 }
 
+/* PlogRate (PlogRate.cpp:92-123 setRates; PlogRate.h:110-164 updateFromStruct,
+ * evalFromStruct).  The rate table is a map from ln P to a group of entries
+ * [first, end) with equal pressure, plus copies of the first and last groups
+ * at ln P = -1000 and +1000; the interval is found with upper_bound and each
+ * end evaluates ln(1e-300 + sum of the group's A T^b exp(-Ea/RT)). */
+static double plog_group(const double* d, int first, int end, double logT, double recipT) {
+    double k = 1e-300;
+    for (int j = first; j < end; j++) k += d[4 * j + 1] * exp(d[4 * j + 2] * logT - d[4 * j + 3] * recipT);
+    return log(k);
+}
+static double plog_kf(const double* d, int n, double logP, double logT, double recipT) {
+    /* keys[g], first[g], end[g] of the map, sentinels included */
+    double keys[n + 2];
+    int first[n + 2], end[n + 2], ng = 1;
+    for (int j = 0; j < n; j++) {
+        double lp = log(d[4 * j]);
+        if (ng == 1 || keys[ng - 1] != lp) { keys[ng] = lp; first[ng] = j; end[ng] = j + 1; ng++; }
+        else end[ng - 1] = j + 1;
+    }
+    keys[0] = -1000.0; first[0] = first[1]; end[0] = end[1];
+    keys[ng] = 1000.0; first[ng] = first[ng - 1]; end[ng] = end[ng - 1]; ng++;
+    int hi = 0;
+    while (hi < ng && !(keys[hi] > logP)) hi++; /* upper_bound */
+    if (hi == 0 || hi == ng) return NAN;        /* |ln P| >= 1000: out of range */
+    int lo = hi - 1;
+    double lk1 = plog_group(d, first[lo], end[lo], logT, recipT);
+    double lk2 = plog_group(d, first[hi], end[hi], logT, recipT);
+    double rdp = 1.0 / (keys[hi] - keys[lo]);
+    return exp(lk1 + (lk2 - lk1) * (logP - keys[lo]) * rdp);
+}
+
+/* ChebyshevRate (ChebyshevRate.cpp:96-112 setLimits; ChebyshevRate.h:133-171
+ * updateFromStruct, evalFromStruct): reduce every temperature row over the
+ * pressure polynomials, then sum the temperature series; k = 10^logk. */
+static double cheb_kf(const double* d, double log10P, double recipT) {
+    const int nT = (int)d[4], nP = (int)d[5];
+    const double* c = d + 6;
+    double lPmin = log10(d[2]), lPmax = log10(d[3]);
+    double TminInv = 1.0 / d[0], TmaxInv = 1.0 / d[1];
+    double TrNum = -TminInv - TmaxInv, TrDen = 1.0 / (TmaxInv - TminInv);
+    double PrNum = -lPmin - lPmax, PrDen = 1.0 / (lPmax - lPmin);
+    double dot[nT];
+    double Pr = (2 * log10P + PrNum) * PrDen;
+    double Cnm1 = Pr, Cn = 1, Cnp1;
+    for (int i = 0; i < nT; i++) dot[i] = c[i * nP];
+    for (int j = 1; j < nP; j++) {
+        Cnp1 = 2 * Pr * Cn - Cnm1;
+        for (int i = 0; i < nT; i++) dot[i] += Cnp1 * c[i * nP + j];
+        Cnm1 = Cn;
+        Cn = Cnp1;
+    }
+    double Tr = (2 * recipT + TrNum) * TrDen;
+    Cnm1 = Tr;
+    Cn = 1;
+    double logk = dot[0];
+    for (int i = 1; i < nT; i++) {
+        Cnp1 = 2 * Tr * Cn - Cnm1;
+        logk += Cnp1 * dot[i];
+        Cnm1 = Cn;
+        Cn = Cnp1;
+    }
+    return pow(10, logk);
+}
+
 /* BulkKinetics::updateROP (BulkKinetics.cpp:457-545) + getNetProductionRates */
 static void update_rop(const or_mech* m, work_t* w) {
     const int K = m->K, I = m->I;
@@ -391,7 +482,11 @@ static void update_rop(const or_mech* m, work_t* w) {
     /* rate constants */
     double logT = log(T), recipT = 1.0 / T;
     for (int i = 0; i < I; i++) {
-        if (m->type[i] <= 1) {
+        if (m->pdep[i] == 4) {
+            w->kf[i] = plog_kf(m->pd[i], m->npd[i], log(w->pressure), logT, recipT);
+        } else if (m->pdep[i] == 5) {
+            w->kf[i] = cheb_kf(m->pd[i], log10(w->pressure), recipT);
+        } else if (m->type[i] <= 1) {
             w->kf[i] = m->A[i] * exp(m->b[i] * logT - m->EaR[i] * recipT);
         } else {
             double work = 0.0;

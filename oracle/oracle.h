@@ -20,6 +20,10 @@
  *               ArrheniusRate::evalFromStruct  include/cantera/kinetics/Arrhenius.h:196-198
  *               FalloffRate::evalFromStruct  include/cantera/kinetics/Falloff.h:154-176
  *               TroeRate::updateTemp / F  src/kinetics/Falloff.cpp:283-300
+ *               PlogRate::setRates / updateFromStruct / evalFromStruct
+ *                 src/kinetics/PlogRate.cpp:92-123, include/cantera/kinetics/PlogRate.h:110-164
+ *               ChebyshevRate::setLimits / updateFromStruct / evalFromStruct
+ *                 src/kinetics/ChebyshevRate.cpp:96-112, include/cantera/kinetics/ChebyshevRate.h:133-171
  *   reactor     IdealGasConstPressureReactor::getState/updateState/eval
  *                 src/zeroD/IdealGasConstPressureReactor.cpp:16-110
  *               ReactorNet::eval  src/zeroD/ReactorNet.cpp:617-635
@@ -36,7 +40,10 @@
  * Pinning (see tests/): NASA7 against the CHEMKIN-II values of
  * test/thermo/nasapoly.cpp:62-78; rates against test/kinetics/rates.cpp:128-150
  * (noxNeg.yaml); Kc against the closed form of rates.cpp:99-115; Troe against
- * its exact special cases; invariants (mass and element conservation, detailed
+ * its exact special cases; PLOG and Chebyshev against the known answers of
+ * test/kinetics/pdep.cpp:52-220 (pdep-test.yaml: pressure limits against the
+ * closed-form Arrhenius expressions, CHEMKIN rates of progress, Chebyshev
+ * rate constants); invariants (mass and element conservation, detailed
  * balance); the integrator against test_problems/cxx_samples/kin1_blessed.csv
  * and opmenmp_ignition_blessed.txt and against scipy Radau on the same RHS;
  * the analytic Jacobian against a finite-difference Jacobian of the RHS.

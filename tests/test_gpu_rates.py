@@ -39,7 +39,7 @@ def normalised_error(mech, oracle_mech, T, P, Y, wdot_gpu):
     return worst
 
 
-@pytest.mark.parametrize("mech", ["h2o2", "gri30", "nDodecane", "noxNeg"])
+@pytest.mark.parametrize("mech", ["h2o2", "gri30", "nDodecane", "noxNeg", "pdep-test"])
 def test_rates_match_oracle(mech, oracle_mod):
     import cantera_amd as ca
     from cantera_amd.workloads import MechInfo
@@ -128,3 +128,33 @@ def test_rates_fp32_option_error_against_oracle(mech, oracle_mod):
     mass = sol.molecular_weights @ w32
     scale = sol.molecular_weights @ np.abs(w32) + 1e-300
     assert np.max(np.abs(mass) / scale) < 1e-12
+
+
+def test_pdep_rates_at_known_answer_states(oracle_mod):
+    """PLOG and Chebyshev (pdep-test.yaml) at the states of
+    test/kinetics/pdep.cpp:52-220: below and above the PLOG table, on its
+    nodes, between nodes, and at the Chebyshev domain corners.  The oracle
+    holds the reference's known answers there (CPU tests); here the GPU must
+    match the oracle to the FP64 bar, in both rate precisions (PLOG and
+    Chebyshev constants are FP64 in both)."""
+    import cantera_amd as ca
+    from cantera_amd.workloads import MechInfo
+    info = MechInfo(mech_file("pdep-test"))
+    om = oracle_mod.Mech(mech_file("pdep-test"))
+    sol = ca.Solution("pdep-test")
+    X = np.zeros(info.n_species)
+    for name in ("H", "R1A", "R1B", "R2", "R3", "R4", "R5", "R6"):
+        X[info.species_index(name)] = 1.0
+    Y1 = info.mass_fractions_from_moles(X / X.sum())
+    atm = 101325.0
+    TP = [(500.0, 1e-7), (500.0, 1e10), (500.0, atm), (1100.0, 20 * atm), (1100.0, 0.5 * atm),
+          (800.0, 70 * atm), (400.0, 0.1 * atm), (1100.0, 30 * atm), (500.0, 1000.0), (500.0, 1e7),
+          (300.0, atm), (2000.0, atm)]
+    T = np.array([t for t, _ in TP])
+    P = np.array([p for _, p in TP])
+    Y = np.tile(Y1, (len(TP), 1))
+    for precision in (ca.CG_FP64, ca.CG_FP32):
+        w = sol.net_production_rates(T, P, Y.T, precision=precision)
+        err = normalised_error("pdep-test", om, T, P, Y, w)
+        print(f"pdep known-answer states {precision}: max normalised error {err:.3e}", flush=True)
+        assert err <= (TOL if precision == ca.CG_FP64 else TOL_FP32)

@@ -106,6 +106,28 @@ def test_ndodecane_batch_matches_oracle(oracle_mod):
     _compare("nDodecane", wl, 5, True, 1e-6, 1e-9, 1e-6, oracle_mod)
 
 
+def test_pdep_batch_matches_oracle(oracle_mod):
+    """PLOG and Chebyshev rate constants inside the integrator and its
+    Jacobian (pdep-test.yaml), pressures from 0.01 to 90 atm across the PLOG
+    nodes and the Chebyshev range.  The reactants are 1% each in P1, P4 and
+    P2A: undiluted, this artificial mechanism runs away within nanoseconds and
+    trajectories then amplify rounding differences beyond the bar."""
+    from cantera_amd.workloads import MechInfo, ReactorWorkload
+    info = MechInfo(mech_file("pdep-test"))
+    g = np.random.default_rng(11)
+    n = 40
+    X = np.zeros((n, info.n_species))
+    for name in ("H", "R1A", "R1B", "R2", "R3", "R4", "R5", "R6"):
+        X[:, info.species_index(name)] = 0.01 * g.uniform(0.5, 1.5, n)
+    for name in ("P1", "P4", "P2A"):
+        X[:, info.species_index(name)] += g.uniform(0.5, 1.5, n)
+    X /= X.sum(axis=1, keepdims=True)
+    P = 101325.0 * np.exp(g.uniform(np.log(0.01), np.log(90.0), n))
+    wl = ReactorWorkload("pdep-test IGCPR", g.uniform(800.0, 1200.0, n), P,
+                         info.mass_fractions_from_moles(X), np.full(n, 1e-6), 10, 1e-8, 1e-15)
+    _compare("pdep-test", wl, 10, True, 1e-6, 1e-9, 1e-6, oracle_mod)
+
+
 def test_gri30_mixed_precision_within_tolerance(oracle_mod):
     """FP32 iteration matrix (FP64 state, residual and error test): the Newton
     corrections converge to the same FP64 solution, so trajectories stay within

@@ -269,3 +269,115 @@ def test_bdf_against_radau_golden(oracle_mod):
     ref = d["states"]
     assert np.max(np.abs(y[:, :, 1] - ref[:, :, 1]) / ref[:, :, 1]) < 1e-6
     assert np.all(np.abs(y[:, :, 2:] - ref[:, :, 2:]) <= 1e-9 + 1e-5 * np.abs(ref[:, :, 2:]))
+
+
+# --- PLOG and Chebyshev (test/kinetics/pdep.cpp, pdep-test.yaml) ---------------
+
+CAL_R = R / 4184.0  # gas constant in cal/mol/K
+
+
+@pytest.fixture(scope="module")
+def pdep(oracle_mod):
+    return oracle_mod.Mech(mech_file("pdep-test"))
+
+
+def _pdep_Y(m):
+    """pdep.cpp:23-27: equal mole fractions of H, R1A, R1B, R2..R6."""
+    X = np.zeros(m.K)
+    for name in ("H", "R1A", "R1B", "R2", "R3", "R4", "R5", "R6"):
+        X[m.index(name)] = 1.0
+    X /= X.sum()
+    Y = X * m.mw
+    return Y / Y.sum()
+
+
+def _k(T, A, n, Ea):
+    """pdep.cpp:39-41: A T^n exp(-Ea/RT), Ea in cal/mol."""
+    return A * T ** n * np.exp(-Ea / (CAL_R * T))
+
+
+def test_mechc_pdep_fixture_matches_committed_tables(tmp_path):
+    """The committed pdep-test tables are what the compiler makes of the
+    reference's pdep-test.yaml (PLOG entries, explicit and local units)."""
+    from cantera_amd import mechc
+    out = tmp_path / "pdep.mech"
+    mechc.main([os.path.join(GOLDEN, "pdep-test.yaml"), str(out)])
+    assert out.read_text() == open(mech_file("pdep-test")).read()
+
+
+@pytest.mark.parametrize("P, expect", [
+    # pdep.cpp:52-70 low-pressure limit (1 kmol/m3 = 1e3 mol/cm3 scaling in A)
+    (1e-7, {0: [(1.212400e+13, -0.5779, 10872.7)], 1: [(1.230000e+05, 1.53, 4737.0)],
+            2: [(2.440000e+7, 1.04, 3980.0)], 3: [(2.889338e-17 * 6.02214076e26 / 1e6, 1.98, 4521.0)]}),
+    # pdep.cpp:72-100 high-pressure limit and duplicate pressures (summed)
+    (1e10, {0: [(5.963200e+53, -11.529, 52599.6)], 1: [(1.37e14, -0.79, 17603.0), (1.28e3, 1.71, 9774.0)],
+            2: [(-7.41e27, -5.54, 12108.0), (1.9e12, -0.29, 8306.0)],
+            3: [(2.889338e-17 * 6.02214076e26 / 1e6, 1.98, 4521.0)]}),
+    # pdep.cpp:102-117 pressure exactly on an interpolation node
+    (ONE_ATM, {0: [(4.910800e+28, -4.8507, 24772.8)], 1: [(1.26e17, -1.83, 15003.0), (1.23e1, 2.68, 6335.0)],
+               2: [(3.46e9, 0.442, 5463.0)]}),
+])
+def test_plog_limits_known_answer(pdep, P, expect):
+    """PLOG outside the table and on its nodes reduces to the closed-form
+    Arrhenius sum of the bounding group (1e-9 relative)."""
+    T = 500.0
+    kf = pdep.rates(T, P, _pdep_Y(pdep))["kf"]
+    for i, terms in expect.items():
+        ref = sum(_k(T, *t) for t in terms)
+        assert abs(kf[i] - ref) <= 1e-9 * abs(ref), (P, i)
+
+
+@pytest.mark.parametrize("T, P, ref, tol", [
+    # pdep.cpp:119-155, CHEMKIN rates of progress, kmol/m3/s
+    (1100.0, 20 * ONE_ATM, [3.100682e+05, 2.006871e+05, 4.468658e+06, 1.774796e+06], [1e2] * 4),
+    (1100.0, 0.5 * ONE_ATM, [5.244649e+02, 2.252537e+02, 2.985338e+03, 1.109248e+03], [5e-2, 2e-2, 3e-1, 1e-1]),
+    (800.0, 70 * ONE_ATM, [2.274501e+04, 2.307191e+05, 2.224601e+07, 1.007440e+07], [1e1, 1e2, 1e3, 1e3]),
+])
+def test_plog_intermediate_rop_known_answer(pdep, T, P, ref, tol):
+    ropf = pdep.rates(T, P, _pdep_Y(pdep))["ropf"]
+    for i in range(4):
+        assert abs(ropf[i] - ref[i]) <= tol[i], (T, P, i)
+
+
+def test_chebyshev_known_answer(pdep):
+    """pdep.cpp:157-220: RMG-py rate constants, CHEMKIN rates of progress,
+    the domain corners, and the 'molec' quantity units of reaction 7."""
+    Y = _pdep_Y(pdep)
+    NA_mol = 6.02214076e23
+    kf = pdep.rates(1100.0, 20 * ONE_ATM, Y)["kf"]
+    assert abs(kf[4] - 3.130698657e+06) <= 1e-1
+    assert abs(kf[5] - 1.187949573e+00) <= 1e-7
+    assert abs(kf[4] - kf[6] / NA_mol) <= 5e2
+    kf = pdep.rates(400.0, 0.1 * ONE_ATM, Y)["kf"]
+    assert abs(kf[4] - 1.713599902e+05) <= 1e-3
+    assert abs(kf[5] - 9.581780687e-24) <= 1e-31
+    assert abs(kf[4] - kf[6] / NA_mol) <= 1e2
+    ropf = pdep.rates(1100.0, 30 * ONE_ATM, Y)["ropf"]
+    assert abs(ropf[4] - 4.552930e+03) <= 1e-1
+    assert abs(ropf[5] - 4.877390e-02) <= 1e-5
+    for T, P, ref, tol in ((500.0, 1000.0, 1.225785655e+06, 1e-2), (500.0, 1.0e7, 1.580981157e+03, 1e-5),
+                           (300.0, ONE_ATM, 5.405987017e+03, 1e-5), (2000.0, ONE_ATM, 3.354054351e+07, 1e-1)):
+        assert abs(pdep.rates(T, P, Y)["kf"][4] - ref) <= tol, (T, P)
+
+
+def test_pdep_mass_conservation_and_jacobian(pdep):
+    """Mass-action reactions with pressure-dependent constants: sum W_k wdot_k = 0,
+    and the analytic Jacobian agrees with central differences of the RHS."""
+    g = np.random.default_rng(3)
+    Y = g.random(pdep.K) + 0.05
+    Y /= Y.sum()
+    for T, P in ((900.0, 0.3 * ONE_ATM), (1400.0, 20 * ONE_ATM)):
+        w = pdep.rates(T, P, Y)["wdot"]
+        assert abs(np.dot(w, pdep.mw)) <= 1e-12 * np.abs(w * pdep.mw).sum()
+        y = np.concatenate([[0.2, T], Y])
+        Ja = pdep.jacobian(P, y, True)
+        N = pdep.K + 2
+        Jf = np.zeros((N, N))
+        for j in range(1, N):
+            d = max(abs(y[j]), 1e-8) * 1e-6
+            yp, ym = y.copy(), y.copy()
+            yp[j] += d
+            ym[j] -= d
+            Jf[:, j] = (pdep.rhs(P, yp) - pdep.rhs(P, ym)) / (2 * d)
+        scale = np.abs(Jf).max(axis=1, keepdims=True) + 1e-30 * np.abs(Jf).max()
+        assert np.max(np.abs(Ja - Jf)[1:, 1:] / scale[1:]) < 1e-6, (T, P)
