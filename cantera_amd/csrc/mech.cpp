@@ -113,7 +113,7 @@ HostMech load_mech(const std::string& path) {
             for (int k : r.rk) if (k < 0 || k >= m.K) throw std::runtime_error("reactant index out of range");
             for (int k : r.pk) if (k < 0 || k >= m.K) throw std::runtime_error("product index out of range");
             for (int k : r.tbk) if (k < 0 || k >= m.K) throw std::runtime_error("efficiency index out of range");
-            if (r.type < 0 || r.type > 5) throw std::runtime_error("unknown reaction type");
+            if (r.type < 0 || r.type > 7) throw std::runtime_error("unknown reaction type");
             last_r = i;
         } else if (key == "PLOG") {
             if (last_r < 0 || m.rxn[last_r].type != kPlog) throw std::runtime_error("PLOG record without a PLOG reaction");
@@ -130,6 +130,13 @@ HostMech load_mech(const std::string& path) {
             c.push_back(nT);
             c.push_back(nP);
             for (int e = 0; e < nT * nP; e++) c.push_back(to_d(next()));
+        } else if (key == "SRI") {
+            if (last_r < 0 || m.rxn[last_r].type != kSri) throw std::runtime_error("SRI record without an SRI reaction");
+            for (int e = 0; e < 5; e++) m.rxn[last_r].sri[e] = to_d(next());
+        } else if (key == "TSANG") {
+            if (last_r < 0 || m.rxn[last_r].type != kTsang)
+                throw std::runtime_error("TSANG record without a Tsang reaction");
+            for (int e = 0; e < 2; e++) m.rxn[last_r].tsang[e] = to_d(next());
         } else if (key == "end") {
             ended = true;
         }
@@ -179,7 +186,10 @@ DeviceMech::DeviceMech(const HostMech& h) : h_(h) {
     // pass that runs the Troe code for a few lanes.  PLOG and Chebyshev
     // reactions (types 4, 5) lead: their rate constants are evaluated once per
     // state by lanes 0..63 before the reaction loop.
-    std::stable_sort(perm_.begin(), perm_.end(), [&](int a, int b) { return h.rxn[a].type > h.rxn[b].type; });
+    // Class order: Chebyshev, PLOG, Tsang, SRI, Troe, Lindemann, three-body, elementary.
+    static const int kRank[8] = {7, 6, 5, 4, 1, 0, 3, 2};   // by RxnType
+    std::stable_sort(perm_.begin(), perm_.end(),
+                     [&](int a, int b) { return kRank[h.rxn[a].type] < kRank[h.rxn[b].type]; });
 
     std::vector<double> rd(4 * static_cast<size_t>(I)), tbdef(I), fod, gend, pdd;
     std::vector<RxnI> ri(I);
@@ -196,10 +206,13 @@ DeviceMech::DeviceMech(const HostMech& h) : h_(h) {
         x.gen = -1;
         // PLOG and Chebyshev reactions are mass-action reactions (type 0 on
         // the device) whose rate constant comes from their pdd record
-        const int dtype = (r.type >= kPlog) ? kElementary : r.type;
+        // SRI and Tsang are falloff reactions of the Troe form (type 3; SRI
+        // marked by bit 8, Tsang expressed through the Troe parameters).
+        const int dtype = (r.type == kPlog || r.type == kChebyshev) ? kElementary
+                          : (r.type == kSri || r.type == kTsang) ? kTroe : r.type;
         x.flags = (r.rev ? 1 : 0) | (dtype << 1) | (dtype != kElementary ? 32 : 0) | (r.type == kPlog ? 64 : 0) |
-                  (r.type == kChebyshev ? 128 : 0);
-        if (r.type >= kPlog) n_pd++;
+                  (r.type == kChebyshev ? 128 : 0) | (r.type == kSri ? 256 : 0);
+        if (r.type == kPlog || r.type == kChebyshev) n_pd++;
         if (dtype == kElementary) n_el++;
         else if (dtype == kThreeBody) n_tb++;
         else n_fo++;
@@ -239,9 +252,16 @@ DeviceMech::DeviceMech(const HostMech& h) : h_(h) {
                 for (int j = 0; j < x.np; j++) g[8 + j] = r.ps[j];
             gend.insert(gend.end(), g, g + 12);
         }
-        if (r.type >= kLindemann) {
+        if (dtype >= kLindemann) {
             x.fo = static_cast<int>(fod.size() / 8);
             double fo[8] = {r.A0, r.b0, r.EaR0, r.troe[0], r.troe[1], r.troe[2], r.troe[3], 0.0};
+            if (r.type == kTsang) {
+                // Fcent = A + B T (TsangRate::updateTemp, Falloff.cpp:480-484) as
+                // (1 - A) e^(-T inf) + A e^(-T 0) + B T
+                fo[3] = r.tsang[0]; fo[4] = INFINITY; fo[5] = 0.0; fo[6] = 0.0; fo[7] = r.tsang[1];
+            } else if (r.type == kSri) {
+                for (int j = 0; j < 5; j++) fo[3 + j] = r.sri[j];
+            }
             fod.insert(fod.end(), fo, fo + 8);
         }
         if (r.type == kPlog) {

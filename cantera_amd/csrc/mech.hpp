@@ -13,13 +13,15 @@ namespace cg {
 constexpr int kMaxSide = 4;   // species entries per reaction side on the device
 constexpr int kMaxTB = 32;    // explicit efficiencies per reaction
 
-enum RxnType { kElementary = 0, kThreeBody = 1, kLindemann = 2, kTroe = 3, kPlog = 4, kChebyshev = 5 };
+enum RxnType { kElementary = 0, kThreeBody = 1, kLindemann = 2, kTroe = 3, kPlog = 4, kChebyshev = 5, kSri = 6, kTsang = 7 };
 
 struct HostReaction {
     int type = 0;
     bool rev = false;
     double dn = 0, A = 0, b = 0, EaR = 0, A0 = 0, b0 = 0, EaR0 = 0;
     double troe[4] = {0, 0, 0, 0};   // a, 1/T3, 1/T1, T2
+    double sri[5] = {0, 0, 0, 1, 0}; // kSri: a, b, c, d, e
+    double tsang[2] = {0, 0};        // kTsang: A, B
     std::vector<int> rk;             // reactant species (name order)
     std::vector<double> rs, ro;      // stoich, order
     std::vector<int> pk;
@@ -51,7 +53,8 @@ HostMech load_mech(const std::string& path);
 // sides are "general" (C_AnyN: species, order, stoich).
 struct RxnI {
     int flags;      // bit0 reversible, bits1-2 type (0..3), bit3 reactant side general, bit4 product side general,
-                    // bit5 third body, bit6 PLOG, bit7 Chebyshev (those two with type 0)
+                    // bit5 third body, bit6 PLOG, bit7 Chebyshev (those two with type 0),
+                    // bit8 SRI (with type 3)
     int nr, np;     // entries in rk/pk
     int tbo, tbn;   // third-body efficiency list offset/count
     int fo;         // falloff record index, or PLOG / Chebyshev record offset in pdd (-1 if none)
@@ -75,7 +78,8 @@ struct DevMech {
     const double* rd;        // I x 4: A, b, EaR, dn
     const double* tbdef;     // I
     const RxnI* ri;          // I
-    const double* fod;       // n_fo x 8: A0, b0, EaR0, a, 1/T3, 1/T1, T2, pad
+    const double* fod;       // n_fo x 8: A0, b0, EaR0, then Troe a, 1/T3, 1/T1, T2, B (Tsang: a = A, 1/T3 = inf,
+                             // 1/T1 = 0, T2 = 0, B) or SRI a, b, c, d, e
     const double* gend;      // n_gen x 12: reactant order[4], reactant stoich[4], product stoich[4]
     const int* tbk;          // efficiency species
     const double* tbe;       // efficiency - default

@@ -116,9 +116,18 @@ __device__ __forceinline__ void scatter_reaction_derivs(const DevMech& m, int p,
             double rc_high = rd.x * exp(rd.y * g.logT - rd.z * g.recipT);
             double Pr = v.M * rc_low / (rc_high + kSmallNumber);
             double F = 1.0, dlogF = 0.0;
-            if (type == kTroe) {
+            if (type == kTroe && (x.flags & 256)) {
+                // SRI: log10 F = log10(w) / (1 + lpr^2) + log10(d T^e)
+                double w = fo[3] * exp(-fo[4] / g.T);
+                if (fo[5] != 0.0) w += exp(-g.T / fo[5]);
+                double lpr = log10(fmax(Pr, kSmallNumber));
+                double u = 1.0 + lpr * lpr;
+                F = pow(w, 1.0 / u) * (fo[6] * pow(g.T, fo[7]));
+                dlogF = (Pr > kSmallNumber) ? -log10(w) * 2.0 * lpr / (u * u) : 0.0;
+            } else if (type == kTroe) {
                 double Fcent = (1.0 - fo[3]) * exp(-g.T * fo[4]) + fo[3] * exp(-g.T * fo[5]);
                 if (fo[6] != 0.0) Fcent += exp(-fo[6] / g.T);
+                Fcent += fo[7] * g.T;
                 double lFc = log10(fmax(Fcent, kSmallNumber));
                 double lpr = log10(fmax(Pr, kSmallNumber));
                 double cc = -0.4 - 0.67 * lFc, nn = 0.75 - 1.27 * lFc;

@@ -360,6 +360,16 @@ __device__ __forceinline__ double pdep_rate_lane(const DevMech& m, const GasStat
     return k;
 }
 
+// SRI falloff function at reduced pressure pr (SriRate::updateTemp and F,
+// Falloff.cpp:389-403); fo[3..7] = a, b, c, d, e.  Out of line: SRI reactions
+// are rare, and inlined this code enlarges every caller's register frame.
+__device__ __noinline__ double sri_falloff(const double* __restrict__ fo, double T, double pr) {
+    double w = fo[3] * exp(-fo[4] / T);
+    if (fo[5] != 0.0) w += exp(-T / fo[5]);
+    const double lpr = log10(fmax(pr, kSmallNumber));
+    return pow(w, 1.0 / (1.0 + lpr * lpr)) * (fo[6] * pow(T, fo[7]));
+}
+
 // Evaluate reaction p (permuted index) at the gas state; C and MU live in LDS.
 // kpd: this lane's pdep_rate_lane value (used for flags bit 6/7 only).
 // kF32: rate constants (Arrhenius, falloff centre and blending) through the
@@ -400,17 +410,22 @@ __device__ __forceinline__ void eval_reaction(const DevMech& m, int p, const Gas
         kf = (x.flags & 192) ? kpd : rd.x * rate_exp<kF32>(rd.y * g.logT - rd.z * g.recipT);
     } else {
         const double* fo = m.fod + 8 * x.fo;
+        const bool sri = x.flags & 256;
         double work = 0.0;
-        if (type == kTroe) {
+        if (type == kTroe && !sri) {
+            // Troe (fo[7] = 0) and Tsang (Fcent = A + B T)
             double Fcent = (1.0 - fo[3]) * rate_exp<kF32>(-g.T * fo[4]) + fo[3] * rate_exp<kF32>(-g.T * fo[5]);
             if (fo[6] != 0.0) Fcent += rate_exp<kF32>(-fo[6] / g.T);
+            Fcent += fo[7] * g.T;
             work = rate_log10<kF32>(fmax(Fcent, kSmallNumber));
         }
         double rc_low = fo[0] * rate_exp<kF32>(fo[1] * g.logT - fo[2] * g.recipT);
         double rc_high = rd.x * rate_exp<kF32>(rd.y * g.logT - rd.z * g.recipT);
         double pr = M * rc_low / (rc_high + kSmallNumber);
         double F = 1.0;
-        if (type == kTroe) {
+        if (type == kTroe && sri) {
+            F = sri_falloff(fo, g.T, pr);
+        } else if (type == kTroe) {
             double lpr = rate_log10<kF32>(fmax(pr, kSmallNumber));
             double cc = -0.4 - 0.67 * work;
             double nn = 0.75 - 1.27 * work;

@@ -38,10 +38,14 @@ What is restated from the reference, and where:
   temperature and pressure ranges converted to K and Pa, and log10 of the
   rate-unit conversion of 1.0 added to coefficient (0, 0); reaction-local
   ``units`` are honoured.
+* SRI -- ``SriRate::setFalloffCoeffs`` ``Falloff.cpp:347-375``: 3 or 5
+  coefficients (d = 1, e = 0 when absent), C and D must not be negative.
+* Tsang -- ``TsangRate::setFalloffCoeffs`` ``Falloff.cpp:454-469``: A, and
+  B = 0 when absent.
 
 Supported rate types: elementary Arrhenius (incl. ``negative-A``), three-body,
-Lindemann and Troe falloff, PLOG, Chebyshev, reaction ``orders``.  Anything
-else raises ``MechanismError`` naming the reaction.
+Lindemann, Troe, SRI and Tsang falloff, PLOG, Chebyshev, reaction ``orders``.
+Anything else raises ``MechanismError`` naming the reaction.
 """
 from __future__ import annotations
 
@@ -255,7 +259,7 @@ class Species:
 @dataclass
 class Reaction:
     equation: str
-    rtype: int                 # 0 elementary, 1 three-body, 2 Lindemann, 3 Troe
+    rtype: int                 # 0 elementary, 1 three-body, 2 Lindemann, 3 Troe, 4 PLOG, 5 Chebyshev, 6 SRI, 7 Tsang
     reversible: bool
     A: float
     b: float
@@ -271,9 +275,13 @@ class Reaction:
     dn: float = 0.0
     plog: list = field(default_factory=list)    # PLOG: [(P [Pa], A, b, Ea/R)] sorted by P
     cheb: tuple = ()                            # Chebyshev: (Tmin, Tmax, Pmin, Pmax, rows)
+    sri: tuple = ()                             # SRI: (a, b, c, d, e)
+    tsang: tuple = ()                           # Tsang: (A, B)
 
 
 TYPE_ELEMENTARY, TYPE_THREE_BODY, TYPE_LINDEMANN, TYPE_TROE, TYPE_PLOG, TYPE_CHEBYSHEV = 0, 1, 2, 3, 4, 5
+TYPE_SRI, TYPE_TSANG = 6, 7
+FALLOFF_TYPES = (TYPE_LINDEMANN, TYPE_TROE, TYPE_SRI, TYPE_TSANG)
 
 
 def _parse_equation(eq: str):
@@ -580,8 +588,29 @@ def _compile_reaction(node, index, units: UnitSystem) -> Reaction:
             T2 = _num(t["T2"]) if "T2" in t else 0.0
             r.troe = (a, rt3, rt1, T2)
             r.rtype = TYPE_TROE
-        elif "SRI" in node or "Tsang" in node:
-            raise MechanismError(f"reaction {eq!r}: SRI/Tsang falloff not supported")
+        elif "SRI" in node:
+            # SriRate::setParameters / setFalloffCoeffs (Falloff.cpp:347-375, 405-428)
+            f = node["SRI"]
+            c = [_num(f["A"]), _num(f["B"]), _num(f["C"])]
+            if "D" in f:
+                c.append(_num(f["D"]))
+            if "E" in f:
+                c.append(_num(f["E"]))
+            if len(c) not in (3, 5):
+                raise MechanismError(f"reaction {eq!r}: SRI needs 3 or 5 coefficients, got {len(c)}")
+            if c[2] < 0.0:
+                raise MechanismError(f"reaction {eq!r}: SRI parameter C is less than zero")
+            if len(c) == 5 and c[3] < 0.0:
+                raise MechanismError(f"reaction {eq!r}: SRI parameter D is less than zero")
+            if len(c) == 3:
+                c += [1.0, 0.0]
+            r.sri = tuple(c)
+            r.rtype = TYPE_SRI
+        elif "Tsang" in node:
+            # TsangRate::setParameters / setFalloffCoeffs (Falloff.cpp:454-469, 496-512)
+            f = node["Tsang"]
+            r.tsang = (_num(f["A"]), _num(f["B"]) if "B" in f else 0.0)
+            r.rtype = TYPE_TSANG
         else:
             r.rtype = TYPE_LINDEMANN
     elif rate_type == "pressure-dependent-Arrhenius":
@@ -683,6 +712,10 @@ def write_mech(mech, out_path):
         lines.append(" ".join(toks))
         if r.rtype == TYPE_PLOG:
             lines.append(" ".join(["PLOG", str(len(r.plog))] + [_f(x) for e in r.plog for x in e]))
+        elif r.rtype == TYPE_SRI:
+            lines.append(" ".join(["SRI"] + [_f(x) for x in r.sri]))
+        elif r.rtype == TYPE_TSANG:
+            lines.append(" ".join(["TSANG"] + [_f(x) for x in r.tsang]))
         elif r.rtype == TYPE_CHEBYSHEV:
             Tmin, Tmax, Pmin, Pmax, rows = r.cheb
             lines.append(" ".join(["CHEB", _f(Tmin), _f(Tmax), _f(Pmin), _f(Pmax), str(len(rows)), str(len(rows[0]))]
@@ -704,11 +737,12 @@ def main(argv=None):
     a = ap.parse_args(argv)
     mech = compile_mechanism(a.yaml, a.phase, a.search)
     write_mech(mech, a.out)
-    n_t = sum(r.rtype == TYPE_TROE for r in mech["reactions"])
-    n_l = sum(r.rtype == TYPE_LINDEMANN for r in mech["reactions"])
-    n_3 = sum(r.rtype == TYPE_THREE_BODY for r in mech["reactions"])
-    print(f"{a.out}: {len(mech['species'])} species, {len(mech['reactions'])} reactions "
-          f"({n_3} three-body, {n_t} Troe, {n_l} Lindemann)", file=sys.stderr)
+    names = {TYPE_THREE_BODY: "three-body", TYPE_LINDEMANN: "Lindemann", TYPE_TROE: "Troe", TYPE_SRI: "SRI",
+             TYPE_TSANG: "Tsang", TYPE_PLOG: "PLOG", TYPE_CHEBYSHEV: "Chebyshev"}
+    counts = [f"{sum(r.rtype == t for r in mech['reactions'])} {n}" for t, n in names.items()
+              if t in (TYPE_THREE_BODY, TYPE_LINDEMANN, TYPE_TROE) or any(r.rtype == t for r in mech["reactions"])]
+    print(f"{a.out}: {len(mech['species'])} species, {len(mech['reactions'])} reactions ({', '.join(counts)})",
+          file=sys.stderr)
 
 
 if __name__ == "__main__":

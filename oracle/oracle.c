@@ -167,6 +167,7 @@ struct or_mech {
     int* pdep;   /* 4 PLOG, 5 Chebyshev (type[] is 0 for those), else 0 */
     double** pd; /* PLOG: (P, A, b, Ea/R) per entry; Chebyshev: Tmin, Tmax, Pmin, Pmax, nT, nP, coefficients */
     int* npd;    /* PLOG entries / Chebyshev record length */
+    double (*fx)[5]; /* SRI a, b, c, d, e; Tsang A, B */
     stoich_t reac, prod, revprod;
 };
 
@@ -234,15 +235,16 @@ or_mech* or_mech_load(const char* path) {
             m->rs = calloc(I, sizeof *m->rs); m->ro = calloc(I, sizeof *m->ro); m->ps = calloc(I, sizeof *m->ps);
             m->tbe = calloc(I, sizeof *m->tbe); m->tbdef = calloc(I, 8);
             m->pdep = calloc(I, 4); m->pd = calloc(I, sizeof *m->pd); m->npd = calloc(I, 4);
+            m->fx = calloc(I, sizeof *m->fx);
             stoich_init(&m->reac, I + 1);
             stoich_init(&m->prod, I + 1);
             stoich_init(&m->revprod, I + 1);
         } else if (!strcmp(tok, "R")) {
             int i = toki(&p);
             m->type[i] = toki(&p); m->rev[i] = toki(&p); m->dn[i] = tokd(&p);
-            if (m->type[i] < 0 || m->type[i] > 5) { set_err("unknown reaction type"); goto fail; }
+            if (m->type[i] < 0 || m->type[i] > 7) { set_err("unknown reaction type"); goto fail; }
             last = i;
-            if (m->type[i] >= 4) { m->pdep[i] = m->type[i]; m->type[i] = 0; }
+            if (m->type[i] == 4 || m->type[i] == 5) { m->pdep[i] = m->type[i]; m->type[i] = 0; }
             m->A[i] = tokd(&p); m->b[i] = tokd(&p); m->EaR[i] = tokd(&p);
             m->A0[i] = tokd(&p); m->b0[i] = tokd(&p); m->EaR0[i] = tokd(&p);
             m->ta[i] = tokd(&p); m->rt3[i] = tokd(&p); m->rt1[i] = tokd(&p); m->t2[i] = tokd(&p);
@@ -267,6 +269,12 @@ or_mech* or_mech_load(const char* path) {
             m->npd[last] = n;
             m->pd[last] = calloc(4 * (size_t)n, 8);
             for (int j = 0; j < 4 * n; j++) m->pd[last][j] = tokd(&p);
+        } else if (!strcmp(tok, "SRI")) {
+            if (last < 0 || m->type[last] != 6) { set_err("SRI record without an SRI reaction"); goto fail; }
+            for (int j = 0; j < 5; j++) m->fx[last][j] = tokd(&p);
+        } else if (!strcmp(tok, "TSANG")) {
+            if (last < 0 || m->type[last] != 7) { set_err("TSANG record without a Tsang reaction"); goto fail; }
+            for (int j = 0; j < 2; j++) m->fx[last][j] = tokd(&p);
         } else if (!strcmp(tok, "CHEB")) {
             if (last < 0 || m->pdep[last] != 5) { set_err("CHEB record without a Chebyshev reaction"); goto fail; }
             double hdr[6];
@@ -301,7 +309,7 @@ void or_mech_free(or_mech* m) {
     free(m->tbk); free(m->rs); free(m->ro); free(m->ps); free(m->tbe); free(m->tbdef);
     if (m->pd)
         for (int i = 0; i < m->I; i++) free(m->pd[i]);
-    free(m->pdep); free(m->pd); free(m->npd);
+    free(m->pdep); free(m->pd); free(m->npd); free(m->fx);
     if (m->reac.cap) { stoich_free(&m->reac); stoich_free(&m->prod); stoich_free(&m->revprod); }
     free(m);
 }
@@ -489,23 +497,37 @@ static void update_rop(const or_mech* m, work_t* w) {
         } else if (m->type[i] <= 1) {
             w->kf[i] = m->A[i] * exp(m->b[i] * logT - m->EaR[i] * recipT);
         } else {
-            double work = 0.0;
+            /* FalloffRate::evalFromStruct (Falloff.h:154-176) with
+             * updateTemp / F of Troe (Falloff.cpp:283-320), SRI (389-403),
+             * Tsang (480-494) */
+            double work = 0.0, work1 = 1.0;
             if (m->type[i] == 3) {
                 double Fcent = (1.0 - m->ta[i]) * exp(-T * m->rt3[i]) + m->ta[i] * exp(-T * m->rt1[i]);
                 if (m->t2[i] != 0.0) Fcent += exp(-m->t2[i] / T);
                 work = log10(Fcent > SMALL_NUMBER ? Fcent : SMALL_NUMBER);
+            } else if (m->type[i] == 7) {
+                double Fcent = m->fx[i][0] + (m->fx[i][1] * T);
+                work = log10(Fcent > SMALL_NUMBER ? Fcent : SMALL_NUMBER);
+            } else if (m->type[i] == 6) {
+                work = m->fx[i][0] * exp(-m->fx[i][1] / T);
+                if (m->fx[i][2] != 0.0) work += exp(-T / m->fx[i][2]);
+                work1 = m->fx[i][3] * pow(T, m->fx[i][4]);
             }
             double rc_low = m->A0[i] * exp(m->b0[i] * logT - m->EaR0[i] * recipT);
             double rc_high = m->A[i] * exp(m->b[i] * logT - m->EaR[i] * recipT);
             double pr = w->concm[i] * rc_low / (rc_high + SMALL_NUMBER);
             double F = 1.0;
-            if (m->type[i] == 3) {
+            if (m->type[i] == 3 || m->type[i] == 7) {
                 double lpr = log10(pr > SMALL_NUMBER ? pr : SMALL_NUMBER);
                 double cc = -0.4 - 0.67 * work;
                 double nn = 0.75 - 1.27 * work;
                 double f1 = (lpr + cc) / (nn - 0.14 * (lpr + cc));
                 double lgf = work / (1.0 + f1 * f1);
                 F = pow(10.0, lgf);
+            } else if (m->type[i] == 6) {
+                double lpr = log10(pr > SMALL_NUMBER ? pr : SMALL_NUMBER);
+                double xx = 1.0 / (1.0 + lpr * lpr);
+                F = pow(work, xx) * work1;
             }
             pr *= F / (1.0 + pr);
             w->kf[i] = pr * rc_high;
@@ -674,9 +696,22 @@ static void jac_analytic(const or_mech* m, work_t* w, double P, const double* y,
                 double rc_high = m->A[i] * exp(m->b[i] * logT - m->EaR[i] * recipT);
                 double Pr = Mi * rc_low / (rc_high + SMALL_NUMBER);
                 double dlogF = 0.0, F = 1.0;
-                if (m->type[i] == 3) {
-                    double Fcent = (1.0 - m->ta[i]) * exp(-T * m->rt3[i]) + m->ta[i] * exp(-T * m->rt1[i]);
-                    if (m->t2[i] != 0.0) Fcent += exp(-m->t2[i] / T);
+                if (m->type[i] == 6) {
+                    /* d log10 F / d log10 Pr of SRI's F = w^(1/(1+lpr^2)) d T^e */
+                    double wv = m->fx[i][0] * exp(-m->fx[i][1] / T);
+                    if (m->fx[i][2] != 0.0) wv += exp(-T / m->fx[i][2]);
+                    double lpr = log10(Pr > SMALL_NUMBER ? Pr : SMALL_NUMBER);
+                    double u = 1.0 + lpr * lpr;
+                    F = pow(wv, 1.0 / u) * m->fx[i][3] * pow(T, m->fx[i][4]);
+                    dlogF = (Pr > SMALL_NUMBER) ? -log10(wv) * 2.0 * lpr / (u * u) : 0.0;
+                } else if (m->type[i] == 3 || m->type[i] == 7) {
+                    double Fcent;
+                    if (m->type[i] == 7) {
+                        Fcent = m->fx[i][0] + m->fx[i][1] * T;
+                    } else {
+                        Fcent = (1.0 - m->ta[i]) * exp(-T * m->rt3[i]) + m->ta[i] * exp(-T * m->rt1[i]);
+                        if (m->t2[i] != 0.0) Fcent += exp(-m->t2[i] / T);
+                    }
                     double lFc = log10(Fcent > SMALL_NUMBER ? Fcent : SMALL_NUMBER);
                     double lpr = log10(Pr > SMALL_NUMBER ? Pr : SMALL_NUMBER);
                     double cc = -0.4 - 0.67 * lFc, nn = 0.75 - 1.27 * lFc;

@@ -53,6 +53,38 @@ def parse_stoich(path):
     return nu_r, nu_p
 
 
+def parse_third_bodies(path):
+    """[(default efficiency, [(k, efficiency - default)])] of the reactions with
+    a third body, from a .mech file."""
+    out = []
+    with open(path) as f:
+        for line in f:
+            t = line.split()
+            if not t or t[0] != "R":
+                continue
+            p = 15
+            nr = int(t[p]); p += 1 + 3 * nr
+            np_ = int(t[p]); p += 1 + 2 * np_
+            tbdef, ntb = float(t[p]), int(t[p + 1])
+            eff = [(int(t[p + 2 + 2 * j]), float(t[p + 3 + 2 * j])) for j in range(ntb)]
+            if tbdef != 0.0 or ntb:
+                out.append((tbdef, eff))
+    return out
+
+
+def third_body_condition(path, X):
+    """Cancellation factor of [M] = d ctot + sum (e_k - d) C_k for mole
+    fractions X: max over reactions of (d + sum |e_k - d| X_k) / |d + sum (e_k - d) X_k|.
+    Rounding in [M] is amplified by this factor in every rate that uses it
+    (the reference sums [M] the same way, ThirdBodyCalc.h:75-91)."""
+    c = 1.0
+    for d, eff in parse_third_bodies(path):
+        num = d + sum(abs(e) * X[k] for k, e in eff)
+        den = abs(d + sum(e * X[k] for k, e in eff))
+        c = max(c, num / max(den, 1e-300))
+    return c
+
+
 @pytest.fixture(scope="session")
 def oracle_mod():
     from oracle import oracle

@@ -3,13 +3,16 @@
 Bar: FP64 agreement with the oracle to 1e-12, measured per species as
 |wdot_gpu - wdot_oracle| / sum_i |nu_ki| (|ropf_i| + |ropr_i|); the denominator
 is the magnitude of the terms that cancel in wdot (plain relative error is
-ill-posed near equilibrium).  Reference values of the known-answer test come
+ill-posed near equilibrium).  A state whose efficiency-weighted [M] nearly
+cancels (a bath of a species with efficiency 0) has the bar widened by that
+cancellation factor: the reference rounds [M] the same way, so only the
+summation order differs.  Reference values of the known-answer test come
 from test/kinetics/rates.cpp:128-150 (noxNeg.yaml, 2e-5 relative + 1e-9).
 """
 import numpy as np
 import pytest
 
-from conftest import mech_file, parse_stoich
+from conftest import mech_file, parse_stoich, third_body_condition
 
 pytestmark = pytest.mark.gpu
 
@@ -28,18 +31,24 @@ def random_states(info, n, seed, T_lo=300.0, T_hi=2500.0):
 
 
 def normalised_error(mech, oracle_mech, T, P, Y, wdot_gpu):
+    """Worst per-species error over the states, each state's error divided by
+    its third-body cancellation factor (1 unless an efficiency-weighted [M]
+    nearly cancels, e.g. a bath of a species with efficiency 0)."""
     nu_r, nu_p = parse_stoich(mech_file(mech))
     absnu = nu_r + nu_p
+    mw = oracle_mech.mw
     worst = 0.0
     for s in range(len(T)):
         r = oracle_mech.rates(T[s], P[s], Y[s])
         scale = absnu @ (np.abs(r["ropf"]) + np.abs(r["ropr"])) + 1e-300
         err = np.abs(wdot_gpu[:, s] - r["wdot"]) / scale
-        worst = max(worst, float(err.max()))
+        X = Y[s] / mw
+        cond = third_body_condition(mech_file(mech), X / X.sum())
+        worst = max(worst, float(err.max()) / cond)
     return worst
 
 
-@pytest.mark.parametrize("mech", ["h2o2", "gri30", "nDodecane", "noxNeg", "pdep-test"])
+@pytest.mark.parametrize("mech", ["h2o2", "gri30", "nDodecane", "noxNeg", "pdep-test", "sri-falloff", "tsang-falloff"])
 def test_rates_match_oracle(mech, oracle_mod):
     import cantera_amd as ca
     from cantera_amd.workloads import MechInfo
@@ -104,7 +113,7 @@ def test_rates_mass_conservation():
 TOL_FP32 = 1e-5
 
 
-@pytest.mark.parametrize("mech", ["h2o2", "gri30", "nDodecane"])
+@pytest.mark.parametrize("mech", ["h2o2", "gri30", "nDodecane", "sri-falloff", "tsang-falloff"])
 def test_rates_fp32_option_error_against_oracle(mech, oracle_mod):
     """CG_FP32: the rate constants' exp/log10/exp10 in FP32 on FP64 arguments.
 

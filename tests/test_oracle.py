@@ -381,3 +381,86 @@ def test_pdep_mass_conservation_and_jacobian(pdep):
             Jf[:, j] = (pdep.rhs(P, yp) - pdep.rhs(P, ym)) / (2 * d)
         scale = np.abs(Jf).max(axis=1, keepdims=True) + 1e-30 * np.abs(Jf).max()
         assert np.max(np.abs(Ja - Jf)[1:, 1:] / scale[1:]) < 1e-6, (T, P)
+
+
+# --- SRI and Tsang falloff (sri-falloff.yaml, tsang-falloff.yaml) -------------
+
+def _falloff_records(path):
+    """(type, kinf params, k0 params, tbdef, efficiency species, extra) per reaction."""
+    recs, lines = {}, open(path).read().splitlines()
+    for i, line in enumerate(lines):
+        t = line.split()
+        if t and t[0] == "R":
+            v = [float(x) for x in t[4:11]]
+            rest = t[15:]
+            nr = int(rest[0]); p = 1 + 3 * nr
+            np_ = int(rest[p]); p += 1 + 2 * np_
+            tbdef, ntb = float(rest[p]), int(rest[p + 1])
+            tbk = [int(rest[p + 2 + 2 * j]) for j in range(ntb)]
+            extra = [float(x) for x in lines[i + 1].split()[1:]]
+            recs[int(t[1])] = (int(t[2]), v[1:4], v[4:7], tbdef, tbk, extra)
+    return recs
+
+
+@pytest.mark.parametrize("mech, bath", [("sri-falloff", "R1A"), ("tsang-falloff", "NO")])
+def test_sri_tsang_special_cases(oracle_mod, mech, bath):
+    """Exact cases of the falloff functions (Falloff.cpp:389-403, 480-494), with
+    a pure bath gas of default efficiency so that [M] = P/RT:
+    SRI at Pr = 1 (exponent 1): F = d (a e^(-b/T) + e^(-T/c)) T^e; at Pr = 100
+    (exponent 1/(1 + 2^2)): F = d (a e^(-b/T) + e^(-T/c))^(1/5) T^e.
+    Tsang at log10 Pr = 0.4 + 0.67 log10(Fcent): F = Fcent = A + B T."""
+    m = oracle_mod.Mech(mech_file(mech))
+    b = m.index(bath)
+    Y = np.zeros(m.K)
+    Y[b] = 1.0
+    checked = 0
+    for idx, (typ, hi, lo, tbdef, tbk, ex) in _falloff_records(mech_file(mech)).items():
+        assert typ in (6, 7)
+        if tbdef != 1.0 or b in tbk:
+            continue
+        for T in (700.0, 1300.0, 2100.0):
+            kinf = hi[0] * np.exp(hi[1] * np.log(T) - hi[2] / T)
+            k0 = lo[0] * np.exp(lo[1] * np.log(T) - lo[2] / T)
+            if typ == 6:
+                a, bb, c, d, e = ex
+                base = a * np.exp(-bb / T) + np.exp(-T / c)
+                cases = [(1.0, d * base * T ** e), (100.0, d * base ** 0.2 * T ** e)]
+            else:
+                Fcent = ex[0] + ex[1] * T
+                cases = [(10 ** (0.4 + 0.67 * np.log10(Fcent)), Fcent)]
+            for Pr, F in cases:
+                P = Pr * kinf / k0 * R * T
+                kf = m.rates(T, P, Y)["kf"][idx]
+                Pr_act = (P / (R * T)) * k0 / kinf
+                expect = kinf * Pr_act / (1 + Pr_act) * F
+                assert abs(kf - expect) <= 1e-10 * abs(expect), (mech, idx, T, Pr)
+        checked += 1
+    assert checked >= 1
+
+
+def test_mechc_sri_fixture_matches_committed_tables(tmp_path):
+    from cantera_amd import mechc
+    out = tmp_path / "sri.mech"
+    mechc.main([os.path.join(GOLDEN, "sri-falloff.yaml"), str(out)])
+    assert out.read_text() == open(mech_file("sri-falloff")).read()
+
+
+@pytest.mark.parametrize("mech", ["sri-falloff", "tsang-falloff"])
+def test_sri_tsang_jacobian_matches_finite_differences(oracle_mod, mech):
+    m = oracle_mod.Mech(mech_file(mech))
+    g = np.random.default_rng(4)
+    Y = g.random(m.K) + 0.05
+    Y /= Y.sum()
+    for T, P in ((900.0, 0.3 * ONE_ATM), (1600.0, 20 * ONE_ATM)):
+        y = np.concatenate([[0.2, T], Y])
+        Ja = m.jacobian(P, y, True)
+        N = m.K + 2
+        Jf = np.zeros((N, N))
+        for j in range(1, N):
+            d = max(abs(y[j]), 1e-8) * 1e-6
+            yp, ym = y.copy(), y.copy()
+            yp[j] += d
+            ym[j] -= d
+            Jf[:, j] = (m.rhs(P, yp) - m.rhs(P, ym)) / (2 * d)
+        scale = np.abs(Jf).max(axis=1, keepdims=True) + 1e-30 * np.abs(Jf).max()
+        assert np.max(np.abs(Ja - Jf)[1:, 1:] / scale[1:]) < 1e-6, (mech, T, P)
