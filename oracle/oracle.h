@@ -20,6 +20,8 @@
  *               ArrheniusRate::evalFromStruct  include/cantera/kinetics/Arrhenius.h:196-198
  *               FalloffRate::evalFromStruct  include/cantera/kinetics/Falloff.h:154-176
  *               TroeRate::updateTemp / F  src/kinetics/Falloff.cpp:283-300
+ *               SriRate::updateTemp / F  src/kinetics/Falloff.cpp:389-403
+ *               TsangRate::updateTemp / F  src/kinetics/Falloff.cpp:480-494
  *               PlogRate::setRates / updateFromStruct / evalFromStruct
  *                 src/kinetics/PlogRate.cpp:92-123, include/cantera/kinetics/PlogRate.h:110-164
  *               ChebyshevRate::setLimits / updateFromStruct / evalFromStruct
@@ -43,7 +45,9 @@
  * its exact special cases; PLOG and Chebyshev against the known answers of
  * test/kinetics/pdep.cpp:52-220 (pdep-test.yaml: pressure limits against the
  * closed-form Arrhenius expressions, CHEMKIN rates of progress, Chebyshev
- * rate constants); invariants (mass and element conservation, detailed
+ * rate constants); SRI and Tsang against their exact cases (SRI exponent 1
+ * at Pr = 1 and 1/5 at Pr = 100, Tsang F = Fcent at log10 Pr = -c) and their
+ * [M] derivatives against finite differences; invariants (mass and element conservation, detailed
  * balance); the integrator against test_problems/cxx_samples/kin1_blessed.csv
  * and opmenmp_ignition_blessed.txt and against scipy Radau on the same RHS;
  * the analytic Jacobian against a finite-difference Jacobian of the RHS.
