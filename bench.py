@@ -271,7 +271,7 @@ def reactors(a):
         },
         "integrator": {k: float(np.mean(v)) for k, v in stats.items() if k != "order"},
     }
-    if a.workload == "gri30-reactors" and a.precision == "fp64" and not a.no_mixed:
+    if a.workload in ("gri30-reactors", "ndodecane-reactors") and a.precision == "fp64" and not a.no_mixed:
         # the mixed form (FP32 iteration matrix / LU / solves, FP64 residual)
         # on a fresh batch, same inputs and steps, after the timed FP64 region;
         # its error is stated against the FP64 result
@@ -533,7 +533,7 @@ def main(argv=None):
     ap.add_argument("--rtol-g", type=float, default=1e-3, help="gri30-surrogate: guard, relative")
     ap.add_argument("--atol-g", type=float, default=1e-9, help="gri30-surrogate: guard, absolute Y")
     ap.add_argument("--no-mixed", action="store_true",
-                    help="gri30-reactors: skip the mixed-precision run after the FP64 line")
+                    help="gri30-/ndodecane-reactors: skip the mixed-precision run after the FP64 line")
     ap.add_argument("--cpu-sample", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
