@@ -115,9 +115,10 @@ class Solution:
 
 class ReactorBatch:
     """n independent constant-pressure ideal-gas reactors, each with its own
-    ReactorNet; ``advance`` mirrors ``ReactorNet::advance``."""
+    ReactorNet; ``advance`` mirrors ``ReactorNet::advance``.  ``const_volume=True``
+    makes them closed rigid IdealGasReactors (V = 1 m^3) started at (T0, P, Y0)."""
 
-    def __init__(self, sol: Solution, T0, P, Y0, rtol=1e-9, atol=1e-15, precision=CG_FP64):
+    def __init__(self, sol: Solution, T0, P, Y0, rtol=1e-9, atol=1e-15, precision=CG_FP64, const_volume=False):
         self.sol = sol
         T0 = np.ascontiguousarray(np.atleast_1d(T0), dtype=np.float64)
         n = T0.shape[0]
@@ -125,7 +126,8 @@ class ReactorBatch:
         Y0 = np.ascontiguousarray(np.asarray(Y0, dtype=np.float64).reshape(n, sol.n_species))
         self.n = n
         self.neq = sol.n_species + 2
-        self._h = check(lib().cg_reactors_create(sol.handle, n, T0, P, Y0, float(rtol), float(atol), int(precision)),
+        create = lib().cg_reactors_create_const_volume if const_volume else lib().cg_reactors_create
+        self._h = check(create(sol.handle, n, T0, P, Y0, float(rtol), float(atol), int(precision)),
                         "cg_reactors_create")
         self.time = np.zeros(n)
 

@@ -48,6 +48,8 @@ SIGNATURES = [
                                         C.c_void_p, C.c_int32, C.c_void_p]),
     ("cg_jac_last_kernel_ms", C.c_double, [C.c_int32]),
     ("cg_reactors_create", C.c_int32, [C.c_int32, C.c_int64, _dp, _dp, _dp, C.c_double, C.c_double, C.c_int32]),
+    ("cg_reactors_create_const_volume", C.c_int32,
+     [C.c_int32, C.c_int64, _dp, _dp, _dp, C.c_double, C.c_double, C.c_int32]),
     ("cg_reactors_free", C.c_int32, [C.c_int32]),
     ("cg_reactors_advance", C.c_int32, [C.c_int32, _dp]),
     ("cg_reactors_advance_device", C.c_int32, [C.c_int32, C.c_void_p]),

@@ -54,6 +54,7 @@ struct Batch {
     MechObj* mech = nullptr;
     int n = 0, N = 0, K = 0, ns = 1, ld = 0;
     bool fp32 = false;
+    bool const_volume = false;   // IdealGasReactor (V = 1 m^3) instead of IdealGasConstPressureReactor
     double rtol = 1e-9, atol = 1e-15;
     double* rec = nullptr;
     double* jsave = nullptr;
@@ -376,8 +377,8 @@ double cg_jac_last_kernel_ms(int32_t h) {
     }
 }
 
-int32_t cg_reactors_create(int32_t h, int64_t n, const double* T0, const double* P, const double* Y0, double rtol,
-                           double atol, int32_t precision) {
+static int32_t create_batch(int32_t h, int64_t n, const double* T0, const double* P, const double* Y0, double rtol,
+                            double atol, int32_t precision, bool const_volume) {
     return guarded([&] {
         MechObj& mo = get_mech(h);
         if (n <= 0 || n > (1LL << 31) - 1) throw std::runtime_error("reactor count out of range");
@@ -393,6 +394,7 @@ int32_t cg_reactors_create(int32_t h, int64_t n, const double* T0, const double*
         b->N = m.N;
         b->ns = mo.ns;
         b->fp32 = precision == CG_MIXED;
+        b->const_volume = const_volume;
         b->ld = (m.N % 2) ? m.N : m.N + 1;
         b->rtol = rtol;
         b->atol = atol;
@@ -416,7 +418,8 @@ int32_t cg_reactors_create(int32_t h, int64_t n, const double* T0, const double*
         check(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "hipStreamCreate");
         check(hipEventCreate(&b->e0), "hipEventCreate");
         check(hipEventCreate(&b->e1), "hipEventCreate");
-        // initial records: zn[0] = y0 in device order [Y, T, m]; V = 1 m^3
+        // initial records: zn[0] = y0 in device order [Y, T, m]; V = 1 m^3.  The
+        // record's pressure slot holds the volume for a constant-volume reactor.
         std::vector<double> rec(b->rec_stride * n, 0.0);
         const double R = 6.02214076e26 * 1.380649e-23;
         for (int64_t s = 0; s < n; s++) {
@@ -431,7 +434,8 @@ int32_t cg_reactors_create(int32_t h, int64_t n, const double* T0, const double*
             r[m.K + 1] = dens * 1.0;
             for (int i = 0; i < m.N; i++) r[6 * NP + i] = r[i];   // yout = y0
             double* c = r + 7 * NP;
-            c[C_Q] = 1; c[C_L] = 2; c[C_QWAIT] = 2; c[C_QPRIME] = 1; c[C_ETAMAX] = 10000.0; c[C_P] = P[s];
+            c[C_Q] = 1; c[C_L] = 2; c[C_QWAIT] = 2; c[C_QPRIME] = 1; c[C_ETAMAX] = 10000.0;
+            c[C_P] = const_volume ? 1.0 : P[s];
             c[C_ETA] = 1.0;
         }
         check(hipMemcpy(b->rec, rec.data(), rec.size() * 8, hipMemcpyHostToDevice), "hipMemcpy(records)");
@@ -441,6 +445,16 @@ int32_t cg_reactors_create(int32_t h, int64_t n, const double* T0, const double*
         g_batches.push_back(std::move(b));
         return static_cast<int32_t>(g_batches.size() - 1);
     });
+}
+
+int32_t cg_reactors_create(int32_t h, int64_t n, const double* T0, const double* P, const double* Y0, double rtol,
+                           double atol, int32_t precision) {
+    return create_batch(h, n, T0, P, Y0, rtol, atol, precision, false);
+}
+
+int32_t cg_reactors_create_const_volume(int32_t h, int64_t n, const double* T0, const double* P, const double* Y0,
+                                        double rtol, double atol, int32_t precision) {
+    return create_batch(h, n, T0, P, Y0, rtol, atol, precision, true);
 }
 
 int32_t cg_reactors_free(int32_t h) {
@@ -487,8 +501,11 @@ int32_t cg_reactors_set_schedule(int32_t h, int32_t longest_first) {
 static BdfArgs advance_args(Batch& b, const int* list, int count) {
     BdfArgs a{};
     a.mech = b.mech->dm->view();
+    a.mech.const_volume = b.const_volume ? 1 : 0;
     a.n = list ? count : b.n;
-    a.neq_norm = b.N;
+    // the reference's IdealGasReactor state also holds V (constant): it counts
+    // in the WRMS norm's length and nowhere else
+    a.neq_norm = b.N + (b.const_volume ? 1 : 0);
     a.ld = b.ld;
     a.max_steps = b.max_steps;
     a.rtol = b.rtol;
@@ -857,6 +874,7 @@ int32_t cg_reactors_advance_surrogate(int32_t bh, int32_t sh, const double* t_ou
         Batch& b = get_batch(bh);
         SurrogateObj& so = get_surr(sh);
         if (so.mech != b.mech) throw std::runtime_error("surrogate and reactor batch use different mechanisms");
+        if (b.const_volume) throw std::runtime_error("the surrogate path supports constant-pressure reactors only");
         if (!t_out) throw std::runtime_error("null t_out");
         if (!(dT_tol > 0) || !(rtol_g > 0) || !(atol_g > 0)) throw std::runtime_error("guard tolerances must be positive");
         const int n = b.n, N = b.N, K = b.K, NP = 64 * b.ns;

@@ -85,6 +85,8 @@ struct DevMech {
     const double* tbe;       // efficiency - default
     const double* comp;      // K x E element composition
     const double* pdd;       // PLOG and Chebyshev records (see DeviceMech)
+    int const_volume;        // reactor model of an integrator launch: 0 IdealGasConstPressureReactor
+                             // (the P argument is the pressure), 1 IdealGasReactor (it is the volume)
 };
 
 // Owns device memory for a DevMech.

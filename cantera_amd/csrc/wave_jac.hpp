@@ -24,7 +24,7 @@ __device__ __forceinline__ void mat_add(double* p, double v) { unsafeAtomicAdd(p
 // of A).  All per-reaction lists use static indices (registers only).
 __device__ __forceinline__ void scatter_reaction_derivs(const DevMech& m, int p, const GasState& g,
                                                         const double* __restrict__ C, const double* __restrict__ MU,
-                                                        double kpd, double* A, int LD) {
+                                                        double kpd, double* A, int LD, double* acc) {
     RxnRegs x;
     load_rxn(m, p, x);
     RxnVals v;
@@ -104,7 +104,11 @@ __device__ __forceinline__ void scatter_reaction_derivs(const DevMech& m, int p,
             }
         }
     }
-    if (type != kElementary && x.tbn > 0) {
+    // acc (constant volume): C_tot = sum_l C_l is not fixed, so d[M]/dC_l also
+    // holds the default efficiency; that part, equal in every column, is
+    // accumulated per species in acc[k] instead of being scattered K times
+    const double tbd = acc ? m.tbdef[p] : 0.0;
+    if (type != kElementary && (x.tbn > 0 || tbd != 0.0)) {
         const double diff = pr_r - v.rkcn * pr_p;
         double q;
         if (type == kThreeBody) {
@@ -142,6 +146,11 @@ __device__ __forceinline__ void scatter_reaction_derivs(const DevMech& m, int p,
             q = k0p * F / (1.0 + Pr) * (1.0 / (1.0 + Pr) + dlogF) * diff;
         }
         for (int t = 0; t < x.tbn; t++) scatter(m.tbk[x.tbo + t], q * m.tbe[x.tbo + t]);
+        if (tbd != 0.0) {
+#pragma unroll
+            for (int r = 0; r < E; r++)
+                if (lead[r]) atomicAdd(&acc[ek[r]], rnu[r] * q * tbd);
+        }
     }
 }
 
@@ -157,6 +166,9 @@ __device__ void reactor_jacobian(const DevMech& m, const WaveScratch& S, double 
                                  const double (&f0)[NS], double* __restrict__ J) {
     const int lane = lane_id();
     const int K = m.K, N = m.N;
+    // constant volume: u_k and c_v in the energy row, rho fixed in the species block
+    const bool cv = m.const_volume;
+    const double hshift = cv ? 1.0 : 0.0;
     // species block, temperature/mass rows and columns
     double* const JC = kRefOrder ? J + 2 * N + 2 : J;
     double* const rowT = kRefOrder ? J + N + 2 : J + K * N;
@@ -186,10 +198,16 @@ __device__ void reactor_jacobian(const DevMech& m, const WaveScratch& S, double 
         if (j < N)
             for (int i = 0; i < N; i++) J[i * N + j] = 0.0;
     }
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        int k = lane + kWave * s;
+        if (k < K) S.WD[k] = 0.0;
+    }
     __threadfence();
     wave_sync();
     const double kpd = pdep_rate_lane(m, g);
-    for (int p = lane; p < m.I; p += kWave) scatter_reaction_derivs(m, p, g, S.C, S.MU, kpd, JC, N);
+    double* const acc = cv ? S.WD : nullptr;
+    for (int p = lane; p < m.I; p += kWave) scatter_reaction_derivs(m, p, g, S.C, S.MU, kpd, JC, N, acc);
     __threadfence();
     wave_sync();
     // D_k = sum_l JC[k][l] C_l ; keep wdot and D in W2, h_k in MU
@@ -200,7 +218,7 @@ __device__ void reactor_jacobian(const DevMech& m, const WaveScratch& S, double 
         if (k < K) {
             double d = 0.0;
             for (int l = 0; l < K; l++) d += JC[k * N + l] * S.C[l];
-            const double hk = sr.hrt[s] * g.RT;
+            const double hk = (sr.hrt[s] - hshift) * g.RT;
             S.W2[k] = sr.wdot[s];
             S.W2[K + k] = d;
             hD += hk * d;
@@ -213,7 +231,7 @@ __device__ void reactor_jacobian(const DevMech& m, const WaveScratch& S, double 
 #pragma unroll
     for (int s = 0; s < NS; s++) {
         int k = lane + kWave * s;
-        if (k < K) S.MU[k] = sr.hrt[s] * g.RT;
+        if (k < K) S.MU[k] = (sr.hrt[s] - hshift) * g.RT;
     }
     wave_sync();
     const double rho = g.dens, Ct = g.ctot, Wbar = g.mmw, cp = rs.cp_mass;
@@ -222,14 +240,25 @@ __device__ void reactor_jacobian(const DevMech& m, const WaveScratch& S, double 
         int j = lane + kWave * s;
         if (j < K) {
             double hJ = 0.0;
-            for (int k = 0; k < K; k++) hJ += S.MU[k] * JC[k * N + j];
             const double wj = m.mw[j];
-            for (int k = 0; k < K; k++) {
-                const double jc = JC[k * N + j];
-                JC[k * N + j] = m.mw[k] / wj * (jc + (S.W2[k] - S.W2[K + k]) / Ct);
+            if (cv) {
+                // rho fixed: dC_l/dY_j = rho / W_j delta_lj
+                for (int k = 0; k < K; k++) {
+                    const double jc = JC[k * N + j] + S.WD[k];
+                    hJ += S.MU[k] * jc;
+                    JC[k * N + j] = m.mw[k] / wj * jc;
+                }
+                const double cvj = kGasConstant * (sr.cpr[s] - 1.0) / wj;
+                rowT[j] = -1.0 / (rho * cp) * (rho * hJ / wj - H * cvj / cp);
+            } else {
+                for (int k = 0; k < K; k++) hJ += S.MU[k] * JC[k * N + j];
+                for (int k = 0; k < K; k++) {
+                    const double jc = JC[k * N + j];
+                    JC[k * N + j] = m.mw[k] / wj * (jc + (S.W2[k] - S.W2[K + k]) / Ct);
+                }
+                const double cpj = kGasConstant * sr.cpr[s] / wj;
+                rowT[j] = -1.0 / (rho * cp) * ((rho * hJ - Wbar * hD + Wbar * H) / wj - H * cpj / cp);
             }
-            const double cpj = kGasConstant * sr.cpr[s] / wj;
-            rowT[j] = -1.0 / (rho * cp) * ((rho * hJ - Wbar * hD + Wbar * H) / wj - H * cpj / cp);
             rowM[j] = 0.0;
         }
     }

@@ -525,8 +525,11 @@ __device__ __forceinline__ void eval_reaction(const DevMech& m, int p, const Gas
 // Gas state + per-species thermo, concentrations and chemical potentials.
 // Y values: lane owns species lane + 64 s in yk[s].  Leaves C and MU in LDS.
 template <int NS>
+// rho > 0: the state is (T, rho) instead of (T, P) (ThermoPhase::setState_TD);
+// the pressure is then R T rho / Wbar.
 __device__ __forceinline__ void species_pass(const DevMech& m, const WaveScratch& S, double T, double P,
-                                             const double (&yk)[NS], GasState& g, SpeciesRegs<NS>& sr) {
+                                             const double (&yk)[NS], GasState& g, SpeciesRegs<NS>& sr,
+                                             double rho = 0.0) {
     const int lane = lane_id();
     double sum = 0.0;
 #pragma unroll
@@ -537,12 +540,12 @@ __device__ __forceinline__ void species_pass(const DevMech& m, const WaveScratch
     }
     sum = uni(wave_sum(sum));
     g.T = T;
-    g.P = P;
     g.mmw = 1.0 / sum;
     g.RT = T * kGasConstant;
-    g.dens = P * g.mmw / g.RT;
+    g.dens = (rho > 0.0) ? rho : P * g.mmw / g.RT;
     g.ctot = g.dens / g.mmw;
     g.pressure = kGasConstant * g.ctot * T;
+    g.P = (rho > 0.0) ? g.pressure : P;
     g.logT = log(T);
     g.recipT = 1.0 / T;
     g.rrt = 1.0 / g.RT;
@@ -615,11 +618,15 @@ __device__ __forceinline__ void production_rates(const DevMech& m, const WaveScr
 
 // Scalars of the reactor energy equation kept for the Jacobian.
 struct RhsScalars {
-    double mass, vol, cp_mass, H;   // H = sum_k h_k wdot_k [J/m^3/s]
+    double mass, vol, cp_mass, H;   // H = sum_k h_k wdot_k [J/m^3/s] (constant volume: c_v and u_k)
 };
 
-// IdealGasConstPressureReactor::eval + ReactorNet::eval: ydot for the lane's
-// components (internal order).  y holds [Y_0..Y_{K-1}, T, m] distributed.
+// IdealGasConstPressureReactor::eval (src/zeroD/IdealGasConstPressureReactor.cpp)
+// or, with m.const_volume, IdealGasReactor::eval (src/zeroD/IdealGasReactor.cpp:50-117,
+// closed rigid adiabatic: m c_v dT/dt = -sum_k u_k wdot_k V, u_k = h_k - R T),
+// divided by the left-hand sides as ReactorNet::eval does: ydot for the lane's
+// components (internal order).  y holds [Y_0..Y_{K-1}, T, m] distributed; P is
+// the pressure, or the volume of a constant-volume reactor.
 template <int NS>
 __device__ __forceinline__ void reactor_rhs(const DevMech& m, const WaveScratch& S, double P, const double (&y)[NS],
                                             double (&f)[NS], GasState& g, SpeciesRegs<NS>& sr, RhsScalars& rs) {
@@ -632,21 +639,24 @@ __device__ __forceinline__ void reactor_rhs(const DevMech& m, const WaveScratch&
     wave_sync();
     const double T = uni(S.yb[m.K]);
     const double mass = uni(S.yb[m.K + 1]);
-    species_pass<NS>(m, S, T, P, y, g, sr);
+    const bool cv = m.const_volume;
+    species_pass<NS>(m, S, T, P, y, g, sr, cv ? mass / P : 0.0);
     production_rates<NS>(m, S, g, sr);
-    const double vol = mass / g.dens;
+    const double vol = cv ? P : mass / g.dens;
+    const double hshift = cv ? 1.0 : 0.0;   // u_k / RT = h_k / RT - 1
     double e = 0.0, inner = 0.0;
 #pragma unroll
     for (int s = 0; s < NS; s++) {
         int k = lane + kWave * s;
         if (k < m.K) {
-            e += sr.wdot[s] * (sr.hrt[s] * g.RT) * vol;
+            e += sr.wdot[s] * ((sr.hrt[s] - hshift) * g.RT) * vol;
             inner += sr.ym[s] * sr.cpr[s];
         }
     }
     e = uni(wave_sum(e));
     inner = uni(wave_sum(inner));
-    const double cp_mass = kGasConstant * (g.mmw * inner) / g.mmw;
+    // c_p (or c_v = (c_p,mole - R) / Wbar) per unit mass
+    const double cp_mass = kGasConstant * (g.mmw * inner - hshift) / g.mmw;
     const double mcpdTdt = -e;
     rs.mass = mass;
     rs.vol = vol;

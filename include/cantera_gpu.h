@@ -110,6 +110,16 @@ double cg_jac_last_kernel_ms(int32_t mech);
  * samples/cxx/openmp_ignition/openmp_ignition.cpp:36-40. */
 int32_t cg_reactors_create(int32_t mech, int64_t n, const double* T0, const double* P,
                            const double* Y0, double rtol, double atol, int32_t precision);
+/* Same for n closed, rigid, adiabatic IdealGasReactor + ReactorNet pairs:
+ * V = 1 m^3 and mass rho(T0, P0, Y0) V, state (T, rho) thereafter
+ * (IdealGasReactor::updateState / eval, src/zeroD/IdealGasReactor.cpp:38-117:
+ * m c_v dT/dt = -sum_k u_k wdot_k V).  The state returned by
+ * cg_reactors_get_state is [m, T, Y] as for the constant-pressure reactor (the
+ * reference's [m, V, T, Y] without the constant V); the WRMS norm counts V, as
+ * CVODES does on the reference's state.  The surrogate path rejects such a
+ * batch.  Reference: IdealGasReactor(sol) + ReactorNet(reactor). */
+int32_t cg_reactors_create_const_volume(int32_t mech, int64_t n, const double* T0, const double* P,
+                                        const double* Y0, double rtol, double atol, int32_t precision);
 int32_t cg_reactors_free(int32_t batch);
 /* Advance every reactor s to t_out[s] (host array of n).  History persists
  * between calls; each reactor steps past t_out[s] and interpolates back, as

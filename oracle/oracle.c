@@ -357,6 +357,7 @@ typedef struct {
     int K, I;
     double *ym, *cp_R, *h_RT, *s_R, *grt, *conc, *concm, *delta, *rkcn, *kf, *ropf, *ropr, *ropnet, *wdot;
     double T, P, mmw, RT, dens, molar_density, pressure, ctot;
+    int cv; /* constant-volume reactor: the P argument of rhs_w is the volume */
 } work_t;
 
 static work_t* work_new(const or_mech* m) {
@@ -603,8 +604,44 @@ int or_rates_batch(const or_mech* m, int n, const double* T, const double* P, co
 
 /* ---- reactor RHS: IdealGasConstPressureReactor::updateState/eval and
  * ReactorNet::eval ---- */
+/* IdealGasReactor::updateState + eval (src/zeroD/IdealGasReactor.cpp:38-117),
+ * closed, rigid, adiabatic, V = the P argument: setState_TD(T, m/V) and
+ * m c_v dT/dt = -sum_k u_k wdot_k V with u_k = (h_k/RT - 1) RT
+ * (IdealGasPhase::getIntEnergy_RT) and c_v = (c_p,mole - R)/Wbar. */
+static int rhs_cv(const or_mech* m, work_t* w, double V, const double* y, double* ydot) {
+    const int K = m->K;
+    double mass = y[0];
+    double sum = 0.0;
+    for (int k = 0; k < K; k++) w->ym[k] = y[2 + k] * m->rmw[k];
+    for (int k = 0; k < K; k++) sum += w->ym[k];
+    w->mmw = 1.0 / sum;
+    w->T = y[1];
+    w->RT = y[1] * GAS_CONSTANT;
+    w->dens = mass / V;
+    w->molar_density = w->dens / w->mmw;
+    w->pressure = GAS_CONSTANT * w->molar_density * y[1];
+    w->P = w->pressure;
+    update_rop(m, w);
+    double mcvdTdt = 0.0;
+    for (int n = 0; n < K; n++) {
+        double uk = (w->h_RT[n] - 1.0) * w->RT;
+        mcvdTdt -= w->wdot[n] * uk * V;
+        ydot[2 + n] = (w->wdot[n] * V) * m->mw[n] / mass;
+    }
+    double inner = 0.0;
+    for (int k = 0; k < K; k++) inner += w->ym[k] * w->cp_R[k];
+    double cv_mole = GAS_CONSTANT * (w->mmw * inner) - GAS_CONSTANT;
+    double cv_mass = cv_mole / w->mmw;
+    ydot[0] = 0.0;
+    ydot[1] = mcvdTdt / (mass * cv_mass);
+    for (int i = 0; i < K + 2; i++)
+        if (!isfinite(ydot[i])) return 1;
+    return 0;
+}
+
 static int rhs_w(const or_mech* m, work_t* w, double P, const double* y, double* ydot) {
     const int K = m->K;
+    if (w->cv) return rhs_cv(m, w, P, y, ydot);
     double mass = y[0];
     set_state(m, w, y[1], P, y + 2);
     double vol = mass / w->dens;
@@ -647,6 +684,7 @@ static void jac_analytic(const or_mech* m, work_t* w, double P, const double* y,
     double* f1 = calloc(N, 8);
     double* y1 = calloc(N, 8);
     double* JC = calloc((size_t)K * K, 8);
+    double* acc = calloc(K, 8); /* constant volume: default-efficiency part of d[M]/dC, same in every column */
     memset(J, 0, 8 * (size_t)N * N);
     /* T column by difference */
     double dT = sqrt(DBL_EPSILON) * y[1];
@@ -685,7 +723,7 @@ static void jac_analytic(const or_mech* m, work_t* w, double P, const double* y,
                 dk[nd] = m->pk[i][a]; dR[nd++] = d;
             }
         }
-        if (m->type[i] != 0) {
+        if (m->type[i] != 0 && (m->ntb[i] > 0 || (w->cv && m->tbdef[i] != 0.0))) {
             double q; /* d ropnet / d[M] */
             double diff = pr_r - w->rkcn[i] * pr_p;
             if (m->type[i] == 1) {
@@ -727,6 +765,11 @@ static void jac_analytic(const or_mech* m, work_t* w, double P, const double* y,
                 q = dkdM * diff;
             }
             for (int a = 0; a < m->ntb[i]; a++) { dk[nd] = m->tbk[i][a]; dR[nd++] = q * m->tbe[i][a]; }
+            if (w->cv) {
+                /* C_tot = sum_l C_l varies at constant volume */
+                for (int a = 0; a < m->nr[i]; a++) acc[m->rk[i][a]] -= m->rs[i][a] * q * m->tbdef[i];
+                for (int a = 0; a < m->np[i]; a++) acc[m->pk[i][a]] += m->ps[i][a] * q * m->tbdef[i];
+            }
         }
         /* scatter nu_ki * dR into JC, with nu the net stoichiometric
          * coefficient (a species on both sides, e.g. an explicit collider
@@ -750,12 +793,14 @@ static void jac_analytic(const or_mech* m, work_t* w, double P, const double* y,
     double rho = w->dens, Ct = w->ctot, Wbar = w->mmw;
     double inner = 0.0;
     for (int k = 0; k < K; k++) inner += w->ym[k] * w->cp_R[k];
-    double cp_mass = GAS_CONSTANT * (w->mmw * inner) / w->mmw;
+    /* constant volume: c_v and u_k = h_k - RT in the energy row */
+    double shift = w->cv ? 1.0 : 0.0;
+    double cp_mass = GAS_CONSTANT * (w->mmw * inner - shift) / w->mmw;
     double H = 0.0, hD = 0.0;
     double* D = calloc(K, 8);
     double* hk = calloc(K, 8);
     for (int k = 0; k < K; k++) {
-        hk[k] = w->h_RT[k] * w->RT;
+        hk[k] = (w->h_RT[k] - shift) * w->RT;
         double s = 0.0;
         for (int l = 0; l < K; l++) s += JC[k * K + l] * w->conc[l];
         D[k] = s;
@@ -765,14 +810,23 @@ static void jac_analytic(const or_mech* m, work_t* w, double P, const double* y,
     for (int j = 0; j < K; j++) {
         double hJ = 0.0;
         for (int k = 0; k < K; k++) {
-            J[(2 + k) * N + 2 + j] = m->mw[k] / m->mw[j] * (JC[k * K + j] + (w->wdot[k] - D[k]) / Ct);
-            hJ += hk[k] * JC[k * K + j];
+            double jc = JC[k * K + j] + acc[k];
+            if (w->cv) /* rho fixed: dC_l/dY_j = rho/W_j delta_lj */
+                J[(2 + k) * N + 2 + j] = m->mw[k] / m->mw[j] * jc;
+            else
+                J[(2 + k) * N + 2 + j] = m->mw[k] / m->mw[j] * (JC[k * K + j] + (w->wdot[k] - D[k]) / Ct);
+            hJ += hk[k] * jc;
         }
-        double cpj = GAS_CONSTANT * w->cp_R[j] / m->mw[j];
-        J[1 * N + 2 + j] = -1.0 / (rho * cp_mass) *
-                           ((rho * hJ - Wbar * hD + Wbar * H) / m->mw[j] - H * cpj / cp_mass);
+        if (w->cv) {
+            double cvj = GAS_CONSTANT * (w->cp_R[j] - 1.0) / m->mw[j];
+            J[1 * N + 2 + j] = -1.0 / (rho * cp_mass) * (rho * hJ / m->mw[j] - H * cvj / cp_mass);
+        } else {
+            double cpj = GAS_CONSTANT * w->cp_R[j] / m->mw[j];
+            J[1 * N + 2 + j] = -1.0 / (rho * cp_mass) *
+                               ((rho * hJ - Wbar * hD + Wbar * H) / m->mw[j] - H * cpj / cp_mass);
+        }
     }
-    free(D); free(hk); free(f0); free(f1); free(y1); free(JC);
+    free(D); free(hk); free(f0); free(f1); free(y1); free(JC); free(acc);
 }
 
 static void jac_fd(const or_mech* m, work_t* w, double P, const double* y, double* J) {
@@ -791,9 +845,18 @@ static void jac_fd(const or_mech* m, work_t* w, double P, const double* y, doubl
     free(f0); free(f1); free(y1);
 }
 
+int or_igr_rhs(const or_mech* m, double V, const double* y, double* ydot) {
+    work_t* w = work_new(m);
+    w->cv = 1;
+    int r = rhs_w(m, w, V, y, ydot);
+    work_free(w);
+    return r;
+}
+
 int or_igcpr_jacobian(const or_mech* m, double P, const double* y, double* J, int mode) {
     work_t* w = work_new(m);
-    if (mode == 1) jac_analytic(m, w, P, y, J);
+    w->cv = (mode & 2) != 0;
+    if (mode & 1) jac_analytic(m, w, P, y, J);
     else jac_fd(m, w, P, y, J);
     work_free(w);
     return 0;
@@ -843,7 +906,7 @@ enum { NO_FAILURES = 0, FAIL_BAD_J = 1, FAIL_OTHER = 2 };
 struct or_reactor {
     const or_mech* m;
     work_t* w;
-    int N;
+    int N, nnorm; /* nnorm: WRMS length (N, or N + 1 for the constant-volume V) */
     double P, rtol, atol;
     int jac_mode;
     double* zn[LMAX + 1];
@@ -915,7 +978,7 @@ static void lu_solve(const double* A, int n, const int* piv, double* b) {
 static int jac_dq(or_reactor* r, const double* y0, const double* fy, double* J) {
     const int N = r->N;
     double srur = sqrt(UROUND);
-    double fnorm = wrms(fy, r->ewt, N);
+    double fnorm = wrms(fy, r->ewt, r->nnorm);
     double minInc = (fnorm != 0.0) ? (MIN_INC_MULT * fabs(r->h) * UROUND * N * fnorm) : 1.0;
     double* yy = calloc(N, 8);
     double* col = calloc(N, 8);
@@ -1116,11 +1179,11 @@ static int nls(or_reactor* r, int nflag) {
             }
             for (int i = 0; i < N; i++) ycor[i] += delta[i];
             /* convergence test */
-            double del = wrms(delta, r->ewt, N);
+            double del = wrms(delta, r->ewt, r->nnorm);
             if (m > 0) r->crate = fmax(CRDOWN * r->crate, del / r->delp);
             double dcon = del * fmin(1.0, r->crate) / r->tq[4];
             if (dcon <= 1.0) {
-                r->acnrm = (m == 0) ? del : wrms(ycor, r->ewt, N);
+                r->acnrm = (m == 0) ? del : wrms(ycor, r->ewt, r->nnorm);
                 status = 0;
                 break;
             }
@@ -1228,13 +1291,13 @@ static void prepare_next_step(or_reactor* r, double dsm) {
     r->qwait = 2;
     double etaqm1 = 0.0, etaqp1 = 0.0;
     if (r->q > 1) {
-        double ddn = wrms(r->zn[r->q], r->ewt, r->N) * r->tq[1];
+        double ddn = wrms(r->zn[r->q], r->ewt, r->nnorm) * r->tq[1];
         etaqm1 = 1.0 / (pow(BIAS1 * ddn, 1.0 / r->q) + ADDON);
     }
     if (r->q != QMAX && r->saved_tq5 != 0.0) {
         double cquot = (r->tq[5] / r->saved_tq5) * pow(r->h / r->tau[2], r->L);
         for (int i = 0; i < r->N; i++) r->tempv[i] = -cquot * r->zn[QMAX][i] + r->acor[i];
-        double dup = wrms(r->tempv, r->ewt, r->N) * r->tq[3];
+        double dup = wrms(r->tempv, r->ewt, r->nnorm) * r->tq[3];
         etaqp1 = 1.0 / (pow(BIAS3 * dup, 1.0 / (r->L + 1)) + ADDON);
     }
     double etam = fmax(etaqm1, fmax(etaq, etaqp1));
@@ -1330,7 +1393,7 @@ static int initial_h(or_reactor* r, double tout) {
             int ret = f_eval(r, yy, ft);
             if (ret == 0) {
                 for (int i = 0; i < N; i++) ft[i] = (ft[i] - r->zn[1][i]) * (1.0 / hg);
-                yddnrm = wrms(ft, r->ewt, N);
+                yddnrm = wrms(ft, r->ewt, r->nnorm);
                 ok = 1;
                 break;
             }
@@ -1362,16 +1425,20 @@ or_reactor* or_reactor_new(const or_mech* m, double T0, double P, const double* 
     r->P = P;
     r->rtol = rtol;
     r->atol = atol;
-    r->jac_mode = jac_mode;
+    /* bit 1 of the mode: IdealGasReactor (V = 1 m^3; r->P then holds V) */
+    r->w->cv = (jac_mode & 2) != 0;
+    r->nnorm = r->N + r->w->cv;
+    r->jac_mode = jac_mode & 1;
     int N = r->N;
     for (int j = 0; j <= LMAX; j++) r->zn[j] = calloc(N, 8);
     r->ewt = calloc(N, 8); r->acor = calloc(N, 8); r->y = calloc(N, 8); r->ftemp = calloc(N, 8);
     r->tempv = calloc(N, 8); r->savedJ = calloc((size_t)N * N, 8); r->M = calloc((size_t)N * N, 8);
     r->piv = calloc(N, sizeof(int));
     r->yout = calloc(N, 8);
-    /* IdealGasConstPressureReactor::getState with V = 1 m^3 */
+    /* IdealGasConstPressureReactor / IdealGasReactor::getState with V = 1 m^3 */
     set_state(m, r->w, T0, P, Y0);
     r->zn[0][0] = r->w->dens * 1.0;
+    if (r->w->cv) r->P = 1.0;
     r->zn[0][1] = T0;
     memcpy(r->zn[0] + 2, Y0, 8 * m->K);
     memcpy(r->yout, r->zn[0], 8 * N);
@@ -1407,7 +1474,7 @@ static int start(or_reactor* r, double tout) {
 /* one CVode(ONE_STEP) call */
 static int one_step(or_reactor* r) {
     if (r->nst > 0) ewt_set(r, r->zn[0]);
-    double nrm = wrms(r->zn[0], r->ewt, r->N);
+    double nrm = wrms(r->zn[0], r->ewt, r->nnorm);
     if (UROUND * nrm > 1.0) return -10;
     int ret = bdf_step(r);
     return ret;

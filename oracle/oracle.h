@@ -91,10 +91,16 @@ int or_igcpr_rhs(const or_mech* m, double P, const double* y, double* ydot);
 /* Jacobian d(ydot)/dy, row-major neq x neq.  mode 0: finite differences
  * (increment sqrt(eps)*max(|y_j|, 1e-... floor)); mode 1: analytic (the same
  * closed form the GPU kernels use). */
+/* IdealGasReactor RHS for y = [m, T, Y] at volume V (the constant V omitted). */
+int or_igr_rhs(const or_mech* m, double V, const double* y, double* ydot);
 int or_igcpr_jacobian(const or_mech* m, double P, const double* y, double* J, int mode);
 
 /* integrator ------------------------------------------------------------ */
-enum { OR_JAC_DQ = 0, OR_JAC_ANALYTIC = 1 };
+/* jac_mode: OR_JAC_DQ or OR_JAC_ANALYTIC, plus OR_CONST_VOLUME for a closed
+ * rigid IdealGasReactor (V = 1 m^3, mass and V from (T0, P, Y0);
+ * src/zeroD/IdealGasReactor.cpp:16-117) instead of the constant-pressure
+ * reactor.  Its WRMS norms count V (CVODES's state holds it). */
+enum { OR_JAC_DQ = 0, OR_JAC_ANALYTIC = 1, OR_CONST_VOLUME = 2 };
 or_reactor* or_reactor_new(const or_mech* m, double T0, double P, const double* Y0,
                            double rtol, double atol, int jac_mode);
 void or_reactor_free(or_reactor* r);

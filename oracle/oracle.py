@@ -41,6 +41,7 @@ def lib():
         L.or_equilibrium_constants.argtypes = [C.c_void_p, C.c_double, C.c_double, _dp, _dp]
         L.or_rates_batch.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp, C.c_int]
         L.or_igcpr_rhs.argtypes = [C.c_void_p, C.c_double, _dp, _dp]
+        L.or_igr_rhs.argtypes = [C.c_void_p, C.c_double, _dp, _dp]
         L.or_igcpr_rhs.restype = C.c_int
         L.or_igcpr_jacobian.argtypes = [C.c_void_p, C.c_double, _dp, _dp, C.c_int]
         L.or_reactor_new.restype = C.c_void_p
@@ -115,27 +116,33 @@ class Mech:
                              np.ascontiguousarray(Y_soa, np.float64), wdot, nthreads)
         return wdot
 
-    def rhs(self, P, y):
+    def rhs(self, P, y, const_volume=False):
+        """ydot of the constant-pressure reactor at P, or of the constant-volume
+        reactor at volume P."""
         ydot = np.zeros(self.K + 2)
-        r = lib().or_igcpr_rhs(self._h, float(P), np.ascontiguousarray(y, np.float64), ydot)
+        f = lib().or_igr_rhs if const_volume else lib().or_igcpr_rhs
+        r = f(self._h, float(P), np.ascontiguousarray(y, np.float64), ydot)
         if r:
             raise FloatingPointError("non-finite RHS")
         return ydot
 
-    def jacobian(self, P, y, analytic=True):
+    def jacobian(self, P, y, analytic=True, const_volume=False):
         N = self.K + 2
         J = np.zeros(N * N)
-        lib().or_igcpr_jacobian(self._h, float(P), np.ascontiguousarray(y, np.float64), J, 1 if analytic else 0)
+        lib().or_igcpr_jacobian(self._h, float(P), np.ascontiguousarray(y, np.float64), J,
+                                (1 if analytic else 0) | (2 if const_volume else 0))
         return J.reshape(N, N)
 
-    def run_reactors(self, T0, P, Y0, dt, nout, rtol=1e-9, atol=1e-15, analytic=False, nthreads=0, keep_states=True):
+    def run_reactors(self, T0, P, Y0, dt, nout, rtol=1e-9, atol=1e-15, analytic=False, nthreads=0, keep_states=True,
+                     const_volume=False):
         n = len(T0)
         N = self.K + 2
         y_out = np.zeros((n, nout + 1, N)) if keep_states else None
         stats = np.zeros((n, 9), dtype=np.int64)
         r = lib().or_reactors_run(self._h, n, np.ascontiguousarray(T0, np.float64), np.ascontiguousarray(P, np.float64),
                                   np.ascontiguousarray(Y0, np.float64), np.ascontiguousarray(dt, np.float64), nout,
-                                  rtol, atol, 1 if analytic else 0, _ptr(y_out), _ptr(stats), nthreads)
+                                  rtol, atol, (1 if analytic else 0) | (2 if const_volume else 0), _ptr(y_out),
+                                  _ptr(stats), nthreads)
         if r:
             raise RuntimeError(f"oracle integration failed for reactor {-r - 1}")
         return y_out, stats
@@ -144,10 +151,10 @@ class Mech:
 class Reactor:
     """One IdealGasConstPressureReactor + ReactorNet (oracle)."""
 
-    def __init__(self, mech: Mech, T0, P, Y0, rtol=1e-9, atol=1e-15, analytic=False):
+    def __init__(self, mech: Mech, T0, P, Y0, rtol=1e-9, atol=1e-15, analytic=False, const_volume=False):
         self.mech = mech
         self._h = lib().or_reactor_new(mech._h, float(T0), float(P), np.ascontiguousarray(Y0, np.float64),
-                                       rtol, atol, 1 if analytic else 0)
+                                       rtol, atol, (1 if analytic else 0) | (2 if const_volume else 0))
 
     def __del__(self):
         if getattr(self, "_h", None) and _LIB is not None:

@@ -58,18 +58,20 @@ def test_kin1_blessed_trajectory():
     assert np.nanmax(rel) < 2e-4
 
 
-def _compare(mech, wl, nout, analytic, tolT, tolY_abs, tolY_rel, oracle_mod, precision=0):
+def _compare(mech, wl, nout, analytic, tolT, tolY_abs, tolY_rel, oracle_mod, precision=0, const_volume=False):
     import cantera_amd as ca
     sol = ca.Solution(mech)
     om = oracle_mod.Mech(mech_file(mech))
     n = len(wl.T0)
-    rb = ca.ReactorBatch(sol, wl.T0, wl.P, wl.Y0, rtol=wl.rtol, atol=wl.atol, precision=precision)
+    rb = ca.ReactorBatch(sol, wl.T0, wl.P, wl.Y0, rtol=wl.rtol, atol=wl.atol, precision=precision,
+                         const_volume=const_volume)
     gpu = np.zeros((n, nout + 1, sol.n_species + 2))
     gpu[:, 0] = rb.state
     for k in range(1, nout + 1):
         rb.advance(k * wl.dt)
         gpu[:, k] = rb.state
-    ref, stats = om.run_reactors(wl.T0, wl.P, wl.Y0, wl.dt, nout, wl.rtol, wl.atol, analytic=analytic, nthreads=8)
+    ref, stats = om.run_reactors(wl.T0, wl.P, wl.Y0, wl.dt, nout, wl.rtol, wl.atol, analytic=analytic, nthreads=8,
+                                 const_volume=const_volume)
     dT = np.abs(gpu[:, :, 1] - ref[:, :, 1]) / ref[:, :, 1]
     dY = np.abs(gpu[:, :, 2:] - ref[:, :, 2:])
     okY = dY <= tolY_abs + tolY_rel * np.abs(ref[:, :, 2:])
@@ -128,6 +130,29 @@ def test_pdep_batch_matches_oracle(oracle_mod):
     _compare("pdep-test", wl, 10, True, 1e-6, 1e-9, 1e-6, oracle_mod)
 
 
+def test_const_volume_batch_matches_oracle(oracle_mod):
+    """IdealGasReactor (closed, rigid, V = 1 m^3) batches against the oracle's
+    constant-volume reactor running the same algorithm; the density (mass per
+    unit volume) stays at its initial value and the specific internal energy is
+    conserved along the GPU trajectories."""
+    from cantera_amd.workloads import MechInfo, config3
+    info = MechInfo(mech_file("gri30"))
+    wl = config3(info, n=48, nsteps=10)
+    gpu, ref = _compare("gri30", wl, 10, True, 1e-6, 1e-9, 1e-6, oracle_mod, const_volume=True)
+    assert np.all(gpu[:, :, 0] == gpu[:, :1, 0])
+    om = oracle_mod.Mech(mech_file("gri30"))
+    R = 6.02214076e26 * 1.380649e-23
+
+    def u_mass(y):
+        _, h, _ = om.thermo(y[1])
+        return float(np.sum(y[2:] / om.mw * (h - 1.0) * R * y[1]))
+
+    for r in range(0, 48, 7):
+        u0 = u_mass(gpu[r, 0])
+        for k in range(1, 11):
+            assert abs(u_mass(gpu[r, k]) - u0) <= 1e-6 * abs(u0) + 10.0, (r, k)
+
+
 def test_gri30_mixed_precision_within_tolerance(oracle_mod):
     """FP32 iteration matrix (FP64 state, residual and error test): the Newton
     corrections converge to the same FP64 solution, so trajectories stay within
@@ -135,6 +160,14 @@ def test_gri30_mixed_precision_within_tolerance(oracle_mod):
     from cantera_amd.workloads import MechInfo, config3
     wl = config3(MechInfo(mech_file("gri30")), n=48, nsteps=10)
     _compare("gri30", wl, 10, True, 1e-4, 1e-7, 1e-3, oracle_mod, precision=1)
+
+
+def test_ndodecane_mixed_precision_within_tolerance(oracle_mod):
+    """N = 102 with the FP32 iteration matrix in LDS (two state slots per
+    lane): within the CVODES-mode bar of the oracle, as for gri30."""
+    from cantera_amd.workloads import MechInfo, config5
+    wl = config5(MechInfo(mech_file("nDodecane")), n=8, nsteps=5)
+    _compare("nDodecane", wl, 5, True, 1e-4, 1e-7, 1e-3, oracle_mod, precision=1)
 
 
 def test_schedule_does_not_change_results():

@@ -464,3 +464,93 @@ def test_sri_tsang_jacobian_matches_finite_differences(oracle_mod, mech):
             Jf[:, j] = (m.rhs(P, yp) - m.rhs(P, ym)) / (2 * d)
         scale = np.abs(Jf).max(axis=1, keepdims=True) + 1e-30 * np.abs(Jf).max()
         assert np.max(np.abs(Ja - Jf)[1:, 1:] / scale[1:]) < 1e-6, (mech, T, P)
+
+
+# --- constant-volume reactor (IdealGasReactor) ---------------------------------
+
+def _gri_mixture(m, phi=1.0, n2=3.76):
+    X = np.zeros(m.K)
+    X[m.index("CH4")] = phi
+    X[m.index("O2")] = 2.0
+    X[m.index("N2")] = 2.0 * n2
+    X /= X.sum()
+    Y = X * m.mw
+    return Y / Y.sum()
+
+
+def test_const_volume_energy_and_density_invariants(gri):
+    """Closed rigid adiabatic reactor (IdealGasReactor.cpp:50-117): the mass
+    density m/V and the specific internal energy u = sum_k Y_k (h_k - RT)/W_k
+    stay at their initial values while T rises by ~2000 K through ignition
+    (u from NASA7 h/RT, independently of the integrator's c_v and u_k terms);
+    the mass fractions keep summing to 1."""
+    from oracle.oracle import Reactor
+    Y0 = _gri_mixture(gri)
+    T0, P0 = 1700.0, ONE_ATM
+    r = Reactor(gri, T0, P0, Y0, rtol=1e-10, atol=1e-18, analytic=True, const_volume=True)
+    y0 = r.state
+
+    def u_mass(y):
+        _, h, _ = gri.thermo(y[1])
+        return float(np.sum(y[2:] / gri.mw * (h - 1.0) * R * y[1]))
+
+    u0 = u_mass(y0)
+    Tmax = T0
+    for k in range(1, 41):
+        r.advance(k * 5e-5)
+        y = r.state
+        Tmax = max(Tmax, y[1])
+        assert abs(u_mass(y) - u0) <= 1e-6 * abs(u0) + 10.0, k
+        assert abs(y[0] - y0[0]) == 0.0
+        assert abs(y[2:].sum() - 1.0) < 1e-8
+    assert Tmax > T0 + 1000.0
+
+
+def test_const_volume_pressure_rises_and_differs_from_const_pressure(gri):
+    """The same mixture burns to a higher temperature at constant volume
+    (c_v < c_p), and its pressure P = rho R T / Wbar rises with T."""
+    from oracle.oracle import Reactor
+    Y0 = _gri_mixture(gri)
+    rv = Reactor(gri, 1700.0, ONE_ATM, Y0, rtol=1e-9, atol=1e-15, analytic=True, const_volume=True)
+    rp = Reactor(gri, 1700.0, ONE_ATM, Y0, rtol=1e-9, atol=1e-15, analytic=True)
+    rv.advance(2e-3)
+    rp.advance(2e-3)
+    yv, yp = rv.state, rp.state
+    assert yv[1] > yp[1] + 100.0
+    Wv = 1.0 / np.sum(yv[2:] / gri.mw)
+    P_end = yv[0] / 1.0 * R * yv[1] / Wv
+    assert P_end > 1.5 * ONE_ATM
+
+
+def test_const_volume_jacobian_matches_finite_differences(oracle_mod):
+    """Analytic species columns of the IdealGasReactor Jacobian (rho fixed, so
+    C_tot and the default third-body efficiency part of [M] vary with Y)
+    against central differences of its RHS."""
+    for mech, fuel in (("h2o2", "H2"), ("gri30", "CH4")):
+        m = oracle_mod.Mech(mech_file(mech))
+        g = np.random.default_rng(2)
+        X = g.random(m.K) * 1e-3
+        X[m.index(fuel)] += 0.1
+        X[m.index("O2")] += 0.2
+        X[m.index("N2")] += 0.7
+        X /= X.sum()
+        Y = X * m.mw
+        Y /= Y.sum()
+        for T in (1000.0, 2200.0):
+            y = np.concatenate([[0.8, T], Y])
+            V = 1.0
+            Ja = m.jacobian(V, y, True, const_volume=True)
+            N = m.K + 2
+            Jf = np.zeros((N, N))
+            for j in range(1, N):
+                d = max(abs(y[j]), 1e-8) * 1e-6
+                yp, ym = y.copy(), y.copy()
+                yp[j] += d
+                ym[j] -= d
+                Jf[:, j] = (m.rhs(V, yp, const_volume=True) - m.rhs(V, ym, const_volume=True)) / (2 * d)
+            scale = np.abs(Jf).max(axis=1, keepdims=True) + 1e-30 * np.abs(Jf).max()
+            err = np.abs(Ja - Jf)[1:] / scale[1:]
+            # species columns are analytic; the T column is a one-sided
+            # difference in the Jacobian itself (sqrt(eps) T increment)
+            assert err[:, 2:].max() < 1e-6, (mech, T)
+            assert err[:, 1].max() < 1e-3, (mech, T)
