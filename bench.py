@@ -134,13 +134,13 @@ def cpu_threads():
     return max(1, min(threads, 16))
 
 
-def cpu_reactors(mech_path, wl, steps, sample, threads):
+def cpu_reactors(mech_path, wl, steps, sample, threads, const_volume=False):
     from oracle.oracle import Mech
     om = Mech(mech_path)
     S = min(sample, len(wl.T0))
     t0 = time.perf_counter()
     om.run_reactors(wl.T0[:S], wl.P[:S], wl.Y0[:S], wl.dt[:S], steps, wl.rtol, wl.atol,
-                    analytic=False, nthreads=threads, keep_states=False)
+                    analytic=False, nthreads=threads, keep_states=False, const_volume=const_volume)
     el = time.perf_counter() - t0
     return {"value": S * steps / el, "unit": "reactor-steps/s", "cores": threads, "kind": "port",
             "sample": f"{S} of the {len(wl.T0)} reactors x {steps} steps, CVODES-method BDF with "
@@ -182,6 +182,11 @@ def reactors(a):
     if a.precision == "fp32":
         raise SystemExit("--precision fp32 applies to gri30-rates only (reactors: fp64 or mixed)")
     prec = ca.CG_FP64 if a.precision == "fp64" else ca.CG_MIXED
+    if a.const_volume:
+        # closed rigid reactors started at the same (T0, P0, Y0): a different
+        # workload from the constant-pressure headline, named as such
+        desc = desc.replace("IdealGasConstPressureReactor", "IdealGasReactor (constant volume, V = 1 m^3)")
+        metric = f"{mech} IdealGasReactor (constant volume) reactor-steps/s (FP64 BDF)"
 
     # all output times resident in HBM before the timed region
     touts = np.ascontiguousarray(np.outer(np.arange(1, steps + 1, dtype=np.float64), wl.dt))
@@ -190,7 +195,8 @@ def reactors(a):
 
     tuned = None
     if a.warmup > 0 or a.autotune:
-        wb = ca.ReactorBatch(sol, wl.T0, wl.P, wl.Y0, rtol=wl.rtol, atol=wl.atol, precision=prec)
+        wb = ca.ReactorBatch(sol, wl.T0, wl.P, wl.Y0, rtol=wl.rtol, atol=wl.atol, precision=prec,
+                         const_volume=a.const_volume)
         wb.set_options(0, a.waves_per_cu)
         done = min(max(a.warmup, 1 if a.autotune else 0), steps)
         for k in range(done):
@@ -205,7 +211,8 @@ def reactors(a):
                      "candidate_ms": [round(float(x), 3) for x in times]}
         del wb
 
-    rb = ca.ReactorBatch(sol, wl.T0, wl.P, wl.Y0, rtol=wl.rtol, atol=wl.atol, precision=prec)
+    rb = ca.ReactorBatch(sol, wl.T0, wl.P, wl.Y0, rtol=wl.rtol, atol=wl.atol, precision=prec,
+                         const_volume=a.const_volume)
     rb.set_options(0, a.waves_per_cu)
     rb.set_schedule(a.schedule == "longest-first")
     rb.set_matrix_storage(a.matrix_storage)
@@ -278,10 +285,12 @@ def reactors(a):
         y64 = rb.state
         del rb
         # warm the mixed kernels on a few reactors, then time all K steps
-        wm = ca.ReactorBatch(sol, wl.T0[:64], wl.P[:64], wl.Y0[:64], rtol=wl.rtol, atol=wl.atol, precision=ca.CG_MIXED)
+        wm = ca.ReactorBatch(sol, wl.T0[:64], wl.P[:64], wl.Y0[:64], rtol=wl.rtol, atol=wl.atol, precision=ca.CG_MIXED,
+                             const_volume=a.const_volume)
         wm.advance(wl.dt[:64])
         del wm
-        rm = ca.ReactorBatch(sol, wl.T0, wl.P, wl.Y0, rtol=wl.rtol, atol=wl.atol, precision=ca.CG_MIXED)
+        rm = ca.ReactorBatch(sol, wl.T0, wl.P, wl.Y0, rtol=wl.rtol, atol=wl.atol, precision=ca.CG_MIXED,
+                             const_volume=a.const_volume)
         check(L.cg_device_synchronize())
         t1 = time.perf_counter()
         for k in range(steps):
@@ -298,7 +307,7 @@ def reactors(a):
             }
     if not a.no_cpu:
         sample = a.cpu_sample or {"gri30": 16384, "nDodecane": 512, "h2o2": 1024}[mech]
-        out["cpu_baseline"] = cpu_reactors(sol.path, wl, steps, sample, cpu_threads())
+        out["cpu_baseline"] = cpu_reactors(sol.path, wl, steps, sample, cpu_threads(), a.const_volume)
     L.cg_dev_free(d_tout)
     copy_calib(a, L, check)
     print(json.dumps(out), flush=True)
@@ -532,6 +541,8 @@ def main(argv=None):
     ap.add_argument("--dT-tol", type=float, default=0.05, help="gri30-surrogate: guard, K")
     ap.add_argument("--rtol-g", type=float, default=1e-3, help="gri30-surrogate: guard, relative")
     ap.add_argument("--atol-g", type=float, default=1e-9, help="gri30-surrogate: guard, absolute Y")
+    ap.add_argument("--const-volume", action="store_true",
+                    help="reactor workloads: closed rigid IdealGasReactor instead of IdealGasConstPressureReactor")
     ap.add_argument("--no-mixed", action="store_true",
                     help="gri30-/ndodecane-reactors: skip the mixed-precision run after the FP64 line")
     ap.add_argument("--cpu-sample", type=int, default=0)
