@@ -2,8 +2,10 @@
 // wavefront into an FP64 matrix in global memory (row-major, internal state
 // order [Y_0..Y_{K-1}, T, m]).
 //
-// Y columns: exact chain rule through C_j = rho Y_j / W_j at fixed T (so the
-// molar density P/RT is fixed), with dwdot/dC from mass action, third-body and
+// Y columns: exact chain rule through C_j = rho Y_j / W_j at fixed T -- at
+// constant pressure the molar density P/RT is fixed; at constant volume rho is
+// fixed and the total concentration, hence the default-efficiency part of [M],
+// varies -- with dwdot/dC from mass action, third-body and
 // falloff [M] dependence -- the structure of BulkKinetics::netProductionRates_ddCi
 // (src/kinetics/BulkKinetics.cpp:761-811) and ThirdBodyCalc::derivatives
 // (include/cantera/kinetics/ThirdBodyCalc.h:97-100), at fixed rate constants.
