@@ -271,7 +271,7 @@ def reactors(a):
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / FP64_PEAK_TFLOPS,
-            "traffic": measured_traffic(f"bdf_advance_{mech}", n),
+            "traffic": measured_traffic(f"bdf_advance_{mech}" + ("_cv" if a.const_volume else ""), n),
             "kernel": "bdf_advance",
             "avg_launch_ms": avg_launch_s * 1e3,
             "flops_per_launch": W / steps,
