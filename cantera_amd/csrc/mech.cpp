@@ -130,6 +130,11 @@ HostMech load_mech(const std::string& path) {
             c.push_back(nT);
             c.push_back(nP);
             for (int e = 0; e < nT * nP; e++) c.push_back(to_d(next()));
+        } else if (key == "CHEMACT") {
+            if (last_r < 0 || !(m.rxn[last_r].type == kLindemann || m.rxn[last_r].type >= kTroe) ||
+                m.rxn[last_r].type == kPlog || m.rxn[last_r].type == kChebyshev)
+                throw std::runtime_error("CHEMACT record without a falloff reaction");
+            m.rxn[last_r].chem_act = true;
         } else if (key == "SRI") {
             if (last_r < 0 || m.rxn[last_r].type != kSri) throw std::runtime_error("SRI record without an SRI reaction");
             for (int e = 0; e < 5; e++) m.rxn[last_r].sri[e] = to_d(next());
@@ -211,7 +216,7 @@ DeviceMech::DeviceMech(const HostMech& h) : h_(h) {
         const int dtype = (r.type == kPlog || r.type == kChebyshev) ? kElementary
                           : (r.type == kSri || r.type == kTsang) ? kTroe : r.type;
         x.flags = (r.rev ? 1 : 0) | (dtype << 1) | (dtype != kElementary ? 32 : 0) | (r.type == kPlog ? 64 : 0) |
-                  (r.type == kChebyshev ? 128 : 0) | (r.type == kSri ? 256 : 0);
+                  (r.type == kChebyshev ? 128 : 0) | (r.type == kSri ? 256 : 0) | (r.chem_act ? 512 : 0);
         if (r.type == kPlog || r.type == kChebyshev) n_pd++;
         if (dtype == kElementary) n_el++;
         else if (dtype == kThreeBody) n_tb++;

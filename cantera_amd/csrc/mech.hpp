@@ -22,6 +22,7 @@ struct HostReaction {
     double troe[4] = {0, 0, 0, 0};   // a, 1/T3, 1/T1, T2
     double sri[5] = {0, 0, 0, 1, 0}; // kSri: a, b, c, d, e
     double tsang[2] = {0, 0};        // kTsang: A, B
+    bool chem_act = false;           // chemically activated falloff: k = k0 F / (1 + Pr)
     std::vector<int> rk;             // reactant species (name order)
     std::vector<double> rs, ro;      // stoich, order
     std::vector<int> pk;
@@ -54,7 +55,7 @@ HostMech load_mech(const std::string& path);
 struct RxnI {
     int flags;      // bit0 reversible, bits1-2 type (0..3), bit3 reactant side general, bit4 product side general,
                     // bit5 third body, bit6 PLOG, bit7 Chebyshev (those two with type 0),
-                    // bit8 SRI (with type 3)
+                    // bit8 SRI (with type 3), bit9 chemically activated (types 2, 3)
     int nr, np;     // entries in rk/pk
     int tbo, tbn;   // third-body efficiency list offset/count
     int fo;         // falloff record index, or PLOG / Chebyshev record offset in pdd (-1 if none)

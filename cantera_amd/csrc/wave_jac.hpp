@@ -144,8 +144,15 @@ __device__ __forceinline__ void scatter_reaction_derivs(const DevMech& m, int p,
                 F = exp10(lFc / w);
                 dlogF = (Pr > kSmallNumber) ? -lFc * 2.0 * f1 * df1 / (w * w) : 0.0;
             }
-            double k0p = rc_low * rc_high / (rc_high + kSmallNumber);
-            q = k0p * F / (1.0 + Pr) * (1.0 / (1.0 + Pr) + dlogF) * diff;
+            if (x.flags & 512) {
+                // k = k0 F/(1+Pr): dk/d[M] = k0 (k0/kinf) F/(1+Pr) (dlogF/Pr - 1/(1+Pr))
+                const double r0 = rc_low / (rc_high + kSmallNumber);
+                const double dl = (Pr > kSmallNumber) ? dlogF / Pr : 0.0;
+                q = rc_low * r0 * F / (1.0 + Pr) * (dl - 1.0 / (1.0 + Pr)) * diff;
+            } else {
+                double k0p = rc_low * rc_high / (rc_high + kSmallNumber);
+                q = k0p * F / (1.0 + Pr) * (1.0 / (1.0 + Pr) + dlogF) * diff;
+            }
         }
         for (int t = 0; t < x.tbn; t++) scatter(m.tbk[x.tbo + t], q * m.tbe[x.tbo + t]);
         if (tbd != 0.0) {

@@ -433,8 +433,10 @@ __device__ __forceinline__ void eval_reaction(const DevMech& m, int p, const Gas
             double lgf = work / (1.0 + f1 * f1);
             F = rate_exp10<kF32>(lgf);
         }
-        pr *= F / (1.0 + pr);
-        kf = pr * rc_high;
+        // FalloffRate::evalFromStruct (Falloff.h:154-176): Pr/(1+Pr) F k_inf, or
+        // F/(1+Pr) k_0 for a chemically activated reaction
+        const double fac = F / (1.0 + pr);
+        kf = (x.flags & 512) ? fac * rc_low : (pr * fac) * rc_high;
     }
     o.kf = kf;
     o.M = M;

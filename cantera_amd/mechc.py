@@ -42,6 +42,9 @@ What is restated from the reference, and where:
   coefficients (d = 1, e = 0 when absent), C and D must not be negative.
 * Tsang -- ``TsangRate::setFalloffCoeffs`` ``Falloff.cpp:454-469``: A, and
   B = 0 when absent.
+* chemically activated falloff (``type: chemically-activated``) -- the falloff
+  record plus a ``CHEMACT`` line; ``FalloffRate::evalFromStruct``
+  (``Falloff.h:154-176``) then returns k0 F / (1 + Pr).
 
 Supported rate types: elementary Arrhenius (incl. ``negative-A``), three-body,
 Lindemann, Troe, SRI and Tsang falloff, PLOG, Chebyshev, reaction ``orders``.
@@ -277,6 +280,7 @@ class Reaction:
     cheb: tuple = ()                            # Chebyshev: (Tmin, Tmax, Pmin, Pmax, rows)
     sri: tuple = ()                             # SRI: (a, b, c, d, e)
     tsang: tuple = ()                           # Tsang: (A, B)
+    chem_act: bool = False                      # chemically activated falloff
 
 
 TYPE_ELEMENTARY, TYPE_THREE_BODY, TYPE_LINDEMANN, TYPE_TROE, TYPE_PLOG, TYPE_CHEBYSHEV = 0, 1, 2, 3, 4, 5
@@ -525,6 +529,9 @@ def _compile_reaction(node, index, units: UnitSystem) -> Reaction:
     rate_type = node.get("type", "")
     if rate_type in ("", "reaction"):
         rate_type = ""
+    chem_act = rate_type == "chemically-activated"
+    if chem_act:
+        rate_type = "falloff"
     if rate_type not in ("", "elementary", "three-body", "falloff", "pressure-dependent-Arrhenius", "Chebyshev"):
         raise MechanismError(f"reaction {eq!r}: rate type {rate_type!r} not supported")
     pdep = rate_type in ("pressure-dependent-Arrhenius", "Chebyshev")
@@ -575,9 +582,14 @@ def _compile_reaction(node, index, units: UnitSystem) -> Reaction:
         return A, b, EaR
 
     r = Reaction(eq, TYPE_ELEMENTARY, reversible, 0.0, 0.0, 0.0)
+    r.chem_act = chem_act
     if rate_type == "falloff":
-        r.A, r.b, r.EaR = arr("high-P-rate-constant", 0.0)
-        r.A0, r.b0, r.EaR0 = arr("low-P-rate-constant", 1.0)
+        # FalloffRate::setParameters (Falloff.cpp:130-160): the low-pressure
+        # rate is one concentration order above the high-pressure one; the
+        # chemically activated form keeps the reaction's order on the low side
+        lo = 0.0 if chem_act else 1.0
+        r.A, r.b, r.EaR = arr("high-P-rate-constant", lo - 1.0)
+        r.A0, r.b0, r.EaR0 = arr("low-P-rate-constant", lo)
         if "Troe" in node:
             t = node["Troe"]
             a = _num(t["A"])
@@ -710,6 +722,8 @@ def write_mech(mech, out_path):
         for k, e in r.tb_eff:
             toks += [str(k), _f(e)]
         lines.append(" ".join(toks))
+        if r.chem_act:
+            lines.append("CHEMACT")
         if r.rtype == TYPE_PLOG:
             lines.append(" ".join(["PLOG", str(len(r.plog))] + [_f(x) for e in r.plog for x in e]))
         elif r.rtype == TYPE_SRI:

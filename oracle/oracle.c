@@ -168,6 +168,7 @@ struct or_mech {
     double** pd; /* PLOG: (P, A, b, Ea/R) per entry; Chebyshev: Tmin, Tmax, Pmin, Pmax, nT, nP, coefficients */
     int* npd;    /* PLOG entries / Chebyshev record length */
     double (*fx)[5]; /* SRI a, b, c, d, e; Tsang A, B */
+    int* ca;         /* chemically activated falloff */
     stoich_t reac, prod, revprod;
 };
 
@@ -236,6 +237,7 @@ or_mech* or_mech_load(const char* path) {
             m->tbe = calloc(I, sizeof *m->tbe); m->tbdef = calloc(I, 8);
             m->pdep = calloc(I, 4); m->pd = calloc(I, sizeof *m->pd); m->npd = calloc(I, 4);
             m->fx = calloc(I, sizeof *m->fx);
+            m->ca = calloc(I, 4);
             stoich_init(&m->reac, I + 1);
             stoich_init(&m->prod, I + 1);
             stoich_init(&m->revprod, I + 1);
@@ -269,6 +271,11 @@ or_mech* or_mech_load(const char* path) {
             m->npd[last] = n;
             m->pd[last] = calloc(4 * (size_t)n, 8);
             for (int j = 0; j < 4 * n; j++) m->pd[last][j] = tokd(&p);
+        } else if (!strcmp(tok, "CHEMACT")) {
+            if (last < 0 || !(m->type[last] == 2 || m->type[last] == 3 || m->type[last] >= 6)) {
+                set_err("CHEMACT record without a falloff reaction"); goto fail;
+            }
+            m->ca[last] = 1;
         } else if (!strcmp(tok, "SRI")) {
             if (last < 0 || m->type[last] != 6) { set_err("SRI record without an SRI reaction"); goto fail; }
             for (int j = 0; j < 5; j++) m->fx[last][j] = tokd(&p);
@@ -309,7 +316,7 @@ void or_mech_free(or_mech* m) {
     free(m->tbk); free(m->rs); free(m->ro); free(m->ps); free(m->tbe); free(m->tbdef);
     if (m->pd)
         for (int i = 0; i < m->I; i++) free(m->pd[i]);
-    free(m->pdep); free(m->pd); free(m->npd); free(m->fx);
+    free(m->pdep); free(m->pd); free(m->npd); free(m->fx); free(m->ca);
     if (m->reac.cap) { stoich_free(&m->reac); stoich_free(&m->prod); stoich_free(&m->revprod); }
     free(m);
 }
@@ -530,8 +537,13 @@ static void update_rop(const or_mech* m, work_t* w) {
                 double xx = 1.0 / (1.0 + lpr * lpr);
                 F = pow(work, xx) * work1;
             }
-            pr *= F / (1.0 + pr);
-            w->kf[i] = pr * rc_high;
+            if (m->ca[i]) {
+                pr = F / (1.0 + pr);
+                w->kf[i] = pr * rc_low;
+            } else {
+                pr *= F / (1.0 + pr);
+                w->kf[i] = pr * rc_high;
+            }
         }
     }
     for (int i = 0; i < I; i++) {
@@ -760,8 +772,15 @@ static void jac_analytic(const or_mech* m, work_t* w, double P, const double* y,
                     dlogF = (Pr > SMALL_NUMBER) ? -lFc * 2.0 * f1v * df1 / ((1.0 + f1v * f1v) * (1.0 + f1v * f1v)) : 0.0;
                 }
                 /* dk/d[M] = k0' F/(1+Pr) * (1/(1+Pr) + dlogF) */
-                double k0p = rc_low * rc_high / (rc_high + SMALL_NUMBER);
-                double dkdM = k0p * F / (1.0 + Pr) * (1.0 / (1.0 + Pr) + dlogF);
+                double dkdM;
+                if (m->ca[i]) {
+                    /* k = k0 F/(1+Pr), Pr = [M] k0/kinf */
+                    double dl = (Pr > SMALL_NUMBER) ? dlogF / Pr : 0.0;
+                    dkdM = rc_low * (rc_low / (rc_high + SMALL_NUMBER)) * F / (1.0 + Pr) * (dl - 1.0 / (1.0 + Pr));
+                } else {
+                    double k0p = rc_low * rc_high / (rc_high + SMALL_NUMBER);
+                    dkdM = k0p * F / (1.0 + Pr) * (1.0 / (1.0 + Pr) + dlogF);
+                }
                 q = dkdM * diff;
             }
             for (int a = 0; a < m->ntb[i]; a++) { dk[nd] = m->tbk[i][a]; dR[nd++] = q * m->tbe[i][a]; }

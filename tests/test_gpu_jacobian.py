@@ -60,7 +60,8 @@ def check_against_oracle(om, T, P, Y, J):
 
 
 @pytest.mark.parametrize("mech,n", [("h2o2", 70), ("gri30", 40), ("nDodecane", 6), ("pdep-test", 40),
-                                    ("sri-falloff", 40), ("tsang-falloff", 40)])
+                                    ("sri-falloff", 40), ("tsang-falloff", 40),
+                                    ("chemically-activated-reaction", 40)])
 def test_jacobian_matches_oracle(mech, n, oracle_mod):
     import cantera_amd as ca
     from cantera_amd.workloads import MechInfo

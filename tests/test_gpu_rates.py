@@ -48,7 +48,8 @@ def normalised_error(mech, oracle_mech, T, P, Y, wdot_gpu):
     return worst
 
 
-@pytest.mark.parametrize("mech", ["h2o2", "gri30", "nDodecane", "noxNeg", "pdep-test", "sri-falloff", "tsang-falloff"])
+@pytest.mark.parametrize("mech", ["h2o2", "gri30", "nDodecane", "noxNeg", "pdep-test", "sri-falloff", "tsang-falloff",
+                                  "chemically-activated-reaction"])
 def test_rates_match_oracle(mech, oracle_mod):
     import cantera_amd as ca
     from cantera_amd.workloads import MechInfo

@@ -554,3 +554,57 @@ def test_const_volume_jacobian_matches_finite_differences(oracle_mod):
             # difference in the Jacobian itself (sqrt(eps) T increment)
             assert err[:, 2:].max() < 1e-6, (mech, T)
             assert err[:, 1].max() < 1e-3, (mech, T)
+
+
+# --- chemically activated falloff (chemically-activated-reaction.yaml) --------
+
+def test_chemically_activated_exact_case(oracle_mod):
+    """FalloffRate::evalFromStruct (Falloff.h:154-176) for a chemically
+    activated Troe reaction: at log10 Pr = 0.4 + 0.67 log10(Fcent) the Troe F
+    equals Fcent exactly, so k = k0 Fcent / (1 + Pr) (k0, not kinf, and
+    1/(1+Pr), not Pr/(1+Pr)).  Pure n2 bath: [M] = P/RT."""
+    mech = "chemically-activated-reaction"
+    m = oracle_mod.Mech(mech_file(mech))
+    recs = _falloff_records(mech_file(mech))
+    typ, hi, lo, tbdef, tbk, ex = recs[0]
+    assert typ == 3 and tbdef == 1.0
+    t = open(mech_file(mech)).read().split("\n")
+    rline = next(l for l in t if l.startswith("R 0 ")).split()
+    a, rt3, rt1, t2 = (float(x) for x in rline[11:15])
+    Y = np.zeros(m.K)
+    Y[m.index("n2")] = 1.0
+    for T in (500.0, 1200.0, 2000.0):
+        kinf = hi[0] * np.exp(hi[1] * np.log(T) - hi[2] / T)
+        k0 = lo[0] * np.exp(lo[1] * np.log(T) - lo[2] / T)
+        Fcent = (1 - a) * np.exp(-T * rt3) + a * np.exp(-T * rt1) + np.exp(-t2 / T)
+        Pr = 10 ** (0.4 + 0.67 * np.log10(Fcent))
+        P = Pr * kinf / k0 * R * T
+        kf = m.rates(T, P, Y)["kf"][0]
+        Pr_act = (P / (R * T)) * k0 / kinf
+        expect = k0 * Fcent / (1 + Pr_act)
+        assert abs(kf - expect) <= 1e-10 * abs(expect), T
+
+
+def test_chemically_activated_jacobian_and_tables(oracle_mod, tmp_path):
+    from cantera_amd import mechc
+    mech = "chemically-activated-reaction"
+    out = tmp_path / "ca.mech"
+    mechc.main([os.path.join(GOLDEN, mech + ".yaml"), str(out)])
+    assert out.read_text() == open(mech_file(mech)).read()
+    m = oracle_mod.Mech(mech_file(mech))
+    Y = np.random.default_rng(6).random(m.K) + 0.05
+    Y /= Y.sum()
+    for T, P in ((800.0, 0.2 * ONE_ATM), (1500.0, 30 * ONE_ATM)):
+        y = np.concatenate([[0.2, T], Y])
+        Ja = m.jacobian(P, y, True)
+        N = m.K + 2
+        Jf = np.zeros((N, N))
+        for j in range(1, N):
+            d = max(abs(y[j]), 1e-8) * 1e-6
+            yp, ym = y.copy(), y.copy()
+            yp[j] += d
+            ym[j] -= d
+            Jf[:, j] = (m.rhs(P, yp) - m.rhs(P, ym)) / (2 * d)
+        scale = np.abs(Jf).max(axis=1, keepdims=True) + 1e-30 * np.abs(Jf).max()
+        err = np.abs(Ja - Jf)[1:] / scale[1:]
+        assert err[:, 2:].max() < 1e-6, (T, P)
