@@ -31,8 +31,9 @@ extern "C" {
 enum cg_precision {
     CG_FP64 = 0,   /* plain: everything in FP64 */
     CG_MIXED = 1,  /* FP32 iteration matrix / LU / solves, FP64 residual, state and step control */
-    CG_FP32 = 2    /* rates only: the rate constants' exp/log10/exp10 (Arrhenius, falloff) in FP32;
-                      thermo, equilibrium constants, mass action and wdot in FP64; FP64 I/O */
+    CG_FP32 = 2    /* rates only: the rate constants' exp/log10/exp10 (Arrhenius, Troe/Tsang/Lindemann
+                      falloff) in FP32; PLOG, Chebyshev and SRI constants, thermo, equilibrium
+                      constants, mass action and wdot in FP64; FP64 I/O */
 };
 
 /* ---- library ------------------------------------------------------------ */
