@@ -99,17 +99,18 @@ class Solution:
     def last_rates_kernel_ms(self) -> float:
         return float(lib().cg_rates_last_kernel_ms(self._h))
 
-    def reactor_jacobians(self, T, P, Y) -> np.ndarray:
+    def reactor_jacobians(self, T, P, Y, precision=CG_FP64) -> np.ndarray:
         """d(ydot)/dy of a constant-pressure reactor at n states, shape (n, K+2, K+2),
         state order [m, T, Y_0..Y_{K-1}] (as ``Reactor.jacobian`` orders it).
-        Y is species-major (K, n)."""
+        Y is species-major (K, n).  precision=CG_FP32 returns float32 (the FP64
+        assembly rounded at the store)."""
         T = np.ascontiguousarray(np.atleast_1d(T), dtype=np.float64)
         n = T.shape[0]
         P = np.ascontiguousarray(np.broadcast_to(np.asarray(P, dtype=np.float64), (n,)))
         Y = np.ascontiguousarray(np.asarray(Y, dtype=np.float64).reshape(self.n_species, n))
         N = self.n_species + 2
-        J = np.zeros((n, N, N))
-        check(lib().cg_jac_batch(self._h, n, T, P, Y, J, CG_FP64), "cg_jac_batch")
+        J = np.zeros((n, N, N), dtype=np.float32 if precision == CG_FP32 else np.float64)
+        check(lib().cg_jac_batch(self._h, n, T, P, Y, J.ctypes.data_as(C.c_void_p), int(precision)), "cg_jac_batch")
         return J
 
 
@@ -145,16 +146,42 @@ class ReactorBatch:
     def set_options(self, max_steps=0, waves_per_cu=-1):
         check(lib().cg_reactors_set_options(self._h, int(max_steps), int(waves_per_cu)))
 
-    def advance_surrogate(self, t, surrogate, dT_tol=0.05, rtol_g=1e-3, atol_g=1e-9) -> np.ndarray:
-        """advance() with guarded surrogate steps where t is a whole number of
-        surrogate steps away; returns [surrogate steps kept, reactors advanced
-        by the surrogate alone, reactors that ran the integrator]."""
+    # guard defaults: the end-to-end budget (1 K, 1e-4 in Y over 100 steps)
+    # divided by the number of steps; gate at 1500 K
+    GUARD = {"tol_T": 0.01, "tol_Y": 1e-6, "gate_T": 1500.0}
+
+    def advance_surrogate(self, t, surrogate, tol_T=GUARD["tol_T"], tol_Y=GUARD["tol_Y"],
+                          gate_T=GUARD["gate_T"]) -> np.ndarray:
+        """advance() with one guarded surrogate step for every reactor whose
+        output time is one surrogate step ahead; the rest (and every rejected
+        step) run the integrator.  Returns [eligible, kept, integrated]."""
         t_out = np.ascontiguousarray(np.broadcast_to(np.asarray(t, dtype=np.float64), (self.n,)))
         counts = (C.c_int64 * 3)()
-        check(lib().cg_reactors_advance_surrogate(self._h, surrogate.handle, t_out, float(dT_tol), float(rtol_g),
-                                                  float(atol_g), counts), "cg_reactors_advance_surrogate")
+        check(lib().cg_reactors_advance_surrogate(self._h, surrogate.handle, t_out, float(tol_T), float(tol_Y),
+                                                  float(gate_T), counts), "cg_reactors_advance_surrogate")
         self.time = t_out.copy()
         return np.array(counts[:], dtype=np.int64)
+
+    def advance_surrogate_device(self, t_dev, surrogate, row=0, tol_T=GUARD["tol_T"], tol_Y=GUARD["tol_Y"],
+                                 gate_T=GUARD["gate_T"]):
+        """Asynchronous advance_surrogate to the output times at t_dev (row `row`
+        of an n_steps x n device array); counts accumulate on the device."""
+        ptr = (t_dev if isinstance(t_dev, int) else t_dev.ptr) + row * self.n * 8
+        check(lib().cg_reactors_advance_surrogate_device(self._h, surrogate.handle, C.c_void_p(ptr), float(tol_T),
+                                                         float(tol_Y), float(gate_T)),
+              "cg_reactors_advance_surrogate_device")
+
+    def surrogate_counts(self, reset=False) -> np.ndarray:
+        """[eligible reactor-steps, kept, integrated] accumulated on the device."""
+        counts = (C.c_int64 * 3)()
+        check(lib().cg_reactors_surrogate_counts(self._h, counts, 1 if reset else 0), "cg_reactors_surrogate_counts")
+        return np.array(counts[:], dtype=np.int64)
+
+    def state_to_device(self, y_dev, row=0):
+        """Queue a copy of the states (n x (K+2), [m, T, Y]) into device memory
+        at row `row` of an n_rows x n x (K+2) array."""
+        ptr = (y_dev if isinstance(y_dev, int) else y_dev.ptr) + row * self.n * self.neq * 8
+        check(lib().cg_reactors_get_state_device(self._h, C.c_void_p(ptr)), "cg_reactors_get_state_device")
 
     def set_matrix_storage(self, storage):
         """'lds' or 'global' (or 'default') iteration-matrix storage for 32 < N <= 64."""

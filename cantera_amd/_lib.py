@@ -43,7 +43,7 @@ SIGNATURES = [
     ("cg_rates_batch", C.c_int32, [C.c_int32, C.c_int64, _dp, _dp, _dp, _dp, C.c_int32]),
     ("cg_rates_batch_device", C.c_int32, [C.c_int32, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p,
                                           C.c_void_p, C.c_int32, C.c_void_p]),
-    ("cg_jac_batch", C.c_int32, [C.c_int32, C.c_int64, _dp, _dp, _dp, _dp, C.c_int32]),
+    ("cg_jac_batch", C.c_int32, [C.c_int32, C.c_int64, _dp, _dp, _dp, C.c_void_p, C.c_int32]),
     ("cg_jac_batch_device", C.c_int32, [C.c_int32, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p,
                                         C.c_void_p, C.c_int32, C.c_void_p]),
     ("cg_jac_last_kernel_ms", C.c_double, [C.c_int32]),
@@ -80,6 +80,10 @@ SIGNATURES = [
     ("cg_surrogate_set_normalization", C.c_int32, [C.c_int32, C.POINTER(C.c_double)]),
     ("cg_reactors_advance_surrogate", C.c_int32, [C.c_int32, C.c_int32, _dp, C.c_double, C.c_double, C.c_double,
                                                   C.POINTER(C.c_int64)]),
+    ("cg_reactors_advance_surrogate_device", C.c_int32, [C.c_int32, C.c_int32, C.c_void_p, C.c_double, C.c_double,
+                                                         C.c_double]),
+    ("cg_reactors_surrogate_counts", C.c_int32, [C.c_int32, C.POINTER(C.c_int64), C.c_int32]),
+    ("cg_reactors_get_state_device", C.c_int32, [C.c_int32, C.c_void_p]),
 ]
 
 

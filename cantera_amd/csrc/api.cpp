@@ -80,6 +80,13 @@ struct Batch {
     int graph_steps = 0;
     const double* graph_tout = nullptr;
     const int* graph_order = nullptr;
+    // guarded surrogate steps: workspace allocated at the first such call
+    double* sg_Y = nullptr;          // n x N current states (reference order)
+    double* sg_Yn = nullptr;         // n x N predictions
+    double* sg_F = nullptr;          // n x N f(y) of the last kept state (internal order)
+    double* sg_P = nullptr;          // n
+    int* sg_ints = nullptr;          // elig (n), fvalid (n), list (n), count (1)
+    unsigned long long* sg_counters = nullptr;   // eligible, kept, integrated (accumulated)
     void drop_graph() {
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
         if (graph) (void)hipGraphDestroy(graph);
@@ -90,7 +97,9 @@ struct Batch {
     ~Batch() {
         drop_graph();
         for (void* p : {static_cast<void*>(rec), static_cast<void*>(jsave), lusave, static_cast<void*>(piv),
-                        static_cast<void*>(tout), static_cast<void*>(queue), static_cast<void*>(lpt), sort_tmp})
+                        static_cast<void*>(tout), static_cast<void*>(queue), static_cast<void*>(lpt), sort_tmp,
+                        static_cast<void*>(sg_Y), static_cast<void*>(sg_Yn), static_cast<void*>(sg_F),
+                        static_cast<void*>(sg_P), static_cast<void*>(sg_ints), static_cast<void*>(sg_counters)})
             if (p) (void)hipFree(p);
         if (e0) (void)hipEventDestroy(e0);
         if (e1) (void)hipEventDestroy(e1);
@@ -322,27 +331,30 @@ double cg_rates_last_kernel_ms(int32_t h) {
     }
 }
 
-int32_t cg_jac_batch_device(int32_t h, int64_t n, const double* T, const double* P, const double* Y, double* J,
+int32_t cg_jac_batch_device(int32_t h, int64_t n, const double* T, const double* P, const double* Y, void* J,
                             int32_t precision, void* stream) {
     return guarded([&] {
         MechObj& mo = get_mech(h);
-        if (precision != CG_FP64) throw std::runtime_error("cg_jac_batch: only CG_FP64 is available");
+        if (precision != CG_FP64 && precision != CG_FP32)
+            throw std::runtime_error("cg_jac_batch: precision must be CG_FP64 or CG_FP32");
+        const bool f32 = precision == CG_FP32;
+        if (f32 && mo.ns != 1) throw std::runtime_error("cg_jac_batch: CG_FP32 output needs K + 2 <= 64");
         if (n <= 0) return 0;
         if (!T || !P || !Y || !J) throw std::runtime_error("null array");
         const DevMech& m = mo.dm->view();
         const size_t lds = jac_lds_bytes(m, mo.ns);
-        const void* kp = jac_kernel_ptr(mo.ns);
+        const void* kp = jac_kernel_ptr(mo.ns, f32);
         set_lds_limit(kp, lds);
         const int grid = grid_for(kp, lds, 0, n);
         hipStream_t s = static_cast<hipStream_t>(stream);
         check(hipEventRecord(mo.j0, s), "hipEventRecord");
-        check(jac_launch(m, mo.ns, n, T, P, Y, J, grid, lds, s), "Jacobian kernel launch");
+        check(jac_launch(m, mo.ns, f32, n, T, P, Y, J, grid, lds, s), "Jacobian kernel launch");
         check(hipEventRecord(mo.j1, s), "hipEventRecord");
         return 0;
     });
 }
 
-int32_t cg_jac_batch(int32_t h, int64_t n, const double* T, const double* P, const double* Y, double* J,
+int32_t cg_jac_batch(int32_t h, int64_t n, const double* T, const double* P, const double* Y, void* J,
                      int32_t precision) {
     return guarded([&] {
         MechObj& mo = get_mech(h);
@@ -350,7 +362,7 @@ int32_t cg_jac_batch(int32_t h, int64_t n, const double* T, const double* P, con
         if (!T || !P || !Y || !J) throw std::runtime_error("null array");
         const int K = mo.dm->view().K, N = mo.dm->view().N;
         const size_t bytes = static_cast<size_t>(n) * sizeof(double);
-        const size_t jbytes = bytes * N * N;
+        const size_t jbytes = static_cast<size_t>(n) * N * N * (precision == CG_FP32 ? 4 : 8);
         double* d = nullptr;
         check(hipMalloc(&d, bytes * (2 + static_cast<size_t>(K)) + jbytes), "hipMalloc");
         std::unique_ptr<double, void (*)(double*)> guard(d, [](double* p) { (void)hipFree(p); });
@@ -387,6 +399,17 @@ static int32_t create_batch(int32_t h, int64_t n, const double* T0, const double
         if (precision != CG_FP64 && precision != CG_MIXED) throw std::runtime_error("unknown precision");
         const DevMech& m = mo.dm->view();
         const HostMech& hm = mo.dm->host();
+        // Phase::setTemperature / Phase::setDensity reject non-positive values
+        // (Phase.cpp); non-finite inputs are rejected here too
+        for (int64_t s = 0; s < n; s++) {
+            if (!(T0[s] > 0.0) || !std::isfinite(T0[s]))
+                throw std::runtime_error("reactor " + std::to_string(s) + ": temperature must be positive and finite");
+            if (!(P[s] > 0.0) || !std::isfinite(P[s]))
+                throw std::runtime_error("reactor " + std::to_string(s) + ": pressure must be positive and finite");
+            for (int k = 0; k < m.K; k++)
+                if (!std::isfinite(Y0[s * m.K + k]))
+                    throw std::runtime_error("reactor " + std::to_string(s) + ": non-finite mass fraction");
+        }
         auto b = std::make_unique<Batch>();
         b->mech = &mo;
         b->n = static_cast<int>(n);
@@ -868,109 +891,112 @@ int32_t cg_surrogate_set_normalization(int32_t s, const double* v) {
     return guarded([&] { get_surr(s).net->set_normalization(v); return 0; });
 }
 
-int32_t cg_reactors_advance_surrogate(int32_t bh, int32_t sh, const double* t_out, double dT_tol, double rtol_g,
-                                      double atol_g, int64_t* counts) {
+// guarded surrogate step for every reactor whose output time is one step dt
+// ahead; the rest are integrated.  Stream-ordered on the batch stream.
+static void advance_surrogate_device(Batch& b, SurrogateObj& so, const double* t_out_dev, double tol_T, double tol_Y,
+                                     double gate_T) {
+    if (so.mech != b.mech) throw std::runtime_error("surrogate and reactor batch use different mechanisms");
+    if (b.const_volume) throw std::runtime_error("the surrogate path supports constant-pressure reactors only");
+    if (!t_out_dev) throw std::runtime_error("null t_out");
+    if (!(tol_T > 0) || !(tol_Y > 0)) throw std::runtime_error("guard tolerances must be positive");
+    const int n = b.n, N = b.N, K = b.K, NP = 64 * b.ns;
+    const size_t nn = static_cast<size_t>(n);
+    if (!b.sg_Y) {
+        check(hipMalloc(&b.sg_Y, nn * N * 8), "hipMalloc(surrogate states)");
+        check(hipMalloc(&b.sg_Yn, nn * N * 8), "hipMalloc(surrogate predictions)");
+        check(hipMalloc(&b.sg_F, nn * N * 8), "hipMalloc(surrogate rhs)");
+        check(hipMalloc(&b.sg_P, nn * 8), "hipMalloc(surrogate pressures)");
+        check(hipMalloc(&b.sg_ints, (3 * nn + 1) * 4), "hipMalloc(surrogate lists)");
+        check(hipMalloc(&b.sg_counters, 3 * sizeof(unsigned long long)), "hipMalloc(surrogate counters)");
+        check(surrogate_clear(b.sg_ints, 3 * n + 1, b.stream), "surrogate clear");
+        check(surrogate_clear(reinterpret_cast<int*>(b.sg_counters), 6, b.stream), "surrogate clear");
+    }
+    int* elig = b.sg_ints;
+    int* fvalid = b.sg_ints + nn;
+    int* list = b.sg_ints + 2 * nn;
+    int* count = b.sg_ints + 3 * nn;
+    const double dt = so.net->dt();
+    const DevMech& m = b.mech->dm->view();
+    const HostMech& hm = b.mech->dm->host();
+    int rad[3];
+    const char* names[3] = {"H", "O", "OH"};
+    for (int j = 0; j < 3; j++) {
+        rad[j] = -1;
+        for (int k = 0; k < K; k++)
+            if (hm.names[k] == names[j]) rad[j] = k;
+    }
+    if (rad[0] < 0 || rad[1] < 0 || rad[2] < 0) rad[0] = -1;   // no gate species: every step integrated
+    if (t_out_dev != b.tout)
+        check(hipMemcpyAsync(b.tout, t_out_dev, nn * 8, hipMemcpyDeviceToDevice, b.stream), "hipMemcpyAsync(tout)");
+    check(surrogate_clear(count, 1, b.stream), "surrogate clear");
+    check(surrogate_gather(n, b.rec, b.rec_stride, NP, N, K, b.tout, dt, b.sg_Y, b.sg_P, elig, list, count,
+                           b.sg_counters, b.stream), "surrogate gather");
+    so.net->predict(n, b.sg_Y, b.sg_P, b.sg_Yn, b.stream);
+    const size_t lds = guard_lds_bytes(m, b.ns);
+    const void* gk = guard_kernel_ptr(b.ns);
+    set_lds_limit(gk, lds);
+    const int grid = grid_for(gk, lds, 0, n);
+    check(surrogate_guard(m, b.ns, n, elig, b.sg_Y, b.sg_Yn, b.sg_F, fvalid, b.sg_P, b.rec, b.rec_stride, NP, b.tout,
+                          dt, tol_T, tol_Y, gate_T, rad, list, count, b.sg_counters, grid, lds, b.stream),
+          "surrogate guard");
+    BdfArgs a = advance_args(b, list, n);
+    a.n_dev = count;
+    check(hipEventRecord(b.e0, b.stream), "hipEventRecord");
+    check(bdf_launch(a, b.fp32, b.global_matrix, advance_grid(b), b.lds, b.stream), "integrator kernel launch");
+    check(hipEventRecord(b.e1, b.stream), "hipEventRecord");
+}
+
+int32_t cg_reactors_advance_surrogate_device(int32_t bh, int32_t sh, const double* t_out_dev, double tol_T,
+                                             double tol_Y, double gate_T) {
+    return guarded([&] {
+        advance_surrogate_device(get_batch(bh), get_surr(sh), t_out_dev, tol_T, tol_Y, gate_T);
+        return 0;
+    });
+}
+
+int32_t cg_reactors_get_state_device(int32_t bh, double* y_dev) {
+    return guarded([&] {
+        Batch& b = get_batch(bh);
+        if (!y_dev) throw std::runtime_error("null state array");
+        check(export_states(b.n, b.rec, b.rec_stride, 64 * b.ns, b.N, b.K, y_dev, b.stream), "state export");
+        return 0;
+    });
+}
+
+int32_t cg_reactors_surrogate_counts(int32_t bh, int64_t* counts, int32_t reset) {
+    return guarded([&] {
+        Batch& b = get_batch(bh);
+        unsigned long long c[3] = {0, 0, 0};
+        if (b.sg_counters) {
+            check(hipMemcpyAsync(c, b.sg_counters, sizeof(c), hipMemcpyDeviceToHost, b.stream), "hipMemcpyAsync");
+            check(hipStreamSynchronize(b.stream), "hipStreamSynchronize");
+            if (reset) check(surrogate_clear(reinterpret_cast<int*>(b.sg_counters), 6, b.stream), "surrogate clear");
+        }
+        if (counts)
+            for (int j = 0; j < 3; j++) counts[j] = static_cast<int64_t>(c[j]);
+        return 0;
+    });
+}
+
+int32_t cg_reactors_advance_surrogate(int32_t bh, int32_t sh, const double* t_out, double tol_T, double tol_Y,
+                                      double gate_T, int64_t* counts) {
     return guarded([&] {
         Batch& b = get_batch(bh);
         SurrogateObj& so = get_surr(sh);
-        if (so.mech != b.mech) throw std::runtime_error("surrogate and reactor batch use different mechanisms");
-        if (b.const_volume) throw std::runtime_error("the surrogate path supports constant-pressure reactors only");
         if (!t_out) throw std::runtime_error("null t_out");
-        if (!(dT_tol > 0) || !(rtol_g > 0) || !(atol_g > 0)) throw std::runtime_error("guard tolerances must be positive");
-        const int n = b.n, N = b.N, K = b.K, NP = 64 * b.ns;
-        const double dt = so.net->dt();
+        unsigned long long before[3] = {0, 0, 0};
+        if (b.sg_counters) {
+            check(hipMemcpyAsync(before, b.sg_counters, sizeof(before), hipMemcpyDeviceToHost, b.stream), "hipMemcpyAsync");
+        }
+        // t_out goes through the batch's own time buffer (copied in stream order)
+        check(hipMemcpyAsync(b.tout, t_out, static_cast<size_t>(b.n) * 8, hipMemcpyHostToDevice, b.stream),
+              "hipMemcpyAsync(t_out)");
+        advance_surrogate_device(b, so, b.tout, tol_T, tol_Y, gate_T);
+        unsigned long long after[3];
+        check(hipMemcpyAsync(after, b.sg_counters, sizeof(after), hipMemcpyDeviceToHost, b.stream), "hipMemcpyAsync");
         check(hipStreamSynchronize(b.stream), "hipStreamSynchronize");
-        std::vector<double> tu(n), st(n);
-        check(hipMemcpy2D(tu.data(), 8, b.rec + 7 * NP + C_TUSER, b.rec_stride * 8, 8, n, hipMemcpyDeviceToHost),
-              "hipMemcpy2D(time)");
-        check(hipMemcpy2D(st.data(), 8, b.rec + 7 * NP + C_STATUS, b.rec_stride * 8, 8, n, hipMemcpyDeviceToHost),
-              "hipMemcpy2D(status)");
-        std::vector<int> list, nsub, bdf;
-        std::vector<double> t0;
-        for (int r = 0; r < n; r++) {
-            if (st[r] < 0) continue;
-            const double span = t_out[r] - tu[r];
-            if (span < 0) throw std::runtime_error("t_out before the current time");
-            const long long k = std::llround(span / dt);
-            if (k >= 1 && std::fabs(k * dt - span) <= 1e-9 * span) {
-                list.push_back(r);
-                nsub.push_back(static_cast<int>(k));
-                t0.push_back(tu[r]);
-            } else if (span > 0) {
-                bdf.push_back(r);
-            }
-        }
-        const int na = static_cast<int>(list.size());
-        int64_t accepted = 0, whole = 0;
-        if (na > 0) {
-            const size_t rows = static_cast<size_t>(na);
-            std::vector<void*> mem;
-            auto dalloc = [&](size_t bytes) {
-                void* p = nullptr;
-                check(hipMalloc(&p, bytes), "hipMalloc(surrogate step)");
-                mem.push_back(p);
-                return p;
-            };
-            struct Free {
-                std::vector<void*>& m;
-                ~Free() { for (void* p : m) (void)hipFree(p); }
-            } fr{mem};
-            int* d_list = static_cast<int*>(dalloc(rows * 4));
-            int* d_nsub = static_cast<int*>(dalloc(rows * 4));
-            int* d_done = static_cast<int*>(dalloc(rows * 4));
-            int* d_rej = static_cast<int*>(dalloc(rows * 4));
-            double* d_t0 = static_cast<double*>(dalloc(rows * 8));
-            double* d_P = static_cast<double*>(dalloc(rows * 8));
-            double* d_Y = static_cast<double*>(dalloc(rows * N * 8));
-            double* d_Yn = static_cast<double*>(dalloc(rows * N * 8));
-            double* d_F = static_cast<double*>(dalloc(rows * N * 8));
-            check(hipMemcpy(d_list, list.data(), rows * 4, hipMemcpyHostToDevice), "hipMemcpy");
-            check(hipMemcpy(d_nsub, nsub.data(), rows * 4, hipMemcpyHostToDevice), "hipMemcpy");
-            check(hipMemcpy(d_t0, t0.data(), rows * 8, hipMemcpyHostToDevice), "hipMemcpy");
-            check(hipMemset(d_done, 0, rows * 4), "hipMemset");
-            check(hipMemset(d_rej, 0, rows * 4), "hipMemset");
-            check(surrogate_gather(na, d_list, b.rec, b.rec_stride, NP, N, K, d_Y, d_P, b.stream), "surrogate gather");
-            const DevMech& m = b.mech->dm->view();
-            const size_t lds = guard_lds_bytes(m, b.ns);
-            const void* gk = guard_kernel_ptr(b.ns);
-            set_lds_limit(gk, lds);
-            const int grid = grid_for(gk, lds, 0, na);
-            const int maxsub = *std::max_element(nsub.begin(), nsub.end());
-            check(hipEventRecord(b.e0, b.stream), "hipEventRecord");
-            for (int k = 0; k < maxsub; k++) {
-                so.net->predict(na, d_Y, d_P, d_Yn, b.stream);
-                check(surrogate_guard(m, b.ns, na, d_nsub, d_done, d_rej, d_Y, d_Yn, d_F, d_P, dt, dT_tol, rtol_g, atol_g,
-                                      grid, lds, b.stream),
-                      "surrogate guard");
-            }
-            check(surrogate_restart(na, d_list, d_done, d_t0, dt, d_Y, b.rec, b.rec_stride, NP, N, K, b.stream),
-                  "surrogate restart");
-            check(hipEventRecord(b.e1, b.stream), "hipEventRecord");
-            std::vector<int> done(na);
-            check(hipMemcpyAsync(done.data(), d_done, rows * 4, hipMemcpyDeviceToHost, b.stream), "hipMemcpyAsync");
-            check(hipStreamSynchronize(b.stream), "hipStreamSynchronize");
-            for (int i = 0; i < na; i++) {
-                accepted += done[i];
-                if (done[i] == nsub[i]) whole++;
-                else bdf.push_back(list[i]);
-            }
-        }
-        if (!bdf.empty()) {
-            int* d_bdf = nullptr;
-            check(hipMalloc(&d_bdf, bdf.size() * 4), "hipMalloc(fallback list)");
-            std::unique_ptr<int, void (*)(int*)> g(d_bdf, [](int* p) { (void)hipFree(p); });
-            check(hipMemcpyAsync(b.tout, t_out, static_cast<size_t>(n) * 8, hipMemcpyHostToDevice, b.stream),
-                  "hipMemcpyAsync(tout)");
-            check(hipMemcpyAsync(d_bdf, bdf.data(), bdf.size() * 4, hipMemcpyHostToDevice, b.stream),
-                  "hipMemcpyAsync(list)");
-            launch_advance(b, d_bdf, static_cast<int>(bdf.size()));
-            check(hipStreamSynchronize(b.stream), "hipStreamSynchronize");
-        }
-        if (counts) {
-            counts[0] = accepted;
-            counts[1] = whole;
-            counts[2] = static_cast<int64_t>(bdf.size());
-        }
+        if (counts)
+            for (int j = 0; j < 3; j++) counts[j] = static_cast<int64_t>(after[j] - before[j]);
         int bad = cg_reactors_status(bh, nullptr);
         if (bad < 0) throw std::runtime_error(g_error);
         if (bad > 0) throw std::runtime_error(std::to_string(bad) + " reactor(s) failed to integrate; see cg_reactors_status");

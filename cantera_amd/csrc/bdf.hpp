@@ -35,6 +35,7 @@ struct BdfArgs {
     size_t piv_stride;
     const double* tout;   // n, device
     const int* order;     // optional processing order (n), device
+    const int* n_dev;     // optional: number of work items, read on the device (replaces n)
     int* queue;           // device work-queue head
     // typed view used by the kernel
     template <class T> __host__ __device__ T* lu() const { return static_cast<T*>(lu_save_raw); }
@@ -66,21 +67,25 @@ size_t order_temp_bytes(int n);
 hipError_t order_update(const double* rec, size_t rec_stride, int ctl_off, int n, int* prev_nst, int* bucket,
                         int* rank, int* order_out, void* temp, hipStream_t stream);
 
-// guarded surrogate steps for reactor batches
+// guarded surrogate steps for reactor batches (all on the device)
 size_t guard_lds_bytes(const DevMech& m, int ns);
 const void* guard_kernel_ptr(int ns);
-hipError_t surrogate_gather(int n, const int* list, const double* rec, size_t stride, int NP, int N, int K, double* Y,
-                            double* P, hipStream_t s);
-hipError_t surrogate_guard(const DevMech& m, int ns, int n, const int* nsub, int* done, int* rejected, double* Ycur,
-                           const double* Ynext, double* Fcur, const double* Pa, double dt, double dT_tol,
-                           double rtol_g, double atol_g, int grid, size_t lds, hipStream_t s);
-hipError_t surrogate_restart(int n, const int* list, const int* done, const double* t0, double dt, const double* Y,
-                             double* rec, size_t stride, int NP, int N, int K, hipStream_t s);
+hipError_t surrogate_clear(int* p, int n, hipStream_t s);
+// output states of all reactors, reference order [m, T, Y], n x N doubles
+hipError_t export_states(int n, const double* rec, size_t stride, int NP, int N, int K, double* Y, hipStream_t s);
+hipError_t surrogate_gather(int n, const double* rec, size_t stride, int NP, int N, int K, const double* tout,
+                            double dt, double* Y, double* P, int* elig, int* list, int* count,
+                            unsigned long long* counters, hipStream_t s);
+hipError_t surrogate_guard(const DevMech& m, int ns, int n, const int* elig, const double* Ycur, const double* Ynext,
+                           double* Fcur, int* fvalid, const double* P, double* rec, size_t stride, int NP,
+                           const double* tout, double dt, double tol_T, double tol_Y, double gate_T, const int* rad,
+                           int* list, int* count, unsigned long long* counters, int grid, size_t lds, hipStream_t s);
 
 // batched reactor Jacobians (inputs as the rates, output N x N per state)
 size_t jac_lds_bytes(const DevMech& m, int ns);
-hipError_t jac_launch(const DevMech& m, int ns, long n, const double* T, const double* P, const double* Y, double* J,
-                      int grid, size_t lds, hipStream_t stream);
-const void* jac_kernel_ptr(int ns);
+// f32: FP32 output (the FP64 assembly rounded at the store; needs ns == 1)
+hipError_t jac_launch(const DevMech& m, int ns, bool f32, long n, const double* T, const double* P, const double* Y,
+                      void* J, int grid, size_t lds, hipStream_t stream);
+const void* jac_kernel_ptr(int ns, bool f32);
 
 }  // namespace cg

@@ -9,6 +9,10 @@
 // ACM TOMS 31 (2005) and the CVODE user guide).  advance(tout) steps past tout
 // and interpolates (CVodesIntegrator::integrate, :479-529); history, the saved
 // Jacobian and the factored iteration matrix persist between calls.
+// The step-control constants and the names of the step helpers (predict,
+// rescale, set_eta, prepare_next_step, ...) follow SUNDIALS CVODE (cvode.c,
+// Copyright (c) 2002-2025, Lawrence Livermore National Security and Southern
+// Methodist University; BSD-3-Clause).
 //
 // Layout: the Nordsieck history and work vectors are lane-distributed
 // registers (component i in lane i % 64, slot i / 64); the step-control block
@@ -902,11 +906,12 @@ __device__ __forceinline__ void bdf_advance_body(const BdfArgs& a) {
     it.rtol = a.rtol;
     it.atol = a.atol;
     it.ls.init(base, a.ld);
+    const int nwork = a.n_dev ? uni(*a.n_dev) : a.n;
     for (;;) {
         int r = 0;
         if (it.lane == 0) r = atomicAdd(a.queue, 1);
         r = uni(__shfl(r, 0, kWave));
-        if (r >= a.n) break;
+        if (r >= nwork) break;
         const int rid = a.order ? uni(a.order[r]) : r;
         double* rec = a.rec + static_cast<size_t>(rid) * a.rec_stride;
         const double status = uni(rec[7 * kWave * NS + C_STATUS]);

@@ -24,10 +24,12 @@ namespace {
 // (L2 atomics) -- used when N x N doubles do not fit next to the scratch.
 // The mechanism is the first parameter: rhs_call reads it through
 // kernarg_mech().
-template <int NS, bool kLdsJ>
+// OT: element type of the output (double, or float for CG_FP32: the FP64
+// assembly rounded once at the store, which halves the HBM bytes written).
+template <int NS, bool kLdsJ, typename OT>
 __global__ __launch_bounds__(64) void jac_batch_kernel(DevMech m, long n, const double* __restrict__ T,
                                                        const double* __restrict__ P, const double* __restrict__ Y,
-                                                       double* __restrict__ J) {
+                                                       OT* __restrict__ J) {
     const int lane = lane_id();
     ScratchLayout L;
     int off = 0;
@@ -65,12 +67,13 @@ __global__ __launch_bounds__(64) void jac_batch_kernel(DevMech m, long n, const 
             f[s] = (i < m.N) ? S.F[i] : 0.0;
         }
         wave_sync();
-        double* Jg = J + static_cast<size_t>(st) * NN;
-        if (kLdsJ) {
+        OT* Jg = J + static_cast<size_t>(st) * NN;
+        if constexpr (kLdsJ) {
             reactor_jacobian<NS, true>(m, S, Ps, y, f, Jl);
-            for (int e = lane; e < NN; e += kWave) Jg[e] = Jl[e];
+            for (int e = lane; e < NN; e += kWave) Jg[e] = static_cast<OT>(Jl[e]);
             wave_sync();
         } else {
+            static_assert(sizeof(OT) == sizeof(double), "in-place assembly needs a double output");
             reactor_jacobian<NS, true>(m, S, Ps, y, f, Jg);
         }
     }
@@ -87,17 +90,25 @@ size_t jac_lds_bytes(const DevMech& m, int ns) {
     return d * sizeof(double);
 }
 
-hipError_t jac_launch(const DevMech& m, int ns, long n, const double* T, const double* P, const double* Y, double* J,
-                      int grid, size_t lds, hipStream_t stream) {
-    if (ns == 1) hipLaunchKernelGGL((jac_batch_kernel<1, true>), dim3(grid), dim3(64), lds, stream, m, n, T, P, Y, J);
-    else if (ns == 2) hipLaunchKernelGGL((jac_batch_kernel<2, false>), dim3(grid), dim3(64), lds, stream, m, n, T, P, Y, J);
+hipError_t jac_launch(const DevMech& m, int ns, bool f32, long n, const double* T, const double* P, const double* Y,
+                      void* J, int grid, size_t lds, hipStream_t stream) {
+    if (ns == 1 && f32)
+        hipLaunchKernelGGL((jac_batch_kernel<1, true, float>), dim3(grid), dim3(64), lds, stream, m, n, T, P, Y,
+                           static_cast<float*>(J));
+    else if (ns == 1)
+        hipLaunchKernelGGL((jac_batch_kernel<1, true, double>), dim3(grid), dim3(64), lds, stream, m, n, T, P, Y,
+                           static_cast<double*>(J));
+    else if (ns == 2 && !f32)
+        hipLaunchKernelGGL((jac_batch_kernel<2, false, double>), dim3(grid), dim3(64), lds, stream, m, n, T, P, Y,
+                           static_cast<double*>(J));
     else return hipErrorInvalidValue;
     return hipGetLastError();
 }
 
-const void* jac_kernel_ptr(int ns) {
-    return ns == 1 ? reinterpret_cast<const void*>(&jac_batch_kernel<1, true>)
-                   : reinterpret_cast<const void*>(&jac_batch_kernel<2, false>);
+const void* jac_kernel_ptr(int ns, bool f32) {
+    if (ns == 1) return f32 ? reinterpret_cast<const void*>(&jac_batch_kernel<1, true, float>)
+                            : reinterpret_cast<const void*>(&jac_batch_kernel<1, true, double>);
+    return reinterpret_cast<const void*>(&jac_batch_kernel<2, false, double>);
 }
 
 }  // namespace cg

@@ -44,7 +44,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void ra
         }
         GasState g;
         SpeciesRegs<NS> sr;
-        species_pass<NS>(m, S, Ts, Ps, yk, g, sr);
+        species_pass<NS, true>(m, S, Ts, Ps, yk, g, sr);
         production_rates<NS, kF32>(m, S, g, sr);
 #pragma unroll
         for (int s = 0; s < NS; s++) {

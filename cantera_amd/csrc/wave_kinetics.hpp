@@ -205,6 +205,15 @@ __device__ __forceinline__ WaveScratch scratch_view(const ScratchLayout& L) {
 
 // Scalar thermodynamic state derived from (T, P, Y): setMassFractions_NoNorm +
 // setState_TP (Phase.cpp:362-372; ThermoPhase.cpp:187-198; IdealGasPhase.h:338-354).
+// c + a*b with the product rounded before the sum (no fused multiply-add):
+// [M] = d C_tot + sum_k (e_k - d) C_k is accumulated exactly as the reference
+// does (ThirdBodyCalc::update, ThirdBodyCalc.h:75-83), so a nearly cancelling
+// [M] (a bath of an efficiency-0 species) rounds the same way.
+__device__ __forceinline__ double add_mul_rn(double c, double a, double b) {
+#pragma clang fp contract(off)
+    return c + a * b;
+}
+
 struct GasState {
     double T, P, mmw, RT, dens, ctot, pressure, logT, recipT, rrt, logC0, tmp;
 };
@@ -400,9 +409,9 @@ __device__ __forceinline__ void eval_reaction(const DevMech& m, int p, const Gas
             for (int b = 0; b < 4; b++) cc[b] = C[kk[b]];
 #pragma unroll
             for (int b = 0; b < 4; b++)
-                if (j0 + b < x.tbn) sum += ee[b] * cc[b];
+                if (j0 + b < x.tbn) sum = add_mul_rn(sum, ee[b], cc[b]);
         }
-        M = m.tbdef[p] * g.ctot + sum;
+        M = add_mul_rn(sum, m.tbdef[p], g.ctot);
     }
     double kf;
     if (type <= kThreeBody) {
@@ -526,9 +535,13 @@ __device__ __forceinline__ void eval_reaction(const DevMech& m, int p, const Gas
 
 // Gas state + per-species thermo, concentrations and chemical potentials.
 // Y values: lane owns species lane + 64 s in yk[s].  Leaves C and MU in LDS.
-template <int NS>
 // rho > 0: the state is (T, rho) instead of (T, P) (ThermoPhase::setState_TD);
 // the pressure is then R T rho / Wbar.
+// kSeqSum: 1/Wbar = sum_k Y_k/W_k summed in species order, as
+// Phase::setMassFractions_NoNorm does (Phase.cpp:362-372), through S.WD (the
+// batched rates and Jacobians, whose results are compared bit for bit);
+// otherwise a wavefront tree sum (the integrator's right-hand side).
+template <int NS, bool kSeqSum = false>
 __device__ __forceinline__ void species_pass(const DevMech& m, const WaveScratch& S, double T, double P,
                                              const double (&yk)[NS], GasState& g, SpeciesRegs<NS>& sr,
                                              double rho = 0.0) {
@@ -540,7 +553,19 @@ __device__ __forceinline__ void species_pass(const DevMech& m, const WaveScratch
         sr.ym[s] = (k < m.K) ? yk[s] * m.rmw[k] : 0.0;
         sum += sr.ym[s];
     }
-    sum = uni(wave_sum(sum));
+    if constexpr (kSeqSum) {
+#pragma unroll
+        for (int s = 0; s < NS; s++) {
+            const int k = lane + kWave * s;
+            if (k < m.K) S.WD[k] = sr.ym[s];
+        }
+        wave_sync();
+        sum = 0.0;
+        for (int k = 0; k < m.K; k++) sum += S.WD[k];
+        wave_sync();
+    } else {
+        sum = uni(wave_sum(sum));
+    }
     g.T = T;
     g.mmw = 1.0 / sum;
     g.RT = T * kGasConstant;
@@ -707,6 +732,56 @@ __device__ __noinline__ void rhs_call(KernargMech* mech, ScratchLayout L, double
     for (int s = 0; s < NS; s++) {
         const int i = lane + kWave * s;
         if (i < m.N) S.F[i] = f[s];
+    }
+    wave_sync();
+}
+
+// Loss frequencies lambda_k = d_k / C_k [1/s] at the state in S.yb (internal
+// order, pressure P), left in S.W2[0..K): d_k = sum_i nu'_ki ropf_i +
+// nu''_ki ropr_i is the destruction rate of species k
+// (Kinetics::getDestructionRates, src/kinetics/Kinetics.cpp:435-445).  Used
+// by the surrogate guard to turn a defect of species k into an error estimate.
+template <int NS>
+__device__ __noinline__ void loss_frequency_call(KernargMech* mech, ScratchLayout L, double P) {
+    const DevMech m = *(const DevMech*)mech;
+    const WaveScratch S = scratch_view(L);
+    const int lane = lane_id();
+    double y[NS];
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        const int i = lane + kWave * s;
+        y[s] = (i < m.K) ? S.yb[i] : 0.0;
+    }
+    const double T = S.yb[m.K];
+    GasState g;
+    SpeciesRegs<NS> sr;
+    species_pass<NS>(m, S, T, P, y, g, sr);
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        const int k = lane + kWave * s;
+        if (k < m.K) S.WD[k] = 0.0;
+    }
+    wave_sync();
+    const double kpd = pdep_rate_lane(m, g);
+    for (int p = lane; p < m.I; p += kWave) {
+        RxnVals v;
+        RxnRegs x;
+        load_rxn(m, p, x);
+        eval_reaction(m, p, g, S.C, S.MU, x, kpd, v);
+        const double* gs = (x.gen >= 0) ? m.gend + 12 * x.gen : m.gend;
+        const bool rgen = x.flags & 8, pgen = x.flags & 16;
+#pragma unroll
+        for (int j = 0; j < kMaxSide; j++)
+            if (j < x.nr) atomicAdd(&S.WD[x.rk[j]], (rgen ? gs[4 + j] : 1.0) * v.ropf);
+#pragma unroll
+        for (int j = 0; j < kMaxSide; j++)
+            if (j < x.np) atomicAdd(&S.WD[x.pk[j]], (pgen ? gs[8 + j] : 1.0) * v.ropr);
+    }
+    wave_sync();
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        const int k = lane + kWave * s;
+        if (k < m.K) S.W2[k] = S.WD[k] / fmax(S.C[k], 1e-300);
     }
     wave_sync();
 }

@@ -151,41 +151,59 @@ def _energy_units_factor(u: str) -> tuple[float, str]:
 
 @dataclass
 class UnitSystem:
+    """Default units of a YAML file or of one reaction (``UnitSystem``,
+    ``Units.cpp:431-522``): factors to SI for bare numbers."""
     length: float = 1.0
     quantity: float = 1.0
     time: float = 1.0
+    energy: float = 1.0
+    pressure: float = 1.0
     ea_factor: float = 1.0      # multiplies an Ea value to J/kmol (or R for K)
+    explicit_ea: bool = False   # activation-energy given, not derived from energy/quantity
 
     @classmethod
     def from_node(cls, node: dict | None) -> "UnitSystem":
-        us = cls()
-        node = node or {}
-        if "length" in node:
-            us.length = _LENGTH[node["length"]]
-        if "quantity" in node:
-            us.quantity = _QUANTITY[node["quantity"]]
-        if "time" in node:
-            us.time = _TIME[node["time"]]
-        energy = _ENERGY[node.get("energy", "J")]
-        if "activation-energy" in node:
-            f, kind = _energy_units_factor(node["activation-energy"])
-            us.ea_factor = GAS_CONSTANT if node["activation-energy"] == "K" else f
-        else:
-            # m_activation_energy_factor = m_energy_factor / m_quantity_factor
-            us.ea_factor = energy / us.quantity
-        return us
+        return cls().merged(node or {}, force=True)
 
-    def merged(self, node: dict | None) -> "UnitSystem":
-        """This system with a reaction's local ``units`` overrides."""
-        if not node:
+    def merged(self, node: dict | None, force: bool = False) -> "UnitSystem":
+        """This system with the overrides of a ``units`` map applied
+        (``UnitSystem::setDefaults(map)``, ``Units.cpp:475-522``): a reaction's
+        own ``units`` entry applies to that reaction's values only.  Without an
+        explicit ``activation-energy``, Ea takes energy / quantity."""
+        if not node and not force:
             return self
-        us = UnitSystem(self.length, self.quantity, self.time, self.ea_factor)
-        if "length" in node:
-            us.length = _LENGTH[node["length"]]
-        if "quantity" in node:
-            us.quantity = _QUANTITY[node["quantity"]]
-        if "time" in node:
-            us.time = _TIME[node["time"]]
+        node = node or {}
+        us = UnitSystem(self.length, self.quantity, self.time, self.energy, self.pressure, self.ea_factor,
+                        self.explicit_ea)
+        for key in node:
+            if key not in ("length", "quantity", "time", "energy", "pressure", "activation-energy", "mass",
+                           "temperature", "current"):
+                raise MechanismError(f"unsupported units entry {key!r}")
+        try:
+            if "length" in node:
+                us.length = _LENGTH[node["length"]]
+            if "quantity" in node:
+                us.quantity = _QUANTITY[node["quantity"]]
+            if "time" in node:
+                us.time = _TIME[node["time"]]
+            if "energy" in node:
+                us.energy = _ENERGY[node["energy"]]
+            if "pressure" in node:
+                us.pressure = _PRESSURE[node["pressure"]]
+        except KeyError as e:
+            raise MechanismError(f"unsupported unit {e.args[0]!r}") from None
+        if node.get("temperature", "K") != "K":
+            raise MechanismError("temperature scales other than K are not supported")
+        if "mass" in node and node["mass"] != "kg":
+            raise MechanismError("mass units other than kg are not supported")
+        if "activation-energy" in node:
+            ea = node["activation-energy"]
+            f, kind = _energy_units_factor(ea)
+            us.ea_factor = GAS_CONSTANT if ea == "K" else f
+            us.explicit_ea = True
+        elif not us.explicit_ea:
+            # m_activation_energy_factor = m_energy_factor / m_quantity_factor
+            us.ea_factor = us.energy / us.quantity
         return us
 
     def rate_coeff(self, value, order: float) -> float:
@@ -237,10 +255,10 @@ def _rate_units_factor(u: str) -> float:
     return factor
 
 
-def _pressure(value) -> float:
+def _pressure(value, units: "UnitSystem | None" = None) -> float:
     v, u = _split_value_units(value)
     if not u:
-        return v
+        return v * (units.pressure if units is not None else 1.0)
     if u not in _PRESSURE:
         raise MechanismError(f"unsupported pressure unit {u!r}")
     return v * _PRESSURE[u]
@@ -526,6 +544,9 @@ def compile_mechanism(path: str, phase_name: str | None = None, search_dirs=()):
 
 def _compile_reaction(node, index, units: UnitSystem) -> Reaction:
     eq = node["equation"]
+    # a reaction's own `units` map overrides the file defaults for every value
+    # of this reaction (AnyMap::applyUnits on the reaction node)
+    units = units.merged(node.get("units"))
     rate_type = node.get("type", "")
     if rate_type in ("", "reaction"):
         rate_type = ""
@@ -553,6 +574,15 @@ def _compile_reaction(node, index, units: UnitSystem) -> Reaction:
         if name not in index:
             raise MechanismError(f"reaction {eq!r}: undeclared species {name!r}")
     orders = {k: _num(v) for k, v in (node.get("orders") or {}).items()}
+    # Reaction::check (Reaction.cpp:135-162): explicit orders need their flags
+    if not node.get("nonreactant-orders", False):
+        for name in orders:
+            if name not in reactants:
+                raise MechanismError(f"reaction {eq!r}: order specified for non-reactant species {name!r}")
+    if not node.get("negative-orders", False):
+        for name, o in orders.items():
+            if o < 0.0:
+                raise MechanismError(f"reaction {eq!r}: negative order specified for species {name!r}")
     if orders and reversible:
         raise MechanismError(f"reaction {eq!r}: orders only allowed on irreversible reactions")
     # rate units: reactant orders (Reaction::calculateRateCoeffUnits)
@@ -628,16 +658,16 @@ def _compile_reaction(node, index, units: UnitSystem) -> Reaction:
     elif rate_type == "pressure-dependent-Arrhenius":
         ents = []
         for rc in node["rate-constants"]:
-            ents.append((_pressure(rc["P"]), units.rate_coeff(rc["A"], order), _num(rc["b"]),
+            ents.append((_pressure(rc["P"], units), units.rate_coeff(rc["A"], order), _num(rc["b"]),
                          units.ea_over_r(rc["Ea"]) if "Ea" in rc else 0.0))
         if not ents:
             raise MechanismError(f"reaction {eq!r}: PLOG without rate constants")
         r.plog = sorted(ents, key=lambda e: e[0])      # stable: equal pressures keep input order
         r.rtype = TYPE_PLOG
     elif rate_type == "Chebyshev":
-        lu = units.merged(node.get("units"))
+        lu = units
         Tr = [_num(_split_value_units(t)[0]) for t in node["temperature-range"]]
-        Pr = [_pressure(x) for x in node["pressure-range"]]
+        Pr = [_pressure(x, units) for x in node["pressure-range"]]
         rows = [[_num(x) for x in row] for row in node["data"]]
         if not rows or any(len(row) != len(rows[0]) for row in rows):
             raise MechanismError(f"reaction {eq!r}: inconsistent Chebyshev coefficient matrix")

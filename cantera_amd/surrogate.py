@@ -131,22 +131,34 @@ class Surrogate:
         check(lib().cg_surrogate_set_normalization(self._h, v.ctypes.data_as(C.POINTER(C.c_double))), "set_norm")
 
 
-def generate_pairs(sol, T0, P, Y0, dt=1e-6, nsteps=100, rtol=1e-8, atol=1e-15, stride=1):
+def regime_gate(sol, y, P, gate_T=1500.0):
+    """The guard's regime gate for states y (rows [m, T, Y]): T > gate_T and
+    the H + O + OH pool being destroyed (sum of their net production rates
+    < 0, the sign of sum_k f_k / W_k that the device guard tests)."""
+    y = np.asarray(y, np.float64)
+    rad = [sol.species_index(nm) for nm in ("H", "O", "OH")]
+    w = sol.net_production_rates(y[:, 1], P, np.ascontiguousarray(y[:, 2:].T))
+    return (y[:, 1] > gate_T) & (w[rad].sum(axis=0) < 0.0)
+
+
+def generate_pairs(sol, T0, P, Y0, dt=1e-6, nsteps=100, rtol=1e-8, atol=1e-15, stride=1, gate_T=None):
     """State pairs (y(t), y(t + dt)) along FP64 GPU trajectories of
     constant-pressure reactors started at (T0, P, Y0); returns y0, P, y1 with
-    rows [m, T, Y].  Every `stride`-th step is kept."""
+    rows [m, T, Y].  Every `stride`-th step is kept; with gate_T, only pairs
+    whose first state passes the guard's regime gate (regime_gate)."""
     from . import ReactorBatch
     rb = ReactorBatch(sol, T0, P, Y0, rtol=rtol, atol=atol)
     n = len(np.atleast_1d(T0))
-    Pn = np.broadcast_to(np.asarray(P, np.float64), (n,))
+    Pn = np.ascontiguousarray(np.broadcast_to(np.asarray(P, np.float64), (n,)))
     prev = rb.state.copy()
     a, b, p = [], [], []
     for k in range(1, nsteps + 1):
         rb.advance(k * dt)
         cur = rb.state.copy()
         if k % stride == 0:
-            a.append(prev)
-            b.append(cur)
-            p.append(Pn.copy())
+            keep = np.ones(n, bool) if gate_T is None else regime_gate(sol, prev, Pn, gate_T)
+            a.append(prev[keep])
+            b.append(cur[keep])
+            p.append(Pn[keep])
         prev = cur
     return np.concatenate(a), np.concatenate(p), np.concatenate(b)

@@ -93,12 +93,14 @@ int32_t cg_rates_batch_device(int32_t mech, int64_t n, const double* T, const do
  * src/kinetics/BulkKinetics.cpp:761-811) and a one-sided difference in T.
  * T[n], P[n], Y[K*n] species-major as cg_rates_batch; J[n*N*N] with N = K+2,
  * row-major per state, state order [m, T, Y_0..Y_{K-1}].  ydot is independent
- * of the mass, so the m row and column are zero.  precision: CG_FP64. */
+ * of the mass, so the m row and column are zero.  precision: CG_FP64 (J holds
+ * doubles) or CG_FP32 (J holds floats: the FP64 assembly rounded once at the
+ * store, half the bytes written; K + 2 <= 64). */
 int32_t cg_jac_batch(int32_t mech, int64_t n, const double* T, const double* P,
-                     const double* Y, double* J, int32_t precision);
+                     const double* Y, void* J, int32_t precision);
 /* Same on device pointers, asynchronous on `stream`. */
 int32_t cg_jac_batch_device(int32_t mech, int64_t n, const double* T, const double* P,
-                            const double* Y, double* J, int32_t precision, void* stream);
+                            const double* Y, void* J, int32_t precision, void* stream);
 /* Duration [ms] of the last batched-Jacobian kernel on `mech`. */
 double cg_jac_last_kernel_ms(int32_t mech);
 
@@ -159,6 +161,11 @@ int32_t cg_reactors_autotune(int32_t batch, const double* t_probe_dev, int32_t* 
 /* State y = [m, T, Y_1..Y_K] per reactor, AoS n x (K+2)
  * (ReactorNet::getState, src/zeroD/ReactorNet.cpp:754-759). */
 int32_t cg_reactors_get_state(int32_t batch, double* y);
+/* The same states written to device memory (n x (K+2) doubles), ordered on
+ * the batch's stream after the work queued before it, without host
+ * synchronisation (no counterpart in the reference: for callers keeping
+ * trajectories on the device). */
+int32_t cg_reactors_get_state_device(int32_t batch, double* y_dev);
 /* Integrator counters per reactor, n x 9 int64: steps, rhs evaluations,
  * Jacobian evaluations, LU factorisations, Newton iterations, nonlinear
  * convergence failures, error-test failures, current order, extra rhs
@@ -168,10 +175,20 @@ int32_t cg_reactors_stats(int32_t batch, int64_t* stats);
 /* Integrator control block per reactor, n x 64 doubles: [0] internal time
  * t_n, [1] current step h, [2] next step, [28] order q, [32] steps, [37] user
  * time, [38..43] rhs/jacobian/LU/Newton/failure counters, [44] status, [45]
- * pressure (CVodesIntegrator::getCurrentStepSize / getLastOrder and the
- * solverStats fields, src/numerics/CVodesIntegrator.cpp:572-621). */
+ * the pressure of a constant-pressure reactor, the volume (1 m^3) of a
+ * constant-volume one (CVodesIntegrator::getCurrentStepSize / getLastOrder and
+ * the solverStats fields, src/numerics/CVodesIntegrator.cpp:572-621). */
 int32_t cg_reactors_get_control(int32_t batch, double* ctl);
-/* Status per reactor: 0 ok, negative = integration failure code. */
+/* Status per reactor: 0 ok, negative = integration failure code, as the
+ * CVODES flags the reference reports (CVodesIntegrator::integrate,
+ * src/numerics/CVodesIntegrator.cpp:491-512): -3 repeated Newton
+ * convergence failures, -4 repeated error-test failures, -5 a non-finite
+ * right-hand side at the first evaluation (FuncEval's fatal error,
+ * src/numerics/FuncEval.cpp:13-43), -1 / -2 / -6 no usable initial step,
+ * -10 tolerances too small for the state, -20 an output time before the
+ * current time, -22 max_steps internal steps taken before reaching the
+ * output time.  A reactor with a negative status is left untouched by later
+ * advances. */
 int32_t cg_reactors_status(int32_t batch, int32_t* status);
 /* Options: max internal steps per advance call (default 20000,
  * CVodesIntegrator.h:120-132) and the persistent grid size in wavefronts per
@@ -225,16 +242,27 @@ int32_t cg_surrogate_get_params(int32_t surrogate, float* params);
 int32_t cg_surrogate_set_params(int32_t surrogate, const float* params);
 int32_t cg_surrogate_get_normalization(int32_t surrogate, double* norm);
 int32_t cg_surrogate_set_normalization(int32_t surrogate, const double* norm);
-/* cg_reactors_advance with surrogate steps where they pass an FP64 guard:
- * reactors whose t_out is a whole number of surrogate steps away take them,
- * each checked (physical range, element conservation, trapezoidal defect of
- * the reactor ODE: |r_T| <= dT_tol K, |r_k| <= rtol_g max(|Y0_k|, |Y1_k|) +
- * atol_g); at the first rejected step the reactor continues with the BDF
- * integrator from its last kept state.  counts[3] (may be NULL): surrogate
- * steps kept, reactors advanced by the surrogate alone, reactors that ran
- * the integrator. */
-int32_t cg_reactors_advance_surrogate(int32_t batch, int32_t surrogate, const double* t_out, double dT_tol,
-                                      double rtol_g, double atol_g, int64_t* counts);
+/* cg_reactors_advance with one guarded surrogate step (no counterpart in the
+ * reference; its path is always the integrator): every reactor whose output
+ * time is exactly one surrogate step dt ahead takes the network's prediction
+ * if it passes the FP64 guard -- physical range, element conservation, the
+ * regime gate (T > gate_T K and the H + O + OH pool being destroyed) and the
+ * step-error estimate from the trapezoidal defect of the reactor ODE
+ * (|r_T| <= tol_T K; |r_k| / (1 + dt/2 lambda_k) <= tol_Y for species, with
+ * lambda_k the species' loss frequency) -- and then restarts from it, as
+ * after CVodeReInit.  Every other reactor (rejected, or a different output
+ * span) is advanced by the BDF integrator in the same call.
+ * _device: t_out in device memory, stream-ordered, no host synchronisation
+ * and no allocation after the first call on a batch.  Counters accumulate on
+ * the device: cg_reactors_surrogate_counts returns [eligible reactor-steps,
+ * kept, integrated] (reset = 1 zeroes them).  The host form returns the
+ * counts of its own call in counts[3] (may be NULL) and an error if any
+ * reactor failed. */
+int32_t cg_reactors_advance_surrogate_device(int32_t batch, int32_t surrogate, const double* t_out_dev, double tol_T,
+                                             double tol_Y, double gate_T);
+int32_t cg_reactors_surrogate_counts(int32_t batch, int64_t* counts, int32_t reset);
+int32_t cg_reactors_advance_surrogate(int32_t batch, int32_t surrogate, const double* t_out, double tol_T,
+                                      double tol_Y, double gate_T, int64_t* counts);
 
 #ifdef __cplusplus
 }

@@ -5,6 +5,10 @@
  * Built with -ffp-contract=off so that a*b+c is never fused: the reference is
  * compiled for baseline x86-64, where no fused multiply-add is emitted, and
  * this file keeps its operation order.
+ *
+ * The BDF integrator's step-control constants and helper structure follow
+ * SUNDIALS CVODE (cvode.c, Copyright (c) 2002-2025, Lawrence Livermore
+ * National Security and Southern Methodist University; BSD-3-Clause).
  */
 #include "oracle.h"
 
@@ -612,6 +616,35 @@ int or_rates_batch(const or_mech* m, int n, const double* T, const double* P, co
         work_free(w);
     }
     return 0;
+}
+
+/* Analytic reactor Jacobians of n states (m = 1 kg), species-major inputs as
+ * or_rates_batch; J[s*N*N + r*N + c] row-major per state. */
+int or_jacobian_batch(const or_mech* m, int n, const double* T, const double* P, const double* Y, double* J,
+                      int nthreads) {
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+    (void)nthreads;
+#endif
+    const int K = m->K, N = K + 2;
+    int err = 0;
+#pragma omp parallel
+    {
+        double* y = malloc(8 * N);
+#pragma omp for schedule(static)
+        for (int s = 0; s < n; s++) {
+            y[0] = 1.0;
+            y[1] = T[s];
+            for (int k = 0; k < K; k++) y[2 + k] = Y[(size_t)k * n + s];
+            if (or_igcpr_jacobian(m, P[s], y, J + (size_t)s * N * N, 1)) {
+#pragma omp atomic write
+                err = 1;
+            }
+        }
+        free(y);
+    }
+    return err ? -1 : 0;
 }
 
 /* ---- reactor RHS: IdealGasConstPressureReactor::updateState/eval and

@@ -86,6 +86,11 @@ int or_equilibrium_constants(const or_mech* m, double T, double P, const double*
 int or_rates_batch(const or_mech* m, int n, const double* T, const double* P,
                    const double* Y, double* wdot, int nthreads);
 
+/* analytic reactor Jacobians (m = 1 kg) of n states, inputs as or_rates_batch,
+ * J[s*N*N + r*N + c] */
+int or_jacobian_batch(const or_mech* m, int n, const double* T, const double* P, const double* Y, double* J,
+                      int nthreads);
+
 /* reactor RHS: y = [m, T, Y_1..Y_K] (neq = K+2), P held fixed */
 int or_igcpr_rhs(const or_mech* m, double P, const double* y, double* ydot);
 /* Jacobian d(ydot)/dy, row-major neq x neq.  mode 0: finite differences

@@ -40,6 +40,8 @@ def lib():
         L.or_rates.argtypes = [C.c_void_p, C.c_double, C.c_double, _dp, opt, opt, opt, opt, opt]
         L.or_equilibrium_constants.argtypes = [C.c_void_p, C.c_double, C.c_double, _dp, _dp]
         L.or_rates_batch.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp, C.c_int]
+        L.or_jacobian_batch.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp, C.c_int]
+        L.or_jacobian_batch.restype = C.c_int
         L.or_igcpr_rhs.argtypes = [C.c_void_p, C.c_double, _dp, _dp]
         L.or_igr_rhs.argtypes = [C.c_void_p, C.c_double, _dp, _dp]
         L.or_igcpr_rhs.restype = C.c_int
@@ -115,6 +117,16 @@ class Mech:
         lib().or_rates_batch(self._h, n, np.ascontiguousarray(T, np.float64), np.ascontiguousarray(P, np.float64),
                              np.ascontiguousarray(Y_soa, np.float64), wdot, nthreads)
         return wdot
+
+    def jacobian_batch(self, T, P, Y_soa, nthreads=0):
+        """Analytic reactor Jacobians (n, N, N) at n states (Y species-major)."""
+        n = len(T)
+        N = self.K + 2
+        J = np.zeros((n, N, N))
+        if lib().or_jacobian_batch(self._h, n, np.ascontiguousarray(T, np.float64), np.ascontiguousarray(P, np.float64),
+                                   np.ascontiguousarray(Y_soa, np.float64), J.reshape(-1), nthreads):
+            raise FloatingPointError("non-finite Jacobian")
+        return J
 
     def rhs(self, P, y, const_volume=False):
         """ydot of the constant-pressure reactor at P, or of the constant-volume

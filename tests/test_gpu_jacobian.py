@@ -92,3 +92,39 @@ def test_jacobian_gri30_config_states(oracle_mod):
     ec, et = check_against_oracle(om, T[pick], P[pick], Y[:, pick].T, J[pick])
     assert ec <= 1e-10
     assert et <= 1e-5
+
+
+@pytest.mark.parametrize("mech", ["h2o2", "gri30"])
+def test_fp32_jacobian_error_against_oracle(mech, oracle_mod):
+    """CG_FP32: the FP64 assembly rounded once to FP32 at the store.  Error
+    model: each entry gains at most one FP32 rounding, |dJ_ij| <= 2^-24 |J_ij|
+    (6e-8) on top of the FP64 bars; written per entry as 1e-7 of the row scale
+    (species columns) and of the T-column scale.  The FP32 output equals the
+    FP64 kernel's output rounded to FP32 exactly (same states, same kernel
+    arithmetic up to the atomic summation order: bar 1e-10 row scale)."""
+    import cantera_amd as ca
+    from cantera_amd.workloads import MechInfo
+    info = MechInfo(mech_file(mech))
+    om = oracle_mod.Mech(mech_file(mech))
+    sol = ca.Solution(mech)
+    T, P, Y = states(info, 40, 12)
+    J32 = sol.reactor_jacobians(T, P, Y.T, precision=ca.CG_FP32)
+    J64 = sol.reactor_jacobians(T, P, Y.T)
+    assert J32.dtype == np.float32 and np.all(np.isfinite(J32))
+    ec, et = check_against_oracle(om, T, P, Y, J32.astype(np.float64))
+    rs = np.abs(J64).max(axis=2, keepdims=True) + 1e-300
+    d = float((np.abs(J32.astype(np.float64) - J64.astype(np.float32).astype(np.float64)) / rs).max())
+    print(f"{mech} FP32: species columns {ec:.2e}, T column {et:.2e}, vs rounded FP64 {d:.2e}", flush=True)
+    assert ec <= 1e-7
+    assert et <= 1e-5
+    assert d <= 1e-10
+
+
+def test_fp32_jacobian_rejects_large_mechanisms():
+    import cantera_amd as ca
+    from cantera_amd.workloads import MechInfo
+    info = MechInfo(mech_file("nDodecane"))
+    sol = ca.Solution("nDodecane")
+    T, P, Y = states(info, 2, 3)
+    with pytest.raises(ca.CanteraGpuError):
+        sol.reactor_jacobians(T, P, Y.T, precision=ca.CG_FP32)

@@ -277,3 +277,73 @@ def test_reactor_invariants():
     assert np.max(np.abs(y[:, 2:].sum(axis=1) - 1.0)) < 1e-7
     E1 = y[:, 2:] / info.molecular_weights @ info.composition
     assert np.max(np.abs(E1 - E0) / (np.abs(E0) + 1e-12)) < 1e-6
+
+
+def test_gri30_config3_256_reactors_all_steps(oracle_mod):
+    """256 reactors of the headline workload (T0 900-2000 K, phi 0.5-3, 1-40
+    atm, per-reactor dt 1e-6..1e-4 s) over all 100 output steps, including its
+    stiff tail, against the oracle running the same algorithm.  Over 100 steps
+    through ignition a last-bit difference can flip one step-size or
+    Jacobian-reuse decision; from then on the two runs differ by the
+    integrator's local error, amplified by the ignition's sensitivity, so the
+    bar is the integrator-tolerance one (|dT| <= 1e-4 T, |dY| <= 1e-7 + 1e-3
+    |Y|).  At least 90% of the reactors must still meet the same-algorithm bar
+    (|dT| <= 1e-6 T, |dY| <= 1e-9 + 1e-6 |Y|) at every step."""
+    from cantera_amd.workloads import MechInfo, config3
+    wl = config3(MechInfo(mech_file("gri30")), n=256, nsteps=100)
+    gpu, ref = _compare("gri30", wl, 100, True, 1e-4, 1e-7, 1e-3, oracle_mod)
+    tight = ((np.abs(gpu[:, :, 1] - ref[:, :, 1]) <= 1e-6 * ref[:, :, 1]).all(axis=1)
+             & (np.abs(gpu[:, :, 2:] - ref[:, :, 2:]) <= 1e-9 + 1e-6 * np.abs(ref[:, :, 2:])).all(axis=(1, 2)))
+    print(f"{tight.sum()} of 256 reactors within the same-algorithm bar at all 100 steps", flush=True)
+    assert tight.mean() >= 0.9
+
+
+def test_const_volume_mixed_and_cvodes_mode(oracle_mod):
+    """Constant-volume batches in the mixed form (FP32 iteration matrix) and
+    against the oracle's CVODES-mode (difference-quotient Jacobian) run: both
+    within the integrator-tolerance bar used for constant pressure."""
+    from cantera_amd.workloads import MechInfo, config3
+    wl = config3(MechInfo(mech_file("gri30")), n=48, nsteps=10)
+    _compare("gri30", wl, 10, True, 1e-4, 1e-7, 1e-3, oracle_mod, precision=1, const_volume=True)
+    _compare("gri30", wl, 10, False, 1e-4, 1e-7, 1e-3, oracle_mod, const_volume=True)
+
+
+def test_max_steps_failure_status():
+    """A reactor that needs more than max_steps internal steps to reach its
+    output time stops with status -22 (CVODES' too-much-work flag, which the
+    reference reports as a CanteraError, CVodesIntegrator.cpp:491-512); the
+    call returns an error, the failed reactors are left untouched by later
+    advances, and other batches keep working."""
+    import cantera_amd as ca
+    from cantera_amd.workloads import MechInfo, config3
+    wl = config3(MechInfo(mech_file("gri30")), n=16, nsteps=2)
+    sol = ca.Solution("gri30")
+    rb = ca.ReactorBatch(sol, wl.T0, wl.P, wl.Y0, rtol=wl.rtol, atol=wl.atol)
+    rb.set_options(max_steps=3)
+    with pytest.raises(ca.CanteraGpuError):
+        rb.advance(1e-3)
+    st = rb.status
+    print("status after a 3-step limit:", st.tolist(), flush=True)
+    assert np.all(st == -22)
+    ctl_t = rb.state
+    with pytest.raises(ca.CanteraGpuError):
+        rb.advance(2e-3)
+    assert np.array_equal(rb.state, ctl_t)
+    ok = ca.ReactorBatch(sol, wl.T0, wl.P, wl.Y0, rtol=wl.rtol, atol=wl.atol)
+    ok.advance(1e-5)
+    assert np.all(ok.status == 0)
+
+
+def test_invalid_initial_state_is_rejected():
+    """Phase::setTemperature / setDensity reject non-positive values; the
+    batch constructor also rejects non-finite inputs, before any kernel runs."""
+    import cantera_amd as ca
+    from cantera_amd.workloads import MechInfo, config3
+    wl = config3(MechInfo(mech_file("gri30")), n=4, nsteps=1)
+    sol = ca.Solution("gri30")
+    for T0, P in ((-300.0, 101325.0), (np.nan, 101325.0), (1000.0, 0.0)):
+        T = wl.T0.copy()
+        Pa = wl.P.copy()
+        T[2], Pa[2] = T0, P
+        with pytest.raises(ca.CanteraGpuError):
+            ca.ReactorBatch(sol, T, Pa, wl.Y0)

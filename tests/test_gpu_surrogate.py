@@ -141,34 +141,90 @@ def test_training_lowers_loss():
     assert losses[-1] < 0.2 * losses[0]
 
 
-def _plain_and_surrogate(sol, wl, sur, nsteps, dT_tol, rtol_g):
+def _plain_and_surrogate(sol, wl, sur, nsteps, dt, precision=0):
+    """The same reactors advanced nsteps outputs of dt by the plain integrator
+    and by the guarded surrogate path; returns both trajectories (n x steps+1
+    x N), the counts [eligible, kept, integrated] and the two wall times."""
+    import time
+
     import cantera_amd as ca
     a = ca.ReactorBatch(sol, wl.T0, wl.P, wl.Y0, rtol=wl.rtol, atol=wl.atol)
-    b = ca.ReactorBatch(sol, wl.T0, wl.P, wl.Y0, rtol=wl.rtol, atol=wl.atol)
+    b = ca.ReactorBatch(sol, wl.T0, wl.P, wl.Y0, rtol=wl.rtol, atol=wl.atol, precision=precision)
+    n = len(wl.T0)
+    ya = np.zeros((n, nsteps + 1, sol.n_species + 2))
+    yb = np.zeros_like(ya)
+    ya[:, 0] = a.state
+    yb[:, 0] = b.state
     tot = np.zeros(3, dtype=np.int64)
+    ta = tb = 0.0
     for k in range(1, nsteps + 1):
-        a.advance(k * 1e-6)
-        tot += b.advance_surrogate(k * 1e-6, sur, dT_tol=dT_tol, rtol_g=rtol_g)
-    return a.state, b.state, tot
+        t0 = time.perf_counter()
+        a.advance(k * dt)
+        ta += time.perf_counter() - t0
+        t0 = time.perf_counter()
+        tot += b.advance_surrogate(k * dt, sur)
+        tb += time.perf_counter() - t0
+        ya[:, k] = a.state
+        yb[:, k] = b.state
+    return ya, yb, tot, ta, tb
 
 
-def test_guarded_advance_within_1K():
+def test_guarded_advance_h2o2_within_budget():
+    """h2o2 at dt = 1e-6 s with a network trained on gated integrator pairs:
+    whatever the guard keeps, every output stays within 1 K and 1e-4 in Y of
+    the plain integrator, and mass fractions keep summing to 1."""
     import cantera_amd as ca
     from cantera_amd.surrogate import Surrogate, generate_pairs
     from cantera_amd.workloads import MechInfo, config1
     info = MechInfo(mech_file("h2o2"))
     sol = ca.Solution("h2o2")
     train = config1(info, n=512)
-    y0, P, y1 = generate_pairs(sol, train.T0, train.P, train.Y0, dt=1e-6, nsteps=80)
+    y0, P, y1 = generate_pairs(sol, train.T0, train.P, train.Y0, dt=1e-6, nsteps=80, gate_T=1500.0)
     sur = Surrogate(sol, dt=1e-6, hidden=128, depth=3, seed=2)
     sur.fit(y0, P, y1, epochs=30, batch=512, lr=1e-3)
     wl = config1(info, n=96)
-    ya, yb, tot = _plain_and_surrogate(sol, wl, sur, 40, dT_tol=0.05, rtol_g=1e-3)
-    dT = np.abs(ya[:, 1] - yb[:, 1])
-    print(f"kept surrogate steps {tot[0]}, reactor-steps by surrogate alone {tot[1]}, by BDF {tot[2]}, "
-          f"max |dT| {dT.max():.3e} K", flush=True)
-    assert dT.max() <= 1.0
-    assert np.all(np.abs(yb[:, 2:].sum(axis=1) - 1) < 1e-5)
+    ya, yb, tot, _, _ = _plain_and_surrogate(sol, wl, sur, 40, 1e-6)
+    dT = np.abs(ya[:, :, 1] - yb[:, :, 1]).max()
+    dY = np.abs(ya[:, :, 2:] - yb[:, :, 2:]).max()
+    print(f"h2o2: {len(y0)} gated training pairs; eligible {tot[0]}, kept {tot[1]}, integrated {tot[2]}; "
+          f"max |dT| {dT:.3e} K, max |dY| {dY:.3e}", flush=True)
+    assert dT <= 1.0 and dY <= 1e-4
+    assert np.all(np.abs(yb[:, -1, 2:].sum(axis=1) - 1) < 1e-5)
+
+
+def test_trained_gri30_surrogate_gain_and_budget():
+    """gri30 at the fixed step dt = 1e-5 s: a network trained on gated pairs
+    of FP64 GPU trajectories (8192 reactors of the headline distribution,
+    another seed, 100 steps) keeps guarded steps on 512 evaluation reactors
+    over 100 steps, where an untrained network keeps none; at every output the
+    guarded trajectories stay within the budget of 1 K in T and 1e-4 in Y of
+    the plain FP64 integrator.  The wall-time ratio of the two paths is
+    printed as measured."""
+    import cantera_amd as ca
+    from cantera_amd.surrogate import Surrogate, generate_pairs
+    from cantera_amd.workloads import MechInfo, config3
+    info = MechInfo(mech_file("gri30"))
+    sol = ca.Solution("gri30")
+    dt = 1e-5
+    tr = config3(info, n=8192, nsteps=100, seed=0x5EED0104)
+    y0, P, y1 = generate_pairs(sol, tr.T0, tr.P, tr.Y0, dt=dt, nsteps=100, rtol=tr.rtol, atol=tr.atol,
+                               gate_T=ca.ReactorBatch.GUARD["gate_T"])
+    sur = Surrogate(sol, dt=dt, hidden=256, depth=4, seed=1)
+    losses = sur.fit(y0, P, y1, epochs=20, batch=4096, lr=1e-3)
+    wl = config3(info, n=512, nsteps=100)
+    ya, yb, tot, ta, tb = _plain_and_surrogate(sol, wl, sur, 100, dt)
+    dT = np.abs(ya[:, :, 1] - yb[:, :, 1]).max()
+    dY = np.abs(ya[:, :, 2:] - yb[:, :, 2:]).max()
+    fresh = Surrogate(sol, dt=dt, hidden=256, depth=4, seed=5)
+    fresh.fit_normalization(y0, P, y1)
+    _, _, tot0, _, _ = _plain_and_surrogate(sol, config3(info, n=512, nsteps=20), fresh, 20, dt)
+    print(f"gri30 dt 1e-5: {len(y0)} gated pairs, loss {losses[0]:.3g} -> {losses[-1]:.3g}; trained: eligible "
+          f"{tot[0]}, kept {tot[1]} ({tot[1] / max(1, tot[0]):.3%}), integrated {tot[2]}; untrained (20 steps): kept "
+          f"{tot0[1]}; max |dT| {dT:.3e} K, max |dY| {dY:.3e}; wall time plain {ta:.2f} s, guarded {tb:.2f} s "
+          f"(gain {ta / tb:.3f}x)", flush=True)
+    assert tot[1] > 0
+    assert tot0[1] == 0
+    assert dT <= 1.0 and dY <= 1e-4
 
 
 def test_untrained_network_falls_back_to_integrator():
@@ -179,10 +235,10 @@ def test_untrained_network_falls_back_to_integrator():
     sol = ca.Solution("gri30")
     wl = config3(info, n=64, nsteps=5)
     sur = Surrogate(sol, dt=1e-6, hidden=256, depth=4, seed=9)
-    ya, yb, tot = _plain_and_surrogate(sol, wl, sur, 5, dT_tol=0.01, rtol_g=1e-4)
+    ya, yb, tot, _, _ = _plain_and_surrogate(sol, wl, sur, 5, 1e-6)
     print("counts", tot, flush=True)
-    assert tot[1] == 0
-    assert np.max(np.abs(ya[:, 1] - yb[:, 1]) / ya[:, 1]) <= 1e-6
+    assert tot[1] == 0 and tot[2] == tot[0] == 5 * 64
+    assert np.max(np.abs(ya[:, -1, 1] - yb[:, -1, 1]) / ya[:, -1, 1]) <= 1e-6
 
 
 def test_prediction_conserves_elements():

@@ -608,3 +608,91 @@ def test_chemically_activated_jacobian_and_tables(oracle_mod, tmp_path):
         scale = np.abs(Jf).max(axis=1, keepdims=True) + 1e-30 * np.abs(Jf).max()
         err = np.abs(Ja - Jf)[1:] / scale[1:]
         assert err[:, 2:].max() < 1e-6, (T, P)
+
+
+def test_chemically_activated_chemkin_known_answer(oracle_mod):
+    """test/python/test_kinetics.py:855-866: forward rate of progress of the
+    chemically activated reaction at 900 K and three pressures, computed with
+    Chemkin (2e-5 relative, as the reference checks it)."""
+    m = oracle_mod.Mech(mech_file("chemically-activated-reaction"))
+    X = np.array([0.01, 0.01, 0.04, 0.10, 0.84])          # ch3, oh, ch2o, h2, n2
+    Y = X * m.mw / (X * m.mw).sum()
+    for P, ref in ((2026.5, 2.851022e-04), (202650.0, 2.775924e+00), (10132500.0, 2.481792e+03)):
+        ropf = m.rates(900.0, P, Y)["ropf"][0]
+        assert ropf == pytest.approx(ref, rel=2e-5), P
+
+
+# --- fractional stoichiometry and explicit orders (frac.yaml) ---------------
+
+@pytest.fixture(scope="module")
+def frac(oracle_mod):
+    return oracle_mod.Mech(mech_file("frac"))
+
+
+def _frac_state(m):
+    """test/kinetics/rates.cpp:12-20: 2000 K, 4 atm, X = H2O:0.5, OH:.05, H:0.1,
+    O2:0.15, H2:0.2; returns (T, P, Y, concentrations)."""
+    X = np.zeros(m.K)
+    for name, v in (("H2O", 0.5), ("OH", 0.05), ("H", 0.1), ("O2", 0.15), ("H2", 0.2)):
+        X[m.index(name)] = v
+    X /= X.sum()
+    T, P = 2000.0, 4 * ONE_ATM
+    Y = X * m.mw / (X * m.mw).sum()
+    return T, P, Y, X * P / (R * T)
+
+
+def test_frac_rate_constants_and_rates_of_progress(frac):
+    """rates.cpp:45-76: k_f of the fractional-order reactions (the sum of
+    orders fixes the units: 1e13 (mol/cm^3)^-2.8/s = 1e13 * 1000^-2.8 SI),
+    irreversible k_r = 0, and ropf = k_f prod C^order with the explicit orders
+    (C_AnyN: H2^0.8 O2 OH^2; H2 O2^-0.25 with negative orders allowed)."""
+    m = frac
+    T, P, Y, C = _frac_state(m)
+    r = m.rates(T, P, Y)
+    kf, ropf = r["kf"], r["ropf"]
+    assert kf[0] == pytest.approx(1e13, rel=1e-15)
+    assert abs(kf[1] - 1e13 * 1e3 ** -2.8) <= 1e-2
+    assert np.all(r["rkcn"] == 0.0) and np.all(r["ropr"] == 0.0)
+    i = {n: m.index(n) for n in ("H2O", "H", "OH", "O2", "H2", "O")}
+    assert ropf[0] == pytest.approx(C[i["H2O"]] * kf[0], rel=1e-14)
+    assert ropf[1] == pytest.approx(C[i["H2"]] ** 0.8 * C[i["O2"]] * C[i["OH"]] ** 2 * kf[1], rel=1e-14)
+    assert ropf[2] == pytest.approx(C[i["H2"]] * C[i["O2"]] ** -0.25 * kf[2], rel=1e-14)
+
+
+def test_frac_creation_and_destruction(frac):
+    """rates.cpp:78-96: fractional stoichiometric coefficients in the net
+    production rates (wdot = creation - destruction; all irreversible)."""
+    m = frac
+    T, P, Y, C = _frac_state(m)
+    r = m.rates(T, P, Y)
+    f, w = r["ropf"], r["wdot"]
+    i = {n: m.index(n) for n in ("H2O", "H", "OH", "O2", "H2", "O")}
+    assert w[i["H"]] == pytest.approx(1.4 * f[0], rel=1e-14)
+    assert w[i["OH"]] == pytest.approx(0.6 * f[0] - 0.6 * f[1], rel=1e-12)
+    assert w[i["O2"]] == pytest.approx(0.2 * f[0] - (0.2 * f[1] + 0.5 * f[2]), rel=1e-12)
+    assert w[i["H2"]] == pytest.approx(-(0.7 * f[1] + f[2]), rel=1e-14)
+    assert w[i["H2O"]] == pytest.approx(f[1] + f[2] - f[0], rel=1e-12)
+    assert w[i["O"]] == 0.0
+
+
+def test_frac_equilibrium_constants(frac):
+    """rates.cpp:98-115: Kc = exp(-dG0/RT) (P_ref/RT)^dn with the fractional
+    net coefficients 1.2 and -0.5 (g0 from the species' NASA7 at P_ref)."""
+    m = frac
+    T, P, Y, C = _frac_state(m)
+    Kc = m.equilibrium_constants(T, P, Y)
+    _, h, s = m.thermo(T)
+    mu0 = (h - s) * R * T       # g0 in J/kmol
+    i = {n: m.index(n) for n in ("H2O", "H", "OH", "O2", "H2")}
+    dG0 = 1.4 * mu0[i["H"]] + 0.6 * mu0[i["OH"]] + 0.2 * mu0[i["O2"]] - mu0[i["H2O"]]
+    dG1 = mu0[i["H2O"]] - 0.7 * mu0[i["H2"]] - 0.6 * mu0[i["OH"]] - 0.2 * mu0[i["O2"]]
+    RT = R * T
+    assert Kc[0] == pytest.approx(np.exp(-dG0 / RT) * (ONE_ATM / RT) ** 1.2, rel=1e-13)
+    assert Kc[1] == pytest.approx(np.exp(-dG1 / RT) * (ONE_ATM / RT) ** -0.5, rel=1e-13)
+
+
+def test_frac_tables_match_compiler(tmp_path):
+    from cantera_amd import mechc
+    out = tmp_path / "frac.mech"
+    mechc.main([os.path.join(GOLDEN, "frac.yaml"), str(out)])
+    assert out.read_text() == open(mech_file("frac")).read()
