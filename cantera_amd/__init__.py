@@ -42,6 +42,13 @@ def mech_path(name: str) -> str:
         out = os.path.join(MECH_DIR, base + ".mech")
         write_mech(compile_mechanism(name), out)
         return out
+    if os.path.exists(name) and name.endswith((".inp", ".ck", ".dat")):
+        # Chemkin format with the thermo data in the same file
+        from .ckc import compile_chemkin
+        from .mechc import write_mech
+        out = os.path.join(MECH_DIR, base + ".mech")
+        write_mech(compile_chemkin(name), out)
+        return out
     raise FileNotFoundError(f"no compiled mechanism for {name!r}")
 
 

@@ -485,7 +485,12 @@ def _collect_species(doc, phase, path, search_dirs=()):
 
 
 def compile_mechanism(path: str, phase_name: str | None = None, search_dirs=()):
-    doc = _load_yaml(path)
+    return compile_document(_load_yaml(path), path, phase_name, search_dirs)
+
+
+def compile_document(doc: dict, path: str, phase_name: str | None = None, search_dirs=()):
+    """Compile a mechanism already held as a Cantera-YAML document (dicts and
+    lists); `path` names its source (for species references and messages)."""
     phase = _find_phase(doc, phase_name)
     if phase.get("thermo") != "ideal-gas":
         raise MechanismError(f"phase {phase['name']!r} is {phase.get('thermo')!r}; only ideal-gas is supported")
