@@ -74,6 +74,8 @@ SIGNATURES = [
     ("cg_surrogate_loss_grad", C.c_int32, [C.c_int32, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p,
                                            C.POINTER(C.c_double), C.POINTER(C.c_float)]),
     ("cg_surrogate_predict", C.c_int32, [C.c_int32, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("cg_surrogate_predict_noalloc", C.c_int32, [C.c_int32, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p,
+                                                 C.c_void_p]),
     ("cg_surrogate_get_params", C.c_int32, [C.c_int32, C.POINTER(C.c_float)]),
     ("cg_surrogate_set_params", C.c_int32, [C.c_int32, C.POINTER(C.c_float)]),
     ("cg_surrogate_get_normalization", C.c_int32, [C.c_int32, C.POINTER(C.c_double)]),

@@ -878,6 +878,17 @@ int32_t cg_surrogate_predict(int32_t s, int64_t n, const double* y0, const doubl
     });
 }
 
+int32_t cg_surrogate_predict_noalloc(int32_t s, int64_t n, const double* y0, const double* P, double* y1,
+                                     void* stream) {
+    return guarded([&] {
+        if (n < 0) throw std::runtime_error("negative row count");
+        if (n == 0) return 0;
+        if (!y0 || !P || !y1) throw std::runtime_error("null array");
+        get_surr(s).net->predict_noalloc(n, y0, P, y1, static_cast<hipStream_t>(stream));
+        return 0;
+    });
+}
+
 int32_t cg_surrogate_get_params(int32_t s, float* p) {
     return guarded([&] { get_surr(s).net->get_params(p); return 0; });
 }

@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <random>
 #include <stdexcept>
 #include <string>
@@ -142,7 +143,7 @@ __global__ __launch_bounds__(256) void gemm_bf16(int M, int N, int K, const uint
                 const size_t o = static_cast<size_t>(m) * e.ldc + n;
                 if (EPI == kFwdGelu) {
                     const float z = v + e.bias[n];
-                    e.zf[o] = z;
+                    if (e.zf) e.zf[o] = z;      // pre-activation kept for training only
                     e.hb[o] = f2bf(gelu(z));
                 } else if (EPI == kFwdLinear) {
                     e.zf[o] = v + e.bias[n];
@@ -453,7 +454,7 @@ __global__ __launch_bounds__(256) void gemm_nt_bf16(int M, int N, int K, const u
                 const size_t o = static_cast<size_t>(m) * e.ldc + n;
                 if (EPI == kFwdGelu) {
                     const float z = v + e.bias[n];
-                    e.zf[o] = z;
+                    if (e.zf) e.zf[o] = z;      // pre-activation kept for training only
                     e.hb[o] = f2bf(gelu(z));
                 } else {
                     e.zf[o] = v + e.bias[n];
@@ -752,6 +753,87 @@ void SurrogateNet::predict(long n, const double* y0, const double* P, double* y1
         check(hipGetLastError(), "surrogate features");
         forward(rows, st);
         hipLaunchKernelGGL(apply_kernel, dim3(rows), dim3(64), 0, st, rows, N, s_.K, y0 + c0 * N, out_, norm_,
+                           s_.in_p, s_.out_p, comp_, rmw_, E_, y1 + c0 * N);
+        check(hipGetLastError(), "surrogate apply");
+    }
+}
+
+// ---- inference without allocation ------------------------------------------
+//
+// Workspace of predict_noalloc: one array per device, part of the code object,
+// held until the process ends.  Its address on the current device comes from
+// hipGetSymbolAddress (the array's name is only a host-side stand-in), once
+// per device, and is kept per device.
+constexpr size_t kNoAllocBytes = size_t{16} << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+void* scratch_on_current_device() {
+    int dev = -1;
+    check(hipGetDevice(&dev), "hipGetDevice");
+    if (dev < 0 || dev >= kMaxDevices) throw std::runtime_error("surrogate: device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        if (!p) throw std::runtime_error("surrogate: no address for the inference workspace");
+        g_scratch_ptr[dev] = p;
+    }
+    return g_scratch_ptr[dev];
+}
+}  // namespace
+
+size_t SurrogateNet::noalloc_workspace_bytes() { return kNoAllocBytes; }
+
+// Rows per pass through the workspace: features (in_p BF16), two hidden
+// activations (BF16, ping-pong), the FP32 output (out_p); a multiple of 128
+// (the GEMM row tile), so every buffer starts 256-byte aligned.
+int SurrogateNet::noalloc_rows() const {
+    const size_t per_row = static_cast<size_t>(s_.in_p) * 2 + 4 * static_cast<size_t>(s_.hidden) +
+                           static_cast<size_t>(s_.out_p) * 4;
+    const size_t rows = kNoAllocBytes / per_row / 128 * 128;
+    return static_cast<int>(std::min<size_t>(rows, 1 << 20));
+}
+
+void SurrogateNet::predict_noalloc(long n, const double* y0, const double* P, double* y1, hipStream_t st) {
+    const int R = noalloc_rows();
+    if (R < 128)
+        throw std::runtime_error("surrogate: network too wide for the 16 MiB inference workspace (fewer than 128 rows)");
+    unsigned char* base = static_cast<unsigned char*>(scratch_on_current_device());
+    const size_t r = static_cast<size_t>(R);
+    uint16_t* feat = reinterpret_cast<uint16_t*>(base);
+    uint16_t* ha = reinterpret_cast<uint16_t*>(base + r * s_.in_p * 2);
+    uint16_t* hb = ha + r * s_.hidden;
+    float* out = reinterpret_cast<float*>(base + r * s_.in_p * 2 + 4 * r * s_.hidden);
+    const int N = s_.K + 2;
+    for (long c0 = 0; c0 < n; c0 += R) {
+        const int rows = static_cast<int>(std::min<long>(R, n - c0));
+        hipLaunchKernelGGL(features_kernel, dim3(rows), dim3(64), 0, st, rows, nullptr, N, s_.K, y0 + c0 * N, P + c0,
+                           norm_, s_.in_p, feat);
+        check(hipGetLastError(), "surrogate features");
+        const uint16_t* H = feat;
+        uint16_t* next = ha;
+        for (int l = 0; l <= s_.depth; l++) {
+            const int fi = s_.fan_in(l), fo = s_.fan_out(l);
+            EpiArgs e{};
+            e.bias = par_ + oB_[l];
+            e.ldc = fo;
+            if (l < s_.depth) {
+                e.zf = nullptr;
+                e.hb = next;
+                gemm<false, false, kFwdGelu>(rows, fo, fi, H, fi, wb_ + oW_[l], fi, e, st);
+                H = next;
+                next = (next == ha) ? hb : ha;
+            } else {
+                e.zf = out;
+                gemm<false, false, kFwdLinear>(rows, fo, fi, H, fi, wb_ + oW_[l], fi, e, st);
+            }
+        }
+        hipLaunchKernelGGL(apply_kernel, dim3(rows), dim3(64), 0, st, rows, N, s_.K, y0 + c0 * N, out, norm_,
                            s_.in_p, s_.out_p, comp_, rmw_, E_, y1 + c0 * N);
         check(hipGetLastError(), "surrogate apply");
     }

@@ -53,6 +53,13 @@ public:
                      hipStream_t stream);
     // y1 = y0 + net(y0) on device rows; m is carried over.
     void predict(long n, const double* y0, const double* P, double* y1, hipStream_t stream);
+    // The same without any allocation: intermediate activations live in a
+    // fixed 16 MiB device workspace (one per GPU, shared by every call on it,
+    // so calls that overlap on one GPU race); rows go through it in passes of
+    // noalloc_rows().  Throws if the network is too wide for 128 rows.
+    void predict_noalloc(long n, const double* y0, const double* P, double* y1, hipStream_t stream);
+    int noalloc_rows() const;
+    static size_t noalloc_workspace_bytes();
 
     // element table (K x E) and 1/W_k of the mechanism (device pointers): with
     // them, predictions conserve elements exactly (projection)

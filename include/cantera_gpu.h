@@ -233,6 +233,19 @@ int32_t cg_surrogate_loss_grad(int32_t surrogate, int64_t n, const double* y0, c
                                const double* y1, double* loss, float* grad);
 /* y1 = y0 + net(y0) (device arrays); m carried over. */
 int32_t cg_surrogate_predict(int32_t surrogate, int64_t n, const double* y0, const double* P, double* y1);
+/* cg_surrogate_predict without any allocation (no counterpart in the
+ * reference: for callers that cannot allocate device memory -- code under
+ * HIP graph capture, serving loops).  Same results as cg_surrogate_predict;
+ * asynchronous on `stream` (NULL: the default stream), no synchronisation.
+ * Intermediate activations live in one fixed 16 MiB workspace per GPU, part
+ * of the library and held until the process ends; rows pass through it in
+ * chunks (11,904 rows for a 4 x 256 gri30 network).  Every call on a GPU
+ * shares that GPU's workspace, so calls that overlap on one GPU (different
+ * streams or threads) race.  Rejects a network too wide for 128 rows per
+ * pass (per row: 2 (K+2 padded to 64) + 4 hidden + 4 (K+1 padded to 64)
+ * bytes). */
+int32_t cg_surrogate_predict_noalloc(int32_t surrogate, int64_t n, const double* y0, const double* P, double* y1,
+                                     void* stream);
 /* Parameters (host arrays): FP32 weights and biases layer by layer, weights
  * row-major [out][in] with inputs/outputs padded to multiples of 64;
  * normalisation = feature means, feature deviations, target means, target
