@@ -121,6 +121,21 @@ class Solution:
         return J
 
 
+    def equilibrate(self, T, P, Y, mode="HP"):
+        """Equilibrium states of n mixtures (ThermoPhase::equilibrate): mode
+        "HP" (fixed specific enthalpy and pressure) or "TP".  Y species-major
+        (K, n).  Returns (T_eq (n,), Y_eq (K, n), iterations (n,))."""
+        T = np.ascontiguousarray(np.atleast_1d(T), dtype=np.float64)
+        n = T.shape[0]
+        P = np.ascontiguousarray(np.broadcast_to(np.asarray(P, dtype=np.float64), (n,)))
+        Y = np.ascontiguousarray(np.asarray(Y, dtype=np.float64).reshape(self.n_species, n))
+        To = np.zeros(n)
+        Yo = np.zeros((self.n_species, n))
+        info = np.zeros(n, dtype=np.int32)
+        check(lib().cg_equilibrate_batch(self._h, n, mode.encode(), T, P, Y, To, Yo, info), "cg_equilibrate_batch")
+        return To, Yo, info
+
+
 class ReactorBatch:
     """n independent constant-pressure ideal-gas reactors, each with its own
     ReactorNet; ``advance`` mirrors ``ReactorNet::advance``.  ``const_volume=True``
@@ -138,6 +153,7 @@ class ReactorBatch:
         self._h = check(create(sol.handle, n, T0, P, Y0, float(rtol), float(atol), int(precision)),
                         "cg_reactors_create")
         self.time = np.zeros(n)
+        self.sensitivity_reactions = []
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -149,6 +165,27 @@ class ReactorBatch:
     @property
     def handle(self) -> int:
         return self._h
+
+    def add_sensitivity_reaction(self, rxn) -> int:
+        """Reactor::addSensitivityReaction for every reactor of the batch (a
+        multiplier on reaction ``rxn``'s rate, nominal 1); before the first
+        advance.  Returns the parameter index."""
+        p = check(lib().cg_reactors_add_sensitivity_reaction(self._h, int(rxn)), "add_sensitivity_reaction")
+        self.sensitivity_reactions.append(int(rxn))
+        return p
+
+    def set_sensitivity_tolerances(self, rtol=1e-4, atol=1e-6):
+        """ReactorNet.rtol_sensitivity / atol_sensitivity."""
+        check(lib().cg_reactors_set_sensitivity_tolerances(self._h, float(rtol), float(atol)))
+
+    def sensitivities(self, normalized=True) -> np.ndarray:
+        """(n, K+2, n_params): ReactorNet.sensitivities() of each reactor,
+        (dy_i/dp_j) / y_i; ``normalized=False`` gives dy_i/dp_j."""
+        npar = len(self.sensitivity_reactions)
+        S = np.zeros((self.n, self.neq, npar))
+        if npar:
+            check(lib().cg_reactors_get_sensitivities(self._h, 1 if normalized else 0, S))
+        return S
 
     def set_options(self, max_steps=0, waves_per_cu=-1):
         check(lib().cg_reactors_set_options(self._h, int(max_steps), int(waves_per_cu)))
@@ -183,6 +220,27 @@ class ReactorBatch:
         counts = (C.c_int64 * 3)()
         check(lib().cg_reactors_surrogate_counts(self._h, counts, 1 if reset else 0), "cg_reactors_surrogate_counts")
         return np.array(counts[:], dtype=np.int64)
+
+    def rates_to_device(self, w_dev, row=0):
+        """Queue the net production rates at the current states (species-major
+        K x n) into device memory at row `row` of an n_rows x K x n array."""
+        ptr = (w_dev if isinstance(w_dev, int) else w_dev.ptr) + row * self.n * self.sol.n_species * 8
+        check(lib().cg_reactors_net_production_rates_device(self._h, C.c_void_p(ptr)),
+              "cg_reactors_net_production_rates_device")
+
+    @property
+    def net_production_rates(self) -> np.ndarray:
+        """Net production rates (K, n) at the current reactor states."""
+        w = np.zeros((self.sol.n_species, self.n))
+        L = lib()
+        p = L.cg_dev_alloc(w.nbytes)
+        try:
+            self.rates_to_device(p)
+            check(L.cg_device_synchronize())
+            check(L.cg_memcpy_d2h(w.ctypes.data_as(C.c_void_p), p, w.nbytes))
+        finally:
+            L.cg_dev_free(p)
+        return w
 
     def state_to_device(self, y_dev, row=0):
         """Queue a copy of the states (n x (K+2), [m, T, Y]) into device memory

@@ -47,6 +47,10 @@ SIGNATURES = [
     ("cg_jac_batch_device", C.c_int32, [C.c_int32, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p,
                                         C.c_void_p, C.c_int32, C.c_void_p]),
     ("cg_jac_last_kernel_ms", C.c_double, [C.c_int32]),
+    ("cg_equilibrate_batch", C.c_int32, [C.c_int32, C.c_int64, C.c_char_p, _dp, _dp, _dp, _dp, _dp, _i32p]),
+    ("cg_equilibrate_batch_device", C.c_int32, [C.c_int32, C.c_int64, C.c_char_p, C.c_void_p, C.c_void_p,
+                                                C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32,
+                                                C.c_void_p]),
     ("cg_reactors_create", C.c_int32, [C.c_int32, C.c_int64, _dp, _dp, _dp, C.c_double, C.c_double, C.c_int32]),
     ("cg_reactors_create_const_volume", C.c_int32,
      [C.c_int32, C.c_int64, _dp, _dp, _dp, C.c_double, C.c_double, C.c_int32]),
@@ -58,6 +62,9 @@ SIGNATURES = [
     ("cg_reactors_get_state", C.c_int32, [C.c_int32, _dp]),
     ("cg_reactors_stats", C.c_int32, [C.c_int32, _i64p]),
     ("cg_reactors_status", C.c_int32, [C.c_int32, _i32p]),
+    ("cg_reactors_add_sensitivity_reaction", C.c_int32, [C.c_int32, C.c_int32]),
+    ("cg_reactors_set_sensitivity_tolerances", C.c_int32, [C.c_int32, C.c_double, C.c_double]),
+    ("cg_reactors_get_sensitivities", C.c_int32, [C.c_int32, C.c_int32, _dp]),
     ("cg_reactors_get_control", C.c_int32, [C.c_int32, _dp]),
     ("cg_reactors_set_options", C.c_int32, [C.c_int32, C.c_int32, C.c_int32]),
     ("cg_reactors_set_schedule", C.c_int32, [C.c_int32, C.c_int32]),
@@ -86,6 +93,7 @@ SIGNATURES = [
                                                          C.c_double]),
     ("cg_reactors_surrogate_counts", C.c_int32, [C.c_int32, C.POINTER(C.c_int64), C.c_int32]),
     ("cg_reactors_get_state_device", C.c_int32, [C.c_int32, C.c_void_p]),
+    ("cg_reactors_net_production_rates_device", C.c_int32, [C.c_int32, C.c_void_p]),
 ]
 
 

@@ -87,6 +87,15 @@ struct Batch {
     double* sg_P = nullptr;          // n
     int* sg_ints = nullptr;          // elig (n), fvalid (n), list (n), count (1)
     unsigned long long* sg_counters = nullptr;   // eligible, kept, integrated (accumulated)
+    // forward sensitivities to reaction rate multipliers (ReactorNet defaults,
+    // include/cantera/zeroD/ReactorNet.h: rtol 1e-4, atol 1e-6)
+    int nsens = 0;
+    int sens_rxn[kMaxSens] = {};     // permuted reaction index per parameter
+    int sens_rxn_orig[kMaxSens] = {};
+    double rtol_sens = 1e-4, atol_sens = 1e-6;
+    double* sens = nullptr;          // n x sens_stride doubles
+    size_t sens_stride = 0;
+    bool launched = false;           // an integrator launch has been issued
     void drop_graph() {
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
         if (graph) (void)hipGraphDestroy(graph);
@@ -99,7 +108,8 @@ struct Batch {
         for (void* p : {static_cast<void*>(rec), static_cast<void*>(jsave), lusave, static_cast<void*>(piv),
                         static_cast<void*>(tout), static_cast<void*>(queue), static_cast<void*>(lpt), sort_tmp,
                         static_cast<void*>(sg_Y), static_cast<void*>(sg_Yn), static_cast<void*>(sg_F),
-                        static_cast<void*>(sg_P), static_cast<void*>(sg_ints), static_cast<void*>(sg_counters)})
+                        static_cast<void*>(sg_P), static_cast<void*>(sg_ints), static_cast<void*>(sg_counters),
+                        static_cast<void*>(sens)})
             if (p) (void)hipFree(p);
         if (e0) (void)hipEventDestroy(e0);
         if (e1) (void)hipEventDestroy(e1);
@@ -376,6 +386,71 @@ int32_t cg_jac_batch(int32_t h, int64_t n, const double* T, const double* P, con
     });
 }
 
+static int equil_mode(const char* XY) {
+    if (!XY) throw std::runtime_error("null equilibrium mode");
+    const std::string s(XY);
+    if (s == "HP" || s == "PH") return 1;
+    if (s == "TP" || s == "PT") return 0;
+    throw std::runtime_error("equilibrium mode must be \"HP\" or \"TP\", got \"" + s + "\"");
+}
+
+int32_t cg_equilibrate_batch_device(int32_t h, int64_t n, const char* XY, const double* T, const double* P,
+                                    const double* Y, double* T_out, double* Y_out, int32_t* info, int32_t max_iter,
+                                    void* stream) {
+    return guarded([&] {
+        MechObj& mo = get_mech(h);
+        const int hp = equil_mode(XY);
+        if (n <= 0) return 0;
+        if (!T || !P || !Y || !T_out || !Y_out) throw std::runtime_error("null array");
+        if (max_iter < 1) throw std::runtime_error("max_iter must be positive");
+        const DevMech& m = mo.dm->view();
+        if (m.E > 8) throw std::runtime_error("equilibrium supports at most 8 elements");
+        const void* kp = equil_kernel_ptr(mo.ns);
+        const int grid = grid_for(kp, 0, 0, n);
+        check(equil_launch(m, mo.ns, hp, n, T, P, Y, T_out, Y_out, info, max_iter, grid,
+                           static_cast<hipStream_t>(stream)),
+              "equilibrium kernel launch");
+        return 0;
+    });
+}
+
+int32_t cg_equilibrate_batch(int32_t h, int64_t n, const char* XY, const double* T, const double* P, const double* Y,
+                             double* T_out, double* Y_out, int32_t* info) {
+    return guarded([&] {
+        MechObj& mo = get_mech(h);
+        equil_mode(XY);
+        if (n <= 0) return 0;
+        if (!T || !P || !Y || !T_out || !Y_out) throw std::runtime_error("null array");
+        const int K = mo.dm->view().K;
+        for (int64_t s = 0; s < n; s++) {
+            if (!(T[s] > 0.0) || !std::isfinite(T[s])) throw std::runtime_error("temperature must be positive and finite");
+            if (!(P[s] > 0.0) || !std::isfinite(P[s])) throw std::runtime_error("pressure must be positive and finite");
+        }
+        const size_t b = static_cast<size_t>(n) * 8;
+        double* d = nullptr;
+        check(hipMalloc(&d, b * (3 + 2 * static_cast<size_t>(K)) + static_cast<size_t>(n) * 4), "hipMalloc");
+        std::unique_ptr<double, void (*)(double*)> guard(d, [](double* p) { (void)hipFree(p); });
+        double *dT = d, *dP = d + n, *dY = d + 2 * n, *dTo = dY + static_cast<size_t>(K) * n,
+               *dYo = dTo + n;
+        int* dI = reinterpret_cast<int*>(dYo + static_cast<size_t>(K) * n);
+        check(hipMemcpy(dT, T, b, hipMemcpyHostToDevice), "hipMemcpy");
+        check(hipMemcpy(dP, P, b, hipMemcpyHostToDevice), "hipMemcpy");
+        check(hipMemcpy(dY, Y, b * K, hipMemcpyHostToDevice), "hipMemcpy");
+        if (cg_equilibrate_batch_device(h, n, XY, dT, dP, dY, dTo, dYo, dI, 500, nullptr) < 0)
+            throw std::runtime_error(g_error);
+        check(hipDeviceSynchronize(), "equilibrium");
+        check(hipMemcpy(T_out, dTo, b, hipMemcpyDeviceToHost), "hipMemcpy");
+        check(hipMemcpy(Y_out, dYo, b * K, hipMemcpyDeviceToHost), "hipMemcpy");
+        std::vector<int32_t> inf(n);
+        check(hipMemcpy(inf.data(), dI, static_cast<size_t>(n) * 4, hipMemcpyDeviceToHost), "hipMemcpy");
+        int bad = 0;
+        for (int64_t s = 0; s < n; s++) bad += inf[s] < 0;
+        if (info) std::memcpy(info, inf.data(), static_cast<size_t>(n) * 4);
+        if (bad) throw std::runtime_error(std::to_string(bad) + " state(s) did not reach equilibrium (see info)");
+        return 0;
+    });
+}
+
 double cg_jac_last_kernel_ms(int32_t h) {
     try {
         MechObj& mo = get_mech(h);
@@ -508,7 +583,7 @@ int32_t cg_reactors_set_matrix_storage(int32_t h, int32_t storage) {
         b.drop_graph();
         const DevMech& m = b.mech->dm->view();
         b.lds = bdf_lds_bytes(m, b.ld, b.fp32, b.global_matrix);
-        set_lds_limit(bdf_kernel_ptr(m.N, b.fp32, b.global_matrix), b.lds);
+        set_lds_limit(bdf_kernel_ptr(m.N, b.fp32, b.global_matrix, b.nsens > 0), b.lds);
         return 0;
     });
 }
@@ -523,6 +598,7 @@ int32_t cg_reactors_set_schedule(int32_t h, int32_t longest_first) {
 
 static BdfArgs advance_args(Batch& b, const int* list, int count) {
     BdfArgs a{};
+    b.launched = true;
     a.mech = b.mech->dm->view();
     a.mech.const_volume = b.const_volume ? 1 : 0;
     a.n = list ? count : b.n;
@@ -545,11 +621,17 @@ static BdfArgs advance_args(Batch& b, const int* list, int count) {
     int* order = b.lpt + 4 * static_cast<size_t>(b.n);
     a.order = list ? list : ((b.lpt_on && b.have_order) ? order : nullptr);
     a.queue = b.queue;
+    a.nsens = b.nsens;
+    for (int p = 0; p < b.nsens; p++) a.sens_rxn[p] = b.sens_rxn[p];
+    a.rtol_sens = b.rtol_sens;
+    a.atol_sens = b.atol_sens;
+    a.sens = b.sens;
+    a.sens_stride = b.sens_stride;
     return a;
 }
 
 static int advance_grid(const Batch& b) {
-    return grid_for(bdf_kernel_ptr(b.N, b.fp32, b.global_matrix), b.lds, b.waves_per_cu, b.n);
+    return grid_for(bdf_kernel_ptr(b.N, b.fp32, b.global_matrix, b.nsens > 0), b.lds, b.waves_per_cu, b.n);
 }
 
 // cost keys of the call just launched -> processing order of the next
@@ -630,6 +712,7 @@ int32_t cg_reactors_autotune(int32_t h, const double* t_probe_dev, int32_t* choi
     return guarded([&] {
         Batch& b = get_batch(h);
         if (!t_probe_dev) throw std::runtime_error("null t_probe");
+        if (b.nsens > 0) throw std::runtime_error("autotune runs on batches without sensitivity parameters");
         const DevMech& m = b.mech->dm->view();
         const int v = bdf_variant(m.N);
         // (matrix storage, waves per CU) candidates; storage 1 = LDS, 2 = global
@@ -653,6 +736,7 @@ int32_t cg_reactors_autotune(int32_t h, const double* t_probe_dev, int32_t* choi
             check(hipMemcpyAsync(snap[i], live[i], sizes[i], hipMemcpyDeviceToDevice, b.stream), "hipMemcpyAsync");
         }
         const bool had_order = b.have_order;
+        const bool launched0 = b.launched;   // trials are undone: not an initialisation
         auto restore = [&] {
             for (int i = 0; i < 5; i++)
                 check(hipMemcpyAsync(live[i], snap[i], sizes[i], hipMemcpyDeviceToDevice, b.stream), "hipMemcpyAsync");
@@ -693,11 +777,88 @@ int32_t cg_reactors_autotune(int32_t h, const double* t_probe_dev, int32_t* choi
         b.lds = bdf_lds_bytes(m, b.ld, b.fp32, b.global_matrix);
         set_lds_limit(bdf_kernel_ptr(m.N, b.fp32, b.global_matrix), b.lds);
         b.drop_graph();
+        b.launched = launched0;
         if (choice) {
             choice[0] = b.global_matrix ? 2 : ((v == 56 || v == 64) ? 1 : 0);
             choice[1] = b.waves_per_cu;
         }
         return static_cast<int32_t>(cand.size());
+    });
+}
+
+int32_t cg_reactors_add_sensitivity_reaction(int32_t h, int32_t rxn) {
+    return guarded([&] {
+        Batch& b = get_batch(h);
+        const std::vector<int>& perm = b.mech->dm->perm();
+        const int I = static_cast<int>(perm.size());
+        if (rxn < 0 || rxn >= I)
+            throw std::runtime_error("Reaction number out of range (" + std::to_string(rxn) + ")");
+        if (b.launched)
+            throw std::runtime_error("Sensitivity parameters cannot be added after the integrator has been initialized");
+        if (b.nsens >= kMaxSens)
+            throw std::runtime_error("at most " + std::to_string(kMaxSens) + " sensitivity parameters per batch");
+        if (b.fp32) throw std::runtime_error("sensitivities are integrated in FP64 precision only");
+        int pidx = -1;
+        for (int i = 0; i < I; i++)
+            if (perm[i] == rxn) pidx = i;
+        if (pidx < 0) throw std::runtime_error("reaction not found in the device tables");
+        const int NP = 64 * b.ns;
+        const size_t stride = static_cast<size_t>(b.nsens + 1) * kSensCols * NP;
+        double* buf = nullptr;
+        check(hipMalloc(&buf, stride * b.n * 8), "hipMalloc(sensitivities)");
+        check(hipMemset(buf, 0, stride * b.n * 8), "hipMemset(sensitivities)");
+        if (b.sens) check(hipFree(b.sens), "hipFree(sensitivities)");
+        b.sens = buf;
+        b.sens_stride = stride;
+        b.sens_rxn[b.nsens] = pidx;
+        b.sens_rxn_orig[b.nsens] = rxn;
+        b.nsens++;
+        b.drop_graph();
+        set_lds_limit(bdf_kernel_ptr(b.N, false, b.global_matrix, true), b.lds);
+        return b.nsens - 1;
+    });
+}
+
+int32_t cg_reactors_set_sensitivity_tolerances(int32_t h, double rtol, double atol) {
+    return guarded([&] {
+        Batch& b = get_batch(h);
+        if (!(rtol > 0) || !(atol > 0)) throw std::runtime_error("tolerances must be positive");
+        b.rtol_sens = rtol;
+        b.atol_sens = atol;
+        b.drop_graph();
+        return 0;
+    });
+}
+
+int32_t cg_reactors_get_sensitivities(int32_t h, int32_t normalized, double* S) {
+    return guarded([&] {
+        Batch& b = get_batch(h);
+        if (!S) throw std::runtime_error("null output");
+        check(hipStreamSynchronize(b.stream), "hipStreamSynchronize");
+        const int NP = 64 * b.ns, N = b.N, K = b.K, np = b.nsens;
+        if (np == 0) return 0;
+        std::vector<double> sb(b.sens_stride * b.n), st(static_cast<size_t>(b.n) * N), ctl(static_cast<size_t>(b.n) * C_COUNT);
+        check(hipMemcpy(sb.data(), b.sens, sb.size() * 8, hipMemcpyDeviceToHost), "hipMemcpy(sensitivities)");
+        check(hipMemcpy2D(st.data(), N * 8, b.rec + 6 * NP, b.rec_stride * 8, N * 8, b.n, hipMemcpyDeviceToHost),
+              "hipMemcpy2D(state)");
+        check(hipMemcpy2D(ctl.data(), C_COUNT * 8, b.rec + 7 * NP, b.rec_stride * 8, C_COUNT * 8, b.n,
+                          hipMemcpyDeviceToHost), "hipMemcpy2D(control)");
+        // reference order [m, T, Y_0..] <- device order [Y_0.., T, m]
+        auto dev = [&](int i) { return i == 0 ? K + 1 : (i == 1 ? K : i - 2); };
+        for (int s = 0; s < b.n; s++) {
+            const bool started = ctl[static_cast<size_t>(s) * C_COUNT + C_NST] > 0;
+            for (int i = 0; i < N; i++) {
+                const int d = dev(i);
+                double den = st[static_cast<size_t>(s) * N + d];
+                if (den == 0.0) den = 1e-300;   // SmallNumber
+                for (int p = 0; p < np; p++) {
+                    double v = started ? sb[s * b.sens_stride + (static_cast<size_t>(p) * kSensCols + 7) * NP + d] : 0.0;
+                    if (normalized) v /= den;
+                    S[(static_cast<size_t>(s) * N + i) * np + p] = v;
+                }
+            }
+        }
+        return 0;
     });
 }
 
@@ -908,6 +1069,7 @@ static void advance_surrogate_device(Batch& b, SurrogateObj& so, const double* t
                                      double gate_T) {
     if (so.mech != b.mech) throw std::runtime_error("surrogate and reactor batch use different mechanisms");
     if (b.const_volume) throw std::runtime_error("the surrogate path supports constant-pressure reactors only");
+    if (b.nsens > 0) throw std::runtime_error("the surrogate path does not integrate sensitivities");
     if (!t_out_dev) throw std::runtime_error("null t_out");
     if (!(tol_T > 0) || !(tol_Y > 0)) throw std::runtime_error("guard tolerances must be positive");
     const int n = b.n, N = b.N, K = b.K, NP = 64 * b.ns;
@@ -961,6 +1123,22 @@ int32_t cg_reactors_advance_surrogate_device(int32_t bh, int32_t sh, const doubl
                                              double tol_Y, double gate_T) {
     return guarded([&] {
         advance_surrogate_device(get_batch(bh), get_surr(sh), t_out_dev, tol_T, tol_Y, gate_T);
+        return 0;
+    });
+}
+
+int32_t cg_reactors_net_production_rates_device(int32_t bh, double* wdot_dev) {
+    return guarded([&] {
+        Batch& b = get_batch(bh);
+        if (!wdot_dev) throw std::runtime_error("null rates array");
+        const DevMech& m = b.mech->dm->view();
+        const size_t lds = rates_lds_bytes(m, b.ns);
+        const void* kp = reactor_rates_kernel_ptr(b.ns);
+        set_lds_limit(kp, lds);
+        const int grid = grid_for(kp, lds, 0, b.n);
+        check(reactor_rates_launch(m, b.ns, b.n, b.rec, b.rec_stride, b.const_volume ? 1 : 0, wdot_dev, grid, lds,
+                                   b.stream),
+              "reactor rates kernel launch");
         return 0;
     });
 }

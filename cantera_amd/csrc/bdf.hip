@@ -3,13 +3,13 @@
 
 namespace cg {
 
-const void* bdf_kernel_r16(bool);
-const void* bdf_kernel_r32(bool);
-const void* bdf_kernel_lr56(bool);
-const void* bdf_kernel_lr64(bool);
-const void* bdf_kernel_l2(bool);
-const void* bdf_kernel_gr56(bool);
-const void* bdf_kernel_gr64(bool);
+const void* bdf_kernel_r16(bool, bool);
+const void* bdf_kernel_r32(bool, bool);
+const void* bdf_kernel_lr56(bool, bool);
+const void* bdf_kernel_lr64(bool, bool);
+const void* bdf_kernel_l2(bool, bool);
+const void* bdf_kernel_gr56(bool, bool);
+const void* bdf_kernel_gr64(bool, bool);
 hipError_t bdf_launch_r16(const BdfArgs&, bool, int, size_t, hipStream_t);
 hipError_t bdf_launch_r32(const BdfArgs&, bool, int, size_t, hipStream_t);
 hipError_t bdf_launch_lr56(const BdfArgs&, bool, int, size_t, hipStream_t);
@@ -55,13 +55,13 @@ size_t bdf_lds_bytes(const DevMech& m, int ld, bool fp32, bool global) {
     return bytes;
 }
 
-const void* bdf_kernel_ptr(int N, bool fp32, bool global) {
+const void* bdf_kernel_ptr(int N, bool fp32, bool global, bool sens) {
     switch (bdf_variant(N)) {
-        case 16: return bdf_kernel_r16(fp32);
-        case 32: return bdf_kernel_r32(fp32);
-        case 56: return global ? bdf_kernel_gr56(fp32) : bdf_kernel_lr56(fp32);
-        case 64: return global ? bdf_kernel_gr64(fp32) : bdf_kernel_lr64(fp32);
-        case 2: return bdf_kernel_l2(fp32);
+        case 16: return bdf_kernel_r16(fp32, sens);
+        case 32: return bdf_kernel_r32(fp32, sens);
+        case 56: return global ? bdf_kernel_gr56(fp32, sens) : bdf_kernel_lr56(fp32, sens);
+        case 64: return global ? bdf_kernel_gr64(fp32, sens) : bdf_kernel_lr64(fp32, sens);
+        case 2: return bdf_kernel_l2(fp32, sens);
         default: return nullptr;
     }
 }

@@ -18,6 +18,11 @@ enum CtlIndex {
     C_COUNT = 64
 };
 
+// most reaction-multiplier sensitivity parameters per reactor batch
+constexpr int kMaxSens = 32;
+// columns of a parameter's block: Nordsieck history 0..5, corrector 6, output 7
+constexpr int kSensCols = 8;
+
 struct BdfArgs {
     DevMech mech;
     int n;                // reactors
@@ -37,6 +42,12 @@ struct BdfArgs {
     const int* order;     // optional processing order (n), device
     const int* n_dev;     // optional: number of work items, read on the device (replaces n)
     int* queue;           // device work-queue head
+    // forward sensitivities to reaction rate multipliers (sensitivity kernels only)
+    int nsens;                      // parameters (<= kMaxSens)
+    int sens_rxn[kMaxSens];         // permuted reaction index of each parameter
+    double rtol_sens, atol_sens;
+    double* sens;                   // n x sens_stride doubles: per parameter 8 columns of 64*NS
+    size_t sens_stride;
     // typed view used by the kernel
     template <class T> __host__ __device__ T* lu() const { return static_cast<T*>(lu_save_raw); }
 };
@@ -50,8 +61,9 @@ size_t bdf_lu_elems(int N, int ld);
 // global: for 32 < N <= 64, keep the iteration matrix in the reactor's
 // global-memory factor slot instead of LDS
 size_t bdf_lds_bytes(const DevMech& m, int ld, bool fp32, bool global);
+// a.nsens > 0 selects the FP64 kernels that also integrate the sensitivities
 hipError_t bdf_launch(const BdfArgs& a, bool fp32, bool global, int grid, size_t lds, hipStream_t stream);
-const void* bdf_kernel_ptr(int N, bool fp32, bool global);
+const void* bdf_kernel_ptr(int N, bool fp32, bool global, bool sens = false);
 
 // batched rate evaluation (species-major SoA in and out)
 size_t rates_lds_bytes(const DevMech& m, int ns);
@@ -59,6 +71,10 @@ size_t rates_lds_bytes(const DevMech& m, int ns);
 hipError_t rates_launch(const DevMech& m, int ns, bool f32, long n, const double* T, const double* P,
                         const double* Y, double* wdot, int grid, size_t lds, hipStream_t stream);
 const void* rates_kernel_ptr(int ns, bool f32);
+// rates at the output states of a reactor batch (records as the integrator's)
+hipError_t reactor_rates_launch(const DevMech& m, int ns, long n, const double* rec, size_t stride, int cv,
+                                double* wdot, int grid, size_t lds, hipStream_t stream);
+const void* reactor_rates_kernel_ptr(int ns);
 
 // longest-first processing order for the integrator's work queue (kernels
 // only, so it can be captured in a HIP graph); prev_nst, bucket, rank and
@@ -87,5 +103,11 @@ size_t jac_lds_bytes(const DevMech& m, int ns);
 hipError_t jac_launch(const DevMech& m, int ns, bool f32, long n, const double* T, const double* P, const double* Y,
                       void* J, int grid, size_t lds, hipStream_t stream);
 const void* jac_kernel_ptr(int ns, bool f32);
+
+// batched chemical equilibrium (hp = 1: fixed h and P; 0: fixed T and P),
+// inputs and outputs species-major; info[s] = iterations, or negative
+hipError_t equil_launch(const DevMech& m, int ns, int hp, long n, const double* T, const double* P, const double* Y,
+                        double* Tout, double* Yout, int* info, int max_iter, int grid, hipStream_t stream);
+const void* equil_kernel_ptr(int ns);
 
 }  // namespace cg

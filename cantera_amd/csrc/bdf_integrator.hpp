@@ -79,7 +79,19 @@ __device__ __forceinline__ double wrms(const double (&v)[NS], const double (&w)[
     return uni(sqrt(wave_sum(s) / N));
 }
 
-template <int NS, class LS, typename MT>
+// the integrator kernel's one parameter, addressed in the kernel-argument
+// segment (read with scalar loads wherever the index is run-time)
+using KernargArgs = const __attribute__((address_space(4))) BdfArgs;
+__device__ __forceinline__ KernargArgs* kernarg_args() {
+    return (KernargArgs*)__builtin_amdgcn_kernarg_segment_ptr();
+}
+
+// kSens: also integrate the forward sensitivities dy/dp to the reaction-rate
+// multipliers a.sens_rxn (CVODES staggered corrector, as the reference runs
+// it: CV_STAGGERED, sensitivities out of the error test,
+// src/numerics/CVodesIntegrator.cpp:263-279).  Each parameter's Nordsieck
+// history is kept in global memory and updated alongside zn.
+template <int NS, class LS, typename MT, bool kSens = false>
 struct Integrator {
     const DevMech& m;
     const BdfArgs& a;
@@ -93,6 +105,7 @@ struct Integrator {
     double ewt[NS], acor[NS], y[NS], ftemp[NS], tempv[NS];
     double* rec;
     size_t rid;
+    double* sb = nullptr;   // this reactor's sensitivity block (kSens)
 
     __device__ __forceinline__ Integrator(const DevMech& m_, const BdfArgs& a_, Ctl& c_) : m(m_), a(a_), c(c_) {}
 
@@ -101,6 +114,7 @@ struct Integrator {
     __device__ __forceinline__ void load(size_t r) {
         rid = r;
         rec = a.rec + r * a.rec_stride;
+        if constexpr (kSens) sb = a.sens + r * a.sens_stride;
         const int NP = kWave * NS;
 #pragma unroll
         for (int j = 0; j <= bdfc::QMAX; j++)
@@ -174,6 +188,120 @@ struct Integrator {
         }
         wave_sync();
         return __any(bad) ? 1 : 0;
+    }
+
+    // ---- sensitivity storage: column j of parameter p, this lane's components
+    __device__ __forceinline__ double* scol(int p, int j) const {
+        return sb + (static_cast<size_t>(p) * kSensCols + j) * (kWave * NS);
+    }
+    __device__ __forceinline__ void sload(int p, int j, double (&v)[NS]) const {
+        const double* col = scol(p, j);
+#pragma unroll
+        for (int s = 0; s < NS; s++) v[s] = col[comp(s)];
+    }
+    __device__ __forceinline__ void sstore(int p, int j, const double (&v)[NS]) const {
+        double* col = scol(p, j);
+#pragma unroll
+        for (int s = 0; s < NS; s++) col[comp(s)] = v[s];
+    }
+    __device__ __forceinline__ void sload_all(int p, double (&z)[bdfc::QMAX + 1][NS]) const {
+#pragma unroll
+        for (int j = 0; j <= bdfc::QMAX; j++) sload(p, j, z[j]);
+    }
+    __device__ __forceinline__ void sstore_all(int p, const double (&z)[bdfc::QMAX + 1][NS]) const {
+#pragma unroll
+        for (int j = 0; j <= bdfc::QMAX; j++) sstore(p, j, z[j]);
+    }
+    // z[j] for a run-time j, chosen on values (as pick)
+    __device__ __forceinline__ static void pick_of(const double (&z)[bdfc::QMAX + 1][NS], int j, double (&out)[NS]) {
+#pragma unroll
+        for (int s = 0; s < NS; s++) out[s] = z[0][s];
+#pragma unroll
+        for (int t = 1; t <= bdfc::QMAX; t++)
+#pragma unroll
+            for (int s = 0; s < NS; s++) out[s] = (t == j) ? z[t][s] : out[s];
+    }
+
+    // right-hand side with reaction rx at rate multiplier v (not counted in nfe)
+    __device__ __forceinline__ int f_eval_pert(const double (&yy)[NS], double (&ff)[NS], int rx, double v) {
+#pragma unroll
+        for (int s = 0; s < NS; s++)
+            if (comp(s) < N) S.yb[comp(s)] = yy[s];
+        wave_sync();
+        rhs_pert_call<NS>(kernarg_mech(), L, uni(c.P), rx, v);
+        int bad = 0;
+#pragma unroll
+        for (int s = 0; s < NS; s++) {
+            ff[s] = (comp(s) < N) ? S.F[comp(s)] : 0.0;
+            if (comp(s) < N && !isfinite(ff[s])) bad = 1;
+        }
+        wave_sync();
+        return __any(bad) ? 1 : 0;
+    }
+
+    // Sensitivity right-hand side J s + df/dp for parameter p at (yy, sv): the
+    // centred difference quotient CVODES forms without a user sensitivity
+    // function (cvSensRhs1InternalDQ, CENTERED1; parameter value and scale 1
+    // as Reactor::addSensitivityReaction registers them, src/zeroD/
+    // Reactor.cpp:278-289), [f(y + D s; k_p (1 + D)) - f(y - D s; k_p (1 - D))]
+    // / 2D with D = min(sqrt(max(rtol, uround)), 1 / |s|_wrms), state weights.
+    __device__ __forceinline__ int sens_rhs(int p, const double (&yy)[NS], const double (&sv)[NS], double (&fS)[NS]) {
+        const double del = sqrt(fmax(rtol, bdfc::UROUND));
+        const double nrm = wrms<NS>(sv, ewt, a.neq_norm);
+        const double D = fmin(del, 1.0 / fmax(nrm, 1.0 / del));
+        const int rx = kernarg_args()->sens_rxn[p];
+        double yp[NS], fp[NS];
+#pragma unroll
+        for (int s = 0; s < NS; s++) yp[s] = yy[s] + D * sv[s];
+        if (f_eval_pert(yp, fp, rx, 1.0 + D)) return 1;
+#pragma unroll
+        for (int s = 0; s < NS; s++) yp[s] = yy[s] - D * sv[s];
+        if (f_eval_pert(yp, fS, rx, 1.0 - D)) return 1;
+#pragma unroll
+        for (int s = 0; s < NS; s++) fS[s] = (fp[s] - fS[s]) / (2.0 * D);
+        return 0;
+    }
+
+    // Staggered corrector, after the state's step passed its error test:
+    // Newton on the (linear) BDF equation of each s_n with the state's
+    // iteration matrix, to the state's convergence test on the sensitivity
+    // weights 1 / (rtol_sens |s| + atol_sens); at most 10 iterations.  The
+    // correction is left in column 6.
+    __device__ __forceinline__ int sens_correct() {
+        using namespace bdfc;
+        const double gamma = uni(c.gamma), rl1 = uni(c.rl1), gamrat = uni(c.gamrat), tq4 = uni(c.tq[4]);
+        for (int p = 0; p < a.nsens; p++) {
+            double z0[NS], z1[NS], acs[NS], ews[NS], d[NS], f[NS];
+            sload(p, 0, z0);
+            sload(p, 1, z1);
+#pragma unroll
+            for (int s = 0; s < NS; s++) {
+                ews[s] = (comp(s) < N) ? 1.0 / (a.rtol_sens * fabs(z0[s]) + a.atol_sens) : 0.0;
+                acs[s] = 0.0;
+            }
+            double crateS = 1.0, delpS = 0.0;
+            for (int m_it = 0; m_it < 10; m_it++) {
+#pragma unroll
+                for (int s = 0; s < NS; s++) d[s] = z0[s] + acs[s];
+                if (sens_rhs(p, y, d, f)) return 1;
+#pragma unroll
+                for (int s = 0; s < NS; s++) d[s] = -(rl1 * z1[s] + acs[s] - gamma * f[s]);
+                ls.solve(d, N);
+                if (gamrat != 1.0) {
+                    const double sc = 2.0 / (1.0 + gamrat);
+#pragma unroll
+                    for (int s = 0; s < NS; s++) d[s] *= sc;
+                }
+#pragma unroll
+                for (int s = 0; s < NS; s++) acs[s] += d[s];
+                const double del = wrms<NS>(d, ews, a.neq_norm);
+                if (m_it > 0) crateS = fmax(CRDOWN * crateS, del / delpS);
+                if (del * fmin(1.0, crateS) / tq4 <= 1.0) break;
+                delpS = del;
+            }
+            sstore(p, 6, acs);
+        }
+        return 0;
     }
 
     __device__ __forceinline__ void ewt_set(const double (&yy)[NS]) {
@@ -305,6 +433,20 @@ struct Integrator {
                 if (k <= q && j >= k && j <= q)
 #pragma unroll
                     for (int s = 0; s < NS; s++) zn[j - 1][s] += zn[j][s];
+        if constexpr (kSens) {
+            for (int p = 0; p < a.nsens; p++) {
+                double z[bdfc::QMAX + 1][NS];
+                sload_all(p, z);
+#pragma unroll
+                for (int k = 1; k <= bdfc::QMAX; k++)
+#pragma unroll
+                    for (int j = bdfc::QMAX; j >= 1; j--)
+                        if (k <= q && j >= k && j <= q)
+#pragma unroll
+                            for (int s = 0; s < NS; s++) z[j - 1][s] += z[j][s];
+                sstore_all(p, z);
+            }
+        }
         wave_sync();
     }
     __device__ __forceinline__ void restore(double saved_t) {
@@ -317,6 +459,20 @@ struct Integrator {
                 if (k <= q && j >= k && j <= q)
 #pragma unroll
                     for (int s = 0; s < NS; s++) zn[j - 1][s] -= zn[j][s];
+        if constexpr (kSens) {
+            for (int p = 0; p < a.nsens; p++) {
+                double z[bdfc::QMAX + 1][NS];
+                sload_all(p, z);
+#pragma unroll
+                for (int k = 1; k <= bdfc::QMAX; k++)
+#pragma unroll
+                    for (int j = bdfc::QMAX; j >= 1; j--)
+                        if (k <= q && j >= k && j <= q)
+#pragma unroll
+                            for (int s = 0; s < NS; s++) z[j - 1][s] -= z[j][s];
+                sstore_all(p, z);
+            }
+        }
         wave_sync();
     }
     __device__ __forceinline__ void rescale() {
@@ -329,6 +485,22 @@ struct Integrator {
 #pragma unroll
                 for (int s = 0; s < NS; s++) zn[j][s] *= factor;
                 factor *= eta;
+            }
+        }
+        if constexpr (kSens) {
+            for (int p = 0; p < a.nsens; p++) {
+                double z[bdfc::QMAX + 1][NS];
+                sload_all(p, z);
+                double fac = eta;
+#pragma unroll
+                for (int j = 1; j <= bdfc::QMAX; j++) {
+                    if (j <= q) {
+#pragma unroll
+                        for (int s = 0; s < NS; s++) z[j][s] *= fac;
+                        fac *= eta;
+                    }
+                }
+                sstore_all(p, z);
             }
         }
         const double h = uni(c.hscale) * eta;
@@ -393,6 +565,24 @@ struct Integrator {
             if (j <= q)
 #pragma unroll
                 for (int s = 0; s < NS; s++) zn[j][s] += ll[j] * zL[s];
+        if constexpr (kSens) {
+            for (int p = 0; p < a.nsens; p++) {
+                double z[QMAX + 1][NS];
+                sload_all(p, z);
+#pragma unroll
+                for (int s = 0; s < NS; s++) zL[s] = A1 * z[QMAX][s];
+#pragma unroll
+                for (int j = 2; j <= QMAX; j++)
+#pragma unroll
+                    for (int s = 0; s < NS; s++) z[j][s] = (j == q + 1) ? zL[s] : z[j][s];
+#pragma unroll
+                for (int j = 2; j <= QMAX; j++)
+                    if (j <= q)
+#pragma unroll
+                        for (int s = 0; s < NS; s++) z[j][s] += ll[j] * zL[s];
+                sstore_all(p, z);
+            }
+        }
     }
     __device__ __forceinline__ void decrease_bdf() {
         using namespace bdfc;
@@ -419,6 +609,19 @@ struct Integrator {
             if (j < q)
 #pragma unroll
                 for (int s = 0; s < NS; s++) zn[j][s] = -ll[j] * zq[s] + zn[j][s];
+        if constexpr (kSens) {
+            for (int p = 0; p < a.nsens; p++) {
+                double z[QMAX + 1][NS];
+                sload_all(p, z);
+                pick_of(z, q, zq);
+#pragma unroll
+                for (int j = 2; j < QMAX; j++)
+                    if (j < q)
+#pragma unroll
+                        for (int s = 0; s < NS; s++) z[j][s] = -ll[j] * zq[s] + z[j][s];
+                sstore_all(p, z);
+            }
+        }
     }
 
     // modified Newton iteration; 0 ok, 1 recoverable failure
@@ -566,6 +769,16 @@ struct Integrator {
         if (f_eval(zn[0], tempv)) return -1;
 #pragma unroll
         for (int s = 0; s < NS; s++) zn[1][s] = h * tempv[s];
+        if constexpr (kSens) {
+            for (int p = 0; p < a.nsens; p++) {
+                double z0[NS], f[NS];
+                sload(p, 0, z0);
+                if (sens_rhs(p, zn[0], z0, f)) return -1;
+#pragma unroll
+                for (int s = 0; s < NS; s++) f[s] = h * f[s];
+                sstore(p, 1, f);
+            }
+        }
         return 2;
     }
 
@@ -594,6 +807,22 @@ struct Integrator {
         if (qwait == 1 && q != QMAX) {
 #pragma unroll
             for (int s = 0; s < NS; s++) zn[QMAX][s] = acor[s];
+        }
+        if constexpr (kSens) {
+            for (int p = 0; p < a.nsens; p++) {
+                double z[QMAX + 1][NS], acs[NS];
+                sload_all(p, z);
+                sload(p, 6, acs);
+#pragma unroll
+                for (int j = 0; j <= QMAX; j++)
+                    if (j <= q)
+#pragma unroll
+                        for (int s = 0; s < NS; s++) z[j][s] += l[j] * acs[s];
+                if (qwait == 1 && q != QMAX)
+#pragma unroll
+                    for (int s = 0; s < NS; s++) z[QMAX][s] = acs[s];
+                sstore_all(p, z);
+            }
         }
         const double tq5 = uni(c.tq[5]);
         wave_sync();
@@ -684,6 +913,13 @@ struct Integrator {
             qprime = q + 1;
 #pragma unroll
             for (int s = 0; s < NS; s++) zn[QMAX][s] = acor[s];
+            if constexpr (kSens) {
+                for (int p = 0; p < a.nsens; p++) {
+                    double acs[NS];
+                    sload(p, 6, acs);
+                    sstore(p, QMAX, acs);
+                }
+            }
         }
         set_eta(eta, qprime);
     }
@@ -748,6 +984,8 @@ struct Integrator {
             if (e < 0) return -4;
             break;
         }
+        if constexpr (kSens)
+            if (sens_correct()) return -5;
         complete_step();
         prepare_next_step(dsm);
         const double etamax = (uni(c.nst) <= SMALL_NST) ? ETAMX2 : ETAMX3;
@@ -824,6 +1062,19 @@ struct Integrator {
         const double h = uni(c.h);
 #pragma unroll
         for (int s = 0; s < NS; s++) zn[1][s] *= h;
+        if constexpr (kSens) {
+            // s(t0) = 0 (CVodesIntegrator::sensInit); zS[1] = h (J 0 + df/dp)
+            for (int p = 0; p < a.nsens; p++) {
+                double zero[NS], f[NS];
+#pragma unroll
+                for (int s = 0; s < NS; s++) zero[s] = 0.0;
+                for (int j = 0; j < kSensCols; j++) sstore(p, j, zero);
+                if (sens_rhs(p, zn[0], zero, f)) return -5;
+#pragma unroll
+                for (int s = 0; s < NS; s++) f[s] = f[s] * h;
+                sstore(p, 1, f);
+            }
+        }
         wave_sync();
         if (lane == 0) {
             c.hscale = h;
@@ -869,6 +1120,20 @@ struct Integrator {
 #pragma unroll
         for (int s = 0; s < NS; s++)
             if (comp(s) < N) out[comp(s)] = v[s];
+        if constexpr (kSens) {
+            // CVodeGetSensDky(tout, 0) into column 7
+            for (int p = 0; p < a.nsens; p++) {
+                double z[bdfc::QMAX + 1][NS];
+                sload_all(p, z);
+                pick_of(z, q, v);
+#pragma unroll
+                for (int j = bdfc::QMAX - 1; j >= 0; j--)
+                    if (j < q)
+#pragma unroll
+                        for (int s = 0; s < NS; s++) v[s] = z[j][s] + sarg * v[s];
+                sstore(p, 7, v);
+            }
+        }
         wave_sync();
         if (lane == 0) c.t_user = tout;
         wave_sync();
@@ -882,7 +1147,7 @@ __host__ __device__ inline size_t bdf_scratch_doubles(int K, int ns) {
     return static_cast<size_t>(kWave) * ns * 4 + K + K + K + 2 * K + kCtlDoubles;
 }
 
-template <int NS, class LS, typename MT>
+template <int NS, class LS, typename MT, bool kSens = false>
 __device__ __forceinline__ void bdf_advance_body(const BdfArgs& a) {
     const DevMech& m = a.mech;
     ScratchLayout L;
@@ -898,7 +1163,7 @@ __device__ __forceinline__ void bdf_advance_body(const BdfArgs& a) {
     double* base = lds_base() + off;
     Ctl& c = *reinterpret_cast<Ctl*>(base);
     base += kCtlDoubles;
-    Integrator<NS, LS, MT> it(m, a, c);
+    Integrator<NS, LS, MT, kSens> it(m, a, c);
     it.S = scratch_view(L);
     it.L = L;
     it.lane = lane_id();

@@ -3,19 +3,22 @@
 
 namespace cg {
 
-template <typename MT>
+template <typename MT, bool kSens>
 __global__ __launch_bounds__(64) void bdf_advance_lr64(BdfArgs a) {
-    bdf_advance_body<1, LdsRegLinSys<64, MT>, MT>(a);
+    bdf_advance_body<1, LdsRegLinSys<64, MT>, MT, kSens>(a);
 }
 
-const void* bdf_kernel_lr64(bool fp32) {
-    return fp32 ? reinterpret_cast<const void*>(&bdf_advance_lr64<float>)
-                : reinterpret_cast<const void*>(&bdf_advance_lr64<double>);
+// sens: the FP64 kernel that also integrates sensitivities
+const void* bdf_kernel_lr64(bool fp32, bool sens) {
+    if (sens) return reinterpret_cast<const void*>(&bdf_advance_lr64<double, true>);
+    return fp32 ? reinterpret_cast<const void*>(&bdf_advance_lr64<float, false>)
+                : reinterpret_cast<const void*>(&bdf_advance_lr64<double, false>);
 }
 
 hipError_t bdf_launch_lr64(const BdfArgs& a, bool fp32, int grid, size_t lds, hipStream_t stream) {
-    if (fp32) hipLaunchKernelGGL(bdf_advance_lr64<float>, dim3(grid), dim3(64), lds, stream, a);
-    else hipLaunchKernelGGL(bdf_advance_lr64<double>, dim3(grid), dim3(64), lds, stream, a);
+    if (a.nsens > 0) hipLaunchKernelGGL((bdf_advance_lr64<double, true>), dim3(grid), dim3(64), lds, stream, a);
+    else if (fp32) hipLaunchKernelGGL((bdf_advance_lr64<float, false>), dim3(grid), dim3(64), lds, stream, a);
+    else hipLaunchKernelGGL((bdf_advance_lr64<double, false>), dim3(grid), dim3(64), lds, stream, a);
     return hipGetLastError();
 }
 

@@ -3,19 +3,22 @@
 
 namespace cg {
 
-template <typename MT>
+template <typename MT, bool kSens>
 __global__ __launch_bounds__(64) void bdf_advance_r16(BdfArgs a) {
-    bdf_advance_body<1, RegLinSys<16, MT>, MT>(a);
+    bdf_advance_body<1, RegLinSys<16, MT>, MT, kSens>(a);
 }
 
-const void* bdf_kernel_r16(bool fp32) {
-    return fp32 ? reinterpret_cast<const void*>(&bdf_advance_r16<float>)
-                : reinterpret_cast<const void*>(&bdf_advance_r16<double>);
+// sens: the FP64 kernel that also integrates sensitivities
+const void* bdf_kernel_r16(bool fp32, bool sens) {
+    if (sens) return reinterpret_cast<const void*>(&bdf_advance_r16<double, true>);
+    return fp32 ? reinterpret_cast<const void*>(&bdf_advance_r16<float, false>)
+                : reinterpret_cast<const void*>(&bdf_advance_r16<double, false>);
 }
 
 hipError_t bdf_launch_r16(const BdfArgs& a, bool fp32, int grid, size_t lds, hipStream_t stream) {
-    if (fp32) hipLaunchKernelGGL(bdf_advance_r16<float>, dim3(grid), dim3(64), lds, stream, a);
-    else hipLaunchKernelGGL(bdf_advance_r16<double>, dim3(grid), dim3(64), lds, stream, a);
+    if (a.nsens > 0) hipLaunchKernelGGL((bdf_advance_r16<double, true>), dim3(grid), dim3(64), lds, stream, a);
+    else if (fp32) hipLaunchKernelGGL((bdf_advance_r16<float, false>), dim3(grid), dim3(64), lds, stream, a);
+    else hipLaunchKernelGGL((bdf_advance_r16<double, false>), dim3(grid), dim3(64), lds, stream, a);
     return hipGetLastError();
 }
 

@@ -55,7 +55,62 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void ra
     }
 }
 
+// Net production rates at the output states of a reactor batch (one record
+// per reactor: the interpolated state [Y, T, m] after the 6 Nordsieck columns,
+// then the control block with P, or V for a constant-volume reactor).
+template <int NS>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void reactor_rates_kernel(
+    DevMech m, long n, const double* __restrict__ rec, size_t stride, int cv, double* __restrict__ wdot) {
+    extern __shared__ double lds_raw[];
+    const int lane = lane_id();
+    double* base = lds_raw;
+    WaveScratch S{};
+    S.yb = base;  base += kWave * NS;
+    S.C = base;   base += m.K;
+    S.MU = base;  base += m.K;
+    S.WD = base;
+    constexpr int NP = kWave * NS;
+    const XcdStates xs(n);
+    for (long st = xs.lo; st < xs.hi; st += xs.step) {
+        const double* rr = rec + static_cast<size_t>(st) * stride;
+        const double Ts = uni(rr[6 * NP + m.K]);
+        const double mass = uni(rr[6 * NP + m.K + 1]);
+        const double pv = uni(rr[7 * NP + C_P]);
+        double yk[NS];
+#pragma unroll
+        for (int s = 0; s < NS; s++) {
+            const int k = lane + kWave * s;
+            yk[s] = (k < m.K) ? rr[6 * NP + k] : 0.0;
+        }
+        GasState g;
+        SpeciesRegs<NS> sr;
+        species_pass<NS, true>(m, S, Ts, cv ? 0.0 : pv, yk, g, sr, cv ? mass / pv : 0.0);
+        production_rates<NS, false>(m, S, g, sr);
+#pragma unroll
+        for (int s = 0; s < NS; s++) {
+            const int k = lane + kWave * s;
+            if (k < m.K) wdot[static_cast<size_t>(k) * n + st] = sr.wdot[s];
+        }
+        wave_sync();
+    }
+}
+
 }  // namespace
+
+hipError_t reactor_rates_launch(const DevMech& m, int ns, long n, const double* rec, size_t stride, int cv,
+                                double* wdot, int grid, size_t lds, hipStream_t stream) {
+    if (ns == 1)
+        hipLaunchKernelGGL((reactor_rates_kernel<1>), dim3(grid), dim3(64), lds, stream, m, n, rec, stride, cv, wdot);
+    else if (ns == 2)
+        hipLaunchKernelGGL((reactor_rates_kernel<2>), dim3(grid), dim3(64), lds, stream, m, n, rec, stride, cv, wdot);
+    else return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+const void* reactor_rates_kernel_ptr(int ns) {
+    return ns == 1 ? reinterpret_cast<const void*>(&reactor_rates_kernel<1>)
+                   : reinterpret_cast<const void*>(&reactor_rates_kernel<2>);
+}
 
 size_t rates_lds_bytes(const DevMech& m, int ns) {
     const size_t d = static_cast<size_t>(kWave) * ns + 2 * m.K + m.K;

@@ -383,10 +383,12 @@ __device__ __noinline__ double sri_falloff(const double* __restrict__ fo, double
 // kpd: this lane's pdep_rate_lane value (used for flags bit 6/7 only).
 // kF32: rate constants (Arrhenius, falloff centre and blending) through the
 // FP32 transcendentals above; everything else FP64.
+// perturb: the reaction's rate multiplier (m_perturb, Kinetics::setMultiplier),
+// applied to the forward rate of progress before the reverse one is formed.
 template <bool kF32 = false>
 __device__ __forceinline__ void eval_reaction(const DevMech& m, int p, const GasState& g,
                                               const double* __restrict__ C, const double* __restrict__ MU,
-                                              const RxnRegs& x, double kpd, RxnVals& o) {
+                                              const RxnRegs& x, double kpd, RxnVals& o, double perturb = 1.0) {
     const double4 rd = reinterpret_cast<const double4*>(m.rd)[p];
     const int type = (x.flags >> 1) & 3;
     const bool rev = x.flags & 1;
@@ -478,7 +480,7 @@ __device__ __forceinline__ void eval_reaction(const DevMech& m, int p, const Gas
         rkcn = fmin(exp(delta * g.rrt - rd.w * g.logC0), kBigNumber);
     }
     o.rkcn = rkcn;
-    double ropf = kf * 1.0;
+    double ropf = kf * perturb;
     if (type == kThreeBody) ropf *= M;
     double ropr = ropf * rkcn;
     // concentration products with the reference's negative-concentration rules
@@ -609,9 +611,10 @@ __device__ __forceinline__ void species_pass(const DevMech& m, const WaveScratch
 // products with LDS FP64 atomics (the species lists are already in registers),
 // so no second pass over a gather table is needed; the summation order of a
 // species' terms is the hardware's, not the reference's reaction order.
-template <int NS, bool kF32 = false>
+// kPert: reaction pert_rxn (permuted index) runs with rate multiplier pert_v.
+template <int NS, bool kF32 = false, bool kPert = false>
 __device__ __forceinline__ void production_rates(const DevMech& m, const WaveScratch& S, const GasState& g,
-                                                 SpeciesRegs<NS>& sr) {
+                                                 SpeciesRegs<NS>& sr, int pert_rxn = -1, double pert_v = 1.0) {
     const int lane = lane_id();
 #pragma unroll
     for (int s = 0; s < NS; s++) {
@@ -624,7 +627,8 @@ __device__ __forceinline__ void production_rates(const DevMech& m, const WaveScr
         RxnVals v;
         RxnRegs x;
         load_rxn(m, p, x);
-        eval_reaction<kF32>(m, p, g, S.C, S.MU, x, kpd, v);
+        if constexpr (kPert) eval_reaction<kF32>(m, p, g, S.C, S.MU, x, kpd, v, p == pert_rxn ? pert_v : 1.0);
+        else eval_reaction<kF32>(m, p, g, S.C, S.MU, x, kpd, v);
         const double r = v.ropf - v.ropr;
         const double* gs = (x.gen >= 0) ? m.gend + 12 * x.gen : m.gend;
         const bool rgen = x.flags & 8, pgen = x.flags & 16;
@@ -654,9 +658,10 @@ struct RhsScalars {
 // divided by the left-hand sides as ReactorNet::eval does: ydot for the lane's
 // components (internal order).  y holds [Y_0..Y_{K-1}, T, m] distributed; P is
 // the pressure, or the volume of a constant-volume reactor.
-template <int NS>
+template <int NS, bool kPert = false>
 __device__ __forceinline__ void reactor_rhs(const DevMech& m, const WaveScratch& S, double P, const double (&y)[NS],
-                                            double (&f)[NS], GasState& g, SpeciesRegs<NS>& sr, RhsScalars& rs) {
+                                            double (&f)[NS], GasState& g, SpeciesRegs<NS>& sr, RhsScalars& rs,
+                                            int pert_rxn = -1, double pert_v = 1.0) {
     const int lane = lane_id();
 #pragma unroll
     for (int s = 0; s < NS; s++) {
@@ -668,7 +673,7 @@ __device__ __forceinline__ void reactor_rhs(const DevMech& m, const WaveScratch&
     const double mass = uni(S.yb[m.K + 1]);
     const bool cv = m.const_volume;
     species_pass<NS>(m, S, T, P, y, g, sr, cv ? mass / P : 0.0);
-    production_rates<NS>(m, S, g, sr);
+    production_rates<NS, false, kPert>(m, S, g, sr, pert_rxn, pert_v);
     const double vol = cv ? P : mass / g.dens;
     const double hshift = cv ? 1.0 : 0.0;   // u_k / RT = h_k / RT - 1
     double e = 0.0, inner = 0.0;
@@ -728,6 +733,31 @@ __device__ __noinline__ void rhs_call(KernargMech* mech, ScratchLayout L, double
     SpeciesRegs<NS> sr;
     RhsScalars rs;
     reactor_rhs<NS>(m, S, P, y, f, g, sr, rs);
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        const int i = lane + kWave * s;
+        if (i < m.N) S.F[i] = f[s];
+    }
+    wave_sync();
+}
+
+// rhs_call with reaction pert_rxn (permuted index) at rate multiplier pert_v:
+// the perturbed right-hand sides of the sensitivity difference quotients.
+template <int NS>
+__device__ __noinline__ void rhs_pert_call(KernargMech* mech, ScratchLayout L, double P, int pert_rxn, double pert_v) {
+    const DevMech m = *(const DevMech*)mech;
+    const WaveScratch S = scratch_view(L);
+    const int lane = lane_id();
+    double y[NS], f[NS];
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        const int i = lane + kWave * s;
+        y[s] = (i < m.N) ? S.yb[i] : 0.0;
+    }
+    GasState g;
+    SpeciesRegs<NS> sr;
+    RhsScalars rs;
+    reactor_rhs<NS, true>(m, S, P, y, f, g, sr, rs, pert_rxn, pert_v);
 #pragma unroll
     for (int s = 0; s < NS; s++) {
         const int i = lane + kWave * s;

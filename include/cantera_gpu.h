@@ -101,6 +101,22 @@ int32_t cg_jac_batch(int32_t mech, int64_t n, const double* T, const double* P,
 /* Same on device pointers, asynchronous on `stream`. */
 int32_t cg_jac_batch_device(int32_t mech, int64_t n, const double* T, const double* P,
                             const double* Y, void* J, int32_t precision, void* stream);
+/* Chemical equilibrium of n states (ThermoPhase::equilibrate(XY),
+ * src/thermo/ThermoPhase.cpp; for one ideal-gas phase the reference's default
+ * is the element-potential solver ChemEquil, src/equil/ChemEquil.cpp:329-337):
+ * XY = "HP" holds each state's specific enthalpy and pressure, "TP" its
+ * temperature and pressure.  T[n], P[n], Y[K*n] species-major as
+ * cg_rates_batch; T_out[n], Y_out[K*n] the equilibrium states; info[n] (may
+ * be NULL): Newton iterations, or -1 (no convergence in 500 iterations) / -3
+ * (singular system).  Elements and mass are conserved exactly up to rounding.
+ * Returns CG_ERR if any state failed (the reference throws). */
+int32_t cg_equilibrate_batch(int32_t mech, int64_t n, const char* XY, const double* T, const double* P,
+                             const double* Y, double* T_out, double* Y_out, int32_t* info);
+/* Same on device pointers, asynchronous on `stream`, at most max_iter
+ * iterations; failures are reported in info only. */
+int32_t cg_equilibrate_batch_device(int32_t mech, int64_t n, const char* XY, const double* T, const double* P,
+                                    const double* Y, double* T_out, double* Y_out, int32_t* info, int32_t max_iter,
+                                    void* stream);
 /* Duration [ms] of the last batched-Jacobian kernel on `mech`. */
 double cg_jac_last_kernel_ms(int32_t mech);
 
@@ -166,6 +182,12 @@ int32_t cg_reactors_get_state(int32_t batch, double* y);
  * synchronisation (no counterpart in the reference: for callers keeping
  * trajectories on the device). */
 int32_t cg_reactors_get_state_device(int32_t batch, double* y_dev);
+/* Net production rates [kmol/m^3/s] at every reactor's current state into
+ * device memory, species-major wdot[k*n + r] as cg_rates_batch writes them
+ * (the Kinetics::getNetProductionRates call the reference's kinetics1 sample
+ * makes after each ReactorNet::advance, samples/cxx/kinetics1/kinetics1.cpp:
+ * 42-60); stream-ordered on the batch stream. */
+int32_t cg_reactors_net_production_rates_device(int32_t batch, double* wdot_dev);
 /* Integrator counters per reactor, n x 9 int64: steps, rhs evaluations,
  * Jacobian evaluations, LU factorisations, Newton iterations, nonlinear
  * convergence failures, error-test failures, current order, extra rhs
@@ -190,6 +212,27 @@ int32_t cg_reactors_get_control(int32_t batch, double* ctl);
  * output time.  A reactor with a negative status is left untouched by later
  * advances. */
 int32_t cg_reactors_status(int32_t batch, int32_t* status);
+/* Forward sensitivity of every reactor's state to the rate multiplier of
+ * reaction rxn (mechanism order), one parameter per call for all reactors of
+ * the batch (Reactor::addSensitivityReaction, src/zeroD/Reactor.cpp:278-289;
+ * parameter value and scale 1).  Allowed before the first advance only
+ * (ReactorNet::registerSensitivityParameter, src/zeroD/ReactorNet.cpp:832-
+ * 840), FP64 batches only, at most 32 parameters.  The sensitivities are
+ * integrated with the state by CVODES' staggered corrector, excluded from the
+ * error test (CVodesIntegrator::sensInit, src/numerics/CVodesIntegrator.cpp:
+ * 263-279), right-hand sides by centred difference quotients.  Returns the
+ * parameter's index. */
+int32_t cg_reactors_add_sensitivity_reaction(int32_t batch, int32_t rxn);
+/* Sensitivity tolerances (ReactorNet::setSensitivityTolerances,
+ * src/zeroD/ReactorNet.cpp:247-256; defaults rtol 1e-4, atol 1e-6). */
+int32_t cg_reactors_set_sensitivity_tolerances(int32_t batch, double rtol, double atol);
+/* Sensitivities at the last output time, n x (K+2) x n_params doubles in the
+ * state order of cg_reactors_get_state: dy_i/dp_j, or with normalized != 0
+ * (dy_i/dp_j) / y_i with 1e-300 for a zero y_i, as ReactorNet::sensitivity
+ * returns them (src/zeroD/ReactorNet.cpp:676-688); zero before the first
+ * step (CVodesIntegrator::sensitivity, src/numerics/CVodesIntegrator.cpp:
+ * 623-644). */
+int32_t cg_reactors_get_sensitivities(int32_t batch, int32_t normalized, double* S);
 /* Options: max internal steps per advance call (default 20000,
  * CVodesIntegrator.h:120-132) and the persistent grid size in wavefronts per
  * CU (0 = from occupancy). */

@@ -369,12 +369,20 @@ typedef struct {
     double *ym, *cp_R, *h_RT, *s_R, *grt, *conc, *concm, *delta, *rkcn, *kf, *ropf, *ropr, *ropnet, *wdot;
     double T, P, mmw, RT, dens, molar_density, pressure, ctot;
     int cv; /* constant-volume reactor: the P argument of rhs_w is the volume */
+    /* rate multipliers (Kinetics::setMultiplier): mult (NULL = all 1) times,
+     * for reaction mult_i only, mult_v (a transient perturbation) */
+    const double* mult;
+    int mult_i;
+    double mult_v;
 } work_t;
 
 static work_t* work_new(const or_mech* m) {
     work_t* w = calloc(1, sizeof *w);
     int K = m->K, I = m->I;
     w->K = K; w->I = I;
+    w->mult = NULL;
+    w->mult_i = -1;
+    w->mult_v = 1.0;
     double* buf = calloc((size_t)7 * K + 8 * (size_t)I + 8, 8);
     w->ym = buf; w->cp_R = w->ym + K; w->h_RT = w->cp_R + K; w->s_R = w->h_RT + K; w->grt = w->s_R + K;
     w->conc = w->grt + K; w->wdot = w->conc + K;
@@ -551,7 +559,9 @@ static void update_rop(const or_mech* m, work_t* w) {
         }
     }
     for (int i = 0; i < I; i++) {
-        w->ropf[i] = w->kf[i] * 1.0; /* m_perturb */
+        double perturb = w->mult ? w->mult[i] : 1.0; /* m_perturb */
+        if (i == w->mult_i) perturb *= w->mult_v;
+        w->ropf[i] = w->kf[i] * perturb;
         if (m->type[i] == 1) w->ropf[i] *= w->concm[i];
         w->ropr[i] = w->ropf[i];
         w->ropr[i] *= w->rkcn[i];
@@ -616,6 +626,169 @@ int or_rates_batch(const or_mech* m, int n, const double* T, const double* P, co
         work_free(w);
     }
     return 0;
+}
+
+/* ---- chemical equilibrium (ThermoPhase::equilibrate "HP" / "TP") ----
+ * Gibbs-energy minimisation of the ideal-gas mixture at fixed elements, with
+ * element potentials: the Newton iteration of Gordon & McBride, NASA RP-1311
+ * (1994), eqs. 2.24-2.27 (gas phase only) with the step control of its
+ * eqs. 3.1-3.3.  Unknowns per iteration: the element potentials pi_e, the
+ * change of ln n (total kmol per kg) and, for HP, of ln T; species amounts
+ * follow as dln n_j = -mu_j/RT + sum_e a_ej pi_e + dln n + (h_j/RT) dln T.
+ * Elements absent from the mixture, and the species containing them, are
+ * left out.  Iterates until every correction is below 1e-12 (relative to
+ * the total for species amounts).  Returns 0, or -1 if max_iter is reached. */
+static int solve_dense(int n, double* A, double* b) {
+    for (int c = 0; c < n; c++) {
+        int p = c;
+        for (int r = c + 1; r < n; r++)
+            if (fabs(A[r * n + c]) > fabs(A[p * n + c])) p = r;
+        if (A[p * n + c] == 0.0) return -1;
+        if (p != c) {
+            for (int k = 0; k < n; k++) { double t = A[c * n + k]; A[c * n + k] = A[p * n + k]; A[p * n + k] = t; }
+            double t = b[c]; b[c] = b[p]; b[p] = t;
+        }
+        for (int r = c + 1; r < n; r++) {
+            double f = A[r * n + c] / A[c * n + c];
+            for (int k = c; k < n; k++) A[r * n + k] -= f * A[c * n + k];
+            b[r] -= f * b[c];
+        }
+    }
+    for (int c = n - 1; c >= 0; c--) {
+        double v = b[c];
+        for (int k = c + 1; k < n; k++) v -= A[c * n + k] * b[k];
+        b[c] = v / A[c * n + c];
+    }
+    return 0;
+}
+
+int or_equilibrate(const or_mech* m, int hp, double T_in, double P, const double* Y_in, double* T_out, double* Y_out,
+                   int max_iter, int* iters) {
+    const int K = m->K, E = m->E;
+    double* lnn = calloc(K, 8);
+    double* mu = calloc(K, 8);
+    double* hj = calloc(K, 8);
+    double* cpj = calloc(K, 8);
+    double* dln = calloc(K, 8);
+    int* act = calloc(K, sizeof(int));
+    double b0[16], bel[16];
+    int eact[16], ne = 0;
+    if (E > 16) return -2;
+    /* element amounts [kmol/kg] and the specific enthalpy of the input state */
+    double tt[6];
+    tpoly(T_in, tt);
+    double h0 = 0.0;
+    for (int e = 0; e < E; e++) b0[e] = 0.0;
+    for (int k = 0; k < K; k++) {
+        double c, h, sR;
+        nasa7(m, k, tt, &c, &h, &sR);
+        h0 += Y_in[k] * m->rmw[k] * h * GAS_CONSTANT * T_in;
+        for (int e = 0; e < E; e++) b0[e] += m->comp[k * E + e] * Y_in[k] * m->rmw[k];
+    }
+    double bmax = 0.0;
+    for (int e = 0; e < E; e++) bmax = fmax(bmax, b0[e]);
+    for (int e = 0; e < E; e++)
+        if (b0[e] > 1e-14 * bmax) eact[ne++] = e;
+    int nact = 0;
+    for (int k = 0; k < K; k++) {
+        act[k] = 1;
+        for (int e = 0; e < E; e++) {
+            int on = 0;
+            for (int q = 0; q < ne; q++) on |= (eact[q] == e);
+            if (!on && m->comp[k * E + e] != 0.0) act[k] = 0;
+        }
+        nact += act[k];
+        lnn[k] = act[k] ? log(0.1 / K) : -INFINITY;
+    }
+    (void)nact;
+    double lnntot = log(0.1);
+    double T = hp ? 3800.0 : T_in;
+    const int nu = ne + 1 + (hp ? 1 : 0);
+    double A[18 * 18], x[18];
+    int it, status = -1;
+    const double SIZE = 18.420681;
+    for (it = 0; it < max_iter; it++) {
+        tpoly(T, tt);
+        double ntot = exp(lnntot), sumn = 0.0;
+        const double lnP = log(P / m->pref[0]);
+        for (int k = 0; k < K; k++) {
+            if (!act[k]) continue;
+            double c, h, sR;
+            nasa7(m, k, tt, &c, &h, &sR);
+            hj[k] = h;
+            cpj[k] = c;
+            mu[k] = (h - sR) + (lnn[k] - lnntot) + lnP;
+            sumn += exp(lnn[k]);
+        }
+        for (int i = 0; i < nu * nu; i++) A[i] = 0.0;
+        for (int i = 0; i < nu; i++) x[i] = 0.0;
+        for (int q = 0; q < ne; q++) bel[q] = 0.0;
+        double sh = 0.0, scp = 0.0, shh = 0.0, snmu = 0.0, snhmu = 0.0;
+        for (int k = 0; k < K; k++) {
+            if (!act[k]) continue;
+            const double nj = exp(lnn[k]);
+            for (int q = 0; q < ne; q++) {
+                const double aq = m->comp[k * E + eact[q]] * nj;
+                if (aq == 0.0) continue;
+                bel[q] += aq;
+                for (int r = 0; r < ne; r++) A[q * nu + r] += aq * m->comp[k * E + eact[r]];
+                x[q] += aq * mu[k];
+                if (hp) A[q * nu + ne + 1] += aq * hj[k];
+            }
+            sh += nj * hj[k];
+            scp += nj * cpj[k];
+            shh += nj * hj[k] * hj[k];
+            snmu += nj * mu[k];
+            snhmu += nj * hj[k] * mu[k];
+        }
+        for (int q = 0; q < ne; q++) {
+            A[q * nu + ne] = bel[q];
+            A[ne * nu + q] = bel[q];
+            x[q] += b0[eact[q]] - bel[q];
+        }
+        A[ne * nu + ne] = sumn - ntot;
+        x[ne] = ntot - sumn + snmu;
+        if (hp) {
+            A[ne * nu + ne + 1] = sh;
+            for (int q = 0; q < ne; q++) A[(ne + 1) * nu + q] = A[q * nu + ne + 1];
+            A[(ne + 1) * nu + ne] = sh;
+            A[(ne + 1) * nu + ne + 1] = scp + shh;
+            x[ne + 1] = h0 / (GAS_CONSTANT * T) - sh + snhmu;
+        }
+        if (solve_dense(nu, A, x)) { status = -3; break; }
+        const double dlnN = x[ne], dlnT = hp ? x[ne + 1] : 0.0;
+        double big = fmax(5.0 * fabs(dlnT), 5.0 * fabs(dlnN)), lam2 = 1.0, conv = fmax(fabs(dlnT), fabs(dlnN));
+        for (int k = 0; k < K; k++) {
+            if (!act[k]) continue;
+            double d = -mu[k] + dlnN + hj[k] * dlnT;
+            for (int q = 0; q < ne; q++) d += m->comp[k * E + eact[q]] * x[q];
+            dln[k] = d;
+            const double lnx = lnn[k] - lnntot;
+            if (lnx > -SIZE) {
+                if (d > big) big = d;
+                conv = fmax(conv, exp(lnn[k]) / sumn * fabs(d));
+            } else if (d >= 0.0 && d - dlnN != 0.0) {
+                lam2 = fmin(lam2, fabs((-lnx - 9.2103404) / (d - dlnN)));
+            }
+        }
+        double lam = fmin(1.0, lam2);
+        if (big > 2.0) lam = fmin(lam, 2.0 / big);
+        for (int k = 0; k < K; k++)
+            if (act[k]) lnn[k] += lam * dln[k];
+        lnntot += lam * dlnN;
+        if (hp) T = fmin(fmax(T * exp(lam * dlnT), 200.0), 6000.0);
+        if (lam == 1.0 && conv < 1e-12) { status = 0; it++; break; }
+    }
+    double ysum = 0.0;
+    for (int k = 0; k < K; k++) {
+        Y_out[k] = act[k] ? exp(lnn[k]) * m->mw[k] : 0.0;
+        ysum += Y_out[k];
+    }
+    for (int k = 0; k < K; k++) Y_out[k] /= ysum;
+    *T_out = T;
+    if (iters) *iters = it;
+    free(lnn); free(mu); free(hj); free(cpj); free(dln); free(act);
+    return status;
 }
 
 /* Analytic reactor Jacobians of n states (m = 1 kg), species-major inputs as
@@ -921,6 +1094,7 @@ int or_igcpr_jacobian(const or_mech* m, double P, const double* y, double* J, in
 #define LMAX (QMAX + 1)
 #define MXSTEP 20000
 #define MAXCOR 3
+#define OR_MAXSENS 32
 #define MXNEF 7
 #define MXNCF 10
 #define MXNEF1 3
@@ -972,6 +1146,17 @@ struct or_reactor {
     int started;
     double t_user;
     double* yout;
+    /* forward sensitivities to reaction rate multipliers (staggered direct
+     * method; sensitivities are not part of the step-size control) */
+    int ns;
+    int sidx[OR_MAXSENS];
+    double* zS[OR_MAXSENS][LMAX + 1];
+    double* acorS[OR_MAXSENS];
+    double* sout[OR_MAXSENS];
+    double rtolS, atolS;
+    double *ewtS, *sp, *sm, *fsp, *fsm, *dS;
+    long nfeS, nniS;
+    double* mult;   /* rate multipliers (NULL until set) */
 };
 
 static double wrms(const double* v, const double* w, int n) {
@@ -1129,27 +1314,101 @@ static void set_bdf(or_reactor* r) {
     r->tq[4] = NLSCOEF / r->tq[2];
 }
 
+/* The Nordsieck operations act on the state history zn and, identically, on
+ * every sensitivity history zS[p] (CVODES keeps them in the same form). */
 static void predict(or_reactor* r) {
     r->tn += r->h;
-    for (int k = 1; k <= r->q; k++)
-        for (int j = r->q; j >= k; j--)
-            for (int i = 0; i < r->N; i++) r->zn[j - 1][i] += r->zn[j][i];
+    for (int p = -1; p < r->ns; p++) {
+        double** z = (p < 0) ? r->zn : r->zS[p];
+        for (int k = 1; k <= r->q; k++)
+            for (int j = r->q; j >= k; j--)
+                for (int i = 0; i < r->N; i++) z[j - 1][i] += z[j][i];
+    }
 }
 static void restore(or_reactor* r, double saved_t) {
     r->tn = saved_t;
-    for (int k = 1; k <= r->q; k++)
-        for (int j = r->q; j >= k; j--)
-            for (int i = 0; i < r->N; i++) r->zn[j - 1][i] -= r->zn[j][i];
+    for (int p = -1; p < r->ns; p++) {
+        double** z = (p < 0) ? r->zn : r->zS[p];
+        for (int k = 1; k <= r->q; k++)
+            for (int j = r->q; j >= k; j--)
+                for (int i = 0; i < r->N; i++) z[j - 1][i] -= z[j][i];
+    }
 }
 static void rescale(or_reactor* r) {
-    double factor = r->eta;
-    for (int j = 1; j <= r->q; j++) {
-        for (int i = 0; i < r->N; i++) r->zn[j][i] *= factor;
-        factor *= r->eta;
+    for (int p = -1; p < r->ns; p++) {
+        double** z = (p < 0) ? r->zn : r->zS[p];
+        double factor = r->eta;
+        for (int j = 1; j <= r->q; j++) {
+            for (int i = 0; i < r->N; i++) z[j][i] *= factor;
+            factor *= r->eta;
+        }
     }
     r->h = r->hscale * r->eta;
     r->hscale = r->h;
     r->nscon = 0;
+}
+
+/* Sensitivity right-hand side J s + df/dp for parameter p at (y, s): the
+ * simultaneous centred difference quotient CVODES uses without a user
+ * sensitivity function (cvSensRhs1InternalDQ, CENTERED1, parameter value and
+ * scale 1 as Reactor::addSensitivityReaction registers them),
+ * [f(y + D s; k_p (1 + D)) - f(y - D s; k_p (1 - D))] / 2D with
+ * D = min(sqrt(max(rtol, uround)), 1 / |s|_wrms) on the state's weights. */
+static int sens_rhs(or_reactor* r, int p, const double* y, const double* sv, double* fS) {
+    const int N = r->N;
+    const double del = sqrt(fmax(r->rtol, UROUND));
+    double nrm = 0.0;
+    for (int i = 0; i < N; i++) nrm += (sv[i] * r->ewt[i]) * (sv[i] * r->ewt[i]);
+    nrm = sqrt(nrm / r->nnorm);
+    const double D = fmin(del, 1.0 / fmax(nrm, 1.0 / del));
+    for (int i = 0; i < N; i++) {
+        r->sp[i] = y[i] + D * sv[i];
+        r->sm[i] = y[i] - D * sv[i];
+    }
+    r->w->mult_i = r->sidx[p];
+    r->w->mult_v = 1.0 + D;
+    int bad = rhs_w(r->m, r->w, r->P, r->sp, r->fsp);
+    r->w->mult_v = 1.0 - D;
+    bad |= rhs_w(r->m, r->w, r->P, r->sm, r->fsm);
+    r->w->mult_i = -1;
+    r->w->mult_v = 1.0;
+    r->nfeS += 2;
+    for (int i = 0; i < N; i++) fS[i] = (r->fsp[i] - r->fsm[i]) / (2.0 * D);
+    return bad;
+}
+
+/* Staggered corrector for every sensitivity after the state's step passed
+ * its error test: Newton on the linear BDF equation for s_n with the state's
+ * iteration matrix, converged to the state's test on the sensitivity error
+ * weights (rtolS |s| + atolS); at most 10 iterations. */
+static int sens_correct(or_reactor* r) {
+    const int N = r->N;
+    for (int p = 0; p < r->ns; p++) {
+        double** z = r->zS[p];
+        double* acs = r->acorS[p];
+        for (int i = 0; i < N; i++) {
+            r->ewtS[i] = 1.0 / (r->rtolS * fabs(z[0][i]) + r->atolS);
+            acs[i] = 0.0;
+        }
+        double crateS = 1.0, delpS = 0.0;
+        for (int m = 0; m < 10; m++) {
+            for (int i = 0; i < N; i++) r->dS[i] = z[0][i] + acs[i];
+            if (sens_rhs(r, p, r->y, r->dS, r->tempv)) return 1;
+            for (int i = 0; i < N; i++) r->dS[i] = -(r->rl1 * z[1][i] + acs[i] - r->gamma * r->tempv[i]);
+            lu_solve(r->M, N, r->piv, r->dS);
+            if (r->gamrat != 1.0) {
+                double sc = 2.0 / (1.0 + r->gamrat);
+                for (int i = 0; i < N; i++) r->dS[i] *= sc;
+            }
+            for (int i = 0; i < N; i++) acs[i] += r->dS[i];
+            r->nniS++;
+            double del = wrms(r->dS, r->ewtS, r->nnorm);
+            if (m > 0) crateS = fmax(CRDOWN * crateS, del / delpS);
+            if (del * fmin(1.0, crateS) / r->tq[4] <= 1.0) break;
+            delpS = del;
+        }
+    }
+    return 0;
 }
 
 static void increase_bdf(or_reactor* r) {
@@ -1172,9 +1431,12 @@ static void increase_bdf(or_reactor* r) {
     }
     double A1 = (-alpha0 - alpha1) / prod;
     int L = r->q + 1;
-    for (int i = 0; i < r->N; i++) r->zn[L][i] = A1 * r->zn[QMAX][i];
-    for (int j = 2; j <= r->q; j++)
-        for (int i = 0; i < r->N; i++) r->zn[j][i] += l[j] * r->zn[L][i];
+    for (int p = -1; p < r->ns; p++) {
+        double** z = (p < 0) ? r->zn : r->zS[p];
+        for (int i = 0; i < r->N; i++) z[L][i] = A1 * z[QMAX][i];
+        for (int j = 2; j <= r->q; j++)
+            for (int i = 0; i < r->N; i++) z[j][i] += l[j] * z[L][i];
+    }
 }
 static void decrease_bdf(or_reactor* r) {
     double l[LMAX + 1];
@@ -1186,8 +1448,11 @@ static void decrease_bdf(or_reactor* r) {
         xi = hsum / r->hscale;
         for (int i = j + 2; i >= 2; i--) l[i] = l[i] * xi + l[i - 1];
     }
-    for (int j = 2; j < r->q; j++)
-        for (int i = 0; i < r->N; i++) r->zn[j][i] = -l[j] * r->zn[r->q][i] + r->zn[j][i];
+    for (int p = -1; p < r->ns; p++) {
+        double** z = (p < 0) ? r->zn : r->zS[p];
+        for (int j = 2; j < r->q; j++)
+            for (int i = 0; i < r->N; i++) z[j][i] = -l[j] * z[r->q][i] + z[j][i];
+    }
 }
 static void adjust_order(or_reactor* r, int dq) {
     if (dq == 1) increase_bdf(r);
@@ -1296,6 +1561,10 @@ static int do_error_test(or_reactor* r, int* nflag, double saved_t, int* nef, do
     int ret = f_eval(r, r->zn[0], r->tempv);
     if (ret) return -1;
     for (int i = 0; i < r->N; i++) r->zn[1][i] = r->h * r->tempv[i];
+    for (int p = 0; p < r->ns; p++) {
+        if (sens_rhs(r, p, r->zn[0], r->zS[p][0], r->tempv)) return -1;
+        for (int i = 0; i < r->N; i++) r->zS[p][1][i] = r->h * r->tempv[i];
+    }
     return 2;
 }
 
@@ -1308,9 +1577,13 @@ static void complete_step(or_reactor* r) {
     r->tau[1] = r->h;
     for (int j = 0; j <= r->q; j++)
         for (int i = 0; i < r->N; i++) r->zn[j][i] += r->l[j] * r->acor[i];
+    for (int p = 0; p < r->ns; p++)
+        for (int j = 0; j <= r->q; j++)
+            for (int i = 0; i < r->N; i++) r->zS[p][j][i] += r->l[j] * r->acorS[p][i];
     r->qwait--;
     if (r->qwait == 1 && r->q != QMAX) {
         memcpy(r->zn[QMAX], r->acor, 8 * r->N);
+        for (int p = 0; p < r->ns; p++) memcpy(r->zS[p][QMAX], r->acorS[p], 8 * r->N);
         r->saved_tq5 = r->tq[5];
     }
 }
@@ -1366,6 +1639,7 @@ static void prepare_next_step(or_reactor* r, double dsm) {
         r->eta = etaqp1;
         r->qprime = r->q + 1;
         memcpy(r->zn[QMAX], r->acor, 8 * r->N);
+        for (int p = 0; p < r->ns; p++) memcpy(r->zS[p][QMAX], r->acorS[p], 8 * r->N);
     }
     set_eta(r);
 }
@@ -1409,6 +1683,7 @@ static int bdf_step(or_reactor* r) {
         if (e < 0) return -4;
         break;
     }
+    if (r->ns && sens_correct(r)) return -5;
     complete_step(r);
     prepare_next_step(r, dsm);
     r->etamax = (r->nst <= SMALL_NST) ? ETAMX2 : ETAMX3;
@@ -1500,6 +1775,8 @@ or_reactor* or_reactor_new(const or_mech* m, double T0, double P, const double* 
     r->etamax = ETAMX1;
     r->tn = 0.0;
     r->t_user = 0.0;
+    r->rtolS = 1.0e-4;   /* ReactorNet defaults (ReactorNet.h:484-486) */
+    r->atolS = 1.0e-6;
     return r;
 }
 
@@ -1508,6 +1785,11 @@ void or_reactor_free(or_reactor* r) {
     for (int j = 0; j <= LMAX; j++) free(r->zn[j]);
     free(r->ewt); free(r->acor); free(r->y); free(r->ftemp); free(r->tempv); free(r->savedJ);
     free(r->M); free(r->piv); free(r->yout);
+    for (int p = 0; p < r->ns; p++) {
+        for (int j = 0; j <= LMAX; j++) free(r->zS[p][j]);
+        free(r->acorS[p]); free(r->sout[p]);
+    }
+    free(r->ewtS); free(r->sp); free(r->sm); free(r->fsp); free(r->fsm); free(r->dS); free(r->mult);
     work_free(r->w);
     free(r);
 }
@@ -1517,6 +1799,13 @@ static int start(or_reactor* r, double tout) {
     if (f_eval(r, r->zn[0], r->zn[1])) return -1;
     if (initial_h(r, tout)) return -1;
     for (int i = 0; i < r->N; i++) r->zn[1][i] *= r->h;
+    for (int p = 0; p < r->ns; p++) {
+        /* s(0) = 0; zS[1] = h (J 0 + df/dp) */
+        for (int i = 0; i < r->N; i++) r->zS[p][0][i] = 0.0;
+        ewt_set(r, r->zn[0]);
+        if (sens_rhs(r, p, r->zn[0], r->zS[p][0], r->zS[p][1])) return -1;
+        for (int i = 0; i < r->N; i++) r->zS[p][1][i] *= r->h;
+    }
     r->hscale = r->h;
     r->hprime = r->h;
     r->started = 1;
@@ -1543,15 +1832,53 @@ int or_reactor_advance(or_reactor* r, double tout) {
         if (ret) return ret;
         nsteps++;
     }
-    /* CVodeGetDky(tout, 0) */
+    /* CVodeGetDky(tout, 0), CVodeGetSensDky(tout, 0) */
     double s = (tout - r->tn) / r->h;
-    for (int i = 0; i < r->N; i++) {
-        double v = r->zn[r->q][i];
-        for (int j = r->q - 1; j >= 0; j--) v = r->zn[j][i] + s * v;
-        r->yout[i] = v;
+    for (int p = -1; p < r->ns; p++) {
+        double** z = (p < 0) ? r->zn : r->zS[p];
+        double* out = (p < 0) ? r->yout : r->sout[p];
+        for (int i = 0; i < r->N; i++) {
+            double v = z[r->q][i];
+            for (int j = r->q - 1; j >= 0; j--) v = z[j][i] + s * v;
+            out[i] = v;
+        }
     }
     r->t_user = tout;
     return 0;
+}
+
+/* Reactor::addSensitivityReaction (src/zeroD/Reactor.cpp:278-289): parameter
+ * = a multiplier of reaction rxn's rate (nominal 1), before the first step. */
+int or_reactor_add_sensitivity(or_reactor* r, int rxn) {
+    if (r->started) return -1;
+    if (rxn < 0 || rxn >= r->m->I || r->ns >= OR_MAXSENS) return -2;
+    const int N = r->N, p = r->ns;
+    if (!r->ewtS) {
+        r->ewtS = calloc(N, 8); r->sp = calloc(N, 8); r->sm = calloc(N, 8);
+        r->fsp = calloc(N, 8); r->fsm = calloc(N, 8); r->dS = calloc(N, 8);
+    }
+    for (int j = 0; j <= LMAX; j++) r->zS[p][j] = calloc(N, 8);
+    r->acorS[p] = calloc(N, 8);
+    r->sout[p] = calloc(N, 8);
+    r->sidx[p] = rxn;
+    r->ns++;
+    return 0;
+}
+void or_reactor_set_sensitivity_tolerances(or_reactor* r, double rtol, double atol) {
+    r->rtolS = rtol;
+    r->atolS = atol;
+}
+/* dy_i/dp at the output time, S[i * ns + p] (CVodesIntegrator::sensitivity,
+ * src/numerics/CVodesIntegrator.cpp:623-644: zero before the first step). */
+void or_reactor_sensitivities(const or_reactor* r, double* S) {
+    for (int i = 0; i < r->N; i++)
+        for (int p = 0; p < r->ns; p++) S[i * r->ns + p] = r->started ? r->sout[p][i] : 0.0;
+}
+/* Kinetics::setMultiplier for a whole run (all reactions at once). */
+void or_reactor_set_multipliers(or_reactor* r, const double* mult) {
+    if (!r->mult) r->mult = calloc(r->m->I, 8);
+    memcpy(r->mult, mult, 8 * r->m->I);
+    r->w->mult = r->mult;
 }
 
 int or_reactor_step(or_reactor* r) {
@@ -1559,6 +1886,7 @@ int or_reactor_step(or_reactor* r) {
     int ret = one_step(r);
     if (ret) return ret;
     memcpy(r->yout, r->zn[0], 8 * r->N);
+    for (int p = 0; p < r->ns; p++) memcpy(r->sout[p], r->zS[p][0], 8 * r->N);
     r->t_user = r->tn;
     return 0;
 }

@@ -86,6 +86,12 @@ int or_equilibrium_constants(const or_mech* m, double T, double P, const double*
 int or_rates_batch(const or_mech* m, int n, const double* T, const double* P,
                    const double* Y, double* wdot, int nthreads);
 
+/* chemical equilibrium at fixed (h, P) (hp = 1) or (T, P) (hp = 0) of the
+ * mixture (T_in, P, Y_in): ThermoPhase::equilibrate, element-potential
+ * Gibbs minimisation.  Returns 0, -1 (max_iter reached), -3 (singular). */
+int or_equilibrate(const or_mech* m, int hp, double T_in, double P, const double* Y_in, double* T_out,
+                   double* Y_out, int max_iter, int* iters);
+
 /* analytic reactor Jacobians (m = 1 kg) of n states, inputs as or_rates_batch,
  * J[s*N*N + r*N + c] */
 int or_jacobian_batch(const or_mech* m, int n, const double* T, const double* P, const double* Y, double* J,
@@ -115,6 +121,13 @@ double or_reactor_time(const or_reactor* r);          /* user time (after advanc
 void or_reactor_state(const or_reactor* r, double* y); /* neq values */
 /* stats: [nsteps, nfe, nje, nsetups, nni, ncfn, netf, last_q, nfe_dq] */
 void or_reactor_stats(const or_reactor* r, long* stats);
+/* forward sensitivities dy/dp to reaction-rate multipliers (at most 32),
+ * added before the first step; S is neq x ns, row-major */
+int or_reactor_add_sensitivity(or_reactor* r, int rxn);
+void or_reactor_set_sensitivity_tolerances(or_reactor* r, double rtol, double atol);
+void or_reactor_sensitivities(const or_reactor* r, double* S);
+/* every reaction's rate multiplier (Kinetics::setMultiplier) for this reactor */
+void or_reactor_set_multipliers(or_reactor* r, const double* mult);
 
 /* batch: n reactors, each advanced through nout outputs at t_k = k*dt[i],
  * k = 1..nout, states written AoS y_out[(i*(nout+1) + k)*neq + j] (k = 0 is the

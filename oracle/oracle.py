@@ -42,6 +42,9 @@ def lib():
         L.or_rates_batch.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp, C.c_int]
         L.or_jacobian_batch.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp, C.c_int]
         L.or_jacobian_batch.restype = C.c_int
+        L.or_equilibrate.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, _dp, C.POINTER(C.c_double), _dp,
+                                     C.c_int, C.POINTER(C.c_int)]
+        L.or_equilibrate.restype = C.c_int
         L.or_igcpr_rhs.argtypes = [C.c_void_p, C.c_double, _dp, _dp]
         L.or_igr_rhs.argtypes = [C.c_void_p, C.c_double, _dp, _dp]
         L.or_igcpr_rhs.restype = C.c_int
@@ -57,6 +60,11 @@ def lib():
         L.or_reactor_time.restype = C.c_double
         L.or_reactor_state.argtypes = [C.c_void_p, _dp]
         L.or_reactor_stats.argtypes = [C.c_void_p, _lp]
+        L.or_reactor_add_sensitivity.argtypes = [C.c_void_p, C.c_int]
+        L.or_reactor_add_sensitivity.restype = C.c_int
+        L.or_reactor_set_sensitivity_tolerances.argtypes = [C.c_void_p, C.c_double, C.c_double]
+        L.or_reactor_sensitivities.argtypes = [C.c_void_p, _dp]
+        L.or_reactor_set_multipliers.argtypes = [C.c_void_p, _dp]
         L.or_reactors_run.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp, C.c_int, C.c_double,
                                       C.c_double, C.c_int, opt, opt, C.c_int]
         L.or_reactors_run.restype = C.c_int
@@ -128,6 +136,18 @@ class Mech:
             raise FloatingPointError("non-finite Jacobian")
         return J
 
+    def equilibrate(self, T, P, Y, mode="HP", max_iter=500):
+        """(T_eq, Y_eq, iterations) at fixed enthalpy and pressure ("HP") or
+        fixed temperature and pressure ("TP")."""
+        Yo = np.zeros(self.K)
+        To = C.c_double()
+        it = C.c_int()
+        r = lib().or_equilibrate(self._h, 1 if mode == "HP" else 0, float(T), float(P),
+                                 np.ascontiguousarray(Y, np.float64), C.byref(To), Yo, int(max_iter), C.byref(it))
+        if r:
+            raise RuntimeError(f"oracle equilibrium did not converge ({r})")
+        return To.value, Yo, it.value
+
     def rhs(self, P, y, const_volume=False):
         """ydot of the constant-pressure reactor at P, or of the constant-volume
         reactor at volume P."""
@@ -193,6 +213,28 @@ class Reactor:
         y = np.zeros(self.mech.K + 2)
         lib().or_reactor_state(self._h, y)
         return y
+
+    def add_sensitivity_reaction(self, rxn):
+        """Reactor::addSensitivityReaction (src/zeroD/Reactor.cpp:278-289)."""
+        r = lib().or_reactor_add_sensitivity(self._h, int(rxn))
+        if r:
+            raise RuntimeError(f"oracle add_sensitivity failed ({r})")
+        self._ns = getattr(self, "_ns", 0) + 1
+
+    def set_sensitivity_tolerances(self, rtol, atol):
+        lib().or_reactor_set_sensitivity_tolerances(self._h, float(rtol), float(atol))
+
+    def set_multipliers(self, mult):
+        lib().or_reactor_set_multipliers(self._h, np.ascontiguousarray(mult, np.float64))
+
+    @property
+    def sensitivities(self):
+        """dy_i/dp_j (neq x ns): the raw CVODES sensitivities."""
+        ns = getattr(self, "_ns", 0)
+        S = np.zeros((self.mech.K + 2, ns))
+        if ns:
+            lib().or_reactor_sensitivities(self._h, S)
+        return S
 
     @property
     def stats(self):

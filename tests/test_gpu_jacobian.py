@@ -38,7 +38,7 @@ def states(info, n, seed):
     return T, P, info.mass_fractions_from_moles(X)
 
 
-def check_against_oracle(om, T, P, Y, J):
+def check_against_oracle(om, T, P, Y, J, colsum_tol=1e-9):
     K = Y.shape[1]
     worst_c = worst_t = 0.0
     for s in range(len(T)):
@@ -54,7 +54,7 @@ def check_against_oracle(om, T, P, Y, J):
         worst_c, worst_t = max(worst_c, ec), max(worst_t, et)
         colsum = np.abs(Jg[2:, :].sum(axis=0))
         colabs = np.abs(Jg[2:, :]).sum(axis=0) + 1e-300
-        assert float((colsum / colabs)[2:].max()) <= 1e-9
+        assert float((colsum / colabs)[2:].max()) <= colsum_tol
         assert float(colsum[1] / colabs[1]) <= 1e-6
     return worst_c, worst_t
 
@@ -111,12 +111,18 @@ def test_fp32_jacobian_error_against_oracle(mech, oracle_mod):
     J32 = sol.reactor_jacobians(T, P, Y.T, precision=ca.CG_FP32)
     J64 = sol.reactor_jacobians(T, P, Y.T)
     assert J32.dtype == np.float32 and np.all(np.isfinite(J32))
-    ec, et = check_against_oracle(om, T, P, Y, J32.astype(np.float64))
+    # rounding each entry to FP32 leaves column sums at <= 2^-24 of the column's
+    # absolute sum (1e-7 bar)
+    ec, et = check_against_oracle(om, T, P, Y, J32.astype(np.float64), colsum_tol=1e-7)
+    _, et64 = check_against_oracle(om, T, P, Y, J64)
     rs = np.abs(J64).max(axis=2, keepdims=True) + 1e-300
     d = float((np.abs(J32.astype(np.float64) - J64.astype(np.float32).astype(np.float64)) / rs).max())
-    print(f"{mech} FP32: species columns {ec:.2e}, T column {et:.2e}, vs rounded FP64 {d:.2e}", flush=True)
+    print(f"{mech} FP32: species columns {ec:.2e}, T column {et:.2e} (FP64 {et64:.2e}), "
+          f"vs rounded FP64 {d:.2e}", flush=True)
     assert ec <= 1e-7
-    assert et <= 1e-5
+    # the T-column scale bounds |J_i1|, so one FP32 rounding adds <= 2^-24 to
+    # the FP64 kernel's own T-column error on these states
+    assert et <= et64 + 1e-7
     assert d <= 1e-10
 
 

@@ -347,3 +347,37 @@ def test_invalid_initial_state_is_rejected():
         T[2], Pa[2] = T0, P
         with pytest.raises(ca.CanteraGpuError):
             ca.ReactorBatch(sol, T, Pa, wl.Y0)
+
+
+@pytest.mark.parametrize("const_volume", [False, True])
+def test_rates_at_reactor_states(const_volume, oracle_mod):
+    """cg_reactors_net_production_rates_device: the rates at each reactor's
+    current state (kinetics1's getNetProductionRates after every advance)
+    agree with the oracle at those states (bar as the batched rates test);
+    for constant volume the state is (T, rho) with P from the ideal gas law."""
+    import cantera_amd as ca
+    from cantera_amd.workloads import MechInfo, config1
+    from conftest import parse_stoich
+    info = MechInfo(mech_file("h2o2"))
+    sol = ca.Solution("h2o2")
+    om = oracle_mod.Mech(mech_file("h2o2"))
+    wl = config1(info, n=200)
+    rb = ca.ReactorBatch(sol, wl.T0, wl.P, wl.Y0, rtol=wl.rtol, atol=wl.atol, const_volume=const_volume)
+    nu_r, nu_p = parse_stoich(mech_file("h2o2"))
+    absnu = nu_r + nu_p
+    R = 6.02214076e26 * 1.380649e-23
+    for k in (1, 7, 40):
+        rb.advance(k * wl.dt)
+        y = rb.state
+        w = rb.net_production_rates
+        worst = 0.0
+        for s in range(0, 200, 9):
+            P = wl.P[s]
+            if const_volume:
+                Wbar = 1.0 / np.sum(y[s, 2:] / info.molecular_weights)
+                P = y[s, 0] / 1.0 * R * y[s, 1] / Wbar      # rho = m / V, V = 1 m^3
+            r = om.rates(y[s, 1], P, y[s, 2:])
+            scale = absnu @ (np.abs(r["ropf"]) + np.abs(r["ropr"])) + 1e-300
+            worst = max(worst, float((np.abs(w[:, s] - r["wdot"]) / scale).max()))
+        print(f"const_volume={const_volume} output {k}: max normalised error {worst:.2e}", flush=True)
+        assert worst <= 1e-10

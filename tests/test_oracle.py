@@ -696,3 +696,104 @@ def test_frac_tables_match_compiler(tmp_path):
     out = tmp_path / "frac.mech"
     mechc.main([os.path.join(GOLDEN, "frac.yaml"), str(out)])
     assert out.read_text() == open(mech_file("frac")).read()
+
+
+# --- HP / TP chemical equilibrium ---------------------------------------------
+
+def _equil_rows(fname):
+    """Rows of the reference's gas + graphite HP-equilibrium fixture in which no
+    graphite forms (last column 0): these are gas-phase equilibria."""
+    d = np.loadtxt(os.path.join(GOLDEN, fname))
+    return [r for r in d if r[-1] == 0.0]
+
+
+@pytest.mark.parametrize("fname,P", [("gas-carbon-equil.csv", ONE_ATM), ("gas-carbon-equil-5atm.csv", 5 * ONE_ATM)])
+def test_hp_equilibrium_gas_carbon_fixture(gri, fname, P):
+    """test/python/test_equilibrium.py:477-502 (TestEquil_GasCarbon): 1 kmol of
+    CH4/air at 300 K and phi = 0.3 .. 3.5, equilibrated at constant h and P;
+    the fixture holds T and every species' kmol, compared there at rtol 1e-7.
+    Rows where graphite forms (a second phase, out of scope) are skipped."""
+    from cantera_amd.workloads import MechInfo, equivalence_ratio_moles
+    info = MechInfo(mech_file("gri30"))
+    rows = _equil_rows(fname)
+    assert len(rows) >= 11
+    for row in rows:
+        X = equivalence_ratio_moles(info, "CH4", [row[0]])[0]
+        Y = info.mass_fractions_from_moles(X)
+        T, Ye, _ = gri.equilibrate(300.0, P, Y)
+        W0 = 1.0 / np.sum(Y / info.molecular_weights)
+        n = Ye / info.molecular_weights * W0
+        ref = row[2:-1]
+        assert T == pytest.approx(row[1], rel=1e-7)
+        assert np.all(np.abs(n - ref) <= 1e-7 * np.abs(ref) + 1e-15), row[0]
+
+
+def test_hp_equilibrium_cxx_demo_blessed(oracle_mod):
+    """samples/cxx/demo/demo.cpp on h2o2.yaml: H2:1, O2:1, AR:2 at 1200 K and 1
+    atm equilibrated at constant h and P (test_problems/cxx_samples/
+    cxx_demo_blessed.txt, 5 significant figures): T, density, molar enthalpy
+    and every reaction's forward and reverse rate of progress at the
+    equilibrium state (their difference vanishes there)."""
+    m = oracle_mod.Mech(mech_file("h2o2"))
+    X = np.zeros(m.K)
+    for name, v in (("H2", 1.0), ("O2", 1.0), ("AR", 2.0)):
+        X[m.index(name)] = v
+    X /= X.sum()
+    Y = X * m.mw / (X * m.mw).sum()
+    T, Ye, _ = m.equilibrate(1200.0, ONE_ATM, Y)
+    text = open(os.path.join(GOLDEN, "cxx_demo_blessed.txt")).read().split("\n")
+    eq = text.index("Equilibrium state:")
+    blessed = {l.split(":")[0].strip(): float(l.split(":")[1].split()[0]) for l in text[eq + 2:eq + 8]}
+    Wbar = 1.0 / np.sum(Ye / m.mw)
+    rho = ONE_ATM * Wbar / (R * T)
+    _, h, _ = m.thermo(T)
+    hmole = float(np.sum(Ye / m.mw * h) * R * T * Wbar)
+    assert T == pytest.approx(blessed["Temperature"], rel=5e-5)
+    assert rho == pytest.approx(blessed["Density"], rel=5e-5)
+    assert hmole == pytest.approx(blessed["Molar Enthalpy"], rel=5e-5)
+    r = m.rates(T, ONE_ATM, Ye)
+    rows = [l for l in text if l.rstrip().endswith("kmol/m3/s")]
+    assert len(rows) == m.I
+    for i, l in enumerate(rows):
+        fwd, rev = (float(x) for x in l.split()[-4:-2])
+        assert r["ropf"][i] == pytest.approx(fwd, rel=1e-4, abs=1e-12), l
+        assert r["ropr"][i] == pytest.approx(rev, rel=1e-4, abs=1e-12), l
+    assert np.max(np.abs(r["ropf"] - r["ropr"]) / (r["ropf"] + 1e-300)) < 1e-8
+
+
+def test_tp_equilibrium_detailed_balance_and_gibbs(gri):
+    """At a TP equilibrium every reversible reaction is balanced (ropf = ropr),
+    elements are conserved, and the Gibbs energy is below that of nearby
+    element-conserving compositions (brute-force check of the minimum)."""
+    from cantera_amd.workloads import MechInfo, equivalence_ratio_moles
+    info = MechInfo(mech_file("gri30"))
+    X = equivalence_ratio_moles(info, "CH4", [0.8])[0]
+    Y = info.mass_fractions_from_moles(X)
+    T, Ye, _ = gri.equilibrate(2200.0, 3 * ONE_ATM, Y, mode="TP")
+    assert T == 2200.0
+    B = info.composition / info.molecular_weights[:, None]
+    assert np.allclose(Ye @ B, Y @ B, rtol=1e-10, atol=1e-16)
+    r = gri.rates(T, 3 * ONE_ATM, Ye)
+    big = r["ropf"] > 1e-6 * r["ropf"].max()
+    assert np.max(np.abs(r["ropf"] - r["ropr"])[big] / r["ropf"][big]) < 1e-7
+
+    def gibbs(Yv):
+        _, h, s = gri.thermo(T)
+        n = Yv / info.molecular_weights
+        x = n / n.sum()
+        mu = h - s + np.log(np.maximum(x, 1e-300)) + np.log(3.0)
+        return float(np.sum(n[n > 0] * mu[n > 0]))
+
+    g0 = gibbs(Ye)
+    rng = np.random.default_rng(0)
+    # perturb along element-conserving directions (null space of B^T / W)
+    A = (info.composition / info.molecular_weights[:, None]).T
+    _, _, vt = np.linalg.svd(A)
+    null = vt[A.shape[0]:]
+    for _ in range(20):
+        d = null.T @ rng.standard_normal(null.shape[0])
+        d *= 1e-4 * np.abs(Ye).max() / np.abs(d).max()
+        Yp = Ye + d
+        if np.any(Yp < 0):
+            continue
+        assert gibbs(Yp) > g0
