@@ -25,7 +25,7 @@ from ._lib import CG_FP32, CG_FP64, CG_MIXED, CanteraGpuError, check, lib
 
 MECH_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "mechanisms")
 
-__all__ = ["Solution", "ReactorBatch", "CanteraGpuError", "CG_FP64", "CG_MIXED", "CG_FP32", "mech_path", "lib"]
+__all__ = ["Solution", "ReactorBatch", "MoleReactorBatch", "CanteraGpuError", "CG_FP64", "CG_MIXED", "CG_FP32", "mech_path", "lib"]
 
 
 def mech_path(name: str) -> str:
@@ -344,3 +344,48 @@ class ReactorBatch:
 
     def last_kernel_ms(self) -> float:
         return float(lib().cg_reactors_last_kernel_ms(self._h))
+
+
+class MoleReactorBatch(ReactorBatch):
+    """n IdealGasConstPressureMoleReactors, each in a ReactorNet with the
+    AdaptivePreconditioner and GMRES (``net.preconditioner =
+    ct.AdaptivePreconditioner()``): state [T, n_1..n_K] in kmol, V = 1 m^3 at
+    (T0, P, Y0).  ``skip_third_bodies`` / ``skip_falloff`` are the derivative
+    settings of the preconditioner Jacobian; ``user_atol`` applies atol to every
+    component, as ``net.atol = ...`` does (otherwise the species' atol is atol
+    times the initial total moles)."""
+
+    def __init__(self, sol: Solution, T0, P, Y0, rtol=1e-9, atol=1e-15, skip_third_bodies=False,
+                 skip_falloff=False, user_atol=False):
+        self.sol = sol
+        T0 = np.ascontiguousarray(np.atleast_1d(T0), dtype=np.float64)
+        n = T0.shape[0]
+        P = np.ascontiguousarray(np.broadcast_to(np.asarray(P, dtype=np.float64), (n,)))
+        Y0 = np.ascontiguousarray(np.asarray(Y0, dtype=np.float64).reshape(n, sol.n_species))
+        self.n = n
+        self.neq = sol.n_species + 1
+        flags = (1 if skip_third_bodies else 0) | (2 if skip_falloff else 0) | (4 if user_atol else 0)
+        self._h = check(lib().cg_reactors_create_mole(sol.handle, n, T0, P, Y0, float(rtol), float(atol), flags),
+                        "cg_reactors_create_mole")
+        self.time = np.zeros(n)
+        self.sensitivity_reactions = []
+
+    @property
+    def T(self):
+        return self.state[:, 0]
+
+    @property
+    def moles(self):
+        return self.state[:, 1:]
+
+    @property
+    def Y(self):
+        m = self.moles * self.sol.molecular_weights
+        return m / m.sum(axis=1, keepdims=True)
+
+    @property
+    def linear_stats(self) -> dict:
+        s = np.zeros((self.n, 6), dtype=np.int64)
+        check(lib().cg_reactors_linear_stats(self._h, s))
+        keys = ("lin_iters", "prec_evals", "prec_solves", "jt_vec_prod_evals", "lin_rhs_evals", "lin_conv_fails")
+        return {k: s[:, i] for i, k in enumerate(keys)}

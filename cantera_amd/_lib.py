@@ -62,6 +62,8 @@ SIGNATURES = [
     ("cg_reactors_get_state", C.c_int32, [C.c_int32, _dp]),
     ("cg_reactors_stats", C.c_int32, [C.c_int32, _i64p]),
     ("cg_reactors_status", C.c_int32, [C.c_int32, _i32p]),
+    ("cg_reactors_create_mole", C.c_int32, [C.c_int32, C.c_int64, _dp, _dp, _dp, C.c_double, C.c_double, C.c_int32]),
+    ("cg_reactors_linear_stats", C.c_int32, [C.c_int32, _i64p]),
     ("cg_reactors_add_sensitivity_reaction", C.c_int32, [C.c_int32, C.c_int32]),
     ("cg_reactors_set_sensitivity_tolerances", C.c_int32, [C.c_int32, C.c_double, C.c_double]),
     ("cg_reactors_get_sensitivities", C.c_int32, [C.c_int32, C.c_int32, _dp]),

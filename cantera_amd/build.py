@@ -18,7 +18,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 SOURCES = ["mech.cpp", "api.cpp", "rates.hip", "jacobian.hip", "order.hip", "bdf.hip", "bdf_r16.hip", "bdf_r32.hip", "bdf_lr56.hip", "bdf_lr64.hip",
            "bdf_gr56.hip", "bdf_gr64.hip",
-           "bdf_l2.hip", "surrogate.hip", "surrogate_reactors.hip", "equil.hip"]
+           "bdf_l2.hip", "bdf_mole.hip", "sparse_lu.cpp", "surrogate.hip", "surrogate_reactors.hip", "equil.hip"]
 
 COMMON = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",

@@ -16,6 +16,7 @@
 #include "../../include/cantera_gpu.h"
 #include "bdf.hpp"
 #include "mech.hpp"
+#include "sparse_lu.hpp"
 #include "surrogate.hpp"
 
 namespace {
@@ -96,6 +97,11 @@ struct Batch {
     double* sens = nullptr;          // n x sens_stride doubles
     size_t sens_stride = 0;
     bool launched = false;           // an integrator launch has been issued
+    // mole-based reactor with the sparse preconditioner and GMRES
+    bool mole = false;
+    int* sp_tables = nullptr;        // device copy of the sparse-LU schedules
+    SparseTables sp{};
+    long sp_updates = 0;
     void drop_graph() {
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
         if (graph) (void)hipGraphDestroy(graph);
@@ -109,7 +115,7 @@ struct Batch {
                         static_cast<void*>(tout), static_cast<void*>(queue), static_cast<void*>(lpt), sort_tmp,
                         static_cast<void*>(sg_Y), static_cast<void*>(sg_Yn), static_cast<void*>(sg_F),
                         static_cast<void*>(sg_P), static_cast<void*>(sg_ints), static_cast<void*>(sg_counters),
-                        static_cast<void*>(sens)})
+                        static_cast<void*>(sens), static_cast<void*>(sp_tables)})
             if (p) (void)hipFree(p);
         if (e0) (void)hipEventDestroy(e0);
         if (e1) (void)hipEventDestroy(e1);
@@ -555,6 +561,168 @@ int32_t cg_reactors_create_const_volume(int32_t h, int64_t n, const double* T0, 
     return create_batch(h, n, T0, P, Y0, rtol, atol, precision, true);
 }
 
+// Sparsity of the mole reactor's preconditioner Jacobian in the internal order
+// [n_0..n_{K-1}, T]: T row and column, the diagonal, species of a reaction
+// coupled to its reactants and products, and (unless skipped) to its explicit
+// colliders and, for a default efficiency, to every species.
+static std::vector<char> mole_pattern(const HostMech& hm, bool skip_tb) {
+    const int K = hm.K, N = K + 1;
+    std::vector<char> P(static_cast<size_t>(N) * N, 0);
+    for (int i = 0; i < N; i++) {
+        P[i * N + i] = 1;
+        P[K * N + i] = 1;
+        P[i * N + K] = 1;
+    }
+    for (const HostReaction& r : hm.rxn) {
+        std::vector<int> sp(r.rk.begin(), r.rk.end());
+        sp.insert(sp.end(), r.pk.begin(), r.pk.end());
+        const bool tb = !skip_tb && (r.type == kThreeBody || r.type == kLindemann || r.type == kTroe ||
+                                     r.type == kSri || r.type == kTsang);
+        for (int k : sp) {
+            for (int j : sp) P[k * N + j] = 1;
+            if (tb) {
+                for (int j : r.tbk) P[k * N + j] = 1;
+                if (r.tbdef != 0.0)
+                    for (int j = 0; j < K; j++) P[k * N + j] = 1;
+            }
+        }
+    }
+    return P;
+}
+
+int32_t cg_reactors_create_mole(int32_t h, int64_t n, const double* T0, const double* P, const double* Y0,
+                                double rtol, double atol, int32_t flags) {
+    return guarded([&] {
+        MechObj& mo = get_mech(h);
+        if (n <= 0 || n > (1LL << 31) - 1) throw std::runtime_error("reactor count out of range");
+        if (!T0 || !P || !Y0) throw std::runtime_error("null array");
+        if (!(rtol > 0) || !(atol > 0)) throw std::runtime_error("tolerances must be positive");
+        if (flags < 0 || flags > 7) throw std::runtime_error("flags: 1 skip third bodies, 2 skip falloff, 4 user atol");
+        const DevMech& m = mo.dm->view();
+        const HostMech& hm = mo.dm->host();
+        const int K = m.K, N = K + 1;
+        if (bdf_mole_ns(N) < 0) throw std::runtime_error("mole-based reactors support at most 127 species");
+        for (int64_t s = 0; s < n; s++) {
+            if (!(T0[s] > 0.0) || !std::isfinite(T0[s]))
+                throw std::runtime_error("reactor " + std::to_string(s) + ": temperature must be positive and finite");
+            if (!(P[s] > 0.0) || !std::isfinite(P[s]))
+                throw std::runtime_error("reactor " + std::to_string(s) + ": pressure must be positive and finite");
+            for (int k = 0; k < K; k++)
+                if (!std::isfinite(Y0[s * K + k]))
+                    throw std::runtime_error("reactor " + std::to_string(s) + ": non-finite mass fraction");
+        }
+        // symbolic analysis of the preconditioner (host, once per batch)
+        // and a device copy of its tables (the internal index layout is
+        // this library's, so the permutation is applied to the reaction
+        // species lists of the host mechanism: reactions in mechanism order)
+        const SparseLUHost sl = sparse_lu_analyse(N, mole_pattern(hm, flags & 1));
+        auto b = std::make_unique<Batch>();
+        b->mech = &mo;
+        b->mole = true;
+        b->n = static_cast<int>(n);
+        b->K = K;
+        b->N = N;
+        b->ns = bdf_mole_ns(N);
+        b->fp32 = false;
+        b->rtol = rtol;
+        b->atol = atol;
+        b->ld = N;
+        const int NP = 64 * b->ns;
+        b->rec_stride = 7 * static_cast<size_t>(NP) + C_COUNT;
+        b->j_stride = static_cast<size_t>(sl.nnz) + static_cast<size_t>(K) * K;
+        b->lu_stride = static_cast<size_t>(sl.nnz);
+        b->piv_stride = 1;
+        b->sp_updates = sl.updates();
+        // tables: diag, lptr, lslot, lrow, optr, ops, uptr, uslot, urow, iperm, srow, scol
+        std::vector<const std::vector<int>*> parts = {&sl.diag, &sl.lptr, &sl.lslot, &sl.lrow, &sl.optr, &sl.ops,
+                                                      &sl.uptr, &sl.uslot, &sl.urow, &sl.iperm, &sl.row, &sl.col};
+        size_t total = 0;
+        for (auto* v : parts) total += v->size() + 1;
+        std::vector<int> flat;
+        flat.reserve(total);
+        std::vector<size_t> off;
+        for (auto* v : parts) {
+            off.push_back(flat.size());
+            flat.insert(flat.end(), v->begin(), v->end());
+            flat.push_back(0);
+        }
+        check(hipMalloc(&b->sp_tables, flat.size() * sizeof(int)), "hipMalloc(sparse tables)");
+        check(hipMemcpy(b->sp_tables, flat.data(), flat.size() * sizeof(int), hipMemcpyHostToDevice),
+              "hipMemcpy(sparse tables)");
+        const int* d = b->sp_tables;
+        b->sp.nnz = sl.nnz;
+        b->sp.skip = flags & 3;
+        b->sp.diag = d + off[0]; b->sp.lptr = d + off[1]; b->sp.lslot = d + off[2]; b->sp.lrow = d + off[3];
+        b->sp.optr = d + off[4]; b->sp.ops = d + off[5]; b->sp.uptr = d + off[6]; b->sp.uslot = d + off[7];
+        b->sp.urow = d + off[8]; b->sp.iperm = d + off[9]; b->sp.srow = d + off[10]; b->sp.scol = d + off[11];
+        check(hipMalloc(&b->rec, b->rec_stride * n * 8), "hipMalloc(records)");
+        check(hipMalloc(&b->jsave, b->j_stride * n * 8), "hipMalloc(jacobians)");
+        check(hipMalloc(&b->lusave, b->lu_stride * n * 8), "hipMalloc(factors)");
+        check(hipMalloc(&b->piv, b->piv_stride * n * 4), "hipMalloc(pivots)");
+        check(hipMalloc(&b->tout, n * 8), "hipMalloc(tout)");
+        check(hipMalloc(&b->queue, 64), "hipMalloc(queue)");
+        check(hipMalloc(&b->lpt, 5 * n * sizeof(int)), "hipMalloc(order)");
+        check(hipMemset(b->lpt, 0, n * sizeof(int)), "hipMemset(order)");
+        b->sort_tmp_bytes = order_temp_bytes(static_cast<int>(n));
+        check(hipMalloc(&b->sort_tmp, b->sort_tmp_bytes > 0 ? b->sort_tmp_bytes : 16), "hipMalloc(sort)");
+        check(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "hipStreamCreate");
+        check(hipEventCreate(&b->e0), "hipEventCreate");
+        check(hipEventCreate(&b->e1), "hipEventCreate");
+        // initial records: V = 1 m^3, n_k = rho V Y_k / W_k, internal [n.., T]
+        std::vector<double> rec(b->rec_stride * n, 0.0);
+        const double R = 6.02214076e26 * 1.380649e-23;
+        for (int64_t s = 0; s < n; s++) {
+            double* r = rec.data() + s * b->rec_stride;
+            const double* y = Y0 + s * K;
+            double sum = 0.0;
+            for (int k = 0; k < K; k++) sum += y[k] * (1.0 / hm.mw[k]);
+            const double mmw = 1.0 / sum;
+            const double mass = P[s] * mmw / (T0[s] * R) * 1.0;
+            double tot = 0.0;
+            for (int k = 0; k < K; k++) {
+                r[k] = mass * (1.0 / hm.mw[k]) * y[k];
+                tot += r[k];
+            }
+            r[K] = T0[s];
+            for (int i = 0; i < N; i++) r[6 * NP + i] = r[i];
+            double* c = r + 7 * NP;
+            c[C_Q] = 1; c[C_L] = 2; c[C_QWAIT] = 2; c[C_QPRIME] = 1; c[C_ETAMAX] = 10000.0;
+            c[C_P] = P[s];
+            c[C_ETA] = 1.0;
+            c[C_ATOL_SP] = (flags & 4) ? atol : atol * tot;
+        }
+        check(hipMemcpy(b->rec, rec.data(), rec.size() * 8, hipMemcpyHostToDevice), "hipMemcpy(records)");
+        b->lds = bdf_mole_lds_bytes(m, sl.nnz);
+        set_lds_limit(bdf_mole_kernel_ptr(N), b->lds);
+        std::lock_guard<std::mutex> lk(g_mu);
+        g_batches.push_back(std::move(b));
+        return static_cast<int32_t>(g_batches.size() - 1);
+    });
+}
+
+int32_t cg_reactors_linear_stats(int32_t h, int64_t* stats) {
+    return guarded([&] {
+        Batch& b = get_batch(h);
+        if (!stats) throw std::runtime_error("null output");
+        check(hipStreamSynchronize(b.stream), "hipStreamSynchronize");
+        const int NP = 64 * b.ns;
+        std::vector<double> c(static_cast<size_t>(b.n) * C_COUNT);
+        check(hipMemcpy2D(c.data(), C_COUNT * 8, b.rec + 7 * NP, b.rec_stride * 8, C_COUNT * 8, b.n, hipMemcpyDeviceToHost),
+              "hipMemcpy2D(stats)");
+        for (int s = 0; s < b.n; s++) {
+            const double* x = c.data() + static_cast<size_t>(s) * C_COUNT;
+            int64_t* o = stats + static_cast<size_t>(s) * 6;
+            o[0] = static_cast<int64_t>(x[C_NLI]);
+            o[1] = b.mole ? static_cast<int64_t>(x[C_NJE]) : 0;
+            o[2] = static_cast<int64_t>(x[C_NPS]);
+            o[3] = static_cast<int64_t>(x[C_NJT]);
+            o[4] = static_cast<int64_t>(x[C_NFE_LS]);
+            o[5] = static_cast<int64_t>(x[C_NCFL]);
+        }
+        return 0;
+    });
+}
+
 int32_t cg_reactors_free(int32_t h) {
     return guarded([&] {
         std::lock_guard<std::mutex> lk(g_mu);
@@ -579,6 +747,7 @@ int32_t cg_reactors_set_matrix_storage(int32_t h, int32_t storage) {
     return guarded([&] {
         Batch& b = get_batch(h);
         if (storage < 0 || storage > 2) throw std::runtime_error("storage must be 0 (default), 1 (LDS) or 2 (global)");
+        if (b.mole) throw std::runtime_error("matrix storage applies to mass-fraction reactor batches");
         b.global_matrix = storage == 2;
         b.drop_graph();
         const DevMech& m = b.mech->dm->view();
@@ -627,11 +796,21 @@ static BdfArgs advance_args(Batch& b, const int* list, int count) {
     a.atol_sens = b.atol_sens;
     a.sens = b.sens;
     a.sens_stride = b.sens_stride;
+    a.sp = b.sp;
+    if (b.mole) a.neq_norm = b.N;
     return a;
 }
 
+static const void* batch_kernel_ptr(const Batch& b) {
+    return b.mole ? bdf_mole_kernel_ptr(b.N) : bdf_kernel_ptr(b.N, b.fp32, b.global_matrix, b.nsens > 0);
+}
+
+static hipError_t batch_launch(const Batch& b, const BdfArgs& a, int grid) {
+    return b.mole ? bdf_mole_launch(a, grid, b.lds, b.stream) : bdf_launch(a, b.fp32, b.global_matrix, grid, b.lds, b.stream);
+}
+
 static int advance_grid(const Batch& b) {
-    return grid_for(bdf_kernel_ptr(b.N, b.fp32, b.global_matrix, b.nsens > 0), b.lds, b.waves_per_cu, b.n);
+    return grid_for(batch_kernel_ptr(b), b.lds, b.waves_per_cu, b.n);
 }
 
 // cost keys of the call just launched -> processing order of the next
@@ -648,7 +827,7 @@ static void launch_advance(Batch& b, const int* list = nullptr, int count = 0) {
     const BdfArgs a = advance_args(b, list, count);
     const int grid = advance_grid(b);
     check(hipEventRecord(b.e0, b.stream), "hipEventRecord");
-    check(bdf_launch(a, b.fp32, b.global_matrix, grid, b.lds, b.stream), "integrator kernel launch");
+    check(batch_launch(b, a, grid), "integrator kernel launch");
     check(hipEventRecord(b.e1, b.stream), "hipEventRecord");
     update_order(b);
 }
@@ -684,7 +863,7 @@ int32_t cg_reactors_advance_graph(int32_t h, int32_t n_steps, const double* t_ou
                 // later one the order its predecessor's step counts give
                 a.order = (k == 0) ? order : (b.lpt_on ? order_buf : nullptr);
                 a.tout = t_out_dev + static_cast<size_t>(k) * nn;
-                err = bdf_launch(a, b.fp32, b.global_matrix, grid, b.lds, b.stream);
+                err = batch_launch(b, a, grid);
                 if (err == hipSuccess && b.lpt_on)
                     err = order_update(b.rec, b.rec_stride, 7 * 64 * b.ns, b.n, b.lpt, b.lpt + nn, b.lpt + 2 * nn,
                                        order_buf, b.sort_tmp, b.stream);
@@ -713,6 +892,7 @@ int32_t cg_reactors_autotune(int32_t h, const double* t_probe_dev, int32_t* choi
         Batch& b = get_batch(h);
         if (!t_probe_dev) throw std::runtime_error("null t_probe");
         if (b.nsens > 0) throw std::runtime_error("autotune runs on batches without sensitivity parameters");
+        if (b.mole) throw std::runtime_error("autotune runs on mass-fraction reactor batches only");
         const DevMech& m = b.mech->dm->view();
         const int v = bdf_variant(m.N);
         // (matrix storage, waves per CU) candidates; storage 1 = LDS, 2 = global
@@ -798,6 +978,7 @@ int32_t cg_reactors_add_sensitivity_reaction(int32_t h, int32_t rxn) {
         if (b.nsens >= kMaxSens)
             throw std::runtime_error("at most " + std::to_string(kMaxSens) + " sensitivity parameters per batch");
         if (b.fp32) throw std::runtime_error("sensitivities are integrated in FP64 precision only");
+        if (b.mole) throw std::runtime_error("sensitivities are not available for mole-based reactor batches");
         int pidx = -1;
         for (int i = 0; i < I; i++)
             if (perm[i] == rxn) pidx = i;
@@ -905,6 +1086,16 @@ int32_t cg_reactors_get_state(int32_t h, double* y) {
         std::vector<double> buf(static_cast<size_t>(b.n) * b.N);
         check(hipMemcpy2D(buf.data(), b.N * 8, b.rec + 6 * NP, b.rec_stride * 8, b.N * 8, b.n, hipMemcpyDeviceToHost),
               "hipMemcpy2D(state)");
+        if (b.mole) {
+            // [T, n_1..n_K] (IdealGasConstPressureMoleReactor::getState) from [n.., T]
+            for (int s = 0; s < b.n; s++) {
+                const double* src = buf.data() + static_cast<size_t>(s) * b.N;
+                double* dst = y + static_cast<size_t>(s) * b.N;
+                dst[0] = src[b.K];
+                for (int k = 0; k < b.K; k++) dst[1 + k] = src[k];
+            }
+            return 0;
+        }
         for (int s = 0; s < b.n; s++) {
             const double* src = buf.data() + static_cast<size_t>(s) * b.N;
             double* dst = y + static_cast<size_t>(s) * b.N;
@@ -1070,6 +1261,7 @@ static void advance_surrogate_device(Batch& b, SurrogateObj& so, const double* t
     if (so.mech != b.mech) throw std::runtime_error("surrogate and reactor batch use different mechanisms");
     if (b.const_volume) throw std::runtime_error("the surrogate path supports constant-pressure reactors only");
     if (b.nsens > 0) throw std::runtime_error("the surrogate path does not integrate sensitivities");
+    if (b.mole) throw std::runtime_error("the surrogate path supports mass-fraction reactor batches only");
     if (!t_out_dev) throw std::runtime_error("null t_out");
     if (!(tol_T > 0) || !(tol_Y > 0)) throw std::runtime_error("guard tolerances must be positive");
     const int n = b.n, N = b.N, K = b.K, NP = 64 * b.ns;
@@ -1131,6 +1323,7 @@ int32_t cg_reactors_net_production_rates_device(int32_t bh, double* wdot_dev) {
     return guarded([&] {
         Batch& b = get_batch(bh);
         if (!wdot_dev) throw std::runtime_error("null rates array");
+        if (b.mole) throw std::runtime_error("rates at the states of a mole-based batch are not available");
         const DevMech& m = b.mech->dm->view();
         const size_t lds = rates_lds_bytes(m, b.ns);
         const void* kp = reactor_rates_kernel_ptr(b.ns);
@@ -1147,6 +1340,7 @@ int32_t cg_reactors_get_state_device(int32_t bh, double* y_dev) {
     return guarded([&] {
         Batch& b = get_batch(bh);
         if (!y_dev) throw std::runtime_error("null state array");
+        if (b.mole) throw std::runtime_error("device state export is for mass-fraction reactor batches");
         check(export_states(b.n, b.rec, b.rec_stride, 64 * b.ns, b.N, b.K, y_dev, b.stream), "state export");
         return 0;
     });

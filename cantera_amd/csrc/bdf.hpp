@@ -15,7 +15,17 @@ enum CtlIndex {
     C_RL1 = 20, C_GAMMA, C_GAMMAP, C_GAMRAT, C_CRATE, C_DELP, C_ACNRM, C_SAVED_TQ5,
     C_Q = 28, C_QPRIME, C_QWAIT, C_L, C_NST, C_NSTLP, C_NSTLJ, C_NSCON, C_STARTED, C_TUSER,
     C_NFE = 38, C_NJE, C_NSETUPS, C_NNI, C_NCFN, C_NETF, C_STATUS, C_P, C_HAVE_LU, C_NFE_DQ,
+    // mole-reactor kernels: linear-solver counters and the species' atol
+    C_NLI = 48, C_NPS, C_NJT, C_NCFL, C_NFE_LS, C_ATOL_SP,
     C_COUNT = 64
+};
+
+// Tables of the mole-reactor preconditioner's sparse LU (device arrays, one
+// set per batch; layout and meaning as SparseLUHost in sparse_lu.hpp).
+struct SparseTables {
+    int nnz;                                    // value slots of L + U
+    int skip;                                   // bit 0: no third-body terms, bit 1: no falloff d k/d[M]
+    const int *diag, *lptr, *lslot, *lrow, *optr, *ops, *uptr, *uslot, *urow, *iperm, *srow, *scol;
 };
 
 // most reaction-multiplier sensitivity parameters per reactor batch
@@ -48,6 +58,8 @@ struct BdfArgs {
     double rtol_sens, atol_sens;
     double* sens;                   // n x sens_stride doubles: per parameter 8 columns of 64*NS
     size_t sens_stride;
+    // mole reactor (IdealGasConstPressureMoleReactor, GMRES + sparse preconditioner)
+    SparseTables sp;
     // typed view used by the kernel
     template <class T> __host__ __device__ T* lu() const { return static_cast<T*>(lu_save_raw); }
 };
@@ -64,6 +76,12 @@ size_t bdf_lds_bytes(const DevMech& m, int ld, bool fp32, bool global);
 // a.nsens > 0 selects the FP64 kernels that also integrate the sensitivities
 hipError_t bdf_launch(const BdfArgs& a, bool fp32, bool global, int grid, size_t lds, hipStream_t stream);
 const void* bdf_kernel_ptr(int N, bool fp32, bool global, bool sens = false);
+
+// mole-reactor integrator: state [n_0..n_{K-1}, T], N = K + 1 <= 256
+int bdf_mole_ns(int N);
+size_t bdf_mole_lds_bytes(const DevMech& m, int nnz);
+hipError_t bdf_mole_launch(const BdfArgs& a, int grid, size_t lds, hipStream_t stream);
+const void* bdf_mole_kernel_ptr(int N);
 
 // batched rate evaluation (species-major SoA in and out)
 size_t rates_lds_bytes(const DevMech& m, int ns);

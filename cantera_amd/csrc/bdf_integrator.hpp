@@ -29,7 +29,9 @@
 
 #include "bdf.hpp"
 #include "linsys.hpp"
+#include "sparse_linsys.hpp"
 #include "wave_jac.hpp"
+#include "wave_mole.hpp"
 
 namespace cg {
 
@@ -91,7 +93,12 @@ __device__ __forceinline__ KernargArgs* kernarg_args() {
 // it: CV_STAGGERED, sensitivities out of the error test,
 // src/numerics/CVodesIntegrator.cpp:263-279).  Each parameter's Nordsieck
 // history is kept in global memory and updated alongside zn.
-template <int NS, class LS, typename MT, bool kSens = false>
+// kMole: IdealGasConstPressureMoleReactor (state [n_0..n_{K-1}, T]) with the
+// adaptive preconditioner (LS = SparseLdsLinSys) inside CVODES' SPGMR linear
+// solver (CVodesIntegrator.cpp:424-434, with ReactorNet::setPreconditioner,
+// ReactorNet.cpp:330-339,881-908): the Newton systems are solved by GMRES,
+// J v by difference quotients of the RHS, M = I - gamma J only preconditions.
+template <int NS, class LS, typename MT, bool kSens = false, bool kMole = false>
 struct Integrator {
     const DevMech& m;
     const BdfArgs& a;
@@ -106,6 +113,7 @@ struct Integrator {
     double* rec;
     size_t rid;
     double* sb = nullptr;   // this reactor's sensitivity block (kSens)
+    double atol_sp = 0.0;   // kMole: absolute tolerance of the species' moles
 
     __device__ __forceinline__ Integrator(const DevMech& m_, const BdfArgs& a_, Ctl& c_) : m(m_), a(a_), c(c_) {}
 
@@ -139,7 +147,12 @@ struct Integrator {
             c.P = x[C_P];
             c.jcur = 0;
             c.lu_dirty = 0;
+            if constexpr (kMole) {
+                double* cnt = ls.gm + kGmCnt;
+                for (int i = 0; i < 5; i++) cnt[i] = x[C_NLI + i];
+            }
         }
+        if constexpr (kMole) atol_sp = uni(x[C_ATOL_SP]);
         const bool have_lu = uni(x[C_HAVE_LU]) != 0.0;
         ls.bind(a.template lu<MT>() + r * a.lu_stride, a.piv_save + r * a.piv_stride);
         wave_sync();
@@ -169,17 +182,30 @@ struct Integrator {
             x[C_NETF] = uni(c.netf); x[C_NFE_DQ] = uni(c.nfe_dq);
             x[C_STATUS] = status;
             if (dirty) x[C_HAVE_LU] = 1.0;
+            if constexpr (kMole) {
+                const double* cnt = ls.gm + kGmCnt;
+                for (int i = 0; i < 5; i++) x[C_NLI + i] = cnt[i];
+            }
         }
         if (dirty) ls.save(a.template lu<MT>() + rid * a.lu_stride, a.piv_save + rid * a.piv_stride, N);
     }
 
-    __device__ __forceinline__ int f_eval(const double (&yy)[NS], double (&ff)[NS]) {
-        if (lane == 0) c.nfe += 1;
+    // ls_count: a J v product of the linear solver (lin_rhs_evals, not nfe)
+    __device__ __forceinline__ int f_eval(const double (&yy)[NS], double (&ff)[NS], bool ls_count = false) {
+        if (lane == 0) {
+            if constexpr (kMole) {
+                if (ls_count) ls.gm[kGmCnt + 4] += 1;
+                else c.nfe += 1;
+            } else {
+                c.nfe += 1;
+            }
+        }
 #pragma unroll
         for (int s = 0; s < NS; s++)
             if (comp(s) < N) S.yb[comp(s)] = yy[s];
         wave_sync();
-        rhs_call<NS>(kernarg_mech(), L, uni(c.P));
+        if constexpr (kMole) mole_rhs_call<NS>(kernarg_mech(), L, uni(c.P));
+        else rhs_call<NS>(kernarg_mech(), L, uni(c.P));
         int bad = 0;
 #pragma unroll
         for (int s = 0; s < NS; s++) {
@@ -304,9 +330,215 @@ struct Integrator {
         return 0;
     }
 
+    __device__ __forceinline__ static int sp_nnz() { return kernarg_args()->sp.nnz; }
+    __device__ __forceinline__ static double wave_dot(const double (&u)[NS], const double (&v)[NS]) {
+        double t = 0.0;
+#pragma unroll
+        for (int s = 0; s < NS; s++) t += u[s] * v[s];
+        return uni(wave_sum(t));
+    }
+
+    // z = (I - gamma J) v, J v by cvLsDQJtimes' difference quotient at the
+    // Newton iterate (y, ftemp): sig = 1 / |v|_wrms, up to 3 tries with sig
+    // quartered after a recoverable RHS failure.  0 ok, 1 recoverable.
+    __device__ __forceinline__ int atimes(const double (&v)[NS], double (&z)[NS]) {
+        double sig = 1.0 / wrms<NS>(v, ewt, a.neq_norm);
+        int ret = 1;
+        double yy[NS];
+        for (int it = 0; it < 3; it++) {
+#pragma unroll
+            for (int s = 0; s < NS; s++) yy[s] = sig * v[s] + y[s];
+            ret = f_eval(yy, z, true);
+            if (ret == 0) break;
+            sig *= 0.25;
+        }
+        if (ret) return 1;
+        const double siginv = 1.0 / sig, gamma = uni(c.gamma);
+#pragma unroll
+        for (int s = 0; s < NS; s++) z[s] = -gamma * (siginv * (z[s] - ftemp[s])) + v[s];
+        if (lane == 0) ls.gm[kGmCnt + 2] += 1;
+        return 0;
+    }
+
+    // cvLsSolve with SPGMR (SUNDIALS, restated from the published algorithm:
+    // GMRES with modified Gram-Schmidt and Givens rotations, zero initial
+    // guess, right preconditioner, both scalings the error weights, tolerance
+    // eplifac tq[4] sqrt(N) in the scaled 2-norm).  Solves (I - gamma J) x = b
+    // in place.  0 converged (or residual reduced on the first Newton
+    // iteration), 1 recoverable failure.
+    __device__ __forceinline__ int gmres(double (&b)[NS], int mnewt) {
+        double* H = ls.gm + kGmH;
+        double* rot = ls.gm + kGmRot;
+        double* cnt = ls.gm + kGmCnt;
+        const double deltar = 0.05 * uni(c.tq[4]);
+        const double bnorm = wrms<NS>(b, ewt, a.neq_norm);
+        if (bnorm <= deltar) {
+            if (mnewt > 0)
+#pragma unroll
+                for (int s = 0; s < NS; s++) b[s] = 0.0;
+            return 0;
+        }
+        const double delta = deltar * sqrt(static_cast<double>(N));
+        double V[kGmMaxl + 1][NS];
+#pragma unroll
+        for (int s = 0; s < NS; s++) V[0][s] = ewt[s] * b[s];
+        const double r_norm = sqrt(wave_dot(V[0], V[0]));
+        if (r_norm <= delta) {
+#pragma unroll
+            for (int s = 0; s < NS; s++) b[s] = 0.0;
+            return 0;
+        }
+        const double beta = r_norm;
+        double rho = r_norm, rotp = 1.0;
+        const double rinv = 1.0 / r_norm;
+#pragma unroll
+        for (int s = 0; s < NS; s++) V[0][s] *= rinv;
+        int krydim = 0;
+        bool conv = false;
+#pragma unroll
+        for (int l = 0; l < kGmMaxl; l++) {
+            wave_sync();
+            if (lane == 0) cnt[0] += 1;
+            krydim = l + 1;
+            double z[NS], w[NS];
+#pragma unroll
+            for (int s = 0; s < NS; s++) z[s] = (comp(s) < N) ? V[l][s] / ewt[s] : 0.0;
+            ls.solve(z, N);
+            if (lane == 0) cnt[1] += 1;
+            if (atimes(z, w)) {
+                if (lane == 0) cnt[3] += 1;
+                return 1;
+            }
+#pragma unroll
+            for (int s = 0; s < NS; s++) w[s] *= ewt[s];
+            // SUNModifiedGS, with one reorthogonalisation pass when the new
+            // norm falls below 1e-3 of the old
+            double h[kGmMaxl + 1];
+            const double vk = sqrt(wave_dot(w, w));
+#pragma unroll
+            for (int i = 0; i <= l; i++) {
+                h[i] = wave_dot(V[i], w);
+#pragma unroll
+                for (int s = 0; s < NS; s++) w[s] = -h[i] * V[i][s] + w[s];
+            }
+            double nn = sqrt(wave_dot(w, w));
+            if ((1000.0 * vk + nn) == 1000.0 * vk) {
+                double n2 = 0.0;
+#pragma unroll
+                for (int i = 0; i <= l; i++) {
+                    const double np = wave_dot(V[i], w);
+                    if ((1000.0 * h[i] + np) == 1000.0 * h[i]) continue;
+                    h[i] += np;
+#pragma unroll
+                    for (int s = 0; s < NS; s++) w[s] = -np * V[i][s] + w[s];
+                    n2 += np * np;
+                }
+                if (n2 != 0.0) {
+                    const double np = nn * nn - n2;
+                    nn = np > 0.0 ? sqrt(np) : 0.0;
+                }
+            }
+            // SUNQRfact update: previous rotations on the new column, then one more
+#pragma unroll
+            for (int k = 0; k < l; k++) {
+                const double t1 = h[k], t2 = h[k + 1], cr = rot[2 * k], sr = rot[2 * k + 1];
+                h[k] = cr * t1 - sr * t2;
+                h[k + 1] = sr * t1 + cr * t2;
+            }
+            double cr, sr;
+            {
+                const double t1 = h[l], t2 = nn;
+                if (t2 == 0.0) {
+                    cr = 1.0;
+                    sr = 0.0;
+                } else if (fabs(t2) >= fabs(t1)) {
+                    const double t3 = t1 / t2;
+                    sr = -1.0 / sqrt(1.0 + t3 * t3);
+                    cr = -sr * t3;
+                } else {
+                    const double t3 = t2 / t1;
+                    cr = 1.0 / sqrt(1.0 + t3 * t3);
+                    sr = -cr * t3;
+                }
+                h[l] = cr * t1 - sr * t2;
+            }
+            wave_sync();
+            if (lane == 0) {
+                rot[2 * l] = cr;
+                rot[2 * l + 1] = sr;
+#pragma unroll
+                for (int i = 0; i <= l; i++) H[i * kGmMaxl + l] = h[i];
+            }
+            wave_sync();
+            if (h[l] == 0.0) {
+                if (lane == 0) cnt[3] += 1;
+                return 1;
+            }
+            rotp *= sr;
+            rho = fabs(rotp * r_norm);
+            if (rho <= delta) {
+                conv = true;
+                break;
+            }
+            if (l + 1 < kGmMaxl + 1) {
+                const double ninv = 1.0 / nn;
+#pragma unroll
+                for (int s = 0; s < NS; s++) V[l + 1][s] = w[s] * ninv;
+            }
+        }
+        // SUNQRsol: g = R^{-1} Q (r_norm e_1); x = sum_k g_k V_k
+        double g[kGmMaxl + 1];
+        g[0] = r_norm;
+#pragma unroll
+        for (int i = 1; i <= kGmMaxl; i++) g[i] = 0.0;
+#pragma unroll
+        for (int k = 0; k < kGmMaxl; k++) {
+            if (k < krydim) {
+                const double cr = rot[2 * k], sr = rot[2 * k + 1], t1 = g[k], t2 = g[k + 1];
+                g[k] = cr * t1 - sr * t2;
+                g[k + 1] = sr * t1 + cr * t2;
+            }
+        }
+#pragma unroll
+        for (int k = kGmMaxl - 1; k >= 0; k--) {
+            if (k < krydim) {
+                g[k] /= H[k * kGmMaxl + k];
+#pragma unroll
+                for (int i = 0; i < k; i++) g[i] -= g[k] * H[i * kGmMaxl + k];
+            }
+        }
+        if (!conv && !(rho < beta)) {
+            if (lane == 0) cnt[3] += 1;
+            return 1;
+        }
+        double xc[NS];
+#pragma unroll
+        for (int s = 0; s < NS; s++) {
+            double t = 0.0;
+#pragma unroll
+            for (int k = 0; k < kGmMaxl; k++)
+                if (k < krydim) t += g[k] * V[k][s];
+            xc[s] = (comp(s) < N) ? t / ewt[s] : 0.0;
+        }
+        ls.solve(xc, N);
+        wave_sync();
+        if (lane == 0) cnt[1] += 1;
+#pragma unroll
+        for (int s = 0; s < NS; s++) b[s] = xc[s];
+        if (conv) return 0;
+        if (lane == 0) cnt[3] += 1;   // SUNLS_RES_REDUCED
+        return mnewt == 0 ? 0 : 1;
+    }
+
+    // absolute tolerance of component comp(s): the mole reactor's species
+    // (MoleReactor::updateDefaultTolerances, MoleReactor.cpp:64-76) use atol_sp
+    __device__ __forceinline__ double atol_of(int s) const {
+        if constexpr (kMole) return (comp(s) == N - 1) ? atol : atol_sp;
+        else return atol;
+    }
     __device__ __forceinline__ void ewt_set(const double (&yy)[NS]) {
 #pragma unroll
-        for (int s = 0; s < NS; s++) ewt[s] = (comp(s) < N) ? 1.0 / (rtol * fabs(yy[s]) + atol) : 0.0;
+        for (int s = 0; s < NS; s++) ewt[s] = (comp(s) < N) ? 1.0 / (rtol * fabs(yy[s]) + atol_of(s)) : 0.0;
     }
 
     // linear-system setup: Jacobian reuse decision, M = I - gamma J, LU
@@ -328,14 +560,25 @@ struct Integrator {
                 c.jcur = 1;
                 c.nfe_dq += 2;
             }
+            if constexpr (kMole) {
+                // ReactorNet::preconditionerSetup: the state clipped at the
+                // preconditioner's atol (0, AdaptivePreconditioner::stateAdjustment)
 #pragma unroll
-            for (int s = 0; s < NS; s++)
-                if (comp(s) < N) {
-                    S.Y0[comp(s)] = zn[0][s];
-                    S.F0[comp(s)] = ftemp[s];
-                }
-            wave_sync();
-            jac_call<NS>(kernarg_mech(), L, uni(c.P), Js);
+                for (int s = 0; s < NS; s++)
+                    if (comp(s) < N) S.Y0[comp(s)] = fmax(zn[0][s], 0.0);
+                wave_sync();
+                const SparseTables* sp = (const SparseTables*)&kernarg_args()->sp;
+                mole_jac_call<NS>(kernarg_mech(), L, uni(c.P), Js + sp_nnz(), Js, sp);
+            } else {
+#pragma unroll
+                for (int s = 0; s < NS; s++)
+                    if (comp(s) < N) {
+                        S.Y0[comp(s)] = zn[0][s];
+                        S.F0[comp(s)] = ftemp[s];
+                    }
+                wave_sync();
+                jac_call<NS>(kernarg_mech(), L, uni(c.P), Js);
+            }
         }
         ls.form(Js, N, gamma);
         const bool ok = ls.factor(N);
@@ -655,7 +898,16 @@ struct Integrator {
             int status;
             double crate = uni(c.crate), delp = uni(c.delp), gamrat = uni(c.gamrat);
             for (;;) {
-                if constexpr (sizeof(MT) == 8) {
+                if constexpr (kMole) {
+                    // iterative solve: no gamma-ratio scaling (CVLS scalesol is off)
+#pragma unroll
+                    for (int s = 0; s < NS; s++) delta[s] = -delta[s];
+                    if (gmres(delta, m_it)) {
+                        m_it++;
+                        status = 1;
+                        break;
+                    }
+                } else if constexpr (sizeof(MT) == 8) {
 #pragma unroll
                     for (int s = 0; s < NS; s++) delta[s] = -delta[s];
                     ls.solve(delta, N);
@@ -667,7 +919,7 @@ struct Integrator {
 #pragma unroll
                     for (int s = 0; s < NS; s++) delta[s] = static_cast<double>(bf[s]);
                 }
-                if (gamrat != 1.0) {
+                if (!kMole && gamrat != 1.0) {
                     const double sc = 2.0 / (1.0 + gamrat);
 #pragma unroll
                     for (int s = 0; s < NS; s++) delta[s] *= sc;
@@ -1006,7 +1258,7 @@ struct Integrator {
 #pragma unroll
         for (int s = 0; s < NS; s++)
             if (comp(s) < N) {
-                const double t1 = HUB_FACTOR * fabs(zn[0][s]) + (rtol * fabs(zn[0][s]) + atol);
+                const double t1 = HUB_FACTOR * fabs(zn[0][s]) + (rtol * fabs(zn[0][s]) + atol_of(s));
                 hub_inv = fmax(hub_inv, fabs(zn[1][s]) / t1);
             }
         hub_inv = uni(wave_max(hub_inv));
@@ -1147,7 +1399,7 @@ __host__ __device__ inline size_t bdf_scratch_doubles(int K, int ns) {
     return static_cast<size_t>(kWave) * ns * 4 + K + K + K + 2 * K + kCtlDoubles;
 }
 
-template <int NS, class LS, typename MT, bool kSens = false>
+template <int NS, class LS, typename MT, bool kSens = false, bool kMole = false>
 __device__ __forceinline__ void bdf_advance_body(const BdfArgs& a) {
     const DevMech& m = a.mech;
     ScratchLayout L;
@@ -1163,11 +1415,11 @@ __device__ __forceinline__ void bdf_advance_body(const BdfArgs& a) {
     double* base = lds_base() + off;
     Ctl& c = *reinterpret_cast<Ctl*>(base);
     base += kCtlDoubles;
-    Integrator<NS, LS, MT, kSens> it(m, a, c);
+    Integrator<NS, LS, MT, kSens, kMole> it(m, a, c);
     it.S = scratch_view(L);
     it.L = L;
     it.lane = lane_id();
-    it.N = m.N;
+    it.N = kMole ? m.K + 1 : m.N;
     it.rtol = a.rtol;
     it.atol = a.atol;
     it.ls.init(base, a.ld);

@@ -24,9 +24,12 @@ __device__ __forceinline__ void mat_add(double* p, double v) { unsafeAtomicAdd(p
 
 // d ropnet / d C_j contributions of reaction p, scattered into JC (K x K block
 // of A).  All per-reaction lists use static indices (registers only).
+// skip (derivative settings of BulkKinetics, BulkKinetics.cpp:266-281):
+// bit 0 leaves out the third-body [M] terms, bit 1 the falloff d k/d[M]
+// (the rate then counts as linear in [M], BulkKinetics.cpp:761-792).
 __device__ __forceinline__ void scatter_reaction_derivs(const DevMech& m, int p, const GasState& g,
                                                         const double* __restrict__ C, const double* __restrict__ MU,
-                                                        double kpd, double* A, int LD, double* acc) {
+                                                        double kpd, double* A, int LD, double* acc, int skip = 0) {
     RxnRegs x;
     load_rxn(m, p, x);
     RxnVals v;
@@ -110,10 +113,10 @@ __device__ __forceinline__ void scatter_reaction_derivs(const DevMech& m, int p,
     // holds the default efficiency; that part, equal in every column, is
     // accumulated per species in acc[k] instead of being scattered K times
     const double tbd = acc ? m.tbdef[p] : 0.0;
-    if (type != kElementary && (x.tbn > 0 || tbd != 0.0)) {
+    if (!(skip & 1) && type != kElementary && (x.tbn > 0 || tbd != 0.0)) {
         const double diff = pr_r - v.rkcn * pr_p;
         double q;
-        if (type == kThreeBody) {
+        if (type == kThreeBody || (skip & 2)) {
             q = v.kf * diff;
         } else {
             const double* fo = m.fod + 8 * x.fo;

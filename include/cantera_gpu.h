@@ -140,6 +140,28 @@ int32_t cg_reactors_create(int32_t mech, int64_t n, const double* T0, const doub
 int32_t cg_reactors_create_const_volume(int32_t mech, int64_t n, const double* T0, const double* P,
                                         const double* Y0, double rtol, double atol, int32_t precision);
 int32_t cg_reactors_free(int32_t batch);
+/* n IdealGasConstPressureMoleReactors (state [T, n_1..n_K], kmol; V = 1 m^3
+ * at (T0, P0, Y0)) in ReactorNets with the AdaptivePreconditioner and the
+ * GMRES linear solver (IdealGasConstPressureMoleReactor.cpp; ReactorNet::
+ * setPreconditioner / preconditionerSetup, src/zeroD/ReactorNet.cpp:330-339,
+ * 881-908; CVodesIntegrator.cpp:424-434 with SPGMR, right preconditioning).
+ * The preconditioner Jacobian is IdealGasConstPressureMoleReactor::
+ * getJacobianElements (:108-174); M = I - gamma J is factored completely on
+ * a fixed fill pattern (minimum-degree order) where the reference uses an
+ * incomplete LUT.  flags: 1 skip third-body derivatives, 2 skip falloff
+ * derivatives (BulkKinetics derivative settings), 4 atol applies to every
+ * component (ReactorNet::setTolerances); without 4 the species' atol is atol
+ * times the initial total moles (MoleReactor::updateDefaultTolerances,
+ * src/zeroD/MoleReactor.cpp:64-76).  FP64; at most 127 species.  Advance,
+ * status, stats and cg_reactors_get_state (which returns [T, n_1..n_K], K+1
+ * values per reactor) work as for the other batches. */
+int32_t cg_reactors_create_mole(int32_t mech, int64_t n, const double* T0, const double* P, const double* Y0,
+                                double rtol, double atol, int32_t flags);
+/* Linear-solver counters per reactor, n x 6 int64: lin_iters, prec_evals,
+ * prec_solves, jt_vec_prod_evals, lin_rhs_evals, lin_conv_fails
+ * (CVodesIntegrator::solverStats, src/numerics/CVodesIntegrator.cpp:
+ * 572-621); zero for batches with a direct linear solver. */
+int32_t cg_reactors_linear_stats(int32_t batch, int64_t* stats);
 /* Advance every reactor s to t_out[s] (host array of n).  History persists
  * between calls; each reactor steps past t_out[s] and interpolates back, as
  * ReactorNet::advance does (src/zeroD/ReactorNet.cpp:351-357;

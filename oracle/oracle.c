@@ -896,22 +896,17 @@ int or_igcpr_rhs(const or_mech* m, double P, const double* y, double* ydot) {
  * falloff [M] dependence (rate constants and Kc at fixed T).
  * T column: one-sided difference with increment sqrt(eps)*T.
  * The GPU kernels evaluate the same closed form. */
-static void jac_analytic(const or_mech* m, work_t* w, double P, const double* y, double* J) {
-    const int K = m->K, I = m->I, N = K + 2;
-    double* f0 = calloc(N, 8);
-    double* f1 = calloc(N, 8);
-    double* y1 = calloc(N, 8);
-    double* JC = calloc((size_t)K * K, 8);
-    double* acc = calloc(K, 8); /* constant volume: default-efficiency part of d[M]/dC, same in every column */
-    memset(J, 0, 8 * (size_t)N * N);
-    /* T column by difference */
-    double dT = sqrt(DBL_EPSILON) * y[1];
-    memcpy(y1, y, 8 * N);
-    y1[1] = y[1] + dT;
-    rhs_w(m, w, P, y1, f1);
-    rhs_w(m, w, P, y, f0); /* leaves the base state in w */
-    for (int i = 0; i < N; i++) J[i * N + 1] = (f1[i] - f0[i]) / dT;
-    /* dwdot/dC */
+/* d wdot_k / d C_j at fixed T into JC (K x K, row-major), concentrations
+ * varying independently: mass action with the reference's stoichiometric
+ * derivative rules, third-body and falloff [M] terms (d[M]/dC_j = the explicit
+ * efficiency for colliders listed, else the default).  With use_default the
+ * default-efficiency part, equal in every column, is accumulated per row in
+ * acc[k] (add it to every column); skip_tb / skip_fo drop the [M] terms or only
+ * the falloff d k/d[M] (the "skip-third-bodies" / "skip-falloff" derivative
+ * settings, BulkKinetics.cpp:266-281, 627-660, 761-811). */
+static void ddc_scatter(const or_mech* m, work_t* w, double* JC, double* acc, int use_default, int skip_tb,
+                        int skip_fo) {
+    const int K = m->K, I = m->I;
     for (int i = 0; i < I; i++) {
         const double* C = w->conc;
         double kfM = w->kf[i];
@@ -941,10 +936,14 @@ static void jac_analytic(const or_mech* m, work_t* w, double P, const double* y,
                 dk[nd] = m->pk[i][a]; dR[nd++] = d;
             }
         }
-        if (m->type[i] != 0 && (m->ntb[i] > 0 || (w->cv && m->tbdef[i] != 0.0))) {
+        if (!skip_tb && m->type[i] != 0 && (m->ntb[i] > 0 || (use_default && m->tbdef[i] != 0.0))) {
             double q; /* d ropnet / d[M] */
             double diff = pr_r - w->rkcn[i] * pr_p;
-            if (m->type[i] == 1) {
+            if (m->type[i] == 1 || skip_fo) {
+                /* three-body mass action; or a falloff reaction with its
+                 * d k/d[M] skipped, whose rate then counts as linear in [M]
+                 * (BulkKinetics::accumulateCompositionDerivatives,
+                 * BulkKinetics.cpp:761-792, with skip-falloff) */
                 q = w->kf[i] * diff;
             } else {
                 double T = w->T, logT = log(T), recipT = 1.0 / T;
@@ -990,8 +989,8 @@ static void jac_analytic(const or_mech* m, work_t* w, double P, const double* y,
                 q = dkdM * diff;
             }
             for (int a = 0; a < m->ntb[i]; a++) { dk[nd] = m->tbk[i][a]; dR[nd++] = q * m->tbe[i][a]; }
-            if (w->cv) {
-                /* C_tot = sum_l C_l varies at constant volume */
+            if (use_default) {
+                /* d[M]/dC_l holds the default efficiency for every l */
                 for (int a = 0; a < m->nr[i]; a++) acc[m->rk[i][a]] -= m->rs[i][a] * q * m->tbdef[i];
                 for (int a = 0; a < m->np[i]; a++) acc[m->pk[i][a]] += m->ps[i][a] * q * m->tbdef[i];
             }
@@ -1014,6 +1013,24 @@ static void jac_analytic(const or_mech* m, work_t* w, double P, const double* y,
             for (int d = 0; d < nd; d++) JC[nk[c] * K + dk[d]] += nuv[c] * dR[d];
         }
     }
+}
+
+static void jac_analytic(const or_mech* m, work_t* w, double P, const double* y, double* J) {
+    const int K = m->K, N = K + 2;
+    double* f0 = calloc(N, 8);
+    double* f1 = calloc(N, 8);
+    double* y1 = calloc(N, 8);
+    double* JC = calloc((size_t)K * K, 8);
+    double* acc = calloc(K, 8); /* constant volume: default-efficiency part of d[M]/dC, same in every column */
+    memset(J, 0, 8 * (size_t)N * N);
+    /* T column by difference */
+    double dT = sqrt(DBL_EPSILON) * y[1];
+    memcpy(y1, y, 8 * N);
+    y1[1] = y[1] + dT;
+    rhs_w(m, w, P, y1, f1);
+    rhs_w(m, w, P, y, f0); /* leaves the base state in w */
+    for (int i = 0; i < N; i++) J[i * N + 1] = (f1[i] - f0[i]) / dT;
+    ddc_scatter(m, w, JC, acc, w->cv, 0, 0);
     /* chain rule to Y and the energy row */
     double rho = w->dens, Ct = w->ctot, Wbar = w->mmw;
     double inner = 0.0;
@@ -1087,6 +1104,109 @@ int or_igcpr_jacobian(const or_mech* m, double P, const double* y, double* J, in
     return 0;
 }
 
+/* ---- IdealGasConstPressureMoleReactor (state [T, n_1..n_K], kmol) ---- */
+
+/* Phase::setMolesNoTruncate + setState_TP (Phase.cpp:550-568); returns the
+ * mass sum_k n_k W_k (MoleReactor::setMassFromMoles, MoleReactor.cpp:43-51) */
+static double set_state_moles(const or_mech* m, work_t* w, double T, double P, const double* n) {
+    const int K = m->K;
+    double totalMoles = 0.0, totalMass = 0.0;
+    for (int k = 0; k < K; k++) totalMoles += n[k];
+    for (int k = 0; k < K; k++) totalMass += n[k] * m->mw[k];
+    double mmw = totalMass / totalMoles;
+    for (int k = 0; k < K; k++) w->ym[k] = n[k] * (1.0 / (mmw * totalMoles));
+    w->mmw = mmw;
+    w->T = T;
+    w->RT = T * GAS_CONSTANT;
+    w->dens = P * w->mmw / w->RT;
+    w->molar_density = w->dens / w->mmw;
+    w->pressure = GAS_CONSTANT * w->molar_density * T;
+    w->P = P;
+    double mass = 0.0;
+    for (int k = 0; k < K; k++) mass += n[k] * m->mw[k];
+    return mass;
+}
+
+/* IdealGasConstPressureMoleReactor::updateState / eval
+ * (IdealGasConstPressureMoleReactor.cpp:38-104) divided by the left-hand side
+ * as ReactorNet::eval does: dT/dt = -sum_k wdot_k h_k V / (m c_p),
+ * dn_k/dt = wdot_k V. */
+static int rhs_mole(const or_mech* m, work_t* w, double P, const double* y, double* ydot) {
+    const int K = m->K;
+    double mass = set_state_moles(m, w, y[0], P, y + 1);
+    double vol = mass / w->dens;
+    update_rop(m, w);
+    double inner = 0.0;
+    for (int k = 0; k < K; k++) inner += w->ym[k] * w->cp_R[k];
+    double cp_mass = GAS_CONSTANT * (w->mmw * inner) / w->mmw;
+    double TotalCp = mass * cp_mass;
+    double mcpdTdt = 0.0;
+    for (int k = 0; k < K; k++) {
+        double hk = w->h_RT[k] * w->RT;
+        mcpdTdt -= w->wdot[k] * hk * vol;
+        ydot[1 + k] = w->wdot[k] * vol;
+    }
+    ydot[0] = mcpdTdt / TotalCp;
+    for (int i = 0; i < K + 1; i++)
+        if (!isfinite(ydot[i])) return 1;
+    return 0;
+}
+
+/* IdealGasConstPressureMoleReactor::getJacobianElements
+ * (IdealGasConstPressureMoleReactor.cpp:108-174) as a dense (K+1)^2 matrix in
+ * state order: species block d wdot/dC (the -n_i/(V N) dC_i/dn_j terms left
+ * out, as there), T column by a one-sided difference with increment
+ * sqrt(eps) T, T row (c_p,j qdot - (m c_p) sum_k h_k dndot_k/dn_j) / (m c_p)^2. */
+static void jac_mole(const or_mech* m, work_t* w, double P, const double* y, double* J, int skip_tb, int skip_fo) {
+    const int K = m->K, N = K + 1;
+    double* f0 = calloc(N, 8);
+    double* f1 = calloc(N, 8);
+    double* y1 = calloc(N, 8);
+    double* JC = calloc((size_t)K * K, 8);
+    double* acc = calloc(K, 8);
+    memset(J, 0, 8 * (size_t)N * N);
+    /* netProductionRates_ddCi at the state (kf, rkcn, C, [M] of y) */
+    rhs_mole(m, w, P, y, f0);
+    ddc_scatter(m, w, JC, acc, 1, skip_tb, skip_fo);
+    for (int k = 0; k < K; k++)
+        for (int j = 0; j < K; j++) J[(1 + k) * N + 1 + j] = JC[k * K + j] + acc[k];
+    double dT = y[0] * sqrt(DBL_EPSILON);
+    memcpy(y1, y, 8 * N);
+    y1[0] += dT;
+    rhs_mole(m, w, P, y1, f1);
+    rhs_mole(m, w, P, y, f0); /* leaves the base state in w */
+    for (int j = 0; j < N; j++) J[j * N + 0] = (f1[j] - f0[j]) / dT;
+    double mass = 0.0;
+    for (int k = 0; k < K; k++) mass += y[1 + k] * m->mw[k];
+    double vol = mass / w->dens;
+    double inner = 0.0;
+    for (int k = 0; k < K; k++) inner += w->ym[k] * w->cp_R[k];
+    double TotalCp = mass * (GAS_CONSTANT * (w->mmw * inner) / w->mmw);
+    double qdot = 0.0;
+    for (int k = 0; k < K; k++) qdot += (w->h_RT[k] * w->RT) * (w->wdot[k] * vol);
+    double denom = 1.0 / (TotalCp * TotalCp);
+    for (int j = 0; j < K; j++) {
+        double hs = 0.0;
+        for (int k = 0; k < K; k++) hs += (w->h_RT[k] * w->RT) * J[(1 + k) * N + 1 + j];
+        J[0 * N + 1 + j] = (GAS_CONSTANT * w->cp_R[j] * qdot - TotalCp * hs) * denom;
+    }
+    free(f0); free(f1); free(y1); free(JC); free(acc);
+}
+
+int or_mole_rhs(const or_mech* m, double P, const double* y, double* ydot) {
+    work_t* w = work_new(m);
+    int r = rhs_mole(m, w, P, y, ydot);
+    work_free(w);
+    return r;
+}
+
+int or_mole_jacobian(const or_mech* m, double P, const double* y, double* J, int flags) {
+    work_t* w = work_new(m);
+    jac_mole(m, w, P, y, J, flags & 1, (flags >> 1) & 1);
+    work_free(w);
+    return 0;
+}
+
 /* ======================================================================
  * Variable-order BDF in Nordsieck form (CVODE method, restated).
  * ====================================================================== */
@@ -1095,6 +1215,8 @@ int or_igcpr_jacobian(const or_mech* m, double P, const double* y, double* J, in
 #define MXSTEP 20000
 #define MAXCOR 3
 #define OR_MAXSENS 32
+#define GM_MAXL 5      /* SPGMR Krylov dimension (SUNDIALS default, no restarts) */
+#define EPLIN 0.05     /* CVLS linear-convergence factor */
 #define MXNEF 7
 #define MXNCF 10
 #define MXNEF1 3
@@ -1157,6 +1279,14 @@ struct or_reactor {
     double *ewtS, *sp, *sm, *fsp, *fsm, *dS;
     long nfeS, nniS;
     double* mult;   /* rate multipliers (NULL until set) */
+    /* IdealGasConstPressureMoleReactor with the AdaptivePreconditioner and
+     * CVODES' SPGMR linear solver (state [T, n_1..n_K]) */
+    int mole, skip_tb, skip_fo;
+    double* atolv;                     /* per-component atol (NULL: atol) */
+    double* V[GM_MAXL + 1];
+    double *gvt, *gxc, *gwork;
+    double Hes[GM_MAXL + 1][GM_MAXL], givens[2 * GM_MAXL], yg[GM_MAXL + 1];
+    long nli, nps, njt, ncfl, nfe_ls;
 };
 
 static double wrms(const double* v, const double* w, int n) {
@@ -1165,13 +1295,18 @@ static double wrms(const double* v, const double* w, int n) {
     return sqrt(s / n);
 }
 
+static int rhs_mole(const or_mech* m, work_t* w, double P, const double* y, double* ydot);
+static int rhs_r(or_reactor* r, const double* y, double* ydot) {
+    return r->mole ? rhs_mole(r->m, r->w, r->P, y, ydot) : rhs_w(r->m, r->w, r->P, y, ydot);
+}
 static int f_eval(or_reactor* r, const double* y, double* ydot) {
     r->nfe++;
-    return rhs_w(r->m, r->w, r->P, y, ydot);
+    return rhs_r(r, y, ydot);
 }
+static double atol_i(const or_reactor* r, int i) { return r->atolv ? r->atolv[i] : r->atol; }
 
 static void ewt_set(or_reactor* r, const double* y) {
-    for (int i = 0; i < r->N; i++) r->ewt[i] = 1.0 / (r->rtol * fabs(y[i]) + r->atol);
+    for (int i = 0; i < r->N; i++) r->ewt[i] = 1.0 / (r->rtol * fabs(y[i]) + atol_i(r, i));
 }
 
 /* dense LU with partial pivoting (column-oriented, first maximal pivot) */
@@ -1248,7 +1383,15 @@ static int lsetup(or_reactor* r, const double* ypred, const double* fpred) {
         r->nje++;
         r->nstlj = r->nst;
         r->jcur = 1;
-        if (r->jac_mode == OR_JAC_ANALYTIC) {
+        if (r->mole) {
+            /* ReactorNet::preconditionerSetup (ReactorNet.cpp:881-908): the
+             * state clipped at the preconditioner's atol (0,
+             * AdaptivePreconditioner::stateAdjustment), the reactor's
+             * Jacobian elements there */
+            for (int i = 0; i < N; i++) r->gwork[i] = ypred[i] > 0.0 ? ypred[i] : 0.0;
+            jac_mole(r->m, r->w, r->P, r->gwork, r->savedJ, r->skip_tb, r->skip_fo);
+            r->nfe_dq += 2;
+        } else if (r->jac_mode == OR_JAC_ANALYTIC) {
             jac_analytic(r->m, r->w, r->P, ypred, r->savedJ);
             r->nfe_dq += 2;
         } else {
@@ -1367,9 +1510,9 @@ static int sens_rhs(or_reactor* r, int p, const double* y, const double* sv, dou
     }
     r->w->mult_i = r->sidx[p];
     r->w->mult_v = 1.0 + D;
-    int bad = rhs_w(r->m, r->w, r->P, r->sp, r->fsp);
+    int bad = rhs_r(r, r->sp, r->fsp);
     r->w->mult_v = 1.0 - D;
-    bad |= rhs_w(r->m, r->w, r->P, r->sm, r->fsm);
+    bad |= rhs_r(r, r->sm, r->fsm);
     r->w->mult_i = -1;
     r->w->mult_v = 1.0;
     r->nfeS += 2;
@@ -1459,6 +1602,170 @@ static void adjust_order(or_reactor* r, int dq) {
     else if (dq == -1) decrease_bdf(r);
 }
 
+/* ---- CVLS + SPGMR for the mole reactor (SUNDIALS, restated from the
+ * published algorithm: Saad & Schultz's GMRES with modified Gram-Schmidt and
+ * Givens rotations, right preconditioning, scaling by the error weights) ---- */
+
+static double dotv(const double* a, const double* b, int n) {
+    double s = 0.0;
+    for (int i = 0; i < n; i++) s += a[i] * b[i];
+    return s;
+}
+
+/* z = P^{-1} v with P = I - gamma_setup J the factored preconditioner */
+static void psolve(or_reactor* r, const double* v, double* z) {
+    if (z != v) memcpy(z, v, 8 * r->N);
+    lu_solve(r->M, r->N, r->piv, z);
+    r->nps++;
+}
+
+/* z = (I - gamma J) v with J v by cvLsDQJtimes' difference quotient at the
+ * Newton iterate (r->y, r->ftemp): sig = 1 / |v|_wrms, up to 3 tries with sig
+ * quartered after a recoverable RHS failure; 0, 1 recoverable, -1 fatal */
+static int atimes(or_reactor* r, const double* v, double* z) {
+    const int N = r->N;
+    double sig = 1.0 / wrms(v, r->ewt, r->nnorm);
+    int ret = 1;
+    for (int it = 0; it < 3; it++) {
+        for (int i = 0; i < N; i++) r->gwork[i] = sig * v[i] + r->y[i];
+        ret = rhs_r(r, r->gwork, z);
+        r->nfe_ls++;
+        if (ret == 0) break;
+        sig *= 0.25;
+    }
+    if (ret) return 1;
+    double siginv = 1.0 / sig;
+    for (int i = 0; i < N; i++) z[i] = siginv * (z[i] - r->ftemp[i]);
+    r->njt++;
+    for (int i = 0; i < N; i++) z[i] = -r->gamma * z[i] + v[i];
+    return 0;
+}
+
+/* SUNModifiedGS: orthogonalise v[k] against v[0..k-1] into h[.][k-1], one
+ * reorthogonalisation pass when the new norm is below 1e-3 of the old */
+static double mod_gs(double** v, double (*h)[GM_MAXL], int k, int N) {
+    const double FACTOR = 1000.0;
+    double vk_norm = sqrt(dotv(v[k], v[k], N));
+    for (int i = 0; i < k; i++) {
+        h[i][k - 1] = dotv(v[i], v[k], N);
+        for (int j = 0; j < N; j++) v[k][j] = -h[i][k - 1] * v[i][j] + v[k][j];
+    }
+    double new_norm = sqrt(dotv(v[k], v[k], N));
+    double temp = FACTOR * vk_norm;
+    if ((temp + new_norm) != temp) return new_norm;
+    double new_norm_2 = 0.0;
+    for (int i = 0; i < k; i++) {
+        double np = dotv(v[i], v[k], N);
+        temp = FACTOR * h[i][k - 1];
+        if ((temp + np) == temp) continue;
+        h[i][k - 1] += np;
+        for (int j = 0; j < N; j++) v[k][j] = -np * v[i][j] + v[k][j];
+        new_norm_2 += np * np;
+    }
+    if (new_norm_2 != 0.0) {
+        double np = new_norm * new_norm - new_norm_2;
+        new_norm = np > 0.0 ? sqrt(np) : 0.0;
+    }
+    return new_norm;
+}
+
+/* SUNQRfact, update form: rotate the new column n-1 of the Hessenberg matrix
+ * by the previous rotations and add one; returns nonzero on a zero pivot */
+static int qr_update(int n, double (*h)[GM_MAXL], double* q) {
+    int nm1 = n - 1;
+    for (int k = 0; k < nm1; k++) {
+        double t1 = h[k][nm1], t2 = h[k + 1][nm1], c = q[2 * k], sn = q[2 * k + 1];
+        h[k][nm1] = c * t1 - sn * t2;
+        h[k + 1][nm1] = sn * t1 + c * t2;
+    }
+    double t1 = h[nm1][nm1], t2 = h[n][nm1], c, sn;
+    if (t2 == 0.0) {
+        c = 1.0; sn = 0.0;
+    } else if (fabs(t2) >= fabs(t1)) {
+        double t3 = t1 / t2;
+        sn = -1.0 / sqrt(1.0 + t3 * t3);
+        c = -sn * t3;
+    } else {
+        double t3 = t2 / t1;
+        c = 1.0 / sqrt(1.0 + t3 * t3);
+        sn = -c * t3;
+    }
+    q[2 * nm1] = c;
+    q[2 * nm1 + 1] = sn;
+    h[nm1][nm1] = c * t1 - sn * t2;
+    return h[nm1][nm1] == 0.0 ? n : 0;
+}
+
+/* SUNQRsol: b <- R^{-1} Q b */
+static int qr_sol(int n, double (*h)[GM_MAXL], const double* q, double* b) {
+    for (int k = 0; k < n; k++) {
+        double c = q[2 * k], sn = q[2 * k + 1], t1 = b[k], t2 = b[k + 1];
+        b[k] = c * t1 - sn * t2;
+        b[k + 1] = sn * t1 + c * t2;
+    }
+    for (int k = n - 1; k >= 0; k--) {
+        if (h[k][k] == 0.0) return k + 1;
+        b[k] /= h[k][k];
+        for (int i = 0; i < k; i++) b[i] -= b[k] * h[i][k];
+    }
+    return 0;
+}
+
+/* cvLsSolve with SPGMR (zero initial guess, right preconditioner, both
+ * scalings the error weights, tolerance eplifac tq[4] sqrt(N) in the scaled
+ * 2-norm): solves (I - gamma J) x = b in place.  Returns 0 (converged, or the
+ * residual reduced on the first Newton iteration), 1 recoverable, -1 fatal. */
+static int gmres_solve(or_reactor* r, double* b, int mnewt) {
+    const int N = r->N;
+    double deltar = EPLIN * r->tq[4];
+    double bnorm = wrms(b, r->ewt, r->nnorm);
+    if (bnorm <= deltar) {
+        if (mnewt > 0) memset(b, 0, 8 * N);
+        return 0;
+    }
+    double delta = deltar * sqrt((double)N);
+    double** V = r->V;
+    for (int i = 0; i < N; i++) V[0][i] = r->ewt[i] * b[i];
+    double r_norm = sqrt(dotv(V[0], V[0], N)), beta = r_norm, rho = r_norm;
+    if (r_norm <= delta) {
+        memset(b, 0, 8 * N);
+        return 0;
+    }
+    memset(r->Hes, 0, sizeof r->Hes);
+    double rot = 1.0;
+    for (int i = 0; i < N; i++) V[0][i] *= 1.0 / r_norm;
+    int krydim = 0, converged = 0;
+    for (int l = 0; l < GM_MAXL; l++) {
+        r->nli++;
+        krydim = l + 1;
+        for (int i = 0; i < N; i++) r->gvt[i] = V[l][i] / r->ewt[i];
+        psolve(r, r->gvt, r->gvt);
+        int ret = atimes(r, r->gvt, V[l + 1]);
+        if (ret) { r->ncfl++; return ret < 0 ? -1 : 1; }
+        for (int i = 0; i < N; i++) V[l + 1][i] *= r->ewt[i];
+        r->Hes[l + 1][l] = mod_gs(V, r->Hes, l + 1, N);
+        if (qr_update(krydim, r->Hes, r->givens)) { r->ncfl++; return 1; }
+        rot *= r->givens[2 * l + 1];
+        rho = fabs(rot * r_norm);
+        if (rho <= delta) { converged = 1; break; }
+        for (int i = 0; i < N; i++) V[l + 1][i] *= 1.0 / r->Hes[l + 1][l];
+    }
+    r->yg[0] = r_norm;
+    for (int i = 1; i <= krydim; i++) r->yg[i] = 0.0;
+    if (qr_sol(krydim, r->Hes, r->givens, r->yg)) { r->ncfl++; return 1; }
+    for (int i = 0; i < N; i++) {
+        double x = 0.0;
+        for (int k = 0; k < krydim; k++) x += r->yg[k] * V[k][i];
+        r->gxc[i] = x;
+    }
+    if (!converged && !(rho < beta)) { r->ncfl++; return 1; }
+    for (int i = 0; i < N; i++) r->gxc[i] /= r->ewt[i];
+    psolve(r, r->gxc, b);
+    if (converged) return 0;
+    r->ncfl++;               /* SUNLS_RES_REDUCED */
+    return mnewt == 0 ? 0 : 1;
+}
+
 /* Newton iteration for the corrector; returns 0 on success, 1 recoverable failure,
  * -1 fatal */
 static int nls(or_reactor* r, int nflag) {
@@ -1489,10 +1796,17 @@ static int nls(or_reactor* r, int nflag) {
         for (;;) {
             r->nni++;
             for (int i = 0; i < N; i++) delta[i] = -delta[i];
-            lu_solve(r->M, N, r->piv, delta);
-            if (r->gamrat != 1.0) {
-                double s = 2.0 / (1.0 + r->gamrat);
-                for (int i = 0; i < N; i++) delta[i] *= s;
+            if (r->mole) {
+                /* iterative solve: no gamma-ratio scaling (CVLS scalesol off) */
+                int ls = gmres_solve(r, delta, m);
+                if (ls < 0) return -1;
+                if (ls > 0) { status = 1; break; }
+            } else {
+                lu_solve(r->M, N, r->piv, delta);
+                if (r->gamrat != 1.0) {
+                    double s = 2.0 / (1.0 + r->gamrat);
+                    for (int i = 0; i < N; i++) delta[i] *= s;
+                }
             }
             for (int i = 0; i < N; i++) ycor[i] += delta[i];
             /* convergence test */
@@ -1701,7 +2015,7 @@ static int initial_h(or_reactor* r, double tout) {
     /* upper bound */
     double hub_inv = 0.0;
     for (int i = 0; i < N; i++) {
-        double t1 = HUB_FACTOR * fabs(r->zn[0][i]) + (r->rtol * fabs(r->zn[0][i]) + r->atol);
+        double t1 = HUB_FACTOR * fabs(r->zn[0][i]) + (r->rtol * fabs(r->zn[0][i]) + atol_i(r, i));
         double v = fabs(r->zn[1][i]) / t1;
         if (v > hub_inv) hub_inv = v;
     }
@@ -1790,6 +2104,9 @@ void or_reactor_free(or_reactor* r) {
         free(r->acorS[p]); free(r->sout[p]);
     }
     free(r->ewtS); free(r->sp); free(r->sm); free(r->fsp); free(r->fsm); free(r->dS); free(r->mult);
+    free(r->atolv);
+    for (int j = 0; j <= GM_MAXL; j++) free(r->V[j]);
+    free(r->gvt); free(r->gxc); free(r->gwork);
     work_free(r->w);
     free(r);
 }
@@ -1845,6 +2162,46 @@ int or_reactor_advance(or_reactor* r, double tout) {
     }
     r->t_user = tout;
     return 0;
+}
+
+/* IdealGasConstPressureMoleReactor in a ReactorNet with an
+ * AdaptivePreconditioner and the GMRES linear solver
+ * (IdealGasConstPressureMoleReactor.cpp; ReactorNet.cpp:330-339,881-908;
+ * CVodesIntegrator.cpp:424-434): V = 1 m^3 at (T0, P, Y0), state
+ * [T, n_1..n_K].  flags: 1 skip third-body derivatives, 2 skip falloff
+ * derivatives, 4 atol as given for every component (ReactorNet::setTolerances);
+ * otherwise the species' atol is atol times the initial total moles
+ * (MoleReactor::updateDefaultTolerances, MoleReactor.cpp:64-76). */
+or_reactor* or_mole_reactor_new(const or_mech* m, double T0, double P, const double* Y0, double rtol, double atol,
+                                int flags) {
+    or_reactor* r = or_reactor_new(m, T0, P, Y0, rtol, atol, 0);
+    const int K = m->K, N = K + 1;
+    r->mole = 1;
+    r->skip_tb = flags & 1;
+    r->skip_fo = (flags >> 1) & 1;
+    r->N = N;
+    r->nnorm = N;
+    set_state(m, r->w, T0, P, Y0);
+    double mass = r->w->dens * 1.0;
+    r->zn[0][0] = T0;
+    double tot = 0.0;
+    for (int k = 0; k < K; k++) {
+        r->zn[0][1 + k] = mass * m->rmw[k] * Y0[k];
+        tot += r->zn[0][1 + k];
+    }
+    memcpy(r->yout, r->zn[0], 8 * N);
+    r->atolv = calloc(N, 8);
+    for (int i = 0; i < N; i++) r->atolv[i] = (i > 0 && !(flags & 4)) ? atol * tot : atol;
+    for (int j = 0; j <= GM_MAXL; j++) r->V[j] = calloc(N, 8);
+    r->gvt = calloc(N, 8); r->gxc = calloc(N, 8); r->gwork = calloc(N, 8);
+    return r;
+}
+
+/* CVodesIntegrator::solverStats linear-solver fields (CVodesIntegrator.cpp:
+ * 572-621): lin_iters, prec_evals, prec_solves, jt_vec_prod_evals,
+ * lin_rhs_evals, lin_conv_fails */
+void or_reactor_lin_stats(const or_reactor* r, long* s) {
+    s[0] = r->nli; s[1] = r->nje; s[2] = r->nps; s[3] = r->njt; s[4] = r->nfe_ls; s[5] = r->ncfl;
 }
 
 /* Reactor::addSensitivityReaction (src/zeroD/Reactor.cpp:278-289): parameter

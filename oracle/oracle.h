@@ -124,6 +124,16 @@ void or_reactor_stats(const or_reactor* r, long* stats);
 /* forward sensitivities dy/dp to reaction-rate multipliers (at most 32),
  * added before the first step; S is neq x ns, row-major */
 int or_reactor_add_sensitivity(or_reactor* r, int rxn);
+/* IdealGasConstPressureMoleReactor with the adaptive preconditioner and SPGMR
+ * (state [T, n_1..n_K]); flags: 1 skip third-body derivatives, 2 skip falloff
+ * derivatives, 4 atol for every component as given */
+or_reactor* or_mole_reactor_new(const or_mech* m, double T0, double P, const double* Y0, double rtol, double atol,
+                                int flags);
+/* lin_iters, prec_evals, prec_solves, jt_vec_prod_evals, lin_rhs_evals, lin_conv_fails */
+void or_reactor_lin_stats(const or_reactor* r, long* stats);
+/* mole-reactor right-hand side and preconditioner Jacobian (state order) */
+int or_mole_rhs(const or_mech* m, double P, const double* y, double* ydot);
+int or_mole_jacobian(const or_mech* m, double P, const double* y, double* J, int flags);
 void or_reactor_set_sensitivity_tolerances(or_reactor* r, double rtol, double atol);
 void or_reactor_sensitivities(const or_reactor* r, double* S);
 /* every reaction's rate multiplier (Kinetics::setMultiplier) for this reactor */

@@ -65,6 +65,13 @@ def lib():
         L.or_reactor_set_sensitivity_tolerances.argtypes = [C.c_void_p, C.c_double, C.c_double]
         L.or_reactor_sensitivities.argtypes = [C.c_void_p, _dp]
         L.or_reactor_set_multipliers.argtypes = [C.c_void_p, _dp]
+        L.or_mole_reactor_new.restype = C.c_void_p
+        L.or_mole_reactor_new.argtypes = [C.c_void_p, C.c_double, C.c_double, _dp, C.c_double, C.c_double, C.c_int]
+        L.or_reactor_lin_stats.argtypes = [C.c_void_p, _lp]
+        L.or_mole_rhs.argtypes = [C.c_void_p, C.c_double, _dp, _dp]
+        L.or_mole_rhs.restype = C.c_int
+        L.or_mole_jacobian.argtypes = [C.c_void_p, C.c_double, _dp, _dp, C.c_int]
+        L.or_mole_jacobian.restype = C.c_int
         L.or_reactors_run.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp, C.c_int, C.c_double,
                                       C.c_double, C.c_int, opt, opt, C.c_int]
         L.or_reactors_run.restype = C.c_int
@@ -126,6 +133,17 @@ class Mech:
                              np.ascontiguousarray(Y_soa, np.float64), wdot, nthreads)
         return wdot
 
+    def mole_rhs(self, P, y):
+        f = np.zeros(self.K + 1)
+        lib().or_mole_rhs(self._h, float(P), np.ascontiguousarray(y, np.float64), f)
+        return f
+
+    def mole_jacobian(self, P, y, skip_third_bodies=False, skip_falloff=False):
+        J = np.zeros((self.K + 1, self.K + 1))
+        lib().or_mole_jacobian(self._h, float(P), np.ascontiguousarray(y, np.float64), J,
+                               (1 if skip_third_bodies else 0) | (2 if skip_falloff else 0))
+        return J
+
     def jacobian_batch(self, T, P, Y_soa, nthreads=0):
         """Analytic reactor Jacobians (n, N, N) at n states (Y species-major)."""
         n = len(T)
@@ -183,6 +201,8 @@ class Mech:
 class Reactor:
     """One IdealGasConstPressureReactor + ReactorNet (oracle)."""
 
+    neq_extra = 2
+
     def __init__(self, mech: Mech, T0, P, Y0, rtol=1e-9, atol=1e-15, analytic=False, const_volume=False):
         self.mech = mech
         self._h = lib().or_reactor_new(mech._h, float(T0), float(P), np.ascontiguousarray(Y0, np.float64),
@@ -210,7 +230,7 @@ class Reactor:
 
     @property
     def state(self):
-        y = np.zeros(self.mech.K + 2)
+        y = np.zeros(self.mech.K + self.neq_extra)
         lib().or_reactor_state(self._h, y)
         return y
 
@@ -231,7 +251,7 @@ class Reactor:
     def sensitivities(self):
         """dy_i/dp_j (neq x ns): the raw CVODES sensitivities."""
         ns = getattr(self, "_ns", 0)
-        S = np.zeros((self.mech.K + 2, ns))
+        S = np.zeros((self.mech.K + self.neq_extra, ns))
         if ns:
             lib().or_reactor_sensitivities(self._h, S)
         return S
@@ -241,3 +261,29 @@ class Reactor:
         s = np.zeros(9, dtype=np.int64)
         lib().or_reactor_stats(self._h, s)
         return dict(zip(("steps", "rhs", "jac", "lu", "newton", "conv_fail", "err_fail", "order", "rhs_jac"), s.tolist()))
+
+
+class MoleReactor(Reactor):
+    """IdealGasConstPressureMoleReactor + ReactorNet with the adaptive
+    preconditioner and GMRES (oracle); state [T, n_1..n_K] in kmol."""
+
+    neq_extra = 1
+
+    def __init__(self, mech: Mech, T0, P, Y0, rtol=1e-9, atol=1e-15, skip_third_bodies=False, skip_falloff=False,
+                 user_atol=False):
+        self.mech = mech
+        flags = (1 if skip_third_bodies else 0) | (2 if skip_falloff else 0) | (4 if user_atol else 0)
+        self._h = lib().or_mole_reactor_new(mech._h, float(T0), float(P), np.ascontiguousarray(Y0, np.float64),
+                                            rtol, atol, flags)
+
+    @property
+    def lin_stats(self):
+        s = np.zeros(6, dtype=np.int64)
+        lib().or_reactor_lin_stats(self._h, s)
+        return dict(zip(("lin_iters", "prec_evals", "prec_solves", "jt_vec_prod_evals", "lin_rhs_evals",
+                         "lin_conv_fails"), s.tolist()))
+
+    def mass_fractions(self, y=None):
+        y = self.state if y is None else y
+        m = y[1:] * self.mech.mw
+        return m / m.sum()
