@@ -681,7 +681,48 @@ def batch_kernels(a):
     return 0
 
 
-def ndodecane_result(a, steps=1, mixed=False):
+def mole_result(ca, L, check, sol, wl, d_tout, steps, n, el_dense, y64, cpu_args=None):
+    """The same reactors as IdealGasConstPressureMoleReactors with the
+    adaptive preconditioner and GMRES (the reference's path for large
+    mechanisms: ReactorNet.preconditioner = AdaptivePreconditioner()), with the
+    reference's default derivative settings and with third-body and falloff
+    derivatives skipped; timed like the dense line, compared with its final
+    states."""
+    out = {}
+    for skip in (False, True):
+        mb = ca.MoleReactorBatch(sol, wl.T0, wl.P, wl.Y0, rtol=wl.rtol, atol=wl.atol, skip_third_bodies=skip,
+                                 skip_falloff=skip)
+        el, kms = _timed_advances(L, check, mb, d_tout, steps, n, log=f"mole+GMRES skip={skip}")
+        st, ls = mb.stats, mb.linear_stats
+        T, Y = mb.T, mb.Y
+        out["skip_third_bodies_and_falloff" if skip else "default_derivatives"] = {
+            "value": n * steps / el, "unit": "reactor-steps/s", "speedup_vs_dense_fp64": el_dense / el,
+            "avg_launch_ms": float(np.mean(kms)), "all_reactors_integrated": bool((mb.status != 0).sum() == 0),
+            "max_abs_dT_vs_dense_K": float(np.abs(T - y64[:, 1]).max()),
+            "max_abs_dY_vs_dense": float(np.abs(Y - y64[:, 2:]).max()),
+            "integrator": {k: float(np.mean(v)) for k, v in st.items() if k != "order"},
+            "linear_solver": {k: float(np.mean(v)) for k, v in ls.items()},
+        }
+        del mb
+    out["note"] = ("state [T, n_k]; the species' atol is atol times the initial total moles "
+                   "(MoleReactor::updateDefaultTolerances), so the mole path takes fewer internal steps")
+    if not getattr(cpu_args, "no_cpu", True):
+        from oracle.oracle import Mech
+        om = Mech(sol.path)
+
+        def run(S, th):
+            S = min(S, n)
+            om.run_mole_reactors(wl.T0[:S], wl.P[:S], wl.Y0[:S], wl.dt[:S], steps, wl.rtol, wl.atol, nthreads=th)
+            return S * steps
+
+        out["cpu_baseline"] = cpu_baseline(
+            run, "reactor-steps/s", 512, 32,
+            lambda S: f"{min(S, n)} of the {n} reactors x {steps} steps, mole reactor with the adaptive "
+                      f"preconditioner and GMRES (oracle, default derivative settings), OpenMP schedule(static,1)")
+    return out
+
+
+def ndodecane_result(a, steps=5, mixed=False, n=16384):
     """n-dodecane, 262,144 reactors x `steps` advances of 1e-6 s from t = 0 (the
     start-up step, the costliest of the 100): FP64, and with mixed=True the
     mixed form on a fresh batch (error against FP64)."""
@@ -691,7 +732,6 @@ def ndodecane_result(a, steps=1, mixed=False):
 
     L = lib()
     sol = ca.Solution("nDodecane")
-    n = 262144
     wl = config5(sol, n=n, nsteps=steps)
     touts = np.ascontiguousarray(np.outer(np.arange(1, steps + 1, dtype=np.float64), wl.dt))
     d_tout = _dev_array(L, check, touts)
@@ -719,6 +759,7 @@ def ndodecane_result(a, steps=1, mixed=False):
     }
     if mixed:
         out["mixed_precision"] = mixed_result(ca, L, check, sol, wl, d_tout, steps, n, el, y64)
+    out["mole_gmres"] = mole_result(ca, L, check, sol, wl, d_tout, steps, n, el, y64, cpu_args=a)
     L.cg_dev_free(d_tout)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_reactors(sol.path, wl, steps, 512, 32)
@@ -824,7 +865,8 @@ def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--workload", default="gri30-reactors",
-                    choices=["gri30-reactors", "gri30-rates", "gri30-jacobian", "ndodecane-reactors", "h2o2-reactors",
+                    choices=["gri30-reactors", "gri30-rates", "gri30-jacobian", "ndodecane-reactors", "ndodecane-mole",
+                             "h2o2-reactors",
                              "gri30-surrogate"],
                     help="gri30-reactors is the headline line; the others measure one workload alone")
     ap.add_argument("--steps", type=int, default=None)
@@ -861,6 +903,11 @@ def main(argv=None):
         return batch_kernels(a)
     if a.workload == "gri30-surrogate":
         return surrogate(a)
+    if a.workload == "ndodecane-mole":
+        # dense mass-fraction integrator and the mole reactor with GMRES on the
+        # same n-dodecane reactors (the default line's n-dodecane sub-result)
+        print(json.dumps(ndodecane_result(a, steps=a.steps or 5, n=a.n or 16384)), flush=True)
+        return 0
     return reactors(a)
 
 

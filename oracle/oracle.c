@@ -2283,3 +2283,32 @@ int or_reactors_run(const or_mech* m, int n, const double* T0, const double* P, 
     }
     return fail;
 }
+
+/* Batch of mole reactors (IdealGasConstPressureMoleReactor + adaptive
+ * preconditioner + GMRES), OpenMP over reactors as or_reactors_run; the final
+ * states [T, n_1..n_K] into y_out (n x (K+1)) when given. */
+int or_mole_reactors_run(const or_mech* m, int n, const double* T0, const double* P, const double* Y0,
+                         const double* dt, int nout, double rtol, double atol, int flags, double* y_out,
+                         int nthreads) {
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+    (void)nthreads;
+#endif
+    const int K = m->K;
+    int fail = 0;
+#pragma omp parallel for schedule(static, 1)
+    for (int i = 0; i < n; i++) {
+        or_reactor* r = or_mole_reactor_new(m, T0[i], P[i], Y0 + (size_t)i * K, rtol, atol, flags);
+        for (int k = 1; k <= nout; k++) {
+            if (or_reactor_advance(r, k * dt[i])) {
+#pragma omp critical
+                { if (!fail) fail = -(i + 1); }
+                break;
+            }
+        }
+        if (y_out) or_reactor_state(r, y_out + (size_t)i * (K + 1));
+        or_reactor_free(r);
+    }
+    return fail;
+}

@@ -134,6 +134,10 @@ void or_reactor_lin_stats(const or_reactor* r, long* stats);
 /* mole-reactor right-hand side and preconditioner Jacobian (state order) */
 int or_mole_rhs(const or_mech* m, double P, const double* y, double* ydot);
 int or_mole_jacobian(const or_mech* m, double P, const double* y, double* J, int flags);
+/* n mole reactors to k dt[i], k = 1..nout (OpenMP); final states n x (K+1) */
+int or_mole_reactors_run(const or_mech* m, int n, const double* T0, const double* P, const double* Y0,
+                         const double* dt, int nout, double rtol, double atol, int flags, double* y_out,
+                         int nthreads);
 void or_reactor_set_sensitivity_tolerances(or_reactor* r, double rtol, double atol);
 void or_reactor_sensitivities(const or_reactor* r, double* S);
 /* every reaction's rate multiplier (Kinetics::setMultiplier) for this reactor */

@@ -72,6 +72,9 @@ def lib():
         L.or_mole_rhs.restype = C.c_int
         L.or_mole_jacobian.argtypes = [C.c_void_p, C.c_double, _dp, _dp, C.c_int]
         L.or_mole_jacobian.restype = C.c_int
+        L.or_mole_reactors_run.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp, C.c_int, C.c_double, C.c_double,
+                                           C.c_int, opt, C.c_int]
+        L.or_mole_reactors_run.restype = C.c_int
         L.or_reactors_run.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp, C.c_int, C.c_double,
                                       C.c_double, C.c_int, opt, opt, C.c_int]
         L.or_reactors_run.restype = C.c_int
@@ -132,6 +135,20 @@ class Mech:
         lib().or_rates_batch(self._h, n, np.ascontiguousarray(T, np.float64), np.ascontiguousarray(P, np.float64),
                              np.ascontiguousarray(Y_soa, np.float64), wdot, nthreads)
         return wdot
+
+    def run_mole_reactors(self, T0, P, Y0, dt, nout, rtol, atol, skip_third_bodies=False, skip_falloff=False,
+                          user_atol=False, nthreads=0):
+        """Mole reactors with the adaptive preconditioner and GMRES to
+        k dt, k = 1..nout; returns the final states [T, n_1..n_K] (n x (K+1))."""
+        n = len(T0)
+        y = np.zeros((n, self.K + 1))
+        flags = (1 if skip_third_bodies else 0) | (2 if skip_falloff else 0) | (4 if user_atol else 0)
+        r = lib().or_mole_reactors_run(self._h, n, np.ascontiguousarray(T0, np.float64),
+                                       np.ascontiguousarray(P, np.float64), np.ascontiguousarray(Y0, np.float64),
+                                       np.ascontiguousarray(dt, np.float64), nout, rtol, atol, flags, _ptr(y), nthreads)
+        if r:
+            raise RuntimeError(f"oracle integration failed for reactor {-r - 1}")
+        return y
 
     def mole_rhs(self, P, y):
         f = np.zeros(self.K + 1)
