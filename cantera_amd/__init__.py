@@ -353,18 +353,21 @@ class MoleReactorBatch(ReactorBatch):
     (T0, P, Y0).  ``skip_third_bodies`` / ``skip_falloff`` are the derivative
     settings of the preconditioner Jacobian; ``user_atol`` applies atol to every
     component, as ``net.atol = ...`` does (otherwise the species' atol is atol
-    times the initial total moles)."""
+    times the initial total moles).  ``const_volume=True``: IdealGasMoleReactors
+    (closed, rigid), state [T, V, n_1..n_K]."""
 
     def __init__(self, sol: Solution, T0, P, Y0, rtol=1e-9, atol=1e-15, skip_third_bodies=False,
-                 skip_falloff=False, user_atol=False):
+                 skip_falloff=False, user_atol=False, const_volume=False):
         self.sol = sol
         T0 = np.ascontiguousarray(np.atleast_1d(T0), dtype=np.float64)
         n = T0.shape[0]
         P = np.ascontiguousarray(np.broadcast_to(np.asarray(P, dtype=np.float64), (n,)))
         Y0 = np.ascontiguousarray(np.asarray(Y0, dtype=np.float64).reshape(n, sol.n_species))
         self.n = n
-        self.neq = sol.n_species + 1
-        flags = (1 if skip_third_bodies else 0) | (2 if skip_falloff else 0) | (4 if user_atol else 0)
+        self.neq = sol.n_species + (2 if const_volume else 1)
+        self._s0 = 2 if const_volume else 1
+        flags = ((1 if skip_third_bodies else 0) | (2 if skip_falloff else 0) | (4 if user_atol else 0) |
+                 (8 if const_volume else 0))
         self._h = check(lib().cg_reactors_create_mole(sol.handle, n, T0, P, Y0, float(rtol), float(atol), flags),
                         "cg_reactors_create_mole")
         self.time = np.zeros(n)
@@ -376,7 +379,7 @@ class MoleReactorBatch(ReactorBatch):
 
     @property
     def moles(self):
-        return self.state[:, 1:]
+        return self.state[:, self._s0:]
 
     @property
     def Y(self):

@@ -562,11 +562,12 @@ int32_t cg_reactors_create_const_volume(int32_t h, int64_t n, const double* T0, 
 }
 
 // Sparsity of the mole reactor's preconditioner Jacobian in the internal order
-// [n_0..n_{K-1}, T]: T row and column, the diagonal, species of a reaction
-// coupled to its reactants and products, and (unless skipped) to its explicit
-// colliders and, for a default efficiency, to every species.
-static std::vector<char> mole_pattern(const HostMech& hm, bool skip_tb) {
-    const int K = hm.K, N = K + 1;
+// [n_0..n_{K-1}, T] (constant volume: [n.., T, V], V coupled to nothing): T row
+// and column, the diagonal, species of a reaction coupled to its reactants and
+// products, and (unless skipped) to its explicit colliders and, for a default
+// efficiency, to every species.
+static std::vector<char> mole_pattern(const HostMech& hm, bool skip_tb, bool cv) {
+    const int K = hm.K, N = K + 1 + (cv ? 1 : 0);
     std::vector<char> P(static_cast<size_t>(N) * N, 0);
     for (int i = 0; i < N; i++) {
         P[i * N + i] = 1;
@@ -597,11 +598,15 @@ int32_t cg_reactors_create_mole(int32_t h, int64_t n, const double* T0, const do
         if (n <= 0 || n > (1LL << 31) - 1) throw std::runtime_error("reactor count out of range");
         if (!T0 || !P || !Y0) throw std::runtime_error("null array");
         if (!(rtol > 0) || !(atol > 0)) throw std::runtime_error("tolerances must be positive");
-        if (flags < 0 || flags > 7) throw std::runtime_error("flags: 1 skip third bodies, 2 skip falloff, 4 user atol");
+        if (flags < 0 || flags > 15)
+            throw std::runtime_error("flags: 1 skip third bodies, 2 skip falloff, 4 user atol, 8 constant volume");
+        const bool cv = (flags & 8) != 0;
         const DevMech& m = mo.dm->view();
         const HostMech& hm = mo.dm->host();
-        const int K = m.K, N = K + 1;
-        if (bdf_mole_ns(N) < 0) throw std::runtime_error("mole-based reactors support at most 127 species");
+        const int K = m.K, N = K + 1 + (cv ? 1 : 0);
+        if (bdf_mole_ns(N) < 0)
+            throw std::runtime_error(std::string("mole-based reactors support at most ") + (cv ? "126" : "127") +
+                                     " species");
         for (int64_t s = 0; s < n; s++) {
             if (!(T0[s] > 0.0) || !std::isfinite(T0[s]))
                 throw std::runtime_error("reactor " + std::to_string(s) + ": temperature must be positive and finite");
@@ -615,10 +620,11 @@ int32_t cg_reactors_create_mole(int32_t h, int64_t n, const double* T0, const do
         // and a device copy of its tables (the internal index layout is
         // this library's, so the permutation is applied to the reaction
         // species lists of the host mechanism: reactions in mechanism order)
-        const SparseLUHost sl = sparse_lu_analyse(N, mole_pattern(hm, flags & 1));
+        const SparseLUHost sl = sparse_lu_analyse(N, mole_pattern(hm, flags & 1, cv));
         auto b = std::make_unique<Batch>();
         b->mech = &mo;
         b->mole = true;
+        b->const_volume = cv;
         b->n = static_cast<int>(n);
         b->K = K;
         b->N = N;
@@ -668,7 +674,7 @@ int32_t cg_reactors_create_mole(int32_t h, int64_t n, const double* T0, const do
         check(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "hipStreamCreate");
         check(hipEventCreate(&b->e0), "hipEventCreate");
         check(hipEventCreate(&b->e1), "hipEventCreate");
-        // initial records: V = 1 m^3, n_k = rho V Y_k / W_k, internal [n.., T]
+        // initial records: V = 1 m^3, n_k = rho V Y_k / W_k, internal [n.., T(, V)]
         std::vector<double> rec(b->rec_stride * n, 0.0);
         const double R = 6.02214076e26 * 1.380649e-23;
         for (int64_t s = 0; s < n; s++) {
@@ -684,6 +690,7 @@ int32_t cg_reactors_create_mole(int32_t h, int64_t n, const double* T0, const do
                 tot += r[k];
             }
             r[K] = T0[s];
+            if (cv) r[K + 1] = 1.0;
             for (int i = 0; i < N; i++) r[6 * NP + i] = r[i];
             double* c = r + 7 * NP;
             c[C_Q] = 1; c[C_L] = 2; c[C_QWAIT] = 2; c[C_QPRIME] = 1; c[C_ETAMAX] = 10000.0;
@@ -1087,12 +1094,15 @@ int32_t cg_reactors_get_state(int32_t h, double* y) {
         check(hipMemcpy2D(buf.data(), b.N * 8, b.rec + 6 * NP, b.rec_stride * 8, b.N * 8, b.n, hipMemcpyDeviceToHost),
               "hipMemcpy2D(state)");
         if (b.mole) {
-            // [T, n_1..n_K] (IdealGasConstPressureMoleReactor::getState) from [n.., T]
+            // [T, n_1..n_K] (IdealGasConstPressureMoleReactor::getState) from
+            // [n.., T]; constant volume [T, V, n_1..n_K] from [n.., T, V]
+            const int s0 = b.const_volume ? 2 : 1;
             for (int s = 0; s < b.n; s++) {
                 const double* src = buf.data() + static_cast<size_t>(s) * b.N;
                 double* dst = y + static_cast<size_t>(s) * b.N;
                 dst[0] = src[b.K];
-                for (int k = 0; k < b.K; k++) dst[1 + k] = src[k];
+                if (b.const_volume) dst[1] = src[b.K + 1];
+                for (int k = 0; k < b.K; k++) dst[s0 + k] = src[k];
             }
             return 0;
         }

@@ -533,7 +533,7 @@ struct Integrator {
     // absolute tolerance of component comp(s): the mole reactor's species
     // (MoleReactor::updateDefaultTolerances, MoleReactor.cpp:64-76) use atol_sp
     __device__ __forceinline__ double atol_of(int s) const {
-        if constexpr (kMole) return (comp(s) == N - 1) ? atol : atol_sp;
+        if constexpr (kMole) return (comp(s) >= m.K) ? atol : atol_sp;
         else return atol;
     }
     __device__ __forceinline__ void ewt_set(const double (&yy)[NS]) {
@@ -1419,7 +1419,7 @@ __device__ __forceinline__ void bdf_advance_body(const BdfArgs& a) {
     it.S = scratch_view(L);
     it.L = L;
     it.lane = lane_id();
-    it.N = kMole ? m.K + 1 : m.N;
+    it.N = kMole ? m.K + 1 + (m.const_volume ? 1 : 0) : m.N;
     it.rtol = a.rtol;
     it.atol = a.atol;
     it.ls.init(base, a.ld);

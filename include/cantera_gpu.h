@@ -152,9 +152,11 @@ int32_t cg_reactors_free(int32_t batch);
  * derivatives (BulkKinetics derivative settings), 4 atol applies to every
  * component (ReactorNet::setTolerances); without 4 the species' atol is atol
  * times the initial total moles (MoleReactor::updateDefaultTolerances,
- * src/zeroD/MoleReactor.cpp:64-76).  FP64; at most 127 species.  Advance,
- * status, stats and cg_reactors_get_state (which returns [T, n_1..n_K], K+1
- * values per reactor) work as for the other batches. */
+ * src/zeroD/MoleReactor.cpp:64-76); 8 the constant-volume IdealGasMoleReactor
+ * (src/zeroD/IdealGasMoleReactor.cpp, state [T, V, n_1..n_K]).  FP64; at
+ * most 127 species (126 at constant volume).  Advance, status, stats and
+ * cg_reactors_get_state (which returns [T, n_1..n_K], or [T, V, n_1..n_K])
+ * work as for the other batches. */
 int32_t cg_reactors_create_mole(int32_t mech, int64_t n, const double* T0, const double* P, const double* Y0,
                                 double rtol, double atol, int32_t flags);
 /* Linear-solver counters per reactor, n x 6 int64: lin_iters, prec_evals,

@@ -1127,27 +1127,48 @@ static double set_state_moles(const or_mech* m, work_t* w, double T, double P, c
     return mass;
 }
 
+/* Phase::setMolesNoTruncate + setState_TD (IdealGasMoleReactor::updateState,
+ * IdealGasMoleReactor.cpp:97-114): density mass / V, pressure from it */
+static double set_state_moles_TV(const or_mech* m, work_t* w, double T, double V, const double* n) {
+    double mass = set_state_moles(m, w, T, 1.0, n);
+    w->dens = mass / V;
+    w->molar_density = w->dens / w->mmw;
+    w->pressure = GAS_CONSTANT * w->molar_density * T;
+    w->P = w->pressure;
+    return mass;
+}
+
 /* IdealGasConstPressureMoleReactor::updateState / eval
  * (IdealGasConstPressureMoleReactor.cpp:38-104) divided by the left-hand side
  * as ReactorNet::eval does: dT/dt = -sum_k wdot_k h_k V / (m c_p),
- * dn_k/dt = wdot_k V. */
+ * dn_k/dt = wdot_k V; state [T, n_1..n_K].  With w->cv the constant-volume
+ * IdealGasMoleReactor (IdealGasMoleReactor.cpp:97-166): state [T, V, n_1..n_K],
+ * dT/dt = -sum_k wdot_k u_k V / (m c_v), dV/dt = 0 (P is unused). */
 static int rhs_mole(const or_mech* m, work_t* w, double P, const double* y, double* ydot) {
-    const int K = m->K;
-    double mass = set_state_moles(m, w, y[0], P, y + 1);
-    double vol = mass / w->dens;
+    const int K = m->K, cv = w->cv, s0 = 1 + cv;
+    double mass, vol;
+    if (cv) {
+        vol = y[1];
+        mass = set_state_moles_TV(m, w, y[0], vol, y + s0);
+    } else {
+        mass = set_state_moles(m, w, y[0], P, y + s0);
+        vol = mass / w->dens;
+    }
     update_rop(m, w);
     double inner = 0.0;
     for (int k = 0; k < K; k++) inner += w->ym[k] * w->cp_R[k];
-    double cp_mass = GAS_CONSTANT * (w->mmw * inner) / w->mmw;
-    double TotalCp = mass * cp_mass;
-    double mcpdTdt = 0.0;
+    /* c_p or c_v per unit mass; h_k or u_k = h_k - RT */
+    double c_mass = GAS_CONSTANT * (w->mmw * inner - cv) / w->mmw;
+    double Total = mass * c_mass;
+    double mcdTdt = 0.0;
     for (int k = 0; k < K; k++) {
-        double hk = w->h_RT[k] * w->RT;
-        mcpdTdt -= w->wdot[k] * hk * vol;
-        ydot[1 + k] = w->wdot[k] * vol;
+        double hk = (w->h_RT[k] - cv) * w->RT;
+        mcdTdt -= w->wdot[k] * hk * vol;
+        ydot[s0 + k] = w->wdot[k] * vol;
     }
-    ydot[0] = mcpdTdt / TotalCp;
-    for (int i = 0; i < K + 1; i++)
+    ydot[0] = mcdTdt / Total;
+    if (cv) ydot[1] = 0.0;
+    for (int i = 0; i < K + s0; i++)
         if (!isfinite(ydot[i])) return 1;
     return 0;
 }
@@ -1158,7 +1179,7 @@ static int rhs_mole(const or_mech* m, work_t* w, double P, const double* y, doub
  * out, as there), T column by a one-sided difference with increment
  * sqrt(eps) T, T row (c_p,j qdot - (m c_p) sum_k h_k dndot_k/dn_j) / (m c_p)^2. */
 static void jac_mole(const or_mech* m, work_t* w, double P, const double* y, double* J, int skip_tb, int skip_fo) {
-    const int K = m->K, N = K + 1;
+    const int cv = w->cv, s0 = 1 + cv, K = m->K, N = K + s0;
     double* f0 = calloc(N, 8);
     double* f1 = calloc(N, 8);
     double* y1 = calloc(N, 8);
@@ -1169,7 +1190,7 @@ static void jac_mole(const or_mech* m, work_t* w, double P, const double* y, dou
     rhs_mole(m, w, P, y, f0);
     ddc_scatter(m, w, JC, acc, 1, skip_tb, skip_fo);
     for (int k = 0; k < K; k++)
-        for (int j = 0; j < K; j++) J[(1 + k) * N + 1 + j] = JC[k * K + j] + acc[k];
+        for (int j = 0; j < K; j++) J[(s0 + k) * N + s0 + j] = JC[k * K + j] + acc[k];
     double dT = y[0] * sqrt(DBL_EPSILON);
     memcpy(y1, y, 8 * N);
     y1[0] += dT;
@@ -1177,24 +1198,26 @@ static void jac_mole(const or_mech* m, work_t* w, double P, const double* y, dou
     rhs_mole(m, w, P, y, f0); /* leaves the base state in w */
     for (int j = 0; j < N; j++) J[j * N + 0] = (f1[j] - f0[j]) / dT;
     double mass = 0.0;
-    for (int k = 0; k < K; k++) mass += y[1 + k] * m->mw[k];
-    double vol = mass / w->dens;
+    for (int k = 0; k < K; k++) mass += y[s0 + k] * m->mw[k];
+    double vol = cv ? y[1] : mass / w->dens;
     double inner = 0.0;
     for (int k = 0; k < K; k++) inner += w->ym[k] * w->cp_R[k];
-    double TotalCp = mass * (GAS_CONSTANT * (w->mmw * inner) / w->mmw);
+    /* m c_p and h_k (constant volume: m c_v, u_k and c_v,j = c_p,j - R) */
+    double Total = mass * (GAS_CONSTANT * (w->mmw * inner - cv) / w->mmw);
     double qdot = 0.0;
-    for (int k = 0; k < K; k++) qdot += (w->h_RT[k] * w->RT) * (w->wdot[k] * vol);
-    double denom = 1.0 / (TotalCp * TotalCp);
+    for (int k = 0; k < K; k++) qdot += ((w->h_RT[k] - cv) * w->RT) * (w->wdot[k] * vol);
+    double denom = 1.0 / (Total * Total);
     for (int j = 0; j < K; j++) {
         double hs = 0.0;
-        for (int k = 0; k < K; k++) hs += (w->h_RT[k] * w->RT) * J[(1 + k) * N + 1 + j];
-        J[0 * N + 1 + j] = (GAS_CONSTANT * w->cp_R[j] * qdot - TotalCp * hs) * denom;
+        for (int k = 0; k < K; k++) hs += ((w->h_RT[k] - cv) * w->RT) * J[(s0 + k) * N + s0 + j];
+        J[0 * N + s0 + j] = (GAS_CONSTANT * (w->cp_R[j] - cv) * qdot - Total * hs) * denom;
     }
     free(f0); free(f1); free(y1); free(JC); free(acc);
 }
 
-int or_mole_rhs(const or_mech* m, double P, const double* y, double* ydot) {
+int or_mole_rhs(const or_mech* m, double P, const double* y, double* ydot, int cv) {
     work_t* w = work_new(m);
+    w->cv = cv;
     int r = rhs_mole(m, w, P, y, ydot);
     work_free(w);
     return r;
@@ -1202,6 +1225,7 @@ int or_mole_rhs(const or_mech* m, double P, const double* y, double* ydot) {
 
 int or_mole_jacobian(const or_mech* m, double P, const double* y, double* J, int flags) {
     work_t* w = work_new(m);
+    w->cv = (flags >> 3) & 1;
     jac_mole(m, w, P, y, J, flags & 1, (flags >> 1) & 1);
     work_free(w);
     return 0;
@@ -2171,27 +2195,30 @@ int or_reactor_advance(or_reactor* r, double tout) {
  * [T, n_1..n_K].  flags: 1 skip third-body derivatives, 2 skip falloff
  * derivatives, 4 atol as given for every component (ReactorNet::setTolerances);
  * otherwise the species' atol is atol times the initial total moles
- * (MoleReactor::updateDefaultTolerances, MoleReactor.cpp:64-76). */
+ * (MoleReactor::updateDefaultTolerances, MoleReactor.cpp:64-76); 8 the
+ * constant-volume IdealGasMoleReactor, state [T, V, n_1..n_K]. */
 or_reactor* or_mole_reactor_new(const or_mech* m, double T0, double P, const double* Y0, double rtol, double atol,
                                 int flags) {
     or_reactor* r = or_reactor_new(m, T0, P, Y0, rtol, atol, 0);
-    const int K = m->K, N = K + 1;
+    const int cv = (flags >> 3) & 1, s0 = 1 + cv, K = m->K, N = K + s0;
     r->mole = 1;
     r->skip_tb = flags & 1;
     r->skip_fo = (flags >> 1) & 1;
+    r->w->cv = cv;
     r->N = N;
     r->nnorm = N;
     set_state(m, r->w, T0, P, Y0);
     double mass = r->w->dens * 1.0;
     r->zn[0][0] = T0;
+    if (cv) r->zn[0][1] = 1.0;   /* V = 1 m^3 */
     double tot = 0.0;
     for (int k = 0; k < K; k++) {
-        r->zn[0][1 + k] = mass * m->rmw[k] * Y0[k];
-        tot += r->zn[0][1 + k];
+        r->zn[0][s0 + k] = mass * m->rmw[k] * Y0[k];
+        tot += r->zn[0][s0 + k];
     }
     memcpy(r->yout, r->zn[0], 8 * N);
     r->atolv = calloc(N, 8);
-    for (int i = 0; i < N; i++) r->atolv[i] = (i > 0 && !(flags & 4)) ? atol * tot : atol;
+    for (int i = 0; i < N; i++) r->atolv[i] = (i >= s0 && !(flags & 4)) ? atol * tot : atol;
     for (int j = 0; j <= GM_MAXL; j++) r->V[j] = calloc(N, 8);
     r->gvt = calloc(N, 8); r->gxc = calloc(N, 8); r->gwork = calloc(N, 8);
     return r;
@@ -2307,7 +2334,7 @@ int or_mole_reactors_run(const or_mech* m, int n, const double* T0, const double
                 break;
             }
         }
-        if (y_out) or_reactor_state(r, y_out + (size_t)i * (K + 1));
+        if (y_out) or_reactor_state(r, y_out + (size_t)i * r->N);
         or_reactor_free(r);
     }
     return fail;

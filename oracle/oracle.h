@@ -126,15 +126,16 @@ void or_reactor_stats(const or_reactor* r, long* stats);
 int or_reactor_add_sensitivity(or_reactor* r, int rxn);
 /* IdealGasConstPressureMoleReactor with the adaptive preconditioner and SPGMR
  * (state [T, n_1..n_K]); flags: 1 skip third-body derivatives, 2 skip falloff
- * derivatives, 4 atol for every component as given */
+ * derivatives, 4 atol for every component as given, 8 the constant-volume
+ * IdealGasMoleReactor (state [T, V, n_1..n_K]) */
 or_reactor* or_mole_reactor_new(const or_mech* m, double T0, double P, const double* Y0, double rtol, double atol,
                                 int flags);
 /* lin_iters, prec_evals, prec_solves, jt_vec_prod_evals, lin_rhs_evals, lin_conv_fails */
 void or_reactor_lin_stats(const or_reactor* r, long* stats);
 /* mole-reactor right-hand side and preconditioner Jacobian (state order) */
-int or_mole_rhs(const or_mech* m, double P, const double* y, double* ydot);
+int or_mole_rhs(const or_mech* m, double P, const double* y, double* ydot, int cv);
 int or_mole_jacobian(const or_mech* m, double P, const double* y, double* J, int flags);
-/* n mole reactors to k dt[i], k = 1..nout (OpenMP); final states n x (K+1) */
+/* n mole reactors to k dt[i], k = 1..nout (OpenMP); final states n x N */
 int or_mole_reactors_run(const or_mech* m, int n, const double* T0, const double* P, const double* Y0,
                          const double* dt, int nout, double rtol, double atol, int flags, double* y_out,
                          int nthreads);

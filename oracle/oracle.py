@@ -68,7 +68,7 @@ def lib():
         L.or_mole_reactor_new.restype = C.c_void_p
         L.or_mole_reactor_new.argtypes = [C.c_void_p, C.c_double, C.c_double, _dp, C.c_double, C.c_double, C.c_int]
         L.or_reactor_lin_stats.argtypes = [C.c_void_p, _lp]
-        L.or_mole_rhs.argtypes = [C.c_void_p, C.c_double, _dp, _dp]
+        L.or_mole_rhs.argtypes = [C.c_void_p, C.c_double, _dp, _dp, C.c_int]
         L.or_mole_rhs.restype = C.c_int
         L.or_mole_jacobian.argtypes = [C.c_void_p, C.c_double, _dp, _dp, C.c_int]
         L.or_mole_jacobian.restype = C.c_int
@@ -137,12 +137,13 @@ class Mech:
         return wdot
 
     def run_mole_reactors(self, T0, P, Y0, dt, nout, rtol, atol, skip_third_bodies=False, skip_falloff=False,
-                          user_atol=False, nthreads=0):
+                          user_atol=False, nthreads=0, const_volume=False):
         """Mole reactors with the adaptive preconditioner and GMRES to
-        k dt, k = 1..nout; returns the final states [T, n_1..n_K] (n x (K+1))."""
+        k dt, k = 1..nout; returns the final states [T, (V,) n_1..n_K]."""
         n = len(T0)
-        y = np.zeros((n, self.K + 1))
-        flags = (1 if skip_third_bodies else 0) | (2 if skip_falloff else 0) | (4 if user_atol else 0)
+        y = np.zeros((n, self.K + (2 if const_volume else 1)))
+        flags = ((1 if skip_third_bodies else 0) | (2 if skip_falloff else 0) | (4 if user_atol else 0) |
+                 (8 if const_volume else 0))
         r = lib().or_mole_reactors_run(self._h, n, np.ascontiguousarray(T0, np.float64),
                                        np.ascontiguousarray(P, np.float64), np.ascontiguousarray(Y0, np.float64),
                                        np.ascontiguousarray(dt, np.float64), nout, rtol, atol, flags, _ptr(y), nthreads)
@@ -150,15 +151,18 @@ class Mech:
             raise RuntimeError(f"oracle integration failed for reactor {-r - 1}")
         return y
 
-    def mole_rhs(self, P, y):
-        f = np.zeros(self.K + 1)
-        lib().or_mole_rhs(self._h, float(P), np.ascontiguousarray(y, np.float64), f)
+    def mole_rhs(self, P, y, const_volume=False):
+        """[T, n] (or, constant volume, [T, V, n]; P unused) -> ydot"""
+        f = np.zeros(len(y))
+        lib().or_mole_rhs(self._h, float(P), np.ascontiguousarray(y, np.float64), f, 1 if const_volume else 0)
         return f
 
-    def mole_jacobian(self, P, y, skip_third_bodies=False, skip_falloff=False):
-        J = np.zeros((self.K + 1, self.K + 1))
+    def mole_jacobian(self, P, y, skip_third_bodies=False, skip_falloff=False, const_volume=False):
+        N = len(y)
+        J = np.zeros((N, N))
         lib().or_mole_jacobian(self._h, float(P), np.ascontiguousarray(y, np.float64), J,
-                               (1 if skip_third_bodies else 0) | (2 if skip_falloff else 0))
+                               (1 if skip_third_bodies else 0) | (2 if skip_falloff else 0) |
+                               (8 if const_volume else 0))
         return J
 
     def jacobian_batch(self, T, P, Y_soa, nthreads=0):
@@ -281,15 +285,16 @@ class Reactor:
 
 
 class MoleReactor(Reactor):
-    """IdealGasConstPressureMoleReactor + ReactorNet with the adaptive
-    preconditioner and GMRES (oracle); state [T, n_1..n_K] in kmol."""
-
-    neq_extra = 1
+    """IdealGasConstPressureMoleReactor (or, const_volume, IdealGasMoleReactor)
+    + ReactorNet with the adaptive preconditioner and GMRES (oracle); state
+    [T, n_1..n_K] (constant volume [T, V, n_1..n_K]) in kmol."""
 
     def __init__(self, mech: Mech, T0, P, Y0, rtol=1e-9, atol=1e-15, skip_third_bodies=False, skip_falloff=False,
-                 user_atol=False):
+                 user_atol=False, const_volume=False):
         self.mech = mech
-        flags = (1 if skip_third_bodies else 0) | (2 if skip_falloff else 0) | (4 if user_atol else 0)
+        self.neq_extra = 2 if const_volume else 1
+        flags = ((1 if skip_third_bodies else 0) | (2 if skip_falloff else 0) | (4 if user_atol else 0) |
+                 (8 if const_volume else 0))
         self._h = lib().or_mole_reactor_new(mech._h, float(T0), float(P), np.ascontiguousarray(Y0, np.float64),
                                             rtol, atol, flags)
 
@@ -302,5 +307,5 @@ class MoleReactor(Reactor):
 
     def mass_fractions(self, y=None):
         y = self.state if y is None else y
-        m = y[1:] * self.mech.mw
+        m = y[self.neq_extra:] * self.mech.mw
         return m / m.sum()

@@ -80,6 +80,47 @@ def test_mole_batch_matches_oracle_and_mass_fraction_batch(oracle_mod, mech, com
           f"oracle {o.stats['steps']}; Krylov iterations {ls['lin_iters'].tolist()}", flush=True)
 
 
+def test_adaptive_precon_integration_constant_volume(oracle_mod):
+    """test_adaptive_precon_integration (test/python/test_reactor.py:1696-1728)
+    on the GPU: gri30 IdealGasMoleReactors with the preconditioner
+    (skip-third-bodies, skip-falloff) at 900-1100 K, 1 atm, CH4/air phi = 1,
+    atol 1e-16, rtol 1e-8, outputs every 0.5 s to 4.5 s; against the GPU's
+    constant-volume mass-fraction reactors with the reference's bars (Y rel
+    5e-4 + 1e-6, T and P rel 1e-5) and against the oracle's same algorithm
+    (T 1e-6, Y 1e-9 + 1e-6 relative)."""
+    import cantera_amd as ca
+    from cantera_amd.workloads import equivalence_ratio_moles
+    sol = ca.Solution("gri30")
+    Y = sol.mass_fractions_from_moles(equivalence_ratio_moles(sol, "CH4", [1.0])[0])
+    T0 = np.array([900.0, 950.0, 1000.0, 1100.0])
+    n = len(T0)
+    mb = ca.MoleReactorBatch(sol, T0, ONE_ATM, np.tile(Y, (n, 1)), rtol=1e-8, atol=1e-16, skip_third_bodies=True,
+                             skip_falloff=True, user_atol=True, const_volume=True)
+    db = ca.ReactorBatch(sol, T0, ONE_ATM, np.tile(Y, (n, 1)), rtol=1e-8, atol=1e-16, const_volume=True)
+    om = oracle_mod.Mech(mech_file("gri30"))
+    refs = [oracle_mod.MoleReactor(om, T0[s], ONE_ATM, Y, rtol=1e-8, atol=1e-16, skip_third_bodies=True,
+                                   skip_falloff=True, user_atol=True, const_volume=True) for s in range(n)]
+    R = 6.02214076e26 * 1.380649e-23
+    for t in np.arange(0.5, 5.0, 0.5):
+        print(f"gri30 IdealGasMoleReactor t={t:g}", flush=True)
+        mb.advance(np.full(n, t))
+        db.advance(np.full(n, t))
+        ym, yd = mb.state, db.state
+        assert np.all(ym[:, 1] == 1.0)
+        for s in range(n):
+            assert mb.Y[s] == pytest.approx(yd[s, 2:], rel=5e-4, abs=1e-6)
+            assert ym[s, 0] == pytest.approx(yd[s, 1], rel=1e-5)
+            Pm = R * ym[s, 0] * ym[s, 2:].sum() / ym[s, 1]
+            Pd = yd[s, 0] * R * yd[s, 1] * np.sum(yd[s, 2:] / sol.molecular_weights)
+            assert Pm == pytest.approx(Pd, rel=1e-5)
+            refs[s].advance(t)
+            yo = refs[s].state
+            assert abs(ym[s, 0] - yo[0]) <= 1e-6 * yo[0]
+            Yo = refs[s].mass_fractions()
+            assert np.all(np.abs(mb.Y[s] - Yo) <= 1e-9 + 1e-6 * np.abs(Yo))
+    assert np.all(mb.linear_stats["lin_iters"] > 0)
+
+
 def test_mole_batch_rejections():
     import cantera_amd as ca
     sol = ca.Solution("h2o2")
@@ -92,4 +133,4 @@ def test_mole_batch_rejections():
     from cantera_amd._lib import check
     with pytest.raises(ca.CanteraGpuError, match="flags"):
         check(ca.lib().cg_reactors_create_mole(sol.handle, 1, np.array([1000.0]), np.array([ONE_ATM]),
-                                               np.ascontiguousarray(Y), 1e-8, 1e-16, 9), "create")
+                                               np.ascontiguousarray(Y), 1e-8, 1e-16, 16), "create")
