@@ -1396,7 +1396,7 @@ struct Integrator {
 // LDS per wavefront: state broadcast, species/reaction scratch, control block,
 // then the linear-system area (LDS policy only).
 __host__ __device__ inline size_t bdf_scratch_doubles(int K, int ns) {
-    return static_cast<size_t>(kWave) * ns * 4 + K + K + K + 2 * K + kCtlDoubles;
+    return static_cast<size_t>(kWave) * ns * 4 + (K + 1) + (K + 1) + K + 2 * K + kCtlDoubles;
 }
 
 template <int NS, class LS, typename MT, bool kSens = false, bool kMole = false>
@@ -1408,8 +1408,8 @@ __device__ __forceinline__ void bdf_advance_body(const BdfArgs& a) {
     L.F = off;   off += kWave * NS;
     L.Y0 = off;  off += kWave * NS;
     L.F0 = off;  off += kWave * NS;
-    L.C = off;   off += m.K;
-    L.MU = off;  off += m.K;
+    L.C = off;   off += m.K + 1;
+    L.MU = off;  off += m.K + 1;
     L.WD = off;  off += m.K;
     L.W2 = off;  off += 2 * m.K;
     double* base = lds_base() + off;

@@ -37,8 +37,8 @@ __global__ __launch_bounds__(64) void jac_batch_kernel(DevMech m, long n, const 
     L.F = off;   off += kWave * NS;
     L.Y0 = off;  off += kWave * NS;
     L.F0 = off;  off += kWave * NS;
-    L.C = off;   off += m.K;
-    L.MU = off;  off += m.K;
+    L.C = off;   off += m.K + 1;
+    L.MU = off;  off += m.K + 1;
     L.WD = off;  off += m.K;
     L.W2 = off;  off += 2 * m.K;
     const WaveScratch S = scratch_view(L);
@@ -85,7 +85,7 @@ __global__ __launch_bounds__(64) void jac_batch_kernel(DevMech m, long n, const 
 static bool jac_in_lds(int ns) { return ns == 1; }
 
 size_t jac_lds_bytes(const DevMech& m, int ns) {
-    size_t d = static_cast<size_t>(kWave) * ns * 4 + 2 * m.K + m.K + 2 * m.K;
+    size_t d = static_cast<size_t>(kWave) * ns * 4 + 2 * (m.K + 1) + m.K + 2 * m.K;
     if (jac_in_lds(ns)) d += static_cast<size_t>(m.N) * m.N;
     return d * sizeof(double);
 }

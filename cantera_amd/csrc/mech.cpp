@@ -205,7 +205,7 @@ DeviceMech::DeviceMech(const HostMech& h) : h_(h) {
         const HostReaction& r = h.rxn[perm_[p]];
         RxnI& x = ri[p];
         std::memset(&x, 0, sizeof x);
-        for (int j = 0; j < kMaxSide; j++) { x.rk[j] = -1; x.pk[j] = -1; }
+        for (int j = 0; j < kMaxSide; j++) { x.rk[j] = static_cast<short>(K); x.pk[j] = static_cast<short>(K); }
         x.orig = perm_[p];
         x.fo = -1;
         x.gen = -1;
@@ -247,6 +247,7 @@ DeviceMech::DeviceMech(const HostMech& h) : h_(h) {
                 for (int j = 0; j < x.np; j++) x.pk[j] = static_cast<short>(r.pk[j]);
             }
         }
+        if (dtype == kElementary && !(x.flags & (64 | 128)) && !rgen && !pgen) x.flags |= 1024;
         if (rgen || pgen) {
             x.flags |= (rgen ? 8 : 0) | (pgen ? 16 : 0);
             x.gen = static_cast<int>(gend.size() / 12);

@@ -51,11 +51,15 @@ HostMech load_mech(const std::string& path);
 // Per-reaction device record (integer part).  Species lists follow the
 // reference StoichManagerN convention: unit-stoichiometry sides with at most
 // three molecules are expanded into repeated species ("kRep", C1/C2/C3); other
-// sides are "general" (C_AnyN: species, order, stoich).
+// sides are "general" (C_AnyN: species, order, stoich).  Unused entries of
+// rk/pk hold K, a sentinel species whose concentration is 1 and chemical
+// potential 0 in the kernels' scratch, so a plain mass-action reaction can be
+// evaluated without tests on nr and np.
 struct RxnI {
     int flags;      // bit0 reversible, bits1-2 type (0..3), bit3 reactant side general, bit4 product side general,
                     // bit5 third body, bit6 PLOG, bit7 Chebyshev (those two with type 0),
-                    // bit8 SRI (with type 3), bit9 chemically activated (types 2, 3)
+                    // bit8 SRI (with type 3), bit9 chemically activated (types 2, 3),
+                    // bit10 plain mass action (type 0, no PLOG/Chebyshev, both sides kRep)
     int nr, np;     // entries in rk/pk
     int tbo, tbn;   // third-body efficiency list offset/count
     int fo;         // falloff record index, or PLOG / Chebyshev record offset in pdd (-1 if none)

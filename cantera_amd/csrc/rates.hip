@@ -29,8 +29,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void ra
     double* base = lds_raw;
     WaveScratch S{};
     S.yb = base;  base += kWave * NS;
-    S.C = base;   base += m.K;
-    S.MU = base;  base += m.K;
+    S.C = base;   base += m.K + 1;
+    S.MU = base;  base += m.K + 1;
     S.WD = base;
     const XcdStates xs(n);
     for (long st = xs.lo; st < xs.hi; st += xs.step) {
@@ -66,8 +66,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void re
     double* base = lds_raw;
     WaveScratch S{};
     S.yb = base;  base += kWave * NS;
-    S.C = base;   base += m.K;
-    S.MU = base;  base += m.K;
+    S.C = base;   base += m.K + 1;
+    S.MU = base;  base += m.K + 1;
     S.WD = base;
     constexpr int NP = kWave * NS;
     const XcdStates xs(n);
@@ -113,7 +113,7 @@ const void* reactor_rates_kernel_ptr(int ns) {
 }
 
 size_t rates_lds_bytes(const DevMech& m, int ns) {
-    const size_t d = static_cast<size_t>(kWave) * ns + 2 * m.K + m.K;
+    const size_t d = static_cast<size_t>(kWave) * ns + 2 * (m.K + 1) + m.K;
     return d * sizeof(double);
 }
 

@@ -132,8 +132,8 @@ __global__ __launch_bounds__(64) void guard_kernel(GuardArgs a) {
     L.F = off;   off += kWave * NS;
     L.Y0 = off;  off += kWave * NS;
     L.F0 = off;  off += kWave * NS;
-    L.C = off;   off += m.K;
-    L.MU = off;  off += m.K;
+    L.C = off;   off += m.K + 1;
+    L.MU = off;  off += m.K + 1;
     L.WD = off;  off += m.K;
     L.W2 = off;
     const WaveScratch S = scratch_view(L);
@@ -288,7 +288,7 @@ __global__ void clear_ints_kernel(int* p, int n) {
 }  // namespace
 
 size_t guard_lds_bytes(const DevMech& m, int ns) {
-    const size_t d = static_cast<size_t>(kWave) * ns * 4 + 2 * m.K + m.K + 2 * m.K;
+    const size_t d = static_cast<size_t>(kWave) * ns * 4 + 2 * (m.K + 1) + m.K + 2 * m.K;
     return d * sizeof(double);
 }
 

@@ -174,8 +174,8 @@ struct XcdStates {
 // Per-wave LDS scratch for one state evaluation.
 struct WaveScratch {
     double* yb;    // N (state broadcast)
-    double* C;     // K concentrations
-    double* MU;    // K standard chemical potentials [J/kmol]
+    double* C;     // K + 1 concentrations (C[K] = 1: the unused-slot sentinel of RxnI)
+    double* MU;    // K + 1 standard chemical potentials [J/kmol] (MU[K] = 0)
     double* WD;    // K production-rate accumulators
     double* W2;    // 2K (Jacobian: wdot, D)
     double* F;     // N (right-hand side out)
@@ -535,6 +535,36 @@ __device__ __forceinline__ void eval_reaction(const DevMech& m, int p, const Gas
     o.ropr = ropr;
 }
 
+// Net rate of progress of a plain mass-action reaction (RxnI flag bit 10:
+// elementary, kRep sides of at most three entries): the same quantities as
+// eval_reaction without its tests -- unused side entries point at the
+// sentinel species (C = 1, MU = 0), which leaves every sum, product and
+// negative-concentration rule of the C1/C2/C3 cases unchanged.  pf: rate
+// multiplier of the forward rate of progress.
+template <bool kF32 = false>
+__device__ __forceinline__ double plain_ropnet(const DevMech& m, int p, const GasState& g,
+                                              const double* __restrict__ C, const double* __restrict__ MU,
+                                              const RxnRegs& x, double pf = 1.0) {
+    const double4 rd = reinterpret_cast<const double4*>(m.rd)[p];
+    const double kf = rd.x * rate_exp<kF32>(rd.y * g.logT - rd.z * g.recipT);
+    const double dr = (MU[x.rk[0]] + MU[x.rk[1]]) + MU[x.rk[2]];
+    const double dp = (MU[x.pk[0]] + MU[x.pk[1]]) + MU[x.pk[2]];
+    const double delta = dp - dr;
+    const bool rev = x.flags & 1;
+    const double rkcn = rev ? fmin(exp(delta * g.rrt - rd.w * g.logC0), kBigNumber) : 0.0;
+    double ropf = kf * pf;
+    double ropr = ropf * rkcn;
+    {
+        const double a = C[x.rk[0]], b = C[x.rk[1]], d = C[x.rk[2]];
+        ropf = ((a < 0 && (b < 0 || d < 0)) || (b < 0 && d < 0)) ? 0.0 : ropf * (a * b * d);
+    }
+    {
+        const double a = C[x.pk[0]], b = C[x.pk[1]], d = C[x.pk[2]];
+        ropr = rev ? (((a < 0 && (b < 0 || d < 0)) || (b < 0 && d < 0)) ? 0.0 : ropr * (a * b * d)) : 0.0;
+    }
+    return ropf - ropr;
+}
+
 // Gas state + per-species thermo, concentrations and chemical potentials.
 // Y values: lane owns species lane + 64 s in yk[s].  Leaves C and MU in LDS.
 // rho > 0: the state is (T, rho) instead of (T, P) (ThermoPhase::setState_TD);
@@ -602,6 +632,10 @@ __device__ __forceinline__ void species_pass(const DevMech& m, const WaveScratch
         } else {
             sr.cpr[s] = 0.0;
             sr.hrt[s] = 0.0;
+            if (k == m.K) {
+                S.MU[k] = 0.0;
+                S.C[k] = 1.0;
+            }
         }
     }
 }
@@ -623,21 +657,55 @@ __device__ __forceinline__ void production_rates(const DevMech& m, const WaveScr
     }
     wave_sync();
     const double kpd = pdep_rate_lane(m, g);
-    for (int p = lane; p < m.I; p += kWave) {
-        RxnVals v;
-        RxnRegs x;
+    // Lane j takes reactions j, j + 64, ...  When the reactions of this pass
+    // and of the next one are all plain mass action (the elementary reactions
+    // come last in the permuted order), the lane evaluates both in one
+    // branch-free block -- two independent chains of loads and exponentials
+    // for the scheduler to interleave -- and the loop advances two passes.
+    for (int p = lane; p < m.I;) {
+        RxnRegs x, x1;
         load_rxn(m, p, x);
-        if constexpr (kPert) eval_reaction<kF32>(m, p, g, S.C, S.MU, x, kpd, v, p == pert_rxn ? pert_v : 1.0);
-        else eval_reaction<kF32>(m, p, g, S.C, S.MU, x, kpd, v);
-        const double r = v.ropf - v.ropr;
-        const double* gs = (x.gen >= 0) ? m.gend + 12 * x.gen : m.gend;
-        const bool rgen = x.flags & 8, pgen = x.flags & 16;
+        const bool two = p + kWave < m.I;
+        const int p1 = two ? p + kWave : p;
+        load_rxn(m, p1, x1);
+        if (__all((x.flags & 1024) && (x1.flags & 1024))) {
+            double pf0 = 1.0, pf1 = 1.0;
+            if constexpr (kPert) {
+                pf0 = (p == pert_rxn) ? pert_v : 1.0;
+                pf1 = (p1 == pert_rxn) ? pert_v : 1.0;
+            }
+            const double r0 = plain_ropnet<kF32>(m, p, g, S.C, S.MU, x, pf0);
+            const double r1 = plain_ropnet<kF32>(m, p1, g, S.C, S.MU, x1, pf1);
 #pragma unroll
-        for (int j = 0; j < kMaxSide; j++)
-            if (j < x.nr) atomicAdd(&S.WD[x.rk[j]], -(rgen ? gs[4 + j] : 1.0) * r);
+            for (int j = 0; j < 3; j++)
+                if (j < x.nr) atomicAdd(&S.WD[x.rk[j]], -r0);
 #pragma unroll
-        for (int j = 0; j < kMaxSide; j++)
-            if (j < x.np) atomicAdd(&S.WD[x.pk[j]], (pgen ? gs[8 + j] : 1.0) * r);
+            for (int j = 0; j < 3; j++)
+                if (j < x.np) atomicAdd(&S.WD[x.pk[j]], r0);
+            if (two) {
+#pragma unroll
+                for (int j = 0; j < 3; j++)
+                    if (j < x1.nr) atomicAdd(&S.WD[x1.rk[j]], -r1);
+#pragma unroll
+                for (int j = 0; j < 3; j++)
+                    if (j < x1.np) atomicAdd(&S.WD[x1.pk[j]], r1);
+            }
+            p += 2 * kWave;
+        } else {
+            RxnVals v;
+            if constexpr (kPert) eval_reaction<kF32>(m, p, g, S.C, S.MU, x, kpd, v, p == pert_rxn ? pert_v : 1.0);
+            else eval_reaction<kF32>(m, p, g, S.C, S.MU, x, kpd, v);
+            const double r = v.ropf - v.ropr;
+            const double* gs = (x.gen >= 0) ? m.gend + 12 * x.gen : m.gend;
+            const bool rgen = x.flags & 8, pgen = x.flags & 16;
+#pragma unroll
+            for (int j = 0; j < kMaxSide; j++)
+                if (j < x.nr) atomicAdd(&S.WD[x.rk[j]], -(rgen ? gs[4 + j] : 1.0) * r);
+#pragma unroll
+            for (int j = 0; j < kMaxSide; j++)
+                if (j < x.np) atomicAdd(&S.WD[x.pk[j]], (pgen ? gs[8 + j] : 1.0) * r);
+            p += kWave;
+        }
     }
     wave_sync();
 #pragma unroll
