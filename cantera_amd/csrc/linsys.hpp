@@ -12,7 +12,9 @@
 //   that load it into registers: the registers are held only inside them, not
 //   across the whole integrator.  Used for 32 < N <= 64.
 // LdsLinSys: the matrix lives in LDS (row-major, leading dimension LD), used
-//   for N > 64 (two lanes' worth of rows per lane).
+//   for N > 64 (two lanes' worth of rows per lane); factored by panels of 4
+//   columns with the trailing rank-4 updates on the matrix cores
+//   (lu_factor_mfma).
 //
 // Both read J from the FP64 saved-Jacobian buffer and form M = I - gamma J in
 // the matrix precision MT (double: plain mode; float: mixed mode).
@@ -378,23 +380,37 @@ struct LdsLinSys {
         A = reinterpret_cast<MT*>(p);
     }
 
+    // rows in blocks of 8 so that the block's loads of J are in flight together
     __device__ __forceinline__ void form(const double* __restrict__ J, int N, double gamma) {
         const int lane = lane_id();
         const MT mg = static_cast<MT>(-gamma);
 #pragma unroll
         for (int s = 0; s < NS; s++) {
             const int j = lane + kWave * s;
-            if (j < N)
-                for (int i = 0; i < N; i++) {
+            if (j < N) {
+                int i = 0;
+                for (; i + 8 <= N; i += 8) {
+                    double x[8];
+#pragma unroll
+                    for (int b = 0; b < 8; b++) x[b] = J[(i + b) * N + j];
+#pragma unroll
+                    for (int b = 0; b < 8; b++) {
+                        MT v = mg * static_cast<MT>(x[b]);
+                        if (i + b == j) v += MT(1);
+                        A[(i + b) * LD + j] = v;
+                    }
+                }
+                for (; i < N; i++) {
                     MT v = mg * static_cast<MT>(J[i * N + j]);
                     if (i == j) v += MT(1);
                     A[i * LD + j] = v;
                 }
+            }
         }
         wave_sync();
     }
 
-    __device__ __forceinline__ bool factor(int N) { return lu_factor<NS, MT>(A, LD, N, pmap); }
+    __device__ __forceinline__ bool factor(int N) { return lu_factor_mfma<NS, MT>(A, LD, N, pmap); }
 
     template <typename V>
     __device__ __forceinline__ void solve(V (&b)[NS], int N) const {
