@@ -23,6 +23,20 @@
 
 namespace cg {
 
+// Entry j of this lane's row of a row-major N x N matrix (lane = row), zero
+// outside the leading N x N block.  The load itself is unconditional (indices
+// clamped into the block) and its value passes through an empty asm: written
+// as a select on the loaded value, the compiler moved each load under the
+// condition with a wait right after it, which serialised the NMAX loads of a
+// row (the iteration-matrix assembly cost ~39k cycles for N = 55 that way).
+template <typename T>
+__device__ __forceinline__ T row_entry(const T* __restrict__ M, int N, int lane, int j) {
+    T x = M[min(lane, N - 1) * N + min(j, N - 1)];
+    if constexpr (sizeof(T) == 8) x = opaque(x);
+    else asm("" : "+v"(x));
+    return (lane < N && j < N) ? x : T(0);
+}
+
 template <typename T>
 __device__ __forceinline__ T shfl_t(T v, int src) {
     if constexpr (sizeof(T) == 8) {
@@ -48,14 +62,12 @@ struct RegLinSys {
 
     __device__ __forceinline__ void form(const double* __restrict__ J, int N, double gamma) {
         const int lane = lane_id();
-        const bool act = lane < N;
 #pragma unroll
         for (int j = 0; j < NMAX; j++) {
-            double v = 0.0;
-            if (act && j < N) {
-                v = -gamma * J[lane * N + j];
-                if (lane == j) v += 1.0;
-            }
+            // 1 - gamma J_jj as one fused multiply-add, as the compiler formed it
+            // when the product and the sum sat in one branch
+            const double v =
+                (lane < N && j < N) ? fma(-gamma, row_entry(J, N, lane, j), lane == j ? 1.0 : 0.0) : 0.0;
             a[j] = static_cast<MT>(v);
         }
     }
@@ -191,7 +203,7 @@ struct RegLinSys {
     __device__ __forceinline__ void restore(const MT* __restrict__ src, const int* __restrict__ psrc, int N) {
         const int lane = lane_id();
 #pragma unroll
-        for (int j = 0; j < NMAX; j++) a[j] = (lane < N && j < N) ? src[lane * N + j] : MT(0);
+        for (int j = 0; j < NMAX; j++) a[j] = row_entry(src, N, lane, j);
         rdiag = src[static_cast<size_t>(N) * N + lane];
         step = psrc[lane];
         pivlane = psrc[kWave + lane];
@@ -250,14 +262,12 @@ struct LdsRegLinSys {
 
     __device__ __forceinline__ void form(const double* __restrict__ J, int N, double gamma) {
         const int lane = lane_id();
-        const bool act = lane < N;
 #pragma unroll
         for (int j = 0; j < NMAX; j++) {
-            double v = 0.0;
-            if (act && j < N) {
-                v = -gamma * J[lane * N + j];
-                if (lane == j) v += 1.0;
-            }
+            // 1 - gamma J_jj as one fused multiply-add, as the compiler formed it
+            // when the product and the sum sat in one branch
+            const double v =
+                (lane < N && j < N) ? fma(-gamma, row_entry(J, N, lane, j), lane == j ? 1.0 : 0.0) : 0.0;
             A[j * kWave + lane] = static_cast<MT>(v);
         }
         wave_sync();
@@ -324,14 +334,10 @@ template <int NMAX, typename MT>
 __device__ __noinline__ void gr_form(const double* __restrict__ J, MT* __restrict__ A, int N, double gamma) {
     N = uni(N);
     const int lane = lane_id();
-    const bool act = lane < N;
 #pragma unroll
     for (int j = 0; j < NMAX; j++) {
-        double v = 0.0;
-        if (act && j < N) {
-            v = -gamma * J[lane * N + j];
-            if (lane == j) v += 1.0;
-        }
+        const double v =
+            (lane < N && j < N) ? fma(-gamma, row_entry(J, N, lane, j), lane == j ? 1.0 : 0.0) : 0.0;
         A[j * kWave + lane] = static_cast<MT>(v);
     }
 }

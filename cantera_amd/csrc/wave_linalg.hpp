@@ -391,7 +391,7 @@ __device__ void lu_solve(const T* A, int LD, int N, const int* pmap, V* bv, V (&
                 for (int s = 0; s < NS; s++) {
                     const int i = lane + kWave * s;
                     const int k = k0 + b2;
-                    lv[b2][s] = (i > k && i < N && k < kend) ? A[i * LD + k] : T(0);
+                    lv[b2][s] = load_or_zero(A, min(i, N - 1) * LD + min(k, N - 1), i > k && i < N && k < kend);
                 }
 #pragma unroll
             for (int b2 = 0; b2 < kB; b2++) {
@@ -411,7 +411,9 @@ __device__ void lu_solve(const T* A, int LD, int N, const int* pmap, V* bv, V (&
 #pragma unroll
     for (int s = 0; s < NS; s++) {
         const int i = lane + kWave * s;
-        rd[s] = (i < N) ? V(1) / static_cast<V>(A[i * LD + i]) : V(1);
+        const int ic = min(i, N - 1);
+        const T d = load_or_zero(A, ic * LD + ic, true);
+        rd[s] = (i < N) ? V(1) / static_cast<V>(d) : V(1);
     }
 #pragma unroll
     for (int t = NS - 1; t >= 0; t--) {
@@ -424,7 +426,7 @@ __device__ void lu_solve(const T* A, int LD, int N, const int* pmap, V* bv, V (&
                 for (int s = 0; s < NS; s++) {
                     const int i = lane + kWave * s;
                     const int k = k0 - b2;
-                    uv[b2][s] = (k >= klo && i < k) ? A[i * LD + k] : T(0);
+                    uv[b2][s] = load_or_zero(A, min(i, N - 1) * LD + max(k, 0), k >= klo && i < k);
                 }
 #pragma unroll
             for (int b2 = 0; b2 < kB; b2++) {
