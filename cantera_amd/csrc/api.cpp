@@ -173,6 +173,8 @@ int grid_for(const void* kernel, size_t lds, int waves_per_cu, long work_items) 
     return static_cast<int>(std::max<long>(1, std::min(g, work_items)));
 }
 
+constexpr size_t kLdsPerCu = 160 * 1024;
+
 void set_lds_limit(const void* kernel, size_t lds) {
     if (lds > 65536)
         check(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)),
@@ -355,6 +357,7 @@ int32_t cg_jac_batch_device(int32_t h, int64_t n, const double* T, const double*
             throw std::runtime_error("cg_jac_batch: precision must be CG_FP64 or CG_FP32");
         const bool f32 = precision == CG_FP32;
         if (f32 && mo.ns != 1) throw std::runtime_error("cg_jac_batch: CG_FP32 output needs K + 2 <= 64");
+        if (mo.ns > 2) throw std::runtime_error("cg_jac_batch: mechanisms with more than 126 species are not supported");
         if (n <= 0) return 0;
         if (!T || !P || !Y || !J) throw std::runtime_error("null array");
         const DevMech& m = mo.dm->view();
@@ -411,6 +414,7 @@ int32_t cg_equilibrate_batch_device(int32_t h, int64_t n, const char* XY, const 
         if (max_iter < 1) throw std::runtime_error("max_iter must be positive");
         const DevMech& m = mo.dm->view();
         if (m.E > 8) throw std::runtime_error("equilibrium supports at most 8 elements");
+        if (mo.ns > 2) throw std::runtime_error("equilibrium: mechanisms with more than 126 species are not supported");
         const void* kp = equil_kernel_ptr(mo.ns);
         const int grid = grid_for(kp, 0, 0, n);
         check(equil_launch(m, mo.ns, hp, n, T, P, Y, T_out, Y_out, info, max_iter, grid,
@@ -605,7 +609,7 @@ int32_t cg_reactors_create_mole(int32_t h, int64_t n, const double* T0, const do
         const HostMech& hm = mo.dm->host();
         const int K = m.K, N = K + 1 + (cv ? 1 : 0);
         if (bdf_mole_ns(N) < 0)
-            throw std::runtime_error(std::string("mole-based reactors support at most ") + (cv ? "126" : "127") +
+            throw std::runtime_error(std::string("mole-based reactors support at most ") + (cv ? "254" : "255") +
                                      " species");
         for (int64_t s = 0; s < n; s++) {
             if (!(T0[s] > 0.0) || !std::isfinite(T0[s]))
@@ -700,6 +704,12 @@ int32_t cg_reactors_create_mole(int32_t h, int64_t n, const double* T0, const do
         }
         check(hipMemcpy(b->rec, rec.data(), rec.size() * 8, hipMemcpyHostToDevice), "hipMemcpy(records)");
         b->lds = bdf_mole_lds_bytes(m, sl.nnz);
+        if (b->lds > kLdsPerCu)
+            throw std::runtime_error("the preconditioner's fill pattern (" + std::to_string(sl.nnz) +
+                                     " entries) needs " + std::to_string(b->lds / 1024) +
+                                     " KiB of LDS per reactor, more than the " + std::to_string(kLdsPerCu / 1024) +
+                                     " KiB of a CU; skipping the third-body and falloff derivatives (flags 1 | 2) "
+                                     "makes it sparser");
         set_lds_limit(bdf_mole_kernel_ptr(N), b->lds);
         std::lock_guard<std::mutex> lk(g_mu);
         g_batches.push_back(std::move(b));
@@ -1173,6 +1183,7 @@ double cg_reactors_last_kernel_ms(int32_t h) {
 int32_t cg_surrogate_create(int32_t mech, double dt, int32_t hidden, int32_t depth, uint64_t seed) {
     return guarded([&] {
         MechObj& mo = get_mech(mech);
+        if (mo.ns > 2) throw std::runtime_error("surrogate: mechanisms with more than 126 species are not supported");
         auto obj = std::make_unique<SurrogateObj>();
         obj->mech = &mo;
         obj->net = std::make_unique<SurrogateNet>(mo.dm->view().K, dt, hidden, depth, seed);

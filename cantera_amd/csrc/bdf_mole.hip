@@ -13,7 +13,7 @@ __global__ __launch_bounds__(64) void bdf_advance_mole(BdfArgs a) {
     bdf_advance_body<NS, SparseLdsLinSys<NS>, double, false, true>(a);
 }
 
-int bdf_mole_ns(int N) { return N <= 64 ? 1 : (N <= 128 ? 2 : -1); }
+int bdf_mole_ns(int N) { return N <= 64 ? 1 : (N <= 128 ? 2 : (N <= 256 ? 4 : -1)); }
 
 size_t bdf_mole_lds_bytes(const DevMech& m, int nnz) {
     const int ns = bdf_mole_ns(m.K + 1);
@@ -25,6 +25,7 @@ const void* bdf_mole_kernel_ptr(int N) {
     switch (bdf_mole_ns(N)) {
         case 1: return reinterpret_cast<const void*>(&bdf_advance_mole<1>);
         case 2: return reinterpret_cast<const void*>(&bdf_advance_mole<2>);
+        case 4: return reinterpret_cast<const void*>(&bdf_advance_mole<4>);
         default: return nullptr;
     }
 }
@@ -34,6 +35,7 @@ hipError_t bdf_mole_launch(const BdfArgs& a, int grid, size_t lds, hipStream_t s
     switch (bdf_mole_ns(a.mech.K + 1)) {
         case 1: hipLaunchKernelGGL(bdf_advance_mole<1>, dim3(grid), dim3(64), lds, stream, a); break;
         case 2: hipLaunchKernelGGL(bdf_advance_mole<2>, dim3(grid), dim3(64), lds, stream, a); break;
+        case 4: hipLaunchKernelGGL(bdf_advance_mole<4>, dim3(grid), dim3(64), lds, stream, a); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

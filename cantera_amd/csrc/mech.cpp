@@ -182,7 +182,9 @@ bool expand_side(const std::vector<int>& k, const std::vector<double>& order,
 
 DeviceMech::DeviceMech(const HostMech& h) : h_(h) {
     const int K = h.K, I = h.I;
-    if (K > 126) throw std::runtime_error("mechanisms with more than 126 species are not supported");
+    // K + 2 <= 256: four state slots per lane (the dense integrators and
+    // Jacobians take K <= 126, the mole-based reactors and the rates K <= 254)
+    if (K > 254) throw std::runtime_error("mechanisms with more than 254 species are not supported");
     perm_.resize(I);
     std::iota(perm_.begin(), perm_.end(), 0);
     // Reactions of the costliest class first: a wavefront evaluates 64

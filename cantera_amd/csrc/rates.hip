@@ -107,13 +107,16 @@ hipError_t reactor_rates_launch(const DevMech& m, int ns, long n, const double* 
     return hipGetLastError();
 }
 
+// state slots per lane of the rate kernels: 1, 2, or 4 for 128 < K + 2 <= 256
+static int rates_ns(int ns) { return ns <= 2 ? ns : 4; }
+
 const void* reactor_rates_kernel_ptr(int ns) {
     return ns == 1 ? reinterpret_cast<const void*>(&reactor_rates_kernel<1>)
                    : reinterpret_cast<const void*>(&reactor_rates_kernel<2>);
 }
 
 size_t rates_lds_bytes(const DevMech& m, int ns) {
-    const size_t d = static_cast<size_t>(kWave) * ns + 2 * (m.K + 1) + m.K;
+    const size_t d = static_cast<size_t>(kWave) * rates_ns(ns) + 2 * (m.K + 1) + m.K;
     return d * sizeof(double);
 }
 
@@ -127,6 +130,10 @@ hipError_t rates_launch(const DevMech& m, int ns, bool f32, long n, const double
         hipLaunchKernelGGL((rates_kernel<2, false>), dim3(grid), dim3(64), lds, stream, m, n, T, P, Y, wdot);
     else if (ns == 2)
         hipLaunchKernelGGL((rates_kernel<2, true>), dim3(grid), dim3(64), lds, stream, m, n, T, P, Y, wdot);
+    else if (rates_ns(ns) == 4 && !f32)
+        hipLaunchKernelGGL((rates_kernel<4, false>), dim3(grid), dim3(64), lds, stream, m, n, T, P, Y, wdot);
+    else if (rates_ns(ns) == 4)
+        hipLaunchKernelGGL((rates_kernel<4, true>), dim3(grid), dim3(64), lds, stream, m, n, T, P, Y, wdot);
     else return hipErrorInvalidValue;
     return hipGetLastError();
 }
@@ -134,8 +141,10 @@ hipError_t rates_launch(const DevMech& m, int ns, bool f32, long n, const double
 const void* rates_kernel_ptr(int ns, bool f32) {
     if (ns == 1) return f32 ? reinterpret_cast<const void*>(&rates_kernel<1, true>)
                             : reinterpret_cast<const void*>(&rates_kernel<1, false>);
-    return f32 ? reinterpret_cast<const void*>(&rates_kernel<2, true>)
-               : reinterpret_cast<const void*>(&rates_kernel<2, false>);
+    if (ns == 2) return f32 ? reinterpret_cast<const void*>(&rates_kernel<2, true>)
+                            : reinterpret_cast<const void*>(&rates_kernel<2, false>);
+    return f32 ? reinterpret_cast<const void*>(&rates_kernel<4, true>)
+               : reinterpret_cast<const void*>(&rates_kernel<4, false>);
 }
 
 }  // namespace cg

@@ -55,6 +55,9 @@ int32_t cg_device_synchronize(void);
  * Cantera YAML file) and upload its tables.  Replaces newSolution(file, phase)
  * for the ideal-gas/bulk-kinetics path (src/base/Solution.cpp:242-282;
  * KineticsFactory "gas"/"bulk", src/kinetics/KineticsFactory.cpp:24-37).
+ * At most 254 species: rates and mole-based reactor batches take all of
+ * them; the dense reactor batches, Jacobians, equilibrium and the surrogate
+ * take at most 126 and refuse larger mechanisms with an error.
  * Returns a mechanism handle. */
 int32_t cg_mech_load(const char* path);
 int32_t cg_mech_free(int32_t mech);
@@ -154,7 +157,9 @@ int32_t cg_reactors_free(int32_t batch);
  * times the initial total moles (MoleReactor::updateDefaultTolerances,
  * src/zeroD/MoleReactor.cpp:64-76); 8 the constant-volume IdealGasMoleReactor
  * (src/zeroD/IdealGasMoleReactor.cpp, state [T, V, n_1..n_K]).  FP64; at
- * most 127 species (126 at constant volume).  Advance, status, stats and
+ * most 254 species (the dense batches take 126); the preconditioner's fill
+ * pattern must fit in one CU's LDS (160 KiB with the reactor's scratch; the
+ * error names the size, and flags 1 | 2 make it sparser).  Advance, status, stats and
  * cg_reactors_get_state (which returns [T, n_1..n_K], or [T, V, n_1..n_K])
  * work as for the other batches. */
 int32_t cg_reactors_create_mole(int32_t mech, int64_t n, const double* T0, const double* P, const double* Y0,
