@@ -6,11 +6,12 @@
 //   Every register index is static (all loops over columns are unrolled), so
 //   nothing spills to scratch.  The arithmetic is the same as a row-swapping
 //   LU (same multipliers, same updates), only the storage differs.
-// LdsRegLinSys: the same register LU and solves, but the matrix is kept in LDS
+// LdsRegLinSys: the register solves of RegLinSys on factors kept in LDS
 //   between calls (row i in lane i, column-major so a lane's row loads are
-//   conflict-free) and the factorisation / solves are out-of-line functions
-//   that load it into registers: the registers are held only inside them, not
-//   across the whole integrator.  Used for 32 < N <= 64.
+//   conflict-free); the factorisation runs on the matrix cores
+//   (lr_factor_mfma: the MFMA-blocked LU of LdsLinSys on a row-major copy,
+//   then the rows stored in the register layout), the solves are out-of-line
+//   functions that load the factors into registers.  Used for 32 < N <= 64.
 // LdsLinSys: the matrix lives in LDS (row-major, leading dimension LD), used
 //   for N > 64 (two lanes' worth of rows per lane); factored by panels of 4
 //   columns with the trailing rank-4 updates on the matrix cores
@@ -210,25 +211,36 @@ struct RegLinSys {
     }
 };
 
-// Out-of-line register LU of the LDS matrix at byte offset offA from the start
-// of dynamic LDS: A[j * 64 + i] = M(i, j); step[64] and pivlane[64] follow A.
+// Out-of-line LU of the iteration matrix that LdsRegLinSys::form leaves
+// row-major in the same LDS area (leading dimension LD): factored by panels
+// with the trailing updates on the matrix cores (lu_factor_mfma, same pivots
+// and, entry by entry, the same fused multiply-adds as the register LU), then
+// stored in the layout the register LU leaves for lr_solve -- A[j * 64 + i] =
+// (PA = LU)(i, j) with row i the i-th pivot row, 1/U_ii after the NMAX
+// columns, step[i] = i, pivlane[i] = the original row of pivot row i.
 template <int NMAX, typename MT>
-__device__ __noinline__ int lr_factor(int offA, int N) {
+__device__ __noinline__ int lr_factor_mfma(int offA, int N, int LD) {
+    N = uni(N);
+    LD = uni(LD);
     MT* A = reinterpret_cast<MT*>(reinterpret_cast<char*>(lds_base()) + offA);
     int* stp = reinterpret_cast<int*>(A + (NMAX + 1) * kWave);
     int* pvl = stp + kWave;
-    RegLinSys<NMAX, MT> r;
+    if (!lu_factor_mfma<1, MT>(A, LD, N, pvl)) return 0;
     const int lane = lane_id();
+    const int row = min(lane, N - 1) * LD;
+    MT a[NMAX];
 #pragma unroll
-    for (int j = 0; j < NMAX; j++) r.a[j] = A[j * kWave + lane];
-    const bool ok = r.factor(N);
-#pragma unroll
-    for (int j = 0; j < NMAX; j++) A[j * kWave + lane] = r.a[j];
-    A[NMAX * kWave + lane] = r.rdiag;
-    stp[lane] = r.step;
-    pvl[lane] = r.pivlane;
+    for (int j = 0; j < NMAX; j++) a[j] = load_or_zero(A, row + min(j, N - 1), lane < N && j < N);
+    const MT dg = load_or_zero(A, row + min(lane, N - 1), lane < N);
+    const int pv = pvl[lane];
     wave_sync();
-    return ok ? 1 : 0;
+#pragma unroll
+    for (int j = 0; j < NMAX; j++) A[j * kWave + lane] = a[j];
+    A[NMAX * kWave + lane] = (lane < N) ? MT(1) / dg : MT(1);
+    stp[lane] = lane;
+    pvl[lane] = (lane < N) ? pv : lane;
+    wave_sync();
+    return 1;
 }
 
 template <int NMAX, typename MT, typename V>
@@ -252,28 +264,38 @@ struct LdsRegLinSys {
     MT* A;        // LDS, (NMAX + 1) x 64: column j at A + 64 j, then 1/U_kk
     int* piv;     // LDS, step[64] then pivlane[64]
     int offA;     // byte offset of A from the start of dynamic LDS
+    int LD;       // leading dimension of the row-major matrix that form writes
 
-    __device__ __forceinline__ void init(void* lds, int) {
+    __device__ __forceinline__ void init(void* lds, int ld) {
+        LD = ld;
         A = static_cast<MT*>(lds);
         piv = reinterpret_cast<int*>(A + (NMAX + 1) * kWave);
         offA = static_cast<int>(static_cast<char*>(lds) - reinterpret_cast<char*>(lds_base()));
     }
     __device__ __forceinline__ void bind(MT*, int*) {}
 
+    // M = I - gamma J written row-major (leading dimension LD) for the
+    // factorisation: lane j takes column j, so each row of J is one coalesced
+    // load; rows in blocks of 8 keep 8 loads in flight
     __device__ __forceinline__ void form(const double* __restrict__ J, int N, double gamma) {
         const int lane = lane_id();
+        if (lane < N) {
+            int i = 0;
+            for (; i + 8 <= N; i += 8) {
+                double x[8];
 #pragma unroll
-        for (int j = 0; j < NMAX; j++) {
-            // 1 - gamma J_jj as one fused multiply-add, as the compiler formed it
-            // when the product and the sum sat in one branch
-            const double v =
-                (lane < N && j < N) ? fma(-gamma, row_entry(J, N, lane, j), lane == j ? 1.0 : 0.0) : 0.0;
-            A[j * kWave + lane] = static_cast<MT>(v);
+                for (int b = 0; b < 8; b++) x[b] = J[(i + b) * N + lane];
+#pragma unroll
+                for (int b = 0; b < 8; b++)
+                    A[(i + b) * LD + lane] = static_cast<MT>(fma(-gamma, x[b], (i + b == lane) ? 1.0 : 0.0));
+            }
+            for (; i < N; i++)
+                A[i * LD + lane] = static_cast<MT>(fma(-gamma, J[i * N + lane], (i == lane) ? 1.0 : 0.0));
         }
         wave_sync();
     }
 
-    __device__ __forceinline__ bool factor(int N) { return lr_factor<NMAX, MT>(offA, N) != 0; }
+    __device__ __forceinline__ bool factor(int N) { return lr_factor_mfma<NMAX, MT>(offA, N, LD) != 0; }
 
     template <typename V, int S>
     __device__ __forceinline__ void solve(V (&b)[S], int N) const {
