@@ -310,12 +310,37 @@ struct LdsRegLinSys {
         pdst[kWave + lane] = piv[kWave + lane];
     }
 
+    // The NMAX + 1 saved columns in blocks of 16: a block's loads are issued
+    // together, and the pointers step by a block so that every load has an
+    // immediate offset.  (Fully unrolled, the compiler kept one address
+    // register per column for the whole kernel and waited for each pair of
+    // loads in turn: 29 round trips to memory per reactor.)
     __device__ __forceinline__ void restore(const MT* __restrict__ src, const int* __restrict__ psrc, int) {
+        constexpr int kCols = NMAX + 1, kB = 16;
         const int lane = lane_id();
+        const MT* s = src + lane;
+        MT* d = A + lane;
+#pragma unroll 1
+        for (int j0 = 0; j0 + kB <= kCols; j0 += kB) {
+            MT x[kB];
 #pragma unroll
-        for (int j = 0; j <= NMAX; j++) A[j * kWave + lane] = src[j * kWave + lane];
-        piv[lane] = psrc[lane];
-        piv[kWave + lane] = psrc[kWave + lane];
+            for (int b = 0; b < kB; b++) x[b] = s[b * kWave];
+#pragma unroll
+            for (int b = 0; b < kB; b++) d[b * kWave] = x[b];
+            s += kB * kWave;
+            d += kB * kWave;
+        }
+        {
+            constexpr int kR = kCols % kB;
+            MT x[kR > 0 ? kR : 1];
+#pragma unroll
+            for (int b = 0; b < kR; b++) x[b] = s[b * kWave];
+            const int p0 = psrc[lane], p1 = psrc[kWave + lane];
+#pragma unroll
+            for (int b = 0; b < kR; b++) d[b * kWave] = x[b];
+            piv[lane] = p0;
+            piv[kWave + lane] = p1;
+        }
         wave_sync();
     }
 };
@@ -460,18 +485,38 @@ struct LdsLinSys {
         }
     }
 
+    // Rows in blocks of 8 with the column index clamped instead of tested,
+    // so that a block's 8 x NS loads are issued together (a load under the
+    // j < N test was waited for before the next one was issued).
     __device__ __forceinline__ void restore(const MT* __restrict__ src, const int* __restrict__ psrc, int N) {
         const int lane = lane_id();
-        for (int i = 0; i < N; i++)
+        int jc[NS];
+#pragma unroll
+        for (int s = 0; s < NS; s++) jc[s] = min(lane + kWave * s, N - 1);
+        int i = 0;
+        for (; i + 8 <= N; i += 8) {
+            MT x[8][NS];
+#pragma unroll
+            for (int b = 0; b < 8; b++)
+#pragma unroll
+                for (int s = 0; s < NS; s++) x[b][s] = src[(i + b) * N + jc[s]];
+#pragma unroll
+            for (int b = 0; b < 8; b++)
+#pragma unroll
+                for (int s = 0; s < NS; s++)
+                    if (lane + kWave * s < N) A[(i + b) * LD + lane + kWave * s] = x[b][s];
+        }
+        for (; i < N; i++)
 #pragma unroll
             for (int s = 0; s < NS; s++) {
-                const int j = lane + kWave * s;
-                if (j < N) A[i * LD + j] = src[i * N + j];
+                const MT v = src[i * N + jc[s]];
+                if (lane + kWave * s < N) A[i * LD + lane + kWave * s] = v;
             }
 #pragma unroll
         for (int s = 0; s < NS; s++) {
             const int j = lane + kWave * s;
-            if (j < N) pmap[j] = psrc[j];
+            const int v = psrc[jc[s]];
+            if (j < N) pmap[j] = v;
         }
         wave_sync();
     }
