@@ -8,8 +8,9 @@
 // dT/dt scale with V/m = 1/rho), so the state is taken with m = 1 kg and the m
 // row and column are zero.
 //
-// One wavefront per state: RHS at the state (out-of-line kinetics), then the
-// Jacobian assembly (in LDS when it fits, else in the state's output block).
+// One wavefront per state: the Jacobian assembly (two RHS evaluations, at the
+// state and with T + dT, then the per-reaction derivatives), in LDS when it
+// fits, else in the state's output block.
 #include <hip/hip_runtime.h>
 
 #include "bdf.hpp"
@@ -22,8 +23,6 @@ namespace {
 // kLdsJ: assemble J in LDS (FP64 LDS atomics for the per-reaction terms), then
 // write it out with coalesced stores; otherwise assemble it in place in HBM
 // (L2 atomics) -- used when N x N doubles do not fit next to the scratch.
-// The mechanism is the first parameter: rhs_call reads it through
-// kernarg_mech().
 // OT: element type of the output (double, or float for CG_FP32: the FP64
 // assembly rounded once at the store, which halves the HBM bytes written).
 template <int NS, bool kLdsJ, typename OT>
@@ -60,21 +59,16 @@ __global__ __launch_bounds__(64) void jac_batch_kernel(DevMech m, long n, const 
             if (i < m.N) S.yb[i] = v;
         }
         wave_sync();
-        rhs_call<NS>(kernarg_mech(), L, Ps);
 #pragma unroll
-        for (int s = 0; s < NS; s++) {
-            const int i = lane + kWave * s;
-            f[s] = (i < m.N) ? S.F[i] : 0.0;
-        }
-        wave_sync();
+        for (int s = 0; s < NS; s++) f[s] = 0.0;   // not read: the Jacobian takes its own base RHS
         OT* Jg = J + static_cast<size_t>(st) * NN;
         if constexpr (kLdsJ) {
-            reactor_jacobian<NS, true>(m, S, Ps, y, f, Jl);
+            reactor_jacobian<NS, true, true>(m, S, Ps, y, f, Jl);
             for (int e = lane; e < NN; e += kWave) Jg[e] = static_cast<OT>(Jl[e]);
             wave_sync();
         } else {
             static_assert(sizeof(OT) == sizeof(double), "in-place assembly needs a double output");
-            reactor_jacobian<NS, true>(m, S, Ps, y, f, Jg);
+            reactor_jacobian<NS, true, true>(m, S, Ps, y, f, Jg);
         }
     }
 }

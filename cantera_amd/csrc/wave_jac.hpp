@@ -171,9 +171,12 @@ __device__ __forceinline__ void scatter_reaction_derivs(const DevMech& m, int p,
 // kRefOrder = false: J in the internal order [Y_0..Y_{K-1}, T, m];
 // kRefOrder = true:  J in the reactor's own order [m, T, Y_0..Y_{K-1}]
 // (IdealGasConstPressureReactor::getState, src/zeroD/IdealGasConstPressureReactor.cpp:16-26).
+// kBaseF0 = true: f0 is not used; the RHS at y that this function evaluates
+// anyway (to leave the species scratch at the expansion point) is the base of
+// the temperature difference, which saves the caller an RHS evaluation.
 // The per-reaction derivatives are accumulated with FP64 global atomics, so
 // their summation order is not fixed.
-template <int NS, bool kRefOrder = false>
+template <int NS, bool kRefOrder = false, bool kBaseF0 = false>
 __device__ void reactor_jacobian(const DevMech& m, const WaveScratch& S, double P, const double (&y)[NS],
                                  const double (&f0)[NS], double* __restrict__ J) {
     const int lane = lane_id();
@@ -200,10 +203,11 @@ __device__ void reactor_jacobian(const DevMech& m, const WaveScratch& S, double 
     for (int s = 0; s < NS; s++)
         if (lane + kWave * s == K) y1[s] = Tval + dT;
     reactor_rhs<NS>(m, S, P, y1, f1, g, sr, rs);
-#pragma unroll
-    for (int s = 0; s < NS; s++) dfdT[s] = (f1[s] - f0[s]) / dT;
     // base state (species scratch, wdot)
-    reactor_rhs<NS>(m, S, P, y, f1, g, sr, rs);
+    double fb[NS];
+    reactor_rhs<NS>(m, S, P, y, fb, g, sr, rs);
+#pragma unroll
+    for (int s = 0; s < NS; s++) dfdT[s] = (f1[s] - (kBaseF0 ? fb[s] : f0[s])) / dT;
 #pragma unroll
     for (int s = 0; s < NS; s++) {
         int j = lane + kWave * s;

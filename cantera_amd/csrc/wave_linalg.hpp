@@ -377,31 +377,45 @@ __device__ void lu_solve(const T* A, int LD, int N, const int* pmap, V* bv, V (&
     // Forward substitution with the unit lower factor, then back substitution.
     // Step k broadcasts b_k (lane k % 64, slot k / 64 -- the slot is a
     // compile-time index in each loop, so no select sits on the chain) and
-    // updates the other rows with the column-k multipliers, loaded kB columns
-    // ahead and masked to zero outside the triangle: a step is readlane -> FMA.
-    constexpr int kB = 4;
+    // updates the other rows with the column-k multipliers, masked to zero
+    // outside the triangle: a step is readlane -> FMA.  The multipliers come in
+    // blocks of kB columns, each block's loads (in-bounds by clamping, masked
+    // only where they are used) issued while the previous block's steps run,
+    // so the LDS latency is off the chain.
+    constexpr int kB = 8;
+    auto column_block = [&](int k0, T (&v)[kB][NS]) {
+#pragma unroll
+        for (int b2 = 0; b2 < kB; b2++)
+#pragma unroll
+            for (int s = 0; s < NS; s++)
+                v[b2][s] = A[min(lane + kWave * s, N - 1) * LD + min(max(k0 + b2, 0), N - 1)];
+    };
 #pragma unroll
     for (int t = 0; t < NS; t++) {
         const int kend = min(N - 1, kWave * (t + 1));
+        T lv[kB][NS];
+        column_block(kWave * t, lv);
         for (int k0 = kWave * t; k0 < kend; k0 += kB) {
-            T lv[kB][NS];
-#pragma unroll
-            for (int b2 = 0; b2 < kB; b2++)
-#pragma unroll
-                for (int s = 0; s < NS; s++) {
-                    const int i = lane + kWave * s;
-                    const int k = k0 + b2;
-                    lv[b2][s] = load_or_zero(A, min(i, N - 1) * LD + min(k, N - 1), i > k && i < N && k < kend);
-                }
+            T nx[kB][NS];
+            column_block(k0 + kB, nx);
 #pragma unroll
             for (int b2 = 0; b2 < kB; b2++) {
                 const int k = k0 + b2;
                 if (k < kend) {
                     const V bk = readlane_t(b[t], k - kWave * t);
+                    // slots s < t hold rows i < 64 t <= k: nothing to update
 #pragma unroll
-                    for (int s = 0; s < NS; s++) b[s] -= static_cast<V>(lv[b2][s]) * bk;
+                    for (int s = t; s < NS; s++) {
+                        const int i = lane + kWave * s;
+                        const V l = (i > k && i < N) ? static_cast<V>(lv[b2][s]) : V(0);
+                        b[s] -= l * bk;
+                    }
                 }
             }
+#pragma unroll
+            for (int b2 = 0; b2 < kB; b2++)
+#pragma unroll
+                for (int s = 0; s < NS; s++) lv[b2][s] = nx[b2][s];
         }
     }
     // Back substitution: 1/U_ii for every row first (in parallel); lane k holds
@@ -415,28 +429,40 @@ __device__ void lu_solve(const T* A, int LD, int N, const int* pmap, V* bv, V (&
         const T d = load_or_zero(A, ic * LD + ic, true);
         rd[s] = (i < N) ? V(1) / static_cast<V>(d) : V(1);
     }
+    // columns k0, k0 - 1, ..., k0 - kB + 1 (clamped at 0)
+    auto column_block_down = [&](int k0, T (&v)[kB][NS]) {
+#pragma unroll
+        for (int b2 = 0; b2 < kB; b2++)
+#pragma unroll
+            for (int s = 0; s < NS; s++)
+                v[b2][s] = A[min(lane + kWave * s, N - 1) * LD + min(max(k0 - b2, 0), N - 1)];
+    };
 #pragma unroll
     for (int t = NS - 1; t >= 0; t--) {
         const int klo = kWave * t;
+        T uv[kB][NS];
+        column_block_down(min(N, kWave * (t + 1)) - 1, uv);
         for (int k0 = min(N, kWave * (t + 1)) - 1; k0 >= klo; k0 -= kB) {
-            T uv[kB][NS];
-#pragma unroll
-            for (int b2 = 0; b2 < kB; b2++)
-#pragma unroll
-                for (int s = 0; s < NS; s++) {
-                    const int i = lane + kWave * s;
-                    const int k = k0 - b2;
-                    uv[b2][s] = load_or_zero(A, min(i, N - 1) * LD + max(k, 0), k >= klo && i < k);
-                }
+            T nx[kB][NS];
+            column_block_down(k0 - kB, nx);
 #pragma unroll
             for (int b2 = 0; b2 < kB; b2++) {
                 const int k = k0 - b2;
                 if (k >= klo) {
                     const V xk = readlane_t(b[t], k - klo) * readlane_t(rd[t], k - klo);
+                    // slots s > t hold rows i >= 64 (t + 1) > k: nothing to update
 #pragma unroll
-                    for (int s = 0; s < NS; s++) b[s] -= static_cast<V>(uv[b2][s]) * xk;
+                    for (int s = 0; s <= t; s++) {
+                        const int i = lane + kWave * s;
+                        const V u = (i < k) ? static_cast<V>(uv[b2][s]) : V(0);
+                        b[s] -= u * xk;
+                    }
                 }
             }
+#pragma unroll
+            for (int b2 = 0; b2 < kB; b2++)
+#pragma unroll
+                for (int s = 0; s < NS; s++) uv[b2][s] = nx[b2][s];
         }
     }
 #pragma unroll
