@@ -336,6 +336,11 @@ DeviceMech::DeviceMech(const HostMech& h) : h_(h) {
     d_.fod = upload(fod);
     d_.gend = upload(gend);
     d_.pdd = upload(pdd);
+    // kTbPad entries of padding (species K, whose concentration slot holds 1,
+    // efficiency 0) after the last list: the first kTbPad efficiencies of any
+    // reaction are loaded without a test against its list length
+    tbk.insert(tbk.end(), kTbPad, K);
+    tbe.insert(tbe.end(), kTbPad, 0.0);
     d_.tbk = upload(tbk);
     d_.tbe = upload(tbe);
     d_.comp = upload(h.comp);

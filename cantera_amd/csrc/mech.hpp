@@ -12,6 +12,7 @@ namespace cg {
 
 constexpr int kMaxSide = 4;   // species entries per reaction side on the device
 constexpr int kMaxTB = 32;    // explicit efficiencies per reaction
+constexpr int kTbPad = 8;     // padding entries after the efficiency table
 
 enum RxnType { kElementary = 0, kThreeBody = 1, kLindemann = 2, kTroe = 3, kPlog = 4, kChebyshev = 5, kSri = 6, kTsang = 7 };
 

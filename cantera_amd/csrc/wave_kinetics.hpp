@@ -394,10 +394,27 @@ __device__ __forceinline__ void eval_reaction(const DevMech& m, int p, const Gas
     const bool rev = x.flags & 1;
     double M = 1.0;
     if (x.flags & 32) {
-        // efficiencies in groups of 4: the table loads of a group, then its
-        // concentration loads, are in flight together
+        // The first kTbPad efficiencies are loaded with no test against the
+        // list length (the table ends with kTbPad padding entries), so their
+        // table loads, then their concentration loads, are all in flight
+        // together; entries past the reaction's own list are not added.  The
+        // rest follow in groups of 4.
         double sum = 0.0;
-        for (int j0 = 0; j0 < x.tbn; j0 += 4) {
+        {
+            int kk[kTbPad];
+            double ee[kTbPad], cc[kTbPad];
+#pragma unroll
+            for (int b = 0; b < kTbPad; b++) {
+                kk[b] = m.tbk[x.tbo + b];
+                ee[b] = m.tbe[x.tbo + b];
+            }
+#pragma unroll
+            for (int b = 0; b < kTbPad; b++) cc[b] = opaque(C[kk[b]]);
+#pragma unroll
+            for (int b = 0; b < kTbPad; b++)
+                if (b < x.tbn) sum = add_mul_rn(sum, ee[b], cc[b]);
+        }
+        for (int j0 = kTbPad; j0 < x.tbn; j0 += 4) {
             int kk[4];
             double ee[4];
 #pragma unroll
