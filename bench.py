@@ -54,9 +54,17 @@ BF16_PEAK_TFLOPS = 2500.0    # MI355X dense BF16 MFMA
 HBM_PEAK_GBS = 8000.0
 METRIC = "GRI30 reactor-steps/s: plain FP64 BDF vs mixed+surrogate at ≤1 K T error"
 VALU_PIPE = "FP64 VALU (no MFMA issued; the MI355X FP64 vector peak is 78.6 TFLOP/s)"
-# N > 64 (n-dodecane): the dense LU's rank-4 trailing updates run on the matrix cores
+# N > 32 with the iteration matrix in LDS (gri30, n-dodecane): the dense LU's
+# rank-4 trailing updates run on the matrix cores
 LDS_LU_PIPE = ("FP64 VALU, the dense LU's trailing updates on FP64 MFMA (v_mfma_f64_16x16x4f64); "
                "the MI355X FP64 peak is 78.6 TFLOP/s for both")
+
+
+def reactor_pipe(n_species, storage):
+    """Pipes of the integrator kernel: systems of N = K + 2 > 32 unknowns with
+    the matrix in LDS factor on the matrix cores; N <= 32 (register LU) and the
+    global-matrix policy factor on the VALU alone."""
+    return LDS_LU_PIPE if n_species + 2 > 32 and storage != "global" else VALU_PIPE
 SURROGATE_DT = 1e-5
 
 # Algorithmic cost convention: every FP64 add/sub/mul/div counts 1, an FMA 2,
@@ -351,7 +359,7 @@ def reactors(a):
         },
         "roofline": {
             "bound": "valu",
-            "pipe": VALU_PIPE if mech != "nDodecane" else LDS_LU_PIPE,
+            "pipe": reactor_pipe(sol.n_species, tuned["matrix_storage"] if tuned else a.matrix_storage),
             "achieved": achieved,
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
