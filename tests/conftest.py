@@ -72,6 +72,48 @@ def parse_third_bodies(path):
     return out
 
 
+def extend_efficiencies(src, dst, seed=7, base=9, span=14):
+    """Write a copy of the .mech file src to dst in which every reaction with a
+    third body lists base..base+span-1 explicit efficiencies (reaction j gets
+    base + j % span): the species it already lists, then species not yet
+    listed with (efficiency - default) drawn from [0.1, 2].  Longer lists than
+    any shipped mechanism holds, for the kernels' loops over efficiencies.
+    Returns {reaction index: (default, [(k, efficiency - default)])}."""
+    g = np.random.default_rng(seed)
+    K = 0
+    out, lists = [], {}
+    with open(src) as f:
+        for line in f:
+            t = line.split()
+            if t and t[0] == "species":
+                K = int(t[1])
+            if not t or t[0] != "R":
+                out.append(line)
+                continue
+            p = 15
+            nr = int(t[p]); p += 1 + 3 * nr
+            np_ = int(t[p]); p += 1 + 2 * np_
+            tbdef, ntb = float(t[p]), int(t[p + 1])
+            if tbdef == 0.0 and ntb == 0:
+                out.append(line)
+                continue
+            j = int(t[1])
+            eff = [(int(t[p + 2 + 2 * i]), float(t[p + 3 + 2 * i])) for i in range(ntb)]
+            have = {k for k, _ in eff}
+            for k in g.permutation(K):
+                if len(eff) >= base + j % span:
+                    break
+                if int(k) not in have:
+                    eff.append((int(k), float(g.uniform(0.1, 2.0))))
+            rest = t[p + 2 + 2 * ntb:]
+            toks = t[:p] + [repr(tbdef), str(len(eff))] + [x for k, e in eff for x in (str(k), repr(e))] + rest
+            out.append(" ".join(toks) + "\n")
+            lists[j] = (tbdef, eff)
+    with open(dst, "w") as f:
+        f.writelines(out)
+    return lists
+
+
 def third_body_condition(path, X):
     """Cancellation factor of [M] = d ctot + sum (e_k - d) C_k for mole
     fractions X: max over reactions of (d + sum |e_k - d| X_k) / |d + sum (e_k - d) X_k|.
