@@ -23,6 +23,10 @@ namespace {
 // kLdsJ: assemble J in LDS (FP64 LDS atomics for the per-reaction terms), then
 // write it out with coalesced stores; otherwise assemble it in place in HBM
 // (L2 atomics) -- used when N x N doubles do not fit next to the scratch.
+// The assembly is inlined here, where the scratch and the matrix are known to
+// be LDS: as an out-of-line function taking them through pointer arguments it
+// used flat loads, stores and atomics for every access, and its callee-saved
+// registers went to scratch memory.
 // OT: element type of the output (double, or float for CG_FP32: the FP64
 // assembly rounded once at the store, which halves the HBM bytes written).
 template <int NS, bool kLdsJ, typename OT>

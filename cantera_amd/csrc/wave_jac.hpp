@@ -177,7 +177,7 @@ __device__ __forceinline__ void scatter_reaction_derivs(const DevMech& m, int p,
 // The per-reaction derivatives are accumulated with FP64 global atomics, so
 // their summation order is not fixed.
 template <int NS, bool kRefOrder = false, bool kBaseF0 = false>
-__device__ void reactor_jacobian(const DevMech& m, const WaveScratch& S, double P, const double (&y)[NS],
+__device__ __forceinline__ void reactor_jacobian(const DevMech& m, const WaveScratch& S, double P, const double (&y)[NS],
                                  const double (&f0)[NS], double* __restrict__ J) {
     const int lane = lane_id();
     const int K = m.K, N = m.N;
