@@ -86,7 +86,7 @@ struct Batch {
     double* sg_Yn = nullptr;         // n x N predictions
     double* sg_F = nullptr;          // n x N f(y) of the last kept state (internal order)
     double* sg_P = nullptr;          // n
-    int* sg_ints = nullptr;          // elig (n), fvalid (n), list (n), count (1)
+    int* sg_ints = nullptr;          // elig (n), fvalid (n)
     unsigned long long* sg_counters = nullptr;   // eligible, kept, integrated (accumulated)
     // forward sensitivities to reaction rate multipliers (ReactorNet defaults,
     // include/cantera/zeroD/ReactorNet.h: rtol 1e-4, atol 1e-6)
@@ -1292,15 +1292,13 @@ static void advance_surrogate_device(Batch& b, SurrogateObj& so, const double* t
         check(hipMalloc(&b.sg_Yn, nn * N * 8), "hipMalloc(surrogate predictions)");
         check(hipMalloc(&b.sg_F, nn * N * 8), "hipMalloc(surrogate rhs)");
         check(hipMalloc(&b.sg_P, nn * 8), "hipMalloc(surrogate pressures)");
-        check(hipMalloc(&b.sg_ints, (3 * nn + 1) * 4), "hipMalloc(surrogate lists)");
+        check(hipMalloc(&b.sg_ints, 2 * nn * 4), "hipMalloc(surrogate flags)");
         check(hipMalloc(&b.sg_counters, 3 * sizeof(unsigned long long)), "hipMalloc(surrogate counters)");
-        check(surrogate_clear(b.sg_ints, 3 * n + 1, b.stream), "surrogate clear");
+        check(surrogate_clear(b.sg_ints, 2 * n, b.stream), "surrogate clear");
         check(surrogate_clear(reinterpret_cast<int*>(b.sg_counters), 6, b.stream), "surrogate clear");
     }
     int* elig = b.sg_ints;
     int* fvalid = b.sg_ints + nn;
-    int* list = b.sg_ints + 2 * nn;
-    int* count = b.sg_ints + 3 * nn;
     const double dt = so.net->dt();
     const DevMech& m = b.mech->dm->view();
     const HostMech& hm = b.mech->dm->host();
@@ -1314,22 +1312,25 @@ static void advance_surrogate_device(Batch& b, SurrogateObj& so, const double* t
     if (rad[0] < 0 || rad[1] < 0 || rad[2] < 0) rad[0] = -1;   // no gate species: every step integrated
     if (t_out_dev != b.tout)
         check(hipMemcpyAsync(b.tout, t_out_dev, nn * 8, hipMemcpyDeviceToDevice, b.stream), "hipMemcpyAsync(tout)");
-    check(surrogate_clear(count, 1, b.stream), "surrogate clear");
-    check(surrogate_gather(n, b.rec, b.rec_stride, NP, N, K, b.tout, dt, b.sg_Y, b.sg_P, elig, list, count,
-                           b.sg_counters, b.stream), "surrogate gather");
+    check(surrogate_gather(n, b.rec, b.rec_stride, NP, N, K, b.tout, dt, b.sg_Y, b.sg_P, elig, b.sg_counters,
+                           b.stream), "surrogate gather");
     so.net->predict(n, b.sg_Y, b.sg_P, b.sg_Yn, b.stream);
     const size_t lds = guard_lds_bytes(m, b.ns);
     const void* gk = guard_kernel_ptr(b.ns);
     set_lds_limit(gk, lds);
     const int grid = grid_for(gk, lds, 0, n);
     check(surrogate_guard(m, b.ns, n, elig, b.sg_Y, b.sg_Yn, b.sg_F, fvalid, b.sg_P, b.rec, b.rec_stride, NP, b.tout,
-                          dt, tol_T, tol_Y, gate_T, rad, list, count, b.sg_counters, grid, lds, b.stream),
+                          dt, tol_T, tol_Y, gate_T, rad, b.sg_counters, grid, lds, b.stream),
           "surrogate guard");
-    BdfArgs a = advance_args(b, list, n);
-    a.n_dev = count;
+    // the rejected and the non-eligible reactors, in the batch's longest-first
+    // order (a list built by atomics would hand them out in arrival order)
+    BdfArgs a = advance_args(b, nullptr, 0);
+    a.work_flag = elig;
+    a.work_value = 2;
     check(hipEventRecord(b.e0, b.stream), "hipEventRecord");
     check(bdf_launch(a, b.fp32, b.global_matrix, advance_grid(b), b.lds, b.stream), "integrator kernel launch");
     check(hipEventRecord(b.e1, b.stream), "hipEventRecord");
+    update_order(b);
 }
 
 int32_t cg_reactors_advance_surrogate_device(int32_t bh, int32_t sh, const double* t_out_dev, double tol_T,

@@ -51,6 +51,8 @@ struct BdfArgs {
     const double* tout;   // n, device
     const int* order;     // optional processing order (n), device
     const int* n_dev;     // optional: number of work items, read on the device (replaces n)
+    const int* work_flag; // optional: only reactors r with work_flag[r] == work_value are advanced
+    int work_value;
     int* queue;           // device work-queue head
     // forward sensitivities to reaction rate multipliers (sensitivity kernels only)
     int nsens;                      // parameters (<= kMaxSens)
@@ -107,13 +109,15 @@ const void* guard_kernel_ptr(int ns);
 hipError_t surrogate_clear(int* p, int n, hipStream_t s);
 // output states of all reactors, reference order [m, T, Y], n x N doubles
 hipError_t export_states(int n, const double* rec, size_t stride, int NP, int N, int K, double* Y, hipStream_t s);
+// elig[r]: 1 one surrogate step ahead, 2 integrate, 0 nothing to do; the
+// guard turns the rejected 1s into 2s, and the integrator then advances the
+// reactors with elig = 2 (BdfArgs::work_flag)
 hipError_t surrogate_gather(int n, const double* rec, size_t stride, int NP, int N, int K, const double* tout,
-                            double dt, double* Y, double* P, int* elig, int* list, int* count,
-                            unsigned long long* counters, hipStream_t s);
-hipError_t surrogate_guard(const DevMech& m, int ns, int n, const int* elig, const double* Ycur, const double* Ynext,
+                            double dt, double* Y, double* P, int* elig, unsigned long long* counters, hipStream_t s);
+hipError_t surrogate_guard(const DevMech& m, int ns, int n, int* elig, const double* Ycur, const double* Ynext,
                            double* Fcur, int* fvalid, const double* P, double* rec, size_t stride, int NP,
                            const double* tout, double dt, double tol_T, double tol_Y, double gate_T, const int* rad,
-                           int* list, int* count, unsigned long long* counters, int grid, size_t lds, hipStream_t s);
+                           unsigned long long* counters, int grid, size_t lds, hipStream_t s);
 
 // batched reactor Jacobians (inputs as the rates, output N x N per state)
 size_t jac_lds_bytes(const DevMech& m, int ns);

@@ -1430,6 +1430,7 @@ __device__ __forceinline__ void bdf_advance_body(const BdfArgs& a) {
         r = uni(__shfl(r, 0, kWave));
         if (r >= nwork) break;
         const int rid = a.order ? uni(a.order[r]) : r;
+        if (a.work_flag && uni(a.work_flag[rid]) != a.work_value) continue;
         double* rec = a.rec + static_cast<size_t>(rid) * a.rec_stride;
         const double status = uni(rec[7 * kWave * NS + C_STATUS]);
         if (status < 0.0) continue;   // failed earlier: leave untouched

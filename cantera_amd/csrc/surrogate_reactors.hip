@@ -8,9 +8,9 @@
 //   2. the network predicts y1 for every row (GEMMs on the matrix cores);
 //   3. guard_kernel checks each eligible prediction in FP64 and either keeps
 //      it (the reactor restarts from y1 at t + dt, as CVodeReInit would after
-//      a jump in state) or appends the reactor to the integrator's list;
-//   4. the BDF kernel advances the listed reactors, reading the list length
-//      from device memory.
+//      a jump in state) or marks the reactor for the integrator;
+//   4. the BDF kernel advances the marked reactors (BdfArgs::work_flag) in
+//      the batch's longest-first order, skipping the others.
 // No host synchronisation and no allocation happen per call.
 //
 // The guard (all FP64, the integrator's own right-hand side f):
@@ -43,11 +43,11 @@ __device__ __forceinline__ int ref_index(int c, int K) { return c < K ? c + 2 : 
 
 // Per reactor (one 64-thread block): Y row (reference order) of its output
 // state, P, and elig = 1 (one surrogate step ahead), 2 (integrate), 0 (nothing
-// to do).  Reactors with elig = 2 are appended to the integrator's list here.
+// to do).
 __global__ void gather_kernel(int n, const double* __restrict__ rec, size_t stride, int NP, int N, int K,
                               const double* __restrict__ tout, double dt, double* __restrict__ Y,
-                              double* __restrict__ P, int* __restrict__ elig, int* __restrict__ list,
-                              int* __restrict__ count, unsigned long long* __restrict__ counters) {
+                              double* __restrict__ P, int* __restrict__ elig,
+                              unsigned long long* __restrict__ counters) {
     const int r = blockIdx.x;
     if (r >= n) return;
     const double* rr = rec + static_cast<size_t>(r) * stride;
@@ -59,10 +59,7 @@ __global__ void gather_kernel(int n, const double* __restrict__ rec, size_t stri
         int e = 0;
         if (x[C_STATUS] >= 0.0 && span > 0.0) e = (fabs(span - dt) <= 1e-9 * dt) ? 1 : 2;
         elig[r] = e;
-        if (e == 2) {
-            list[atomicAdd(count, 1)] = r;
-            atomicAdd(&counters[2], 1ull);
-        }
+        if (e == 2) atomicAdd(&counters[2], 1ull);
         if (e == 1) atomicAdd(&counters[0], 1ull);
     }
 }
@@ -103,7 +100,7 @@ __device__ __forceinline__ void restart_record(double* rr, int NP, int N, const 
 struct GuardArgs {
     DevMech m;                      // first: rhs_call reads it through kernarg_mech()
     int n;
-    const int* elig;
+    int* elig;
     const double* Ycur;             // n x N, reference order
     const double* Ynext;            // n x N, reference order (prediction)
     double* Fcur;                   // n x N, internal order: f(y0) when fvalid
@@ -115,8 +112,6 @@ struct GuardArgs {
     const double* tout;
     double dt, tol_T, tol_Y, gate_T;
     int rad[3];                     // H, O, OH (-1: absent)
-    int* list;
-    int* count;
     unsigned long long* counters;   // [eligible, kept, integrated]
 };
 
@@ -253,7 +248,7 @@ __global__ __launch_bounds__(64) void guard_kernel(GuardArgs a) {
         if (bad) {
             if (lane == 0) {
                 a.fvalid[i] = 0;
-                a.list[atomicAdd(a.count, 1)] = i;
+                a.elig[i] = 2;
                 atomicAdd(&a.counters[2], 1ull);
             }
         } else {
@@ -307,17 +302,16 @@ hipError_t surrogate_clear(int* p, int n, hipStream_t s) {
 }
 
 hipError_t surrogate_gather(int n, const double* rec, size_t stride, int NP, int N, int K, const double* tout,
-                            double dt, double* Y, double* P, int* elig, int* list, int* count,
-                            unsigned long long* counters, hipStream_t s) {
-    hipLaunchKernelGGL(gather_kernel, dim3(n), dim3(64), 0, s, n, rec, stride, NP, N, K, tout, dt, Y, P, elig, list,
-                       count, counters);
+                            double dt, double* Y, double* P, int* elig, unsigned long long* counters, hipStream_t s) {
+    hipLaunchKernelGGL(gather_kernel, dim3(n), dim3(64), 0, s, n, rec, stride, NP, N, K, tout, dt, Y, P, elig,
+                       counters);
     return hipGetLastError();
 }
 
-hipError_t surrogate_guard(const DevMech& m, int ns, int n, const int* elig, const double* Ycur, const double* Ynext,
+hipError_t surrogate_guard(const DevMech& m, int ns, int n, int* elig, const double* Ycur, const double* Ynext,
                            double* Fcur, int* fvalid, const double* P, double* rec, size_t stride, int NP,
                            const double* tout, double dt, double tol_T, double tol_Y, double gate_T, const int* rad,
-                           int* list, int* count, unsigned long long* counters, int grid, size_t lds, hipStream_t s) {
+                           unsigned long long* counters, int grid, size_t lds, hipStream_t s) {
     GuardArgs a{};
     a.m = m;
     a.n = n;
@@ -336,8 +330,6 @@ hipError_t surrogate_guard(const DevMech& m, int ns, int n, const int* elig, con
     a.tol_Y = tol_Y;
     a.gate_T = gate_T;
     for (int j = 0; j < 3; j++) a.rad[j] = rad[j];
-    a.list = list;
-    a.count = count;
     a.counters = counters;
     if (ns == 1) hipLaunchKernelGGL((guard_kernel<1>), dim3(grid), dim3(64), lds, s, a);
     else hipLaunchKernelGGL((guard_kernel<2>), dim3(grid), dim3(64), lds, s, a);
