@@ -176,6 +176,13 @@ __global__ __launch_bounds__(64) void guard_kernel(GuardArgs a) {
             t = uni(wave_sum(t));
             if (!(fabs(d) <= 1e-6 * t + 1e-300)) bad = 1;
         }
+        // regime gate, first half: hot enough (no RHS needed to tell)
+        double tl = 0.0;
+#pragma unroll
+        for (int s = 0; s < NS; s++)
+            if (lane + kWave * s == K) tl = y0[s];
+        const double T0 = uni(wave_sum(tl));
+        if (!(T0 > a.gate_T) || a.rad[0] < 0) bad = 1;
         bad = __any(bad);
         double* fr = a.Fcur + static_cast<size_t>(i) * N;
         if (!bad) {
@@ -201,7 +208,7 @@ __global__ __launch_bounds__(64) void guard_kernel(GuardArgs a) {
                 }
                 wave_sync();
             }
-            // regime gate: hot, and the radical pool is being destroyed
+            // regime gate, second half: the radical pool is being destroyed
             double rsum = 0.0;
 #pragma unroll
             for (int s = 0; s < NS; s++) {
@@ -209,12 +216,7 @@ __global__ __launch_bounds__(64) void guard_kernel(GuardArgs a) {
                 if (c < K && (c == a.rad[0] || c == a.rad[1] || c == a.rad[2])) rsum += f0[s] * m.rmw[c];
             }
             rsum = uni(wave_sum(rsum));
-            double tl = 0.0;
-#pragma unroll
-            for (int s = 0; s < NS; s++)
-                if (lane + kWave * s == K) tl = y0[s];
-            const double T0 = uni(wave_sum(tl));
-            if (!(T0 > a.gate_T) || !(rsum < 0.0) || a.rad[0] < 0) bad = 1;
+            if (!(rsum < 0.0)) bad = 1;
         }
         bad = __any(bad);
         if (!bad) {
