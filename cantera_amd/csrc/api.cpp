@@ -782,12 +782,12 @@ int32_t cg_reactors_set_schedule(int32_t h, int32_t longest_first) {
     });
 }
 
-static BdfArgs advance_args(Batch& b, const int* list, int count) {
+static BdfArgs advance_args(Batch& b) {
     BdfArgs a{};
     b.launched = true;
     a.mech = b.mech->dm->view();
     a.mech.const_volume = b.const_volume ? 1 : 0;
-    a.n = list ? count : b.n;
+    a.n = b.n;
     // the reference's IdealGasReactor state also holds V (constant): it counts
     // in the WRMS norm's length and nowhere else
     a.neq_norm = b.N + (b.const_volume ? 1 : 0);
@@ -805,7 +805,7 @@ static BdfArgs advance_args(Batch& b, const int* list, int count) {
     a.piv_stride = b.piv_stride;
     a.tout = b.tout;
     int* order = b.lpt + 4 * static_cast<size_t>(b.n);
-    a.order = list ? list : ((b.lpt_on && b.have_order) ? order : nullptr);
+    a.order = (b.lpt_on && b.have_order) ? order : nullptr;
     a.queue = b.queue;
     a.nsens = b.nsens;
     for (int p = 0; p < b.nsens; p++) a.sens_rxn[p] = b.sens_rxn[p];
@@ -840,8 +840,8 @@ static void update_order(Batch& b) {
     b.have_order = true;
 }
 
-static void launch_advance(Batch& b, const int* list = nullptr, int count = 0) {
-    const BdfArgs a = advance_args(b, list, count);
+static void launch_advance(Batch& b) {
+    const BdfArgs a = advance_args(b);
     const int grid = advance_grid(b);
     check(hipEventRecord(b.e0, b.stream), "hipEventRecord");
     check(batch_launch(b, a, grid), "integrator kernel launch");
@@ -875,7 +875,7 @@ int32_t cg_reactors_advance_graph(int32_t h, int32_t n_steps, const double* t_ou
             check(hipStreamBeginCapture(b.stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
             hipError_t err = hipSuccess;
             for (int k = 0; k < n_steps && err == hipSuccess; k++) {
-                BdfArgs a = advance_args(b, nullptr, 0);
+                BdfArgs a = advance_args(b);
                 // the first launch uses the order from before the call; each
                 // later one the order its predecessor's step counts give
                 a.order = (k == 0) ? order : (b.lpt_on ? order_buf : nullptr);
@@ -1324,7 +1324,7 @@ static void advance_surrogate_device(Batch& b, SurrogateObj& so, const double* t
           "surrogate guard");
     // the rejected and the non-eligible reactors, in the batch's longest-first
     // order (a list built by atomics would hand them out in arrival order)
-    BdfArgs a = advance_args(b, nullptr, 0);
+    BdfArgs a = advance_args(b);
     a.work_flag = elig;
     a.work_value = 2;
     check(hipEventRecord(b.e0, b.stream), "hipEventRecord");

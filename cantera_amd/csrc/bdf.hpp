@@ -50,7 +50,6 @@ struct BdfArgs {
     size_t piv_stride;
     const double* tout;   // n, device
     const int* order;     // optional processing order (n), device
-    const int* n_dev;     // optional: number of work items, read on the device (replaces n)
     const int* work_flag; // optional: only reactors r with work_flag[r] == work_value are advanced
     int work_value;
     int* queue;           // device work-queue head

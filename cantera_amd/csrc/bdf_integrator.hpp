@@ -1423,7 +1423,7 @@ __device__ __forceinline__ void bdf_advance_body(const BdfArgs& a) {
     it.rtol = a.rtol;
     it.atol = a.atol;
     it.ls.init(base, a.ld);
-    const int nwork = a.n_dev ? uni(*a.n_dev) : a.n;
+    const int nwork = a.n;
     for (;;) {
         int r = 0;
         if (it.lane == 0) r = atomicAdd(a.queue, 1);
