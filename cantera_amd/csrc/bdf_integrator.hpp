@@ -98,6 +98,13 @@ __device__ __forceinline__ KernargArgs* kernarg_args() {
 // solver (CVodesIntegrator.cpp:424-434, with ReactorNet::setPreconditioner,
 // ReactorNet.cpp:330-339,881-908): the Newton systems are solved by GMRES,
 // J v by difference quotients of the RHS, M = I - gamma J only preconditions.
+// linear-system policies that can take a fresh Jacobian in their own matrix
+// area (jac_in_place, jac_offset, form_in_place)
+template <class LS, class = void>
+struct has_jac_in_place : std::false_type {};
+template <class LS>
+struct has_jac_in_place<LS, std::void_t<decltype(&LS::form_in_place)>> : std::true_type {};
+
 template <int NS, class LS, typename MT, bool kSens = false, bool kMole = false>
 struct Integrator {
     const DevMech& m;
@@ -550,6 +557,7 @@ struct Integrator {
         const bool jbad = (nst == 0) || (nst >= uni(c.nstlj) + MSBJ) ||
                           ((convfail == FAIL_BAD_J) && (dgamma < DGMAX_JBAD)) || (convfail == FAIL_OTHER);
         double* Js = a.j_save + rid * a.j_stride;
+        bool formed = false;   // M formed in place over a Jacobian assembled in the matrix area
         wave_sync();
         if (!jbad) {
             if (lane == 0) c.jcur = 0;
@@ -577,10 +585,17 @@ struct Integrator {
                         S.F0[comp(s)] = ftemp[s];
                     }
                 wave_sync();
-                jac_call<NS>(kernarg_mech(), L, uni(c.P), Js);
+                if constexpr (has_jac_in_place<LS>::value) {
+                    if (ls.jac_in_place(N)) {
+                        jac_lds_call<NS>(kernarg_mech(), L, uni(c.P), ls.jac_offset(), Js);
+                        ls.form_in_place(N, gamma);
+                        formed = true;
+                    }
+                }
+                if (!formed) jac_call<NS>(kernarg_mech(), L, uni(c.P), Js);
             }
         }
-        ls.form(Js, N, gamma);
+        if (!formed) ls.form(Js, N, gamma);
         const bool ok = ls.factor(N);
         if (lane == 0) {
             c.nsetups += 1;

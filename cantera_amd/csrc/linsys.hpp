@@ -295,6 +295,30 @@ struct LdsRegLinSys {
         wave_sync();
     }
 
+    // A fresh Jacobian can be assembled in this matrix area itself (FP64, and
+    // M's row stride equal to N, the Jacobian's): the assembly then uses LDS
+    // atomics instead of global ones, and M = I - gamma J is formed in place.
+    // The area holds (NMAX + 1) x 64 >= NMAX^2 doubles.
+    __device__ __forceinline__ bool jac_in_place(int N) const { return sizeof(MT) == 8 && LD == N; }
+    __device__ __forceinline__ int jac_offset() const { return offA / static_cast<int>(sizeof(double)); }
+    __device__ __forceinline__ void form_in_place(int N, double gamma) {
+        static_assert(NMAX * NMAX <= (NMAX + 1) * kWave, "the Jacobian must fit the matrix area");
+        const int lane = lane_id();
+        double* J = reinterpret_cast<double*>(A);
+        if (lane < N) {
+            int i = 0;
+            for (; i + 8 <= N; i += 8) {
+                double x[8];
+#pragma unroll
+                for (int b = 0; b < 8; b++) x[b] = J[(i + b) * N + lane];
+#pragma unroll
+                for (int b = 0; b < 8; b++) J[(i + b) * N + lane] = fma(-gamma, x[b], (i + b == lane) ? 1.0 : 0.0);
+            }
+            for (; i < N; i++) J[i * N + lane] = fma(-gamma, J[i * N + lane], (i == lane) ? 1.0 : 0.0);
+        }
+        wave_sync();
+    }
+
     __device__ __forceinline__ bool factor(int N) { return lr_factor_mfma<NMAX, MT>(offA, N, LD) != 0; }
 
     template <typename V, int S>

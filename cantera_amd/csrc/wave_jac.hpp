@@ -307,4 +307,27 @@ __device__ __noinline__ void jac_call(KernargMech* mech, ScratchLayout L, double
     reactor_jacobian<NS>(m, S, P, y, f0, J);
 }
 
+// jac_call assembling into LDS at double offset offJ (N x N row-major), then
+// copying the result to Jsave (global, the copy reused while the Jacobian is
+// current).  The LDS pointer is formed here, so the assembly's accesses and
+// atomics are LDS instructions.
+template <int NS>
+__device__ __noinline__ void jac_lds_call(KernargMech* mech, ScratchLayout L, double P, int offJ, double* Jsave) {
+    const DevMech m = *(const DevMech*)mech;
+    const WaveScratch S = scratch_view(L);
+    const int lane = lane_id();
+    double y[NS], f0[NS];
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        const int i = lane + kWave * s;
+        y[s] = (i < m.N) ? S.Y0[i] : 0.0;
+        f0[s] = (i < m.N) ? S.F0[i] : 0.0;
+    }
+    double* J = lds_base() + offJ;
+    reactor_jacobian<NS>(m, S, P, y, f0, J);
+    const int NN = m.N * m.N;
+    for (int e = lane; e < NN; e += kWave) Jsave[e] = J[e];
+    wave_sync();
+}
+
 }  // namespace cg
