@@ -487,6 +487,48 @@ struct LdsLinSys {
         wave_sync();
     }
 
+    // A fresh FP64 Jacobian can be assembled in this matrix area (N x N
+    // row-major, inside the N x LD area): LDS atomics instead of global ones,
+    // then M = I - gamma J formed in place, each row moved to stride LD.
+    __device__ __forceinline__ bool jac_in_place(int) const { return sizeof(MT) == 8; }
+    __device__ __forceinline__ int jac_offset() const {
+        return static_cast<int>((reinterpret_cast<char*>(A) - reinterpret_cast<char*>(lds_base())) / sizeof(double));
+    }
+    // Rows from the last to the first, 8 at a time, every load of a block
+    // before any store: a row's new place (stride LD >= N) overlaps only the
+    // old places of rows below it, which have been read already.
+    __device__ __forceinline__ void form_in_place(int N, double gamma) {
+        const int lane = lane_id();
+        double* J = reinterpret_cast<double*>(A);
+        int i1 = N;   // rows i1 .. N-1 are done
+        while (i1 > 0) {
+            const int i0 = max(i1 - 8, 0);
+            double x[8][NS];
+#pragma unroll
+            for (int b = 0; b < 8; b++)
+#pragma unroll
+                for (int s = 0; s < NS; s++) {
+                    const int i = min(i0 + b, N - 1), j = min(lane + kWave * s, N - 1);
+                    x[b][s] = J[i * N + j];
+                }
+            wave_sync();
+#pragma unroll
+            for (int b = 0; b < 8; b++)
+#pragma unroll
+                for (int s = 0; s < NS; s++) {
+                    const int i = i0 + b, j = lane + kWave * s;
+                    if (i < i1 && j < N) {
+                        // as form: (-gamma) J, plus 1 on the diagonal
+                        double v = -gamma * x[b][s];
+                        if (i == j) v += 1.0;
+                        J[i * LD + j] = v;
+                    }
+                }
+            wave_sync();
+            i1 = i0;
+        }
+    }
+
     __device__ __forceinline__ bool factor(int N) { return lu_factor_mfma<NS, MT>(A, LD, N, pmap); }
 
     template <typename V>
